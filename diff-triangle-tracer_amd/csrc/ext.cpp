@@ -1,0 +1,30 @@
+// pybind11 module diff_tritracer._C: the bindings of the reference's ext.cpp:11-27
+// (FwdTraceResult / BwdTraceResult with .data, OptixTracer with two constructors,
+// build_tri_accel, trace_forward, trace_backward), plus the tracer's options.
+#include <pybind11/pybind11.h>
+#include <torch/extension.h>
+
+#include "tracer.h"
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+  pybind11::class_<FwdTraceResult>(m, "FwdTraceResult")
+      .def(pybind11::init<>())
+      .def_readonly("data", &FwdTraceResult::data);
+
+  pybind11::class_<BwdTraceResult>(m, "BwdTraceResult")
+      .def(pybind11::init<>())
+      .def_readonly("data", &BwdTraceResult::data);
+
+  pybind11::class_<OptixTracer>(m, "OptixTracer")
+      .def(pybind11::init<const std::string &>())
+      .def(pybind11::init<const std::string &, const std::string &, const std::string &>())
+      .def("build_tri_accel", &OptixTracer::build_tri_accel)
+      .def("trace_forward", &OptixTracer::trace_fwd)
+      .def("trace_backward", &OptixTracer::trace_bwd)
+      .def("set_options", &OptixTracer::set_options, pybind11::arg("k") = 16, pybind11::arg("cutoff") = 1e-4,
+           pybind11::arg("chunk_mode") = 0)
+      .def("get_options", &OptixTracer::get_options)
+      .def_property_readonly("variant", &OptixTracer::variant_name);
+
+  m.def("version", []() { return std::string(tt_version()); });
+}

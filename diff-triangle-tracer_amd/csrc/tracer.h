@@ -1,0 +1,89 @@
+// Drop-in host tracer: the reference's C++ interface (tritracer/tracer.h:14-126),
+// same class name, constructors and method signatures, implemented over the
+// C ABI of libtritracer_hip.so (include/tritracer_hip.h) instead of OptiX.
+#pragma once
+#include <torch/extension.h>
+
+#include <stdexcept>
+#include <string>
+#include <tuple>
+
+#include "../../include/tritracer_hip.h"
+
+typedef torch::Tensor Tensor;
+
+// Errors surface as Exception (a std::runtime_error), which pybind11 turns into
+// a Python RuntimeError -- as the reference's exception.h:16-51.
+class Exception : public std::runtime_error {
+ public:
+  explicit Exception(const char *msg) : std::runtime_error(msg) {}
+  explicit Exception(const std::string &msg) : std::runtime_error(msg) {}
+};
+
+// Per-program state. The reference keeps the OptiX GAS, index buffer and a CUDA
+// stream here (tracer.h:30-57); this tracer keeps the vertex/cumsum buffers the
+// acceleration structure was last built from. Work runs on torch's current
+// HIP stream.
+struct TracerState {
+  const float *d_vertices = nullptr;
+  const int *d_cumsum = nullptr;
+  Tensor indexbuf_tensor;  // keeps the cumsum the structure was built from alive
+};
+
+struct FwdTraceResult {
+  std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> data;
+};
+
+struct BwdTraceResult {
+  std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> data;
+};
+
+class OptixTracer {
+ public:
+  TracerState fwdState;
+  TracerState bwdState;
+
+  // `forward` selects the forward program: forward | reflect | refract | envlight
+  // (any extension, e.g. "reflect.ptx"); `backward` must name the backward
+  // program. An unknown name throws, as a missing PTX file does in the reference.
+  OptixTracer(const std::string &pkgDir) : OptixTracer(pkgDir, "forward.ptx", "backward.ptx") {}
+  OptixTracer(const std::string &pkgDir, const std::string &forward, const std::string &backward);
+
+  void build_tri_accel(TracerState &state, uint32_t numVertices, uint32_t numTri, bool rebuild);
+
+  FwdTraceResult trace_fwd(const Tensor &ray_o, const Tensor &ray_d, const Tensor &triangles_points,
+                           const Tensor &sigma, const Tensor &num_points_per_triangle,
+                           const Tensor &cumsum_of_points_per_triangle, const int number_of_points, const Tensor &shs,
+                           const int sh_degree, const Tensor &colors_precomp, const Tensor &opacities, Tensor &scaling,
+                           Tensor &density_factor, const Tensor &bg_color, const Tensor &extra,
+                           const int rebuild_gas_opt, const bool debug);
+
+  BwdTraceResult trace_bwd(const Tensor &ray_o, const Tensor &ray_d, const Tensor &background,
+                           const Tensor &triangles_points, const Tensor &sigma,
+                           const torch::Tensor &num_points_per_triangle,
+                           const torch::Tensor &cumsum_of_points_per_triangle, const torch::Tensor &colors,
+                           const torch::Tensor &others, const int number_of_points, const Tensor &shs,
+                           const int degree, const Tensor &opacities, const Tensor &dL_dpixels,
+                           const Tensor &dL_dothers, const Tensor &scaling, const bool debug);
+
+  // Options this tracer adds (the reference hard-codes CHUNK_SIZE 16 and a 1e-4
+  // transmittance cutoff, config.h:10 and forward.cu:134).
+  void set_options(int k, double cutoff, int chunk_mode);
+  std::tuple<int, double, int> get_options() const;
+  std::string variant_name() const;
+
+  ~OptixTracer();
+
+  OptixTracer(OptixTracer &&other) noexcept;
+  OptixTracer &operator=(OptixTracer &&other) noexcept;
+  OptixTracer(const OptixTracer &) = delete;
+  OptixTracer &operator=(const OptixTracer &) = delete;
+
+ private:
+  tt_tracer *tr_ = nullptr;
+  int variant_ = TT_VARIANT_FORWARD;
+  int k_ = 16;
+  float cutoff_ = 1e-4f;
+  int chunk_mode_ = TT_CHUNK_COMPAT;
+  int device_ = -1;
+};

@@ -1,0 +1,409 @@
+// LBVH builder and refit for gfx950.
+//
+// Replaces optixAccelBuild (reference tritracer/pipeline.cpp:206-265), which is
+// closed driver code. Pipeline (Karras, "Maximizing Parallelism in the
+// Construction of BVHs, Octrees, and k-d Trees", HPG 2012):
+//   1. per-triangle AABB centroids + scene centroid bounds (wave/LDS reduction,
+//      ordered-uint atomics)
+//   2. 30-bit Morton codes over the centroid bounds
+//   3. LSD radix sort, 4 x 8-bit passes: LDS digit histograms, one-block
+//      exclusive scan, stable LDS-ranked scatter (ballot peer masks, wave64)
+//   4. Karras hierarchy (index tie-break for equal codes), parent links
+//   5. bottom-up child boxes with per-node atomic arrival counters; the same
+//      kernel is the refit (topology kept, leaves re-gathered from new vertices)
+// Child boxes live in the parent node, so traversal reads one 64 B node to
+// test both children.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "tt_internal.h"
+
+namespace tt {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_init_bounds(uint32_t *bounds) {
+  int i = threadIdx.x;
+  if (i < 3) bounds[i] = 0xffffffffu;        // min
+  else if (i < 6) bounds[i] = 0u;            // max
+}
+
+__global__ __launch_bounds__(256) void k_centroids(const float *__restrict__ verts, const int *__restrict__ cumsum,
+                                                   int P, float4 *__restrict__ cen, uint32_t *bounds) {
+  __shared__ float s_lo[3][4], s_hi[3][4];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float lo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
+  float hi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
+  if (i < P) {
+    const float *v = verts + 3 * (size_t)cumsum[i];
+    float c[3];
+    for (int a = 0; a < 3; a++) {
+      float mn = fminf(v[a], fminf(v[3 + a], v[6 + a]));
+      float mx = fmaxf(v[a], fmaxf(v[3 + a], v[6 + a]));
+      c[a] = 0.5f * (mn + mx);
+      lo[a] = c[a];
+      hi[a] = c[a];
+    }
+    cen[i] = make_float4(c[0], c[1], c[2], 0.0f);
+  }
+  for (int a = 0; a < 3; a++) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
+    }
+  }
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int a = 0; a < 3; a++) s_lo[a][wave] = lo[a], s_hi[a][wave] = hi[a];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    int a = threadIdx.x;
+    float l = s_lo[a][0], h = s_hi[a][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) l = fminf(l, s_lo[a][w]), h = fmaxf(h, s_hi[a][w]);
+    if (l <= h) {
+      atomicMin(bounds + a, f2ord(l));
+      atomicMax(bounds + 3 + a, f2ord(h));
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float4 *__restrict__ cen, int P, const uint32_t *bounds,
+                                                uint32_t *keys, uint32_t *vals) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  float4 c = cen[i];
+  float cc[3] = {c.x, c.y, c.z};
+  uint32_t q[3];
+  for (int a = 0; a < 3; a++) {
+    float lo = ord2f(bounds[a]), hi = ord2f(bounds[3 + a]);
+    float ext = hi - lo;
+    float x = ext > 0.0f ? (cc[a] - lo) / ext : 0.5f;
+    x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+    q[a] = (uint32_t)x;
+  }
+  keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+  vals[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(kSortThreads) void k_hist(const uint32_t *__restrict__ keys, int n, int shift,
+                                                        uint32_t *__restrict__ hist, int nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int base = blockIdx.x * kSortTile;
+  int end = min(base + kSortTile, n);
+  for (int i = base + threadIdx.x; i < end; i += kSortThreads) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  __syncthreads();
+  hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of `n` counts in place, one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan(uint32_t *data, int n) {
+  __shared__ uint32_t s[1024];
+  int per = (n + 1023) / 1024;
+  int b = threadIdx.x * per;
+  int e = min(b + per, n);
+  uint32_t sum = 0;
+  for (int i = b; i < e; i++) sum += data[i];
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - sum;
+  for (int i = b; i < e; i++) {
+    uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_scatter(const uint32_t *__restrict__ kin,
+                                                           const uint32_t *__restrict__ vin, uint32_t *kout,
+                                                           uint32_t *vout, int n, int shift,
+                                                           const uint32_t *__restrict__ offs, int nblocks) {
+  constexpr int W = kSortThreads / 64;
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t cnt[256];
+  __shared__ uint32_t wcnt[W][256];
+  int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  base[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
+  cnt[tid] = 0;
+  for (int w = 0; w < W; w++) wcnt[w][tid] = 0;
+  __syncthreads();
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int r = 0; r < kSortItems; r++) {
+    int i = blockIdx.x * kSortTile + r * kSortThreads + tid;
+    bool valid = i < n;
+    uint32_t key = valid ? kin[i] : 0u;
+    uint32_t val = valid ? vin[i] : 0u;
+    uint32_t dg = (key >> shift) & 255u;
+    unsigned long long peers = __ballot(valid);
+    for (int bit = 0; bit < 8; bit++) {
+      unsigned long long bb = __ballot(valid && ((dg >> bit) & 1u));
+      peers &= ((dg >> bit) & 1u) ? bb : ~bb;
+    }
+    int rank = __popcll(peers & lt_mask);
+    if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = base[dg] + cnt[dg] + (uint32_t)rank;
+      for (int w = 0; w < wave; w++) pos += wcnt[w][dg];
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < W; w++) add += wcnt[w][tid], wcnt[w][tid] = 0;
+    cnt[tid] += add;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- hierarchy
+__device__ __forceinline__ int delta(const uint32_t *keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = keys[i], b = keys[j];
+  if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+  return __clz((int)(a ^ b));
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ keys, int n, int4 *links,
+                                                int *leaf_parent) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = delta(keys, n, i, j);
+  int s = 0;
+  int t = l;
+  do {
+    t = (t + 1) / 2;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  int gamma = i + s * d + min(d, 0);
+  int left = (min(i, j) == gamma) ? ~gamma : gamma;
+  int right = (max(i, j) == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+  int *li = reinterpret_cast<int *>(links + 4 * (size_t)i + 3);
+  li[0] = left;
+  li[1] = right;
+  li[3] = 0;
+  if (i == 0) li[2] = -1;
+  if (left < 0)
+    leaf_parent[~left] = i;
+  else
+    reinterpret_cast<int *>(links + 4 * (size_t)left + 3)[2] = i;
+  if (right < 0)
+    leaf_parent[~right] = i;
+  else
+    reinterpret_cast<int *>(links + 4 * (size_t)right + 3)[2] = i;
+}
+
+// Two-leaf tree for P == 1: leaf 0 is the triangle, leaf 1 an unhittable dummy.
+__global__ void k_single(int4 *links, int *leaf_parent, uint32_t *vals) {
+  int *li = reinterpret_cast<int *>(links + 3);
+  li[0] = ~0;
+  li[1] = ~1;
+  li[2] = -1;
+  li[3] = 0;
+  leaf_parent[0] = 0;
+  leaf_parent[1] = 0;
+  vals[0] = 0u;
+  vals[1] = 0xFFFFFFFFu;
+}
+
+__global__ void k_zero_int(int *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__device__ __forceinline__ float ld_acq(const float *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Leaves (re)gathered in Morton order, then boxes propagated to the root: the
+// second child to arrive at a node unions both child boxes and moves up.
+__global__ __launch_bounds__(256) void k_bottom_up(const float *__restrict__ verts, const int *__restrict__ cumsum,
+                                                   const uint32_t *__restrict__ order, int n_leaves,
+                                                   float4 *leaves, float4 *nodes, const int *__restrict__ leaf_parent,
+                                                   int *flags) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_leaves) return;
+  uint32_t prim = order[s];
+  float A[3], B[3], C[3];
+  if (prim != 0xFFFFFFFFu) {
+    const float *v = verts + 3 * (size_t)cumsum[prim];
+    for (int a = 0; a < 3; a++) A[a] = v[a], B[a] = v[3 + a], C[a] = v[6 + a];
+  } else {
+    for (int a = 0; a < 3; a++) A[a] = B[a] = C[a] = 1e30f;
+  }
+  leaves[3 * (size_t)s + 0] = make_float4(A[0], A[1], A[2], __uint_as_float(prim));
+  leaves[3 * (size_t)s + 1] = make_float4(B[0], B[1], B[2], 0.0f);
+  leaves[3 * (size_t)s + 2] = make_float4(C[0], C[1], C[2], 0.0f);
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = fminf(A[a], fminf(B[a], C[a]));
+    hi[a] = fmaxf(A[a], fmaxf(B[a], C[a]));
+    float pad = 1e-5f * (1.0f + fmaxf(fabsf(lo[a]), fabsf(hi[a])));
+    lo[a] -= pad;
+    hi[a] += pad;
+  }
+  int child = ~s;
+  int p = leaf_parent[s];
+  while (true) {
+    float4 *nd = nodes + 4 * (size_t)p;
+    const int *li = reinterpret_cast<const int *>(nd + 3);
+    int slot = (li[0] == child) ? 0 : 1;
+    float *b = reinterpret_cast<float *>(nd + slot);
+    b[0] = lo[0], b[1] = hi[0], b[2] = lo[1], b[3] = hi[1];
+    float *bz = reinterpret_cast<float *>(nd + 2) + 2 * slot;
+    bz[0] = lo[2], bz[1] = hi[2];
+    __threadfence();
+    int old = atomicAdd(flags + p, 1);
+    if (old == 0) return;
+    __threadfence();
+    const float *f0 = reinterpret_cast<const float *>(nd);
+    const float *f1 = reinterpret_cast<const float *>(nd + 1);
+    const float *f2 = reinterpret_cast<const float *>(nd + 2);
+    lo[0] = fminf(ld_acq(f0 + 0), ld_acq(f1 + 0));
+    hi[0] = fmaxf(ld_acq(f0 + 1), ld_acq(f1 + 1));
+    lo[1] = fminf(ld_acq(f0 + 2), ld_acq(f1 + 2));
+    hi[1] = fmaxf(ld_acq(f0 + 3), ld_acq(f1 + 3));
+    lo[2] = fminf(ld_acq(f2 + 0), ld_acq(f2 + 2));
+    hi[2] = fmaxf(ld_acq(f2 + 1), ld_acq(f2 + 3));
+    int parent = li[2];
+    if (parent < 0) return;
+    child = p;
+    p = parent;
+  }
+}
+
+template <class T>
+hipError_t grow(T *&ptr, size_t &cap, size_t need) {
+  if (cap >= need && ptr) return hipSuccess;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&ptr), need * sizeof(T));
+  if (e == hipSuccess) cap = need;
+  return e;
+}
+
+}  // namespace
+
+void bvh_free(Bvh &b) {
+  void *ps[] = {b.nodes, b.leaves, b.leaf_parent, b.flags, b.keys[0], b.keys[1], b.vals[0], b.vals[1],
+                b.hist, b.bounds, b.centroids};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  b = Bvh();
+}
+
+#define TT_TRY(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return std::string(#x " failed: ") + hipGetErrorString(e_);     \
+  } while (0)
+
+std::string bvh_build(Bvh &b, const float *verts, int num_vertices, const int *cumsum, int P, int mode,
+                      hipStream_t stream) {
+  (void)num_vertices;
+  if (mode == 1) {
+    if (b.P < 0) return "refit requested but no acceleration structure has been built";
+    if (b.P != P) return "refit requires the same number of triangles as the last build";
+  }
+  if (P == 0) {
+    b.P = 0;
+    b.n_leaves = 0;
+    b.n_internal = 0;
+    return "";
+  }
+  const int nl = P == 1 ? 2 : P;
+  const int ni = nl - 1;
+  if (mode == 0) {
+    size_t need = (size_t)nl;
+    if (b.cap_leaves < need || !b.nodes) {
+      void *ps[] = {b.nodes, b.leaves, b.leaf_parent, b.flags, b.keys[0], b.keys[1], b.vals[0], b.vals[1], b.centroids};
+      for (void *p : ps)
+        if (p) (void)hipFree(p);
+      b.nodes = nullptr, b.leaves = nullptr, b.leaf_parent = nullptr, b.flags = nullptr;
+      b.keys[0] = b.keys[1] = b.vals[0] = b.vals[1] = nullptr;
+      b.centroids = nullptr;
+      b.cap_leaves = 0;
+      TT_TRY(hipMalloc(&b.nodes, sizeof(float4) * 4 * need));
+      TT_TRY(hipMalloc(&b.leaves, sizeof(float4) * 3 * need));
+      TT_TRY(hipMalloc(&b.leaf_parent, sizeof(int) * need));
+      TT_TRY(hipMalloc(&b.flags, sizeof(int) * need));
+      TT_TRY(hipMalloc(&b.keys[0], sizeof(uint32_t) * need));
+      TT_TRY(hipMalloc(&b.keys[1], sizeof(uint32_t) * need));
+      TT_TRY(hipMalloc(&b.vals[0], sizeof(uint32_t) * need));
+      TT_TRY(hipMalloc(&b.vals[1], sizeof(uint32_t) * need));
+      TT_TRY(hipMalloc(&b.centroids, sizeof(float4) * need));
+      b.cap_leaves = need;
+    }
+    if (!b.bounds) TT_TRY(hipMalloc(&b.bounds, sizeof(uint32_t) * 8));
+    if (P == 1) {
+      hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, stream, reinterpret_cast<int4 *>(b.nodes), b.leaf_parent,
+                         b.vals[0]);
+    } else {
+      const int nblk = (P + 255) / 256;
+      hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, stream, b.bounds);
+      hipLaunchKernelGGL(k_centroids, dim3(nblk), dim3(256), 0, stream, verts, cumsum, P, b.centroids, b.bounds);
+      hipLaunchKernelGGL(k_morton, dim3(nblk), dim3(256), 0, stream, b.centroids, P, b.bounds, b.keys[0], b.vals[0]);
+      const int sblk = (P + kSortTile - 1) / kSortTile;
+      TT_TRY(grow(b.hist, b.cap_hist, (size_t)256 * sblk));
+      int cur = 0;
+      for (int shift = 0; shift < 32; shift += 8) {
+        hipLaunchKernelGGL(k_hist, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], P, shift, b.hist, sblk);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, b.hist, 256 * sblk);
+        hipLaunchKernelGGL(k_scatter, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], b.vals[cur],
+                           b.keys[cur ^ 1], b.vals[cur ^ 1], P, shift, b.hist, sblk);
+        cur ^= 1;
+      }
+      // four passes: sorted data is back in buffer 0
+      hipLaunchKernelGGL(k_karras, dim3((ni + 255) / 256), dim3(256), 0, stream, b.keys[0], P,
+                         reinterpret_cast<int4 *>(b.nodes), b.leaf_parent);
+    }
+    b.P = P;
+    b.n_leaves = nl;
+    b.n_internal = ni;
+    b.cumsum_seen = cumsum;
+  }
+  hipLaunchKernelGGL(k_zero_int, dim3((ni + 255) / 256), dim3(256), 0, stream, b.flags, ni);
+  hipLaunchKernelGGL(k_bottom_up, dim3((nl + 255) / 256), dim3(256), 0, stream, verts, cumsum, b.vals[0], nl,
+                     b.leaves, b.nodes, b.leaf_parent, b.flags);
+  TT_TRY(hipGetLastError());
+  return "";
+}
+
+}  // namespace tt

@@ -1,0 +1,251 @@
+// C ABI of libtritracer_hip.so (declared in include/tritracer_hip.h).
+//
+// Thin layer: validates arguments, owns the LBVH buffers, launches the kernels
+// on the caller's stream, and reports failures as status codes plus a
+// thread-local message (the reference throws Exception from CUDA_CHECK /
+// OPTIX_CHECK, tritracer/exception.h:32-51; its C++ caller does the same from
+// these codes).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/tritracer_hip.h"
+#include "tt_internal.h"
+
+struct tt_tracer {
+  int device = 0;
+  tt::Bvh bvh;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool build_timed = false;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+int ok() {
+  g_err.clear();
+  return TT_OK;
+}
+
+#define TT_ARG(cond, msg) \
+  do {                    \
+    if (!(cond)) return fail(TT_ERR_INVALID_ARG, msg); \
+  } while (0)
+
+int check_common(const tt_tracer *tr, int W, int H, const float *ro, const float *rd, const float *verts,
+                 const int *npts, const int *cumsum, int P, const float *shs, int M, int deg, const float *opac,
+                 const float *sigma, int k, int chunk_mode) {
+  TT_ARG(tr, "tracer is NULL");
+  TT_ARG(W >= 0 && H >= 0, "image size must be non-negative");
+  TT_ARG((long long)W * H < (1ll << 31), "too many rays for one launch");
+  TT_ARG(P >= 0, "number of triangles must be non-negative");
+  TT_ARG(k == 8 || k == 16 || k == 32, "k (hits per chunk) must be 8, 16 or 32");
+  TT_ARG(chunk_mode == TT_CHUNK_COMPAT || chunk_mode == TT_CHUNK_EXACT, "unknown chunk mode");
+  if ((long long)W * H > 0) TT_ARG(ro && rd, "ray buffers are NULL");
+  if (P > 0) {
+    TT_ARG(verts && npts && cumsum && opac && sigma, "scene buffers are NULL");
+    TT_ARG(deg >= 0 && deg <= 3, "sh_degree must be in [0, 3]");
+    TT_ARG(shs != nullptr && M >= (deg + 1) * (deg + 1),
+           "SH coefficients are required (precomputed colours are not supported by the tracer, as in the "
+           "reference where params.cols is never set): shs must be [P, M, 3] with M >= (sh_degree+1)^2");
+    if (tr->bvh.P < 0) return fail(TT_ERR_STATE, "no acceleration structure: build it first (rebuild_gas = 0)");
+    if (tr->bvh.P != P)
+      return fail(TT_ERR_STATE, "the acceleration structure holds " + std::to_string(tr->bvh.P) +
+                                    " triangles but " + std::to_string(P) + " were passed; rebuild it");
+  }
+  return TT_OK;
+}
+
+void fill_common(tt::TraceParams &p, const tt_tracer *tr, int W, int H, const float *ro, const float *rd,
+                 const float *verts, const int *npts, const int *cumsum, int P, const float *shs, int M, int deg,
+                 const float *opac, const float *sigma, int chunk_mode, float cutoff, float near_t, float far_t) {
+  p = tt::TraceParams{};
+  p.W = W;
+  p.H = H;
+  p.R = W * H;
+  p.ray_o = ro;
+  p.ray_d = rd;
+  p.verts = verts;
+  p.npts = npts;
+  p.cumsum = cumsum;
+  p.P = P;
+  p.shs = shs;
+  p.M = M;
+  p.deg = deg;
+  p.opac = opac;
+  p.sigma = sigma;
+  p.nodes = tr->bvh.nodes;
+  p.leaves = tr->bvh.leaves;
+  p.n_leaves = P > 0 ? tr->bvh.n_leaves : 0;
+  p.chunk_mode = chunk_mode;
+  p.cutoff = cutoff;
+  p.near_t = near_t;
+  p.far_t = far_t;
+}
+}  // namespace
+
+extern "C" {
+
+const char *tt_last_error(void) {
+  return g_err.c_str();
+}
+
+const char *tt_version(void) {
+  return "tritracer_hip 0.1 (gfx950)";
+}
+
+int tt_create(tt_tracer **out) {
+  if (!out) return fail(TT_ERR_INVALID_ARG, "out is NULL");
+  tt_tracer *tr = new tt_tracer();
+  hipError_t e = hipGetDevice(&tr->device);
+  if (e != hipSuccess) {
+    delete tr;
+    return fail(TT_ERR_HIP, std::string("hipGetDevice failed: ") + hipGetErrorString(e));
+  }
+  *out = tr;
+  return ok();
+}
+
+int tt_destroy(tt_tracer *tr) {
+  if (!tr) return ok();
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(tr->device);
+  tt::bvh_free(tr->bvh);
+  if (tr->ev0) (void)hipEventDestroy(tr->ev0);
+  if (tr->ev1) (void)hipEventDestroy(tr->ev1);
+  (void)hipSetDevice(dev);
+  delete tr;
+  return ok();
+}
+
+int tt_build(tt_tracer *tr, const float *vertices_dev, int num_vertices, const int *cumsum_dev, int P, int mode,
+             void *stream) {
+  TT_ARG(tr, "tracer is NULL");
+  TT_ARG(mode == TT_BUILD || mode == TT_REFIT || mode == TT_REUSE, "mode must be 0 (build), 1 (refit) or 2 (reuse)");
+  TT_ARG(P >= 0 && num_vertices >= 0, "sizes must be non-negative");
+  if (mode == TT_REUSE) {
+    if (tr->bvh.P < 0) return fail(TT_ERR_STATE, "rebuild_gas = 2 (reuse) but no acceleration structure exists");
+    return ok();
+  }
+  if (P > 0) TT_ARG(vertices_dev && cumsum_dev, "vertex / cumsum buffers are NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!tr->ev0) {
+    if (hipEventCreate(&tr->ev0) != hipSuccess || hipEventCreate(&tr->ev1) != hipSuccess)
+      return fail(TT_ERR_HIP, "hipEventCreate failed");
+  }
+  (void)hipEventRecord(tr->ev0, s);
+  std::string err = tt::bvh_build(tr->bvh, vertices_dev, num_vertices, cumsum_dev, P, mode, s);
+  if (!err.empty()) {
+    tr->build_timed = false;
+    return fail(mode == TT_REFIT && err.find("refit") != std::string::npos ? TT_ERR_STATE : TT_ERR_HIP, err);
+  }
+  (void)hipEventRecord(tr->ev1, s);
+  tr->build_timed = true;
+  return ok();
+}
+
+int tt_num_primitives(const tt_tracer *tr) {
+  return tr ? tr->bvh.P : -1;
+}
+
+float tt_last_build_ms(const tt_tracer *tr) {
+  if (!tr || !tr->build_timed) return -1.0f;
+  if (hipEventSynchronize(tr->ev1) != hipSuccess) return -1.0f;
+  float ms = -1.0f;
+  if (hipEventElapsedTime(&ms, tr->ev0, tr->ev1) != hipSuccess) return -1.0f;
+  return ms;
+}
+
+int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
+  TT_ARG(a, "args is NULL");
+  int rc = check_common(tr, a->width, a->height, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev,
+                        a->cumsum_dev, a->P, a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->k,
+                        a->chunk_mode);
+  if (rc != TT_OK) return rc;
+  TT_ARG(a->variant >= TT_VARIANT_FORWARD && a->variant <= TT_VARIANT_ENVLIGHT, "unknown variant");
+  const long long R = (long long)a->width * a->height;
+  if (R > 0) {
+    TT_ARG(a->color_dev && a->bg_dev, "colour / background buffers are NULL");
+    if (a->variant == TT_VARIANT_FORWARD) {
+      TT_ARG(a->others_dev, "others buffer is NULL");
+      if (a->P > 0) TT_ARG(a->max_blending_dev && a->density_dev && a->scaling_dev, "statistics buffers are NULL");
+    }
+    if (a->variant == TT_VARIANT_REFLECT || a->variant == TT_VARIANT_REFRACT)
+      TT_ARG(a->extra_dev && a->extra_len >= 4, "reflect/refract need extra_params = (r, cx, cy, cz)");
+    if (a->variant == TT_VARIANT_ENVLIGHT)
+      TT_ARG(a->extra_dev && a->extra_len >= 5, "envlight needs extra_params = (r, cx, cy, cz, samples)");
+    if (a->hit_log_cap > 0)
+      TT_ARG(a->hit_log_prim_dev && a->hit_log_t_dev && a->hit_log_code_dev && a->hit_log_count_dev,
+             "hit log buffers are NULL");
+  }
+  if (R == 0) return ok();
+  tt::TraceParams p;
+  fill_common(p, tr, a->width, a->height, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev, a->cumsum_dev,
+              a->P, a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->chunk_mode, a->cutoff,
+              a->near_t, a->far_t);
+  p.bg = a->bg_dev;
+  p.extra = a->extra_dev;
+  p.extra_len = a->extra_len;
+  p.color = a->color_dev;
+  p.others = a->others_dev;
+  p.maxb = a->max_blending_dev;
+  p.density = a->density_dev;
+  p.scaling = a->scaling_dev;
+  p.stats = a->stats_dev;
+  p.log_cap = a->hit_log_cap > 0 ? a->hit_log_cap : 0;
+  p.log_prim = a->hit_log_prim_dev;
+  p.log_t = a->hit_log_t_dev;
+  p.log_code = a->hit_log_code_dev;
+  p.log_count = a->hit_log_count_dev;
+  std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
+  if (!err.empty()) return fail(TT_ERR_HIP, err);
+  return ok();
+}
+
+int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
+  TT_ARG(a, "args is NULL");
+  int rc = check_common(tr, a->width, a->height, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev,
+                        a->cumsum_dev, a->P, a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->k,
+                        a->chunk_mode);
+  if (rc != TT_OK) return rc;
+  const long long R = (long long)a->width * a->height;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (a->P > 0) {
+    TT_ARG(a->dL_dvertices_dev && a->dL_dsigma_dev && a->dL_dopacity_dev && a->dL_dsh_dev,
+           "gradient buffers are NULL");
+    TT_ARG(a->dL_dvertices_rows >= 0, "dL_dvertices_rows must be non-negative");
+    if (a->zero_grads) {
+      std::string e;
+      e = tt::launch_fill(a->dL_dvertices_dev, (size_t)a->dL_dvertices_rows * 3, 0.0f, s);
+      if (e.empty()) e = tt::launch_fill(a->dL_dsigma_dev, (size_t)a->P, 0.0f, s);
+      if (e.empty()) e = tt::launch_fill(a->dL_dopacity_dev, (size_t)a->P, 0.0f, s);
+      if (e.empty()) e = tt::launch_fill(a->dL_dsh_dev, (size_t)a->P * a->M * 3, 0.0f, s);
+      if (!e.empty()) return fail(TT_ERR_HIP, e);
+    }
+  }
+  if (R == 0 || a->P == 0) return ok();
+  TT_ARG(a->color_dev && a->others_dev && a->dL_dpix_dev && a->dL_dothers_dev,
+         "forward outputs / upstream gradients are NULL");
+  tt::TraceParams p;
+  fill_common(p, tr, a->width, a->height, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev, a->cumsum_dev,
+              a->P, a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->chunk_mode, a->cutoff,
+              a->near_t, a->far_t);
+  p.fcolor = a->color_dev;
+  p.fothers = a->others_dev;
+  p.dpix = a->dL_dpix_dev;
+  p.dothers = a->dL_dothers_dev;
+  p.dverts = a->dL_dvertices_dev;
+  p.dsigma = a->dL_dsigma_dev;
+  p.dopac = a->dL_dopacity_dev;
+  p.dsh = a->dL_dsh_dev;
+  std::string err = tt::launch_backward(p, a->k, s);
+  if (!err.empty()) return fail(TT_ERR_HIP, err);
+  return ok();
+}
+
+}  // extern "C"

@@ -1,0 +1,395 @@
+// Device-side math of the triangle-splat tracer (gfx950).
+//
+// Everything here is compiled with -ffp-contract=off: the arithmetic rounds
+// where the CPU oracle's FP32 path rounds, so hit lists and per-hit decisions
+// compare bit-exactly. The formulas restate the reference:
+//   window function ........ tritracer/common.h:57-114 (computeWindowFunc)
+//   SH colour .............. tritracer/common.h:16-55 (computeColorFromSH)
+//   SH / window backward ... tritracer/backward.cu:25-189, vecmath.h:2785-2859
+//   sphere helpers ......... tritracer/common.h:116-148, reflect.cu, refract.cu, envlight.cu
+// The ray/triangle test (inside optixTrace in the reference) is the watertight
+// test of Woop, Benthin and Wald (JCGT 2013), double-sided.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TT_D __device__ __forceinline__
+
+namespace tt {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;  // reference RayHit::invalid_id (params.h:119)
+constexpr float kInfDist = 1e6f;            // reference RayHit::infinite_dist (params.h:118)
+constexpr double kStepEps = 1e-9;           // reference STEP_EPSILON (config.h:11), a double literal
+
+// SH constants (auxiliary.h:33-50): real spherical-harmonic normalisations.
+constexpr float SH_C0 = 0.28209479177387814f;
+constexpr float SH_C1 = 0.4886025119029199f;
+constexpr float SH_C2_0 = 1.0925484305920792f, SH_C2_1 = -1.0925484305920792f, SH_C2_2 = 0.31539156525252005f,
+                SH_C2_3 = -1.0925484305920792f, SH_C2_4 = 0.5462742152960396f;
+constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f, SH_C3_2 = -0.4570457994644658f,
+                SH_C3_3 = 0.3731763325901154f, SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
+                SH_C3_6 = -0.5900435899266435f;
+
+// ------------------------------------------------------------------ vectors
+struct v3f {
+  float x, y, z;
+};
+struct v3d {
+  double x, y, z;
+};
+TT_D v3f mk(float x, float y, float z) {
+  return v3f{x, y, z};
+}
+TT_D v3f operator+(v3f a, v3f b) {
+  return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+TT_D v3f operator-(v3f a, v3f b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+TT_D v3f operator-(v3f a) {
+  return {-a.x, -a.y, -a.z};
+}
+TT_D v3f operator*(v3f a, float s) {
+  return {a.x * s, a.y * s, a.z * s};
+}
+TT_D v3f operator*(float s, v3f a) {
+  return {s * a.x, s * a.y, s * a.z};
+}
+TT_D v3f hmul(v3f a, v3f b) {
+  return {a.x * b.x, a.y * b.y, a.z * b.z};
+}
+TT_D float dot(v3f a, v3f b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+TT_D v3f cross(v3f a, v3f b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+TT_D float vlen(v3f v) {
+  return sqrtf(dot(v, v));
+}
+TT_D v3f normalize(v3f v) {
+  float inv = 1.0f / sqrtf(dot(v, v));
+  return v * inv;
+}
+TT_D v3f divs(v3f a, float s) {
+  float inv = 1.0f / s;
+  return a * inv;
+}
+TT_D float sumv(v3f a) {
+  return a.x + a.y + a.z;
+}
+TT_D float comp(v3f v, int i) {
+  return i == 0 ? v.x : (i == 1 ? v.y : v.z);
+}
+TT_D v3d operator+(v3d a, v3d b) {
+  return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+TT_D v3d operator-(v3d a, v3d b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+TT_D v3d operator*(v3d a, double s) {
+  return {a.x * s, a.y * s, a.z * s};
+}
+TT_D double dot(v3d a, v3d b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+TT_D v3d cross(v3d a, v3d b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+TT_D v3d todbl(v3f a) {
+  return {(double)a.x, (double)a.y, (double)a.z};
+}
+TT_D v3f toflt(v3d a) {
+  return {(float)a.x, (float)a.y, (float)a.z};
+}
+TT_D v3f ld3(const float *p) {
+  return {p[0], p[1], p[2]};
+}
+
+// ---------------------------------------------- project-written FP32 functions
+// log2: x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh((m-1)/(m+1)) to s^9.
+// Arguments below the smallest normal count as zero.
+TT_D float log2_tt(float x) {
+  if (!(x > 0.0f)) return (x == 0.0f) ? -__builtin_huge_valf() : __builtin_nanf("");
+  if (x == __builtin_huge_valf()) return x;
+  if (x < 1.17549435e-38f) return -__builtin_huge_valf();
+  uint32_t b = __float_as_uint(x);
+  int e = (int)((b >> 23) & 0xffu) - 127;
+  float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e = e + 1;
+  }
+  float f = m - 1.0f;
+  float s = f / (2.0f + f);
+  float z = s * s;
+  float r = z * (0.666666667f + z * (0.4f + z * (0.285714286f + z * 0.222222222f)));
+  float ln = (s + s) + s * r;
+  return (float)e + ln * 1.44269504f;
+}
+// exp2: n = rint(y), f = y - n, e^(f ln2) Taylor to degree 7, exact 2^n scaling;
+// results below 2^-126 flush to zero.
+TT_D float exp2_tt(float y) {
+  if (y != y) return y;
+  if (y >= 128.0f) return __builtin_huge_valf();
+  if (y < -126.0f) return 0.0f;
+  float n = rintf(y);
+  float f = y - n;
+  float g = f * 0.693147181f;
+  float p = 1.0f +
+            g * (1.0f +
+                 g * (0.5f + g * (0.166666667f +
+                                  g * (0.0416666667f + g * (0.00833333333f + g * (0.00138888889f + g * 0.000198412698f))))));
+  int ni = (int)n;
+  int h = ni / 2;
+  float s1 = __uint_as_float((uint32_t)(h + 127) << 23);
+  float s2 = __uint_as_float((uint32_t)(ni - h + 127) << 23);
+  return (p * s1) * s2;
+}
+TT_D float pow_tt(float x, float y) {
+  return exp2_tt(y * log2_tt(x));
+}
+TT_D float log_tt(float x) {
+  return log2_tt(x) * 0.693147181f;
+}
+TT_D void sincos_tt(float x, float &sn, float &cs) {
+  float q = rintf(x * 0.636619772f);
+  float r = (x - q * 1.5703125f) - q * 4.83826794e-4f;
+  float z = r * r;
+  float s = r + r * z * (-0.166666667f + z * (0.00833333333f + z * (-0.000198412698f + z * 2.75573192e-6f)));
+  float c = 1.0f + z * (-0.5f + z * (0.0416666667f + z * (-0.00138888889f + z * (2.48015873e-5f + z * -2.75573192e-7f))));
+  int qi = ((int)q) & 3;
+  if (qi == 0) {
+    sn = s, cs = c;
+  } else if (qi == 1) {
+    sn = c, cs = -s;
+  } else if (qi == 2) {
+    sn = -s, cs = -c;
+  } else {
+    sn = -c, cs = s;
+  }
+}
+
+// ------------------------------------------------------------ intersection
+struct RayPre {
+  v3f o, d, inv;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+TT_D RayPre prep_ray(v3f o, v3f d) {
+  RayPre r;
+  r.o = o;
+  r.d = d;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  r.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  r.kx = r.kz + 1;
+  if (r.kx == 3) r.kx = 0;
+  r.ky = r.kx + 1;
+  if (r.ky == 3) r.ky = 0;
+  if (comp(d, r.kz) < 0.0f) {
+    int tmp = r.kx;
+    r.kx = r.ky;
+    r.ky = tmp;
+  }
+  float dz = comp(d, r.kz);
+  r.Sx = comp(d, r.kx) / dz;
+  r.Sy = comp(d, r.ky) / dz;
+  r.Sz = 1.0f / dz;
+  float sx = fabsf(d.x) > 1e-30f ? d.x : copysignf(1e-30f, d.x);
+  float sy = fabsf(d.y) > 1e-30f ? d.y : copysignf(1e-30f, d.y);
+  float sz = fabsf(d.z) > 1e-30f ? d.z : copysignf(1e-30f, d.z);
+  r.inv = mk(1.0f / sx, 1.0f / sy, 1.0f / sz);
+  return r;
+}
+
+// Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013.
+TT_D bool intersect(const RayPre &r, v3f v0, v3f v1, v3f v2, float &t) {
+  v3f A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+  float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+  float Ax = comp(A, r.kx) - r.Sx * Akz;
+  float Ay = comp(A, r.ky) - r.Sy * Akz;
+  float Bx = comp(B, r.kx) - r.Sx * Bkz;
+  float By = comp(B, r.ky) - r.Sy * Bkz;
+  float Cx = comp(C, r.kx) - r.Sx * Ckz;
+  float Cy = comp(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  float det = U + V + W;
+  if (det == 0.0f) return false;
+  float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  float T = U * Az + V * Bz + W * Cz;
+  t = T / det;
+  return true;
+}
+
+// Chunk key: a candidate must satisfy t <= far and (t > tlo or (t == tlo and prim > plo)).
+TT_D bool after_key(float t, uint32_t prim, float tlo, long long plo) {
+  return t > tlo || (t == tlo && (long long)prim > plo);
+}
+TT_D bool hit_less(float t, uint32_t p, float bt, uint32_t bp) {
+  return t < bt || (t == bt && p < bp);
+}
+
+// Conservative slab test: boxes are padded at build time and the interval is
+// widened, so a triangle whose computed t lies in [tlo, thi] is never culled.
+TT_D void slab(const RayPre &r, float lx, float hx, float ly, float hy, float lz, float hz, float &tn, float &tf) {
+  float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
+  float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
+  float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
+  tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+  tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+  tf = tf + fabsf(tf) * 1e-6f + 1e-7f;
+  tn = tn - fabsf(tn) * 1e-6f - 1e-7f;
+}
+TT_D bool box_ok(float tn, float tf, float tlo_c, float thi_c) {
+  return tn <= tf && tf >= tlo_c && tn <= thi_c;
+}
+
+// ------------------------------------------------------------- window function
+struct Win {
+  float Cx, phi_final, phi_max, dist;
+  float ed[3];
+  v3f n;
+  v3f rel[3];  // hit - vertex i
+};
+
+// computeWindowFunc (common.h:57-114) in the kernels' FP32 form: relative to
+// vertex A, with the three edge-distance numerators ec x (p_i - p_{i+1}) . (hit - p_i)
+// formed in FP64 (add/sub/mul only) -- in FP32 that dot product cancels to
+// ~eps*|hit - p_i| absolute, which the window's diverging slope near an edge
+// amplifies into percent-level gradient error.
+TT_D void window32(v3f A, v3f B, v3f C, v3f o, v3f d, float t, float sigma, Win &w) {
+  v3f eAB = B - A, eAC = C - A, eBC = C - B;
+  float a = vlen(eBC);
+  float b = vlen(eAC);
+  float c = vlen(eAB);
+  v3f incA = divs(b * eAB + c * eAC, (a + b) + c);
+  v3f ec = cross(eAB, eAC);
+  w.n = normalize(ec);
+  v3d Ad = todbl(A), Bd = todbl(B), Cd = todbl(C);
+  v3d od = todbl(o), td = todbl(d) * (double)t;
+  v3d rA = (od - Ad) + td, rB = (od - Bd) + td, rC = (od - Cd) + td;
+  v3d ecd = cross(Bd - Ad, Cd - Ad);
+  v3d n0 = cross(ecd, Ad - Bd), n1 = cross(ecd, Bd - Cd), n2 = cross(ecd, Cd - Ad);
+  float q0 = (float)dot(n0, rA), q1 = (float)dot(n1, rB), q2 = (float)dot(n2, rC);
+  v3f m0 = toflt(n0), m1 = toflt(n1), m2 = toflt(n2);
+  float il0 = 1.0f / sqrtf(dot(m0, m0));
+  float il1 = 1.0f / sqrtf(dot(m1, m1));
+  float il2 = 1.0f / sqrtf(dot(m2, m2));
+  float dist = dot(m0 * il0, incA);
+  float ed0 = q0 * il0;
+  if (dist > 0.0f) {
+    dist = -dist;
+    ed0 = -ed0;
+  }
+  w.ed[0] = ed0;
+  w.ed[1] = q1 * il1;
+  w.ed[2] = q2 * il2;
+  w.rel[0] = toflt(rA);
+  w.rel[1] = toflt(rB);
+  w.rel[2] = toflt(rC);
+  float phi = fmaxf(fmaxf(w.ed[0], w.ed[1]), w.ed[2]);
+  w.phi_max = phi;
+  w.dist = dist;
+  w.phi_final = phi / dist;
+  w.Cx = fmaxf(0.0f, pow_tt(w.phi_final, sigma));
+}
+
+// --------------------------------------------------------------- SH colour
+TT_D v3f shc(const float *sh, int i) {
+  return v3f{sh[3 * i], sh[3 * i + 1], sh[3 * i + 2]};
+}
+// computeColorFromSH (common.h:16-55).
+TT_D v3f sh_color(const float *sh, int deg, v3f dir, bool &c0, bool &c1, bool &c2) {
+  dir = divs(dir, vlen(dir));
+  v3f res = SH_C0 * shc(sh, 0);
+  if (deg > 0) {
+    float x = dir.x, y = dir.y, z = dir.z;
+    res = res - (SH_C1 * y) * shc(sh, 1) + (SH_C1 * z) * shc(sh, 2) - (SH_C1 * x) * shc(sh, 3);
+    if (deg > 1) {
+      float xx = x * x, yy = y * y, zz = z * z;
+      float xy = x * y, yz = y * z, xz = x * z;
+      res = res + (SH_C2_0 * xy) * shc(sh, 4) + (SH_C2_1 * yz) * shc(sh, 5) +
+            (SH_C2_2 * (2.0f * zz - xx - yy)) * shc(sh, 6) + (SH_C2_3 * xz) * shc(sh, 7) +
+            (SH_C2_4 * (xx - yy)) * shc(sh, 8);
+      if (deg > 2) {
+        res = res + ((SH_C3_0 * y) * (3.0f * xx - yy)) * shc(sh, 9) + ((SH_C3_1 * xy) * z) * shc(sh, 10) +
+              ((SH_C3_2 * y) * (4.0f * zz - xx - yy)) * shc(sh, 11) +
+              ((SH_C3_3 * z) * (2.0f * zz - 3.0f * xx - 3.0f * yy)) * shc(sh, 12) +
+              ((SH_C3_4 * x) * (4.0f * zz - xx - yy)) * shc(sh, 13) + ((SH_C3_5 * z) * (xx - yy)) * shc(sh, 14) +
+              ((SH_C3_6 * x) * (xx - 3.0f * yy)) * shc(sh, 15);
+      }
+    }
+  }
+  res = res + mk(0.5f, 0.5f, 0.5f);
+  c0 = res.x < 0.0f;
+  c1 = res.y < 0.0f;
+  c2 = res.z < 0.0f;
+  return mk(fmaxf(res.x, 0.0f), fmaxf(res.y, 0.0f), fmaxf(res.z, 0.0f));
+}
+
+// --------------------------------------------------------- sphere helpers
+// hit_sphere (common.h:116-141)
+TT_D bool hit_sphere(float rad, v3f cnt, v3f org, v3f dir, float &t) {
+  v3f op = org - cnt;
+  float a = dot(dir, dir);
+  if (a <= 0.0f) return false;
+  float b = dot(op, dir);
+  float c = dot(op, op) - rad * rad;
+  float det = b * b - c * a;
+  if (det < 0.0f) return false;
+  det = sqrtf(det);
+  t = (-b - det) / a;
+  if ((double)t < kStepEps) {
+    t = (-b + det) / a;
+    if ((double)t < kStepEps) return false;
+  }
+  return true;
+}
+TT_D v3f reflect3(v3f i, v3f n) {
+  return i - (2.0f * n) * dot(n, i);
+}
+// refract.cu:166-187, with its formula as written.
+TT_D bool refract_quirk(v3f &out, v3f in, v3f n, float eta_org) {
+  float cosi = dot(in, n);
+  float eta = eta_org;
+  if (cosi > 0.0) {
+    eta = (float)(1.0 / (double)eta_org);
+    n = -n;
+  }
+  float s2i = (float)fmax(0.0, 1.0 - (double)sqrtf(cosi));
+  float s2t = s2i / sqrtf(eta);
+  if (s2t >= 1) return false;
+  float cost = (float)sqrt(fmax(0.0, 1.0 - (double)s2t));
+  out = divs(in, eta) + (-cosi / eta - cost) * n;
+  return true;
+}
+// envlight.cu:25-51: LCG and cosine-weighted hemisphere sample.
+TT_D float lcg(uint32_t &u) {
+  u = u * 1664525u + 1013904223u;
+  return (float)(u & 0x00FFFFFFu) * (1.0f / 16777216.0f);
+}
+TT_D v3f cos_hemi(uint32_t &seed, v3f nrm) {
+  float u = lcg(seed);
+  float v = lcg(seed);
+  float z = sqrtf(u);
+  float rr = sqrtf(fmaxf(0.0f, (float)(1.0 - (double)(z * z))));
+  float phi = v * 6.2831853071795864769f;
+  float sp, cp;
+  sincos_tt(phi, sp, cp);
+  v3f dl = mk(rr * cp, rr * sp, z);
+  v3f up = fabsf(nrm.z) < 0.999f ? mk(0.0f, 0.0f, 1.0f) : mk(0.0f, 1.0f, 0.0f);
+  v3f tt = normalize(cross(up, nrm));
+  v3f bb = normalize(cross(nrm, tt));
+  v3f wd = dl.x * tt + dl.y * bb + dl.z * nrm;
+  return normalize(wd);
+}
+
+}  // namespace tt

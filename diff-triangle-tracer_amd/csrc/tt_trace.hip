@@ -1,0 +1,662 @@
+// Traversal kernels: k-closest gather over the LBVH, front-to-back compositing,
+// and the re-traversing backward.
+//
+// Replaces the OptiX programs of the reference:
+//   __raygen__rg forward ........ tritracer/forward.cu:76-215
+//   __raygen__rg backward ....... tritracer/backward.cu:191-391
+//   reflect / refract / envlight  tritracer/reflect.cu, refract.cu, envlight.cu
+//   optixTrace + __anyhit__ah ... tritracer/closestk.h:17-149 (k-buffer insertion)
+//
+// Mapping: one ray per lane; each wave64 takes an 8x8 pixel tile, a block of
+// 128 lanes a 16x8 tile. Blocks are remapped so that the tiles one XCD works
+// on form a contiguous band of the image (workgroups are dispatched to the 8
+// XCDs round-robin and each XCD has its own L2). The traversal stack is a
+// per-lane stack in LDS ([slot][lane] layout, bank-conflict free) of 64
+// entries: a Karras LBVH over 30-bit Morton codes with index tie-break has at
+// most 63 internal levels (the common prefix of a node's key range grows by at
+// least one bit per level, over 62 key bits), and the ordered traversal defers
+// at most one sibling per level, so the stack cannot overflow.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "tt_device.h"
+#include "tt_internal.h"
+
+namespace tt {
+namespace {
+
+constexpr int kBlock = 128;   // lanes per block: two 8x8 wave tiles -> 16x8 pixels
+constexpr int kTileW = 16, kTileH = 8;
+constexpr int kStack = 64;    // entries per lane (deferred node index)
+
+template <int K>
+struct KBuf {
+  float t[K];
+  uint32_t p[K];
+};
+
+template <int K>
+TT_D void kinit(KBuf<K> &b) {
+#pragma unroll
+  for (int i = 0; i < K; i++) b.t[i] = kInfDist, b.p[i] = kInvalid;
+}
+
+// Sorted insertion (the anyhit compare-and-swap chain of closestk.h:111-147) with a
+// deterministic (t, prim) order instead of traversal order for equal t.
+template <int K>
+TT_D void kinsert(KBuf<K> &b, float t, uint32_t p) {
+  if (!hit_less(t, p, b.t[K - 1], b.p[K - 1])) return;
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    bool lt = hit_less(t, p, b.t[i], b.p[i]);
+    float ot = b.t[i];
+    uint32_t op = b.p[i];
+    b.t[i] = lt ? t : ot;
+    b.p[i] = lt ? p : op;
+    t = lt ? ot : t;
+    p = lt ? op : p;
+  }
+}
+template <int K>
+TT_D void kpop(KBuf<K> &b) {
+#pragma unroll
+  for (int i = 0; i < K - 1; i++) b.t[i] = b.t[i + 1], b.p[i] = b.p[i + 1];
+  b.t[K - 1] = kInfDist;
+  b.p[K - 1] = kInvalid;
+}
+
+// One chunk: the K smallest (t, prim) with t <= far and (t, prim) after the key.
+// Ordered traversal: both children of a node are tested from the node itself;
+// hit leaves are intersected at once, the nearer internal child is descended
+// and the farther one deferred on the LDS stack.
+template <int K>
+TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo, KBuf<K> &b, int *s_node) {
+  kinit(b);
+  if (P.n_leaves == 0) return;
+  const int lane = threadIdx.x;
+  const float tlo_c = tlo - fabsf(tlo) * 1e-6f - 1e-7f;
+  const float far_t = P.far_t;
+  int node = 0;
+  int sp = 0;
+  auto leaf = [&](int s) {
+    const float4 a = P.leaves[3 * s], bb = P.leaves[3 * s + 1], c = P.leaves[3 * s + 2];
+    const uint32_t prim = __float_as_uint(a.w);
+    float t;
+    if (!intersect(r, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), mk(c.x, c.y, c.z), t)) return;
+    if (!(t <= far_t)) return;
+    if (!after_key(t, prim, tlo, plo)) return;
+    kinsert(b, t, prim);
+  };
+  while (true) {
+    if (node >= 0) {
+      const float4 f0 = P.nodes[4 * node], f1 = P.nodes[4 * node + 1], f2 = P.nodes[4 * node + 2];
+      const float4 f3 = P.nodes[4 * node + 3];
+      const int c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+      const float thi = fminf(far_t, b.t[K - 1]);
+      const float thi_c = thi + fabsf(thi) * 1e-6f + 1e-7f;
+      float tn0, tf0, tn1, tf1;
+      slab(r, f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, tn0, tf0);
+      slab(r, f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, tn1, tf1);
+      bool h0 = box_ok(tn0, tf0, tlo_c, thi_c);
+      bool h1 = box_ok(tn1, tf1, tlo_c, thi_c);
+      if (h0 && c0 < 0) {
+        leaf(~c0);
+        h0 = false;
+      }
+      if (h1 && c1 < 0) {
+        leaf(~c1);
+        h1 = false;
+      }
+      if (h0 && h1) {
+        const bool first0 = tn0 <= tn1;
+        s_node[(sp & (kStack - 1)) * kBlock + lane] = first0 ? c1 : c0;
+        sp++;
+        node = first0 ? c0 : c1;
+      } else {
+        node = h0 ? c0 : (h1 ? c1 : -1);
+      }
+    }
+    if (node < 0) {
+      if (sp == 0) return;
+      sp--;
+      node = s_node[(sp & (kStack - 1)) * kBlock + lane];
+    }
+  }
+}
+
+// Pixel of this lane: block -> tile with the XCD band remap, wave -> 8x8 quadrant.
+TT_D bool pixel_of(const TraceParams &P, int &x, int &y) {
+  const int nb = P.n_tiles;
+  const int i = blockIdx.x;
+  const int xcd = i & 7, j = i >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  const int L = xcd * per + min(xcd, rem) + j;
+  const int tx = L % P.tiles_x, ty = L / P.tiles_x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  x = tx * kTileW + (wave & 1) * 8 + (lane & 7);
+  y = ty * kTileH + (lane >> 3);
+  return x < P.W && y < P.H;
+}
+
+// forward.cu:34-74 computeScaling
+TT_D void compute_scaling(const TraceParams &P, int idx, v3f org) {
+  if (P.scaling[idx] > 0.0f) return;
+  const int off = P.cumsum[idx];
+  const int np = P.npts[idx];
+  v3f cen = mk(0.0f, 0.0f, 0.0f);
+  for (int i = 0; i < np; i++) cen = cen + ld3(P.verts + 3 * (size_t)(off + i));
+  cen = cen * (1.0f / (float)np);
+  v3f tri2org = cen - org;
+  float mx = 0.0f;
+  for (int i = 0; i < np; i++) {
+    v3f a = ld3(P.verts + 3 * (size_t)(off + i)) - org;
+    float len_ab = vlen(a) * vlen(tri2org);
+    float ang = 0.0f;
+    if (!(len_ab < 1e-12f)) {
+      float c = fmaxf(-1.0f, fminf(1.0f, dot(a, tri2org) / len_ab));
+      ang = acosf(c);
+    }
+    mx = fmaxf(ang, mx);
+  }
+  P.scaling[idx] = mx;
+}
+
+struct SegState {
+  float C[3];
+  float T;
+  float N[3];
+  int consumed, blended;
+};
+
+// The chunk loop of forward.cu:76-155 (FULL: with normal/statistics outputs) and
+// of the variants (colour only).
+template <int K, bool FULL>
+TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node) {
+  RayPre r = prep_ray(o, d);
+  KBuf<K> b;
+  float last = 0.0f, tlo = 0.0f;
+  long long plo = -1;
+  const bool logging = FULL && P.log_cap > 0;
+  while (true) {
+    gather<K>(P, r, tlo, plo, b, s_node);
+    float lt = 0.0f;
+    uint32_t lp = 0;
+    for (int i = 0; i < K; i++) {
+      const float t = b.t[0];
+      const uint32_t prim = b.p[0];
+      kpop(b);
+      if (prim == kInvalid) return;
+      lt = t;
+      lp = prim;
+      const int slot = st.consumed;
+      st.consumed++;
+      last = fmaxf((float)((double)last + kStepEps), t);
+      int code;
+      if (t < P.near_t) {
+        code = 0;
+      } else {
+        const int off = P.cumsum[prim];
+        const float *vp = P.verts + 3 * (size_t)off;
+        Win w;
+        window32(ld3(vp), ld3(vp + 3), ld3(vp + 6), o, d, t, P.sigma[prim], w);
+        const float alpha = fminf(0.99f, P.opac[prim] * w.Cx);
+        if (alpha < 1.0f / 255.0f) {
+          code = 1;
+        } else {
+          const float testT = st.T * (1.0f - alpha);
+          if (testT < P.cutoff) {
+            if (logging && slot < P.log_cap) {
+              P.log_prim[(size_t)ray * P.log_cap + slot] = (int)prim;
+              P.log_t[(size_t)ray * P.log_cap + slot] = t;
+              P.log_code[(size_t)ray * P.log_cap + slot] = 3;
+            }
+            return;
+          }
+          const float bw = alpha * st.T;
+          if (FULL) {
+            P.density[prim] = 1.0f;
+            compute_scaling(P, prim, scaling_org);
+            atomicMax(reinterpret_cast<int *>(P.maxb) + prim, __float_as_int(bw));
+            st.N[0] += w.n.x * bw;
+            st.N[1] += w.n.y * bw;
+            st.N[2] += w.n.z * bw;
+          }
+          bool c0, c1, c2;
+          const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
+          st.C[0] += col.x * bw;
+          st.C[1] += col.y * bw;
+          st.C[2] += col.z * bw;
+          st.T = testT;
+          st.blended++;
+          code = 2;
+        }
+      }
+      if (logging && slot < P.log_cap) {
+        P.log_prim[(size_t)ray * P.log_cap + slot] = (int)prim;
+        P.log_t[(size_t)ray * P.log_cap + slot] = t;
+        P.log_code[(size_t)ray * P.log_cap + slot] = code;
+      }
+    }
+    if (P.chunk_mode == 1) {
+      tlo = lt;
+      plo = (long long)lp;
+    } else if (last > tlo) {
+      tlo = last;
+      plo = -1;
+    } else {
+      plo = (long long)lp;
+    }
+  }
+}
+
+template <int K, int VARIANT>
+__global__ __launch_bounds__(kBlock) void k_forward(TraceParams P) {
+  __shared__ int s_node[kStack * kBlock];
+  int x, y;
+  if (!pixel_of(P, x, y)) return;
+  const int pix = y * P.W + x;
+  v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
+  SegState st;
+  st.C[0] = st.C[1] = st.C[2] = 0.0f;
+  st.N[0] = st.N[1] = st.N[2] = 0.0f;
+  st.T = 1.0f;
+  st.consumed = st.blended = 0;
+  if (VARIANT == 0) {
+    trace_segment<K, true>(P, o, d, st, pix, o, s_node);
+  } else {
+    const float rad = P.extra_len > 0 ? P.extra[0] : 1.0f;
+    const v3f cnt = P.extra_len > 3 ? mk(P.extra[1], P.extra[2], P.extra[3]) : mk(0.0f, 0.0f, 0.0f);
+    const float eps = (float)kStepEps;
+    if (VARIANT == 1) {  // reflect.cu:25-48
+      float sth;
+      if (hit_sphere(rad, cnt, o, d, sth)) {
+        o = o + d * sth;
+        const v3f nrm = normalize(o - cnt);
+        o = o + nrm * eps;
+        d = reflect3(normalize(d), nrm);
+      }
+      trace_segment<K, false>(P, o, d, st, pix, o, s_node);
+    } else if (VARIANT == 2) {  // refract.cu:189-213
+      float sth;
+      if (hit_sphere(rad, cnt, o, d, sth)) {
+        const v3f hp = o + d * sth;
+        const v3f nrm = normalize(hp - cnt);
+        const v3f in = normalize(d);
+        const float ior = 1.5f / 1.0003f;
+        v3f nd = in;
+        if (!refract_quirk(nd, in, nrm, ior)) {
+          const v3f rn = dot(in, nrm) < 0.0f ? nrm : -nrm;
+          nd = reflect3(in, rn);
+        }
+        o = hp + nrm * eps;
+        d = nd;
+      }
+      trace_segment<K, false>(P, o, d, st, pix, o, s_node);
+    } else {  // envlight.cu:116-184
+      const int samples = (int)(P.extra_len > 4 ? P.extra[4] : 0.0f);
+      float sth;
+      if (hit_sphere(rad, cnt, o, d, sth)) {
+        o = o + d * sth;
+        const v3f nrm0 = normalize(o - cnt);
+        o = o + nrm0 * eps;
+        float radc[3] = {0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < samples; i++) {
+          const v3f nrm = normalize(o - cnt);
+          uint32_t seed = (((uint32_t)pix + 1u) * 9781u) ^ (((uint32_t)i + 1u) * 6271u);
+          const v3f dn = cos_hemi(seed, nrm);
+          const v3f on = o + dn * eps;
+          SegState ss;
+          ss.C[0] = ss.C[1] = ss.C[2] = 0.0f;
+          ss.N[0] = ss.N[1] = ss.N[2] = 0.0f;
+          ss.T = 1.0f;
+          ss.consumed = ss.blended = 0;
+          trace_segment<K, false>(P, on, dn, ss, pix, on, s_node);
+          for (int c = 0; c < 3; c++) radc[c] += ss.C[c] + ss.T * P.bg[c];
+          st.consumed += ss.consumed;
+          st.blended += ss.blended;
+        }
+        const float inv = 1.0f / (float)samples;
+        for (int c = 0; c < 3; c++) st.C[c] += radc[c] * inv;
+        st.T = 0.0f;
+      } else {
+        trace_segment<K, false>(P, o, d, st, pix, o, s_node);
+      }
+    }
+  }
+  const size_t R = (size_t)P.R;
+  for (int c = 0; c < 3; c++) P.color[c * R + pix] = st.C[c] + st.T * P.bg[c];
+  if (VARIANT == 0) {
+    P.others[0 * R + pix] = 0.0f;
+    P.others[1 * R + pix] = 1.0f - st.T;
+    P.others[2 * R + pix] = st.N[0];
+    P.others[3 * R + pix] = st.N[1];
+    P.others[4 * R + pix] = st.N[2];
+    P.others[5 * R + pix] = 0.0f;
+    P.others[6 * R + pix] = 0.0f;
+    if (P.log_cap > 0) P.log_count[pix] = st.consumed;
+  }
+  if (P.stats) {
+    P.stats[2 * (size_t)pix] = st.consumed;
+    P.stats[2 * (size_t)pix + 1] = st.blended;
+  }
+}
+
+// ------------------------------------------------------------------ backward
+TT_D v3f vecmat(v3f v, const float A[9]) {
+  return {A[0] * v.x + A[3] * v.y + A[6] * v.z, A[1] * v.x + A[4] * v.y + A[7] * v.z,
+          A[2] * v.x + A[5] * v.y + A[8] * v.z};
+}
+TT_D void skew(v3f v, float S[9]) {
+  S[0] = 0.0f, S[1] = -v.z, S[2] = v.y;
+  S[3] = v.z, S[4] = 0.0f, S[5] = -v.x;
+  S[6] = -v.y, S[7] = v.x, S[8] = 0.0f;
+}
+TT_D void outer(v3f u, v3f v, float M[9]) {
+  M[0] = u.x * v.x, M[1] = u.x * v.y, M[2] = u.x * v.z;
+  M[3] = u.y * v.x, M[4] = u.y * v.y, M[5] = u.y * v.z;
+  M[6] = u.z * v.x, M[7] = u.z * v.y, M[8] = u.z * v.z;
+}
+TT_D void eye_scaled(float s, float M[9]) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) M[i] = ((i % 4) == 0 ? 1.0f : 0.0f) * s;
+}
+// d(v/|v|)/dv = (I - n n^T)/|v| (vecmath.h:2798-2809); scale formed as 1/|v| in double.
+TT_D void dnormvdv(v3f nv, v3f v, float out[9]) {
+  float op[9];
+  outer(nv, nv, op);
+#pragma unroll
+  for (int i = 0; i < 9; i++) out[i] = ((i % 4) == 0 ? 1.0f : 0.0f) - op[i];
+  const float s = (float)(1.0 / (double)vlen(v));
+#pragma unroll
+  for (int i = 0; i < 9; i++) out[i] *= s;
+}
+// Jacobians of ne = ((p1-p3) x (p2-p3)) x (p1-p2) (vecmath.h:2811-2859).
+TT_D void dne_dp1(v3f p1, v3f p2, v3f p3, float out[9]) {
+  v3f b = p2 - p3, c = p1 - p2;
+  v3f axb = cross(p1 - p3, b);
+  float m[9], d[9], s[9];
+  outer(b, c, m);
+  eye_scaled(dot(b, c), d);
+  skew(axb, s);
+#pragma unroll
+  for (int i = 0; i < 9; i++) out[i] = (m[i] - d[i]) + s[i];
+}
+TT_D void dne_dp2(v3f p1, v3f p2, v3f p3, float out[9]) {
+  v3f a = p1 - p3, b = p2 - p3, c = p1 - p2;
+  v3f axb = cross(p1 - p3, b);
+  float d[9], m[9], s[9];
+  eye_scaled(dot(a, c), d);
+  outer(a, c, m);
+  skew(axb, s);
+#pragma unroll
+  for (int i = 0; i < 9; i++) out[i] = (d[i] - m[i]) - s[i];
+}
+TT_D void dne_dp3(v3f p1, v3f p2, v3f p3, float out[9]) {
+  v3f c = p1 - p2;
+  float d[9], m[9];
+  eye_scaled(dot(c, c), d);
+  outer(c, c, m);
+#pragma unroll
+  for (int i = 0; i < 9; i++) out[i] = m[i] - d[i];
+}
+
+TT_D void atomic_add3(float *p, v3f g) {
+  unsafeAtomicAdd(p, g.x);
+  unsafeAtomicAdd(p + 1, g.y);
+  unsafeAtomicAdd(p + 2, g.z);
+}
+
+// computeWindowFuncBwd (backward.cu:104-189): gradient of the max edge distance
+// (|d_k - phi| <= 1e-6) and of the face normal w.r.t. the three vertices, hit
+// point and incenter distance held constant. The per-vertex sums are formed in
+// registers and scattered once (9 atomics instead of up to 27).
+TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, float *dverts_tri) {
+  v3f g[3] = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
+  for (int k = 0; k < 3; k++) {
+    if (fabsf(w.ed[k] - w.phi_max) > 1e-6f) continue;
+    const int k1 = k == 2 ? 0 : k + 1, k2 = k == 0 ? 2 : k - 1;
+    const v3f p1 = V[k], p2 = V[k1], p3 = V[k2];
+    const v3f n = cross(p1 - p3, p2 - p3);
+    const v3f ne = cross(n, p1 - p2);
+    const v3f Ne = normalize(ne);
+    float J[9];
+    dnormvdv(Ne, ne, J);
+    const v3f dL_dNe = dL_dphi * w.rel[k];
+    const v3f dL_dne = vecmat(dL_dNe, J);
+    float j1[9], j2[9], j3[9];
+    dne_dp1(p1, p2, p3, j1);
+    dne_dp2(p1, p2, p3, j2);
+    dne_dp3(p1, p2, p3, j3);
+    const v3f g1 = vecmat(dL_dne, j1) - dL_dphi * Ne;
+    const v3f g2 = vecmat(dL_dne, j2);
+    const v3f g3 = vecmat(dL_dne, j3);
+    if (k == 0) {
+      g[0] = g[0] + g1, g[1] = g[1] + g2, g[2] = g[2] + g3;
+    } else if (k == 1) {
+      g[1] = g[1] + g1, g[2] = g[2] + g2, g[0] = g[0] + g3;
+    } else {
+      g[2] = g[2] + g1, g[0] = g[0] + g2, g[1] = g[1] + g3;
+    }
+  }
+  if (dL_dn3.x != 0.0f || dL_dn3.y != 0.0f || dL_dn3.z != 0.0f) {
+    const v3f p1 = V[0], p2 = V[1], p3 = V[2];
+    const v3f n = cross(p1 - p3, p2 - p3);
+    const v3f Nn = normalize(n);
+    float J[9];
+    dnormvdv(Nn, n, J);
+    const v3f dL_dn = vecmat(dL_dn3, J);
+    float s1[9], s2[9], s3[9];
+    skew(p3 - p2, s1);
+    skew(p1 - p3, s2);
+    skew(p2 - p1, s3);
+    g[0] = g[0] + vecmat(dL_dn, s1);
+    g[1] = g[1] + vecmat(dL_dn, s2);
+    g[2] = g[2] + vecmat(dL_dn, s3);
+  }
+  atomic_add3(dverts_tri, g[0]);
+  atomic_add3(dverts_tri + 3, g[1]);
+  atomic_add3(dverts_tri + 6, g[2]);
+}
+
+// computeColorFromSHBwd (backward.cu:33-102); clamped channels carry no gradient.
+TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3], float *dsh) {
+  const v3f dir = divs(dir0, vlen(dir0));
+  v3f g = mk(dc[0], dc[1], dc[2]);
+  g.x *= c0 ? 0.0f : 1.0f;
+  g.y *= c1 ? 0.0f : 1.0f;
+  g.z *= c2 ? 0.0f : 1.0f;
+  const float x = dir.x, y = dir.y, z = dir.z;
+  auto add = [&](int j, v3f v) {
+    float *p = dsh + 3 * j;
+    if (!c0) unsafeAtomicAdd(p, v.x);
+    if (!c1) unsafeAtomicAdd(p + 1, v.y);
+    if (!c2) unsafeAtomicAdd(p + 2, v.z);
+  };
+  add(0, SH_C0 * g);
+  if (deg > 0) {
+    add(1, (-SH_C1 * y) * g);
+    add(2, (SH_C1 * z) * g);
+    add(3, (-SH_C1 * x) * g);
+    if (deg > 1) {
+      const float xx = x * x, yy = y * y, zz = z * z;
+      const float xy = x * y, yz = y * z, xz = x * z;
+      add(4, (SH_C2_0 * xy) * g);
+      add(5, (SH_C2_1 * yz) * g);
+      add(6, (SH_C2_2 * (2.0f * zz - xx - yy)) * g);
+      add(7, (SH_C2_3 * xz) * g);
+      add(8, (SH_C2_4 * (xx - yy)) * g);
+      if (deg > 2) {
+        add(9, ((SH_C3_0 * y) * (3.0f * xx - yy)) * g);
+        add(10, ((SH_C3_1 * xy) * z) * g);
+        add(11, ((SH_C3_2 * y) * (4.0f * zz - xx - yy)) * g);
+        add(12, ((SH_C3_3 * z) * (2.0f * zz - 3.0f * xx - 3.0f * yy)) * g);
+        add(13, ((SH_C3_4 * x) * (4.0f * zz - xx - yy)) * g);
+        add(14, ((SH_C3_5 * z) * (xx - yy)) * g);
+        add(15, ((SH_C3_6 * x) * (xx - 3.0f * yy)) * g);
+      }
+    }
+  }
+}
+
+// backward.cu:191-391: re-traverse the ray with the same chunk logic, recompute
+// the forward state hit by hit and scatter the gradients.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
+  __shared__ int s_node[kStack * kBlock];
+  int x, y;
+  if (!pixel_of(P, x, y)) return;
+  const int pix = y * P.W + x;
+  const size_t R = (size_t)P.R;
+  const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
+  const float final_depth = P.fothers[pix];
+  const v3f final_normal = mk(P.fothers[2 * R + pix], P.fothers[3 * R + pix], P.fothers[4 * R + pix]);
+  float final_color[3], dL_dpixel[3];
+  for (int c = 0; c < 3; c++) {
+    final_color[c] = P.fcolor[c * R + pix];
+    dL_dpixel[c] = P.dpix[c * R + pix];
+  }
+  const float dL_ddepth = P.dothers[pix];
+  const v3f dL_dn2 = mk(P.dothers[2 * R + pix], P.dothers[3 * R + pix], P.dothers[4 * R + pix]);
+  float C[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f}, D = 0.0f, T = 1.0f;
+  const RayPre r = prep_ray(o, d);
+  KBuf<K> b;
+  float last = 0.0f, tlo = 0.0f;
+  long long plo = -1;
+  while (true) {
+    gather<K>(P, r, tlo, plo, b, s_node);
+    float lt = 0.0f;
+    uint32_t lp = 0;
+    for (int i = 0; i < K; i++) {
+      const float t = b.t[0];
+      const uint32_t prim = b.p[0];
+      kpop(b);
+      if (prim == kInvalid) return;
+      lt = t;
+      lp = prim;
+      last = fmaxf((float)((double)last + kStepEps), t);
+      if (t < P.near_t) continue;
+      const int off = P.cumsum[prim];
+      const float *vp = P.verts + 3 * (size_t)off;
+      const v3f V[3] = {ld3(vp), ld3(vp + 3), ld3(vp + 6)};
+      Win w;
+      window32(V[0], V[1], V[2], o, d, t, P.sigma[prim], w);
+      const float opacity = P.opac[prim];
+      const float alpha = fminf(0.99f, opacity * w.Cx);
+      if (alpha < 1.0f / 255.0f) continue;
+      const float testT = T * (1.0f - alpha);
+      if (testT < P.cutoff) return;
+      const float bw = alpha * T;
+      bool c0, c1, c2;
+      const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
+      C[0] += col.x * bw, C[1] += col.y * bw, C[2] += col.z * bw;
+      N[0] += w.n.x * bw, N[1] += w.n.y * bw, N[2] += w.n.z * bw;
+      D += bw * t;
+      float dL_dcolor[3];
+      const float cc[3] = {col.x, col.y, col.z};
+      float dL_dalpha = 0.0f;
+      const float inv1a = 1.0f / (1.0f - alpha);
+      for (int c = 0; c < 3; c++) {
+        dL_dcolor[c] = dL_dpixel[c] * bw;
+        dL_dalpha += dL_dpixel[c] * (T * cc[c] - (final_color[c] - C[c]) * inv1a);
+      }
+      dL_dalpha += dL_ddepth * (T * t - (final_depth - D) * inv1a);
+      const v3f Nf = mk(N[0], N[1], N[2]);
+      dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
+      unsafeAtomicAdd(P.dopac + prim, dL_dalpha * w.Cx);
+      const float dL_dC = opacity * dL_dalpha;
+      if (w.phi_final > 0.0f) unsafeAtomicAdd(P.dsigma + prim, (dL_dC * w.Cx) * log_tt(w.phi_final));
+      const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
+      const v3f dL_dn3 = dL_dn2 * bw;
+      window_bwd(V, w, dL_dphi, dL_dn3, P.dverts + 3 * (size_t)off);
+      sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, P.dsh + (size_t)prim * P.M * 3);
+      T = testT;
+    }
+    if (P.chunk_mode == 1) {
+      tlo = lt;
+      plo = (long long)lp;
+    } else if (last > tlo) {
+      tlo = last;
+      plo = -1;
+    } else {
+      plo = (long long)lp;
+    }
+  }
+}
+
+__global__ void k_fill(float *p, size_t n, float v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <int K>
+hipError_t fwd_k(const TraceParams &p, int variant, hipStream_t s) {
+  dim3 g(p.n_tiles), bl(kBlock);
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL((k_forward<K, 0>), g, bl, 0, s, p);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_forward<K, 1>), g, bl, 0, s, p);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_forward<K, 2>), g, bl, 0, s, p);
+      break;
+    default:
+      hipLaunchKernelGGL((k_forward<K, 3>), g, bl, 0, s, p);
+      break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+static void tiles(TraceParams &p) {
+  p.tiles_x = (p.W + kTileW - 1) / kTileW;
+  p.n_tiles = p.tiles_x * ((p.H + kTileH - 1) / kTileH);
+}
+
+std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_t stream) {
+  TraceParams p = p0;
+  tiles(p);
+  if (p.n_tiles == 0) return "";
+  hipError_t e;
+  if (k == 8)
+    e = fwd_k<8>(p, variant, stream);
+  else if (k == 16)
+    e = fwd_k<16>(p, variant, stream);
+  else if (k == 32)
+    e = fwd_k<32>(p, variant, stream);
+  else
+    return "k must be 8, 16 or 32";
+  if (e != hipSuccess) return std::string("forward launch failed: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string launch_backward(const TraceParams &p0, int k, hipStream_t stream) {
+  TraceParams p = p0;
+  tiles(p);
+  if (p.n_tiles == 0) return "";
+  dim3 g(p.n_tiles), bl(kBlock);
+  if (k == 8)
+    hipLaunchKernelGGL(k_backward<8>, g, bl, 0, stream, p);
+  else if (k == 16)
+    hipLaunchKernelGGL(k_backward<16>, g, bl, 0, stream, p);
+  else if (k == 32)
+    hipLaunchKernelGGL(k_backward<32>, g, bl, 0, stream, p);
+  else
+    return "k must be 8, 16 or 32";
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("backward launch failed: ") + hipGetErrorString(e);
+  return "";
+}
+
+std::string launch_fill(float *ptr, size_t n, float v, hipStream_t stream) {
+  if (n == 0) return "";
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ptr, n, v);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("fill launch failed: ") + hipGetErrorString(e);
+  return "";
+}
+
+}  // namespace tt

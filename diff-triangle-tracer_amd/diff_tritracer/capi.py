@@ -1,0 +1,188 @@
+"""ctypes binding of the C ABI in include/tritracer_hip.h (libtritracer_hip.so).
+
+This is the binding a caller without the torch extension would write; the
+benchmark and the GPU parity tests drive the kernels through it so that what
+they measure is exactly the C ABI. Device buffers are passed as raw pointers
+(here taken from torch tensors on the current HIP device); every call is
+asynchronous on the given stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import PACKAGE_DIR
+
+LIB_PATH = os.path.join(PACKAGE_DIR, "libtritracer_hip.so")
+
+TT_OK = 0
+VARIANTS = {"forward": 0, "reflect": 1, "refract": 2, "envlight": 3}
+BUILD, REFIT, REUSE = 0, 1, 2
+CHUNK = {"compat": 0, "exact": 1}
+
+EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build", "tt_trace_fwd", "tt_trace_bwd",
+            "tt_num_primitives", "tt_last_build_ms"]
+
+_vp = C.c_void_p
+_fp = C.c_void_p  # device pointers travel as void*
+
+
+class FwdArgs(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int),
+        ("ray_o_dev", _fp), ("ray_d_dev", _fp),
+        ("vertices_dev", _fp), ("num_vertices", C.c_int),
+        ("npts_dev", _fp), ("cumsum_dev", _fp), ("P", C.c_int),
+        ("shs_dev", _fp), ("M", C.c_int), ("sh_degree", C.c_int),
+        ("opacities_dev", _fp), ("sigma_dev", _fp),
+        ("bg_dev", _fp), ("extra_dev", _fp), ("extra_len", C.c_int),
+        ("variant", C.c_int), ("k", C.c_int), ("chunk_mode", C.c_int),
+        ("cutoff", C.c_float), ("near_t", C.c_float), ("far_t", C.c_float),
+        ("color_dev", _fp), ("others_dev", _fp), ("max_blending_dev", _fp), ("density_dev", _fp),
+        ("scaling_dev", _fp), ("stats_dev", _fp),
+        ("hit_log_cap", C.c_int), ("hit_log_prim_dev", _fp), ("hit_log_t_dev", _fp), ("hit_log_code_dev", _fp),
+        ("hit_log_count_dev", _fp),
+    ]
+
+
+class BwdArgs(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int),
+        ("ray_o_dev", _fp), ("ray_d_dev", _fp),
+        ("vertices_dev", _fp), ("num_vertices", C.c_int),
+        ("npts_dev", _fp), ("cumsum_dev", _fp), ("P", C.c_int),
+        ("shs_dev", _fp), ("M", C.c_int), ("sh_degree", C.c_int),
+        ("opacities_dev", _fp), ("sigma_dev", _fp),
+        ("color_dev", _fp), ("others_dev", _fp), ("dL_dpix_dev", _fp), ("dL_dothers_dev", _fp),
+        ("k", C.c_int), ("chunk_mode", C.c_int),
+        ("cutoff", C.c_float), ("near_t", C.c_float), ("far_t", C.c_float),
+        ("zero_grads", C.c_int), ("dL_dvertices_rows", C.c_int),
+        ("dL_dvertices_dev", _fp), ("dL_dsigma_dev", _fp), ("dL_dopacity_dev", _fp), ("dL_dsh_dev", _fp),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libtritracer_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with diff-triangle-tracer_amd/build_native.py")
+        L = C.CDLL(LIB_PATH)
+        L.tt_create.argtypes = [C.POINTER(_vp)]
+        L.tt_destroy.argtypes = [_vp]
+        L.tt_last_error.restype = C.c_char_p
+        L.tt_version.restype = C.c_char_p
+        L.tt_build.argtypes = [_vp, _fp, C.c_int, _fp, C.c_int, C.c_int, _vp]
+        L.tt_trace_fwd.argtypes = [_vp, C.POINTER(FwdArgs), _vp]
+        L.tt_trace_bwd.argtypes = [_vp, C.POINTER(BwdArgs), _vp]
+        L.tt_num_primitives.argtypes = [_vp]
+        L.tt_last_build_ms.restype = C.c_float
+        L.tt_last_build_ms.argtypes = [_vp]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().tt_last_error().decode()
+
+
+def check(rc: int, what: str):
+    if rc != TT_OK:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+
+
+def ptr(t) -> int | None:
+    """Device address of a torch tensor (None for None / empty)."""
+    if t is None or t.numel() == 0:
+        return None
+    return t.data_ptr()
+
+
+class TracerHandle:
+    """Owns a tt_tracer*."""
+
+    def __init__(self):
+        h = _vp()
+        check(lib().tt_create(C.byref(h)), "tt_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().tt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build(self, verts, cumsum, P, mode=BUILD, stream=None):
+        check(lib().tt_build(self.h, ptr(verts), int(verts.numel() // 3), ptr(cumsum), int(P), int(mode), stream),
+              "tt_build")
+
+    def last_build_ms(self) -> float:
+        return float(lib().tt_last_build_ms(self.h))
+
+    def num_primitives(self) -> int:
+        return int(lib().tt_num_primitives(self.h))
+
+    def fwd(self, args: FwdArgs, stream=None):
+        check(lib().tt_trace_fwd(self.h, C.byref(args), stream), "tt_trace_fwd")
+
+    def bwd(self, args: BwdArgs, stream=None):
+        check(lib().tt_trace_bwd(self.h, C.byref(args), stream), "tt_trace_bwd")
+
+
+def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
+             scaling, k=16, cutoff=1e-4, chunk_mode="compat", variant="forward", extra=None, stats=None,
+             log=None, near=0.2, far=1000.0) -> FwdArgs:
+    a = FwdArgs()
+    a.width, a.height = int(W), int(H)
+    a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
+    a.vertices_dev, a.num_vertices = ptr(verts), int(verts.numel() // 3)
+    a.npts_dev, a.cumsum_dev, a.P = ptr(npts), ptr(cumsum), int(cumsum.numel())
+    a.shs_dev = ptr(shs)
+    a.M = int(shs.shape[1]) if shs is not None and shs.numel() else 0
+    a.sh_degree = int(sh_degree)
+    a.opacities_dev, a.sigma_dev, a.bg_dev = ptr(opac), ptr(sigma), ptr(bg)
+    a.extra_dev = ptr(extra)
+    a.extra_len = int(extra.numel()) if extra is not None else 0
+    a.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    a.k = int(k)
+    a.chunk_mode = CHUNK[chunk_mode] if isinstance(chunk_mode, str) else int(chunk_mode)
+    a.cutoff, a.near_t, a.far_t = float(cutoff), float(near), float(far)
+    a.color_dev, a.others_dev = ptr(color), ptr(others)
+    a.max_blending_dev, a.density_dev, a.scaling_dev = ptr(maxb), ptr(density), ptr(scaling)
+    a.stats_dev = ptr(stats)
+    if log is not None:
+        prim, t, code, count = log
+        a.hit_log_cap = int(prim.shape[-1])
+        a.hit_log_prim_dev, a.hit_log_t_dev, a.hit_log_code_dev, a.hit_log_count_dev = (
+            ptr(prim), ptr(t), ptr(code), ptr(count))
+    return a
+
+
+def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, color, others, dL_dpix,
+             dL_dothers, dverts, dsigma, dopac, dsh, k=16, cutoff=1e-4, chunk_mode="compat", zero_grads=True,
+             near=0.2, far=1000.0) -> BwdArgs:
+    a = BwdArgs()
+    a.width, a.height = int(W), int(H)
+    a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
+    a.vertices_dev, a.num_vertices = ptr(verts), int(verts.numel() // 3)
+    a.npts_dev, a.cumsum_dev, a.P = ptr(npts), ptr(cumsum), int(cumsum.numel())
+    a.shs_dev = ptr(shs)
+    a.M = int(shs.shape[1]) if shs is not None and shs.numel() else 0
+    a.sh_degree = int(sh_degree)
+    a.opacities_dev, a.sigma_dev = ptr(opac), ptr(sigma)
+    a.color_dev, a.others_dev, a.dL_dpix_dev, a.dL_dothers_dev = ptr(color), ptr(others), ptr(dL_dpix), ptr(dL_dothers)
+    a.k = int(k)
+    a.chunk_mode = CHUNK[chunk_mode] if isinstance(chunk_mode, str) else int(chunk_mode)
+    a.cutoff, a.near_t, a.far_t = float(cutoff), float(near), float(far)
+    a.zero_grads = 1 if zero_grads else 0
+    a.dL_dvertices_rows = int(dverts.numel() // 3)
+    a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
+    return a

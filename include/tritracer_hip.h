@@ -1,0 +1,156 @@
+/*
+ * tritracer_hip.h -- C ABI of the MI355X (gfx950) triangle-splat tracer.
+ *
+ * The library (libtritracer_hip.so) replaces everything the reference
+ * waizui/diff-triangle-tracer runs through OptiX: the GAS build, optixTrace with
+ * its anyhit k-buffer, and the forward/backward/reflect/refract/envlight
+ * raygen programs. Its only caller in this project is the C++ drop-in
+ * OptixTracer (tracer.h/tracer.cpp), which keeps the reference's torch-facing
+ * signatures and turns nonzero status codes into exceptions.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only. Every pointer named *_dev below is device
+ *     memory (HBM) on the device that was current at tt_create.
+ *   - Every call takes a HIP stream as `void *stream` (NULL = the null stream)
+ *     and is asynchronous on it; nothing synchronises the host unless stated.
+ *   - Every entry point returns TT_OK (0) or a TT_ERR_* code; tt_last_error()
+ *     returns the message of the last failure on the calling thread.
+ *   - A tracer object is not thread-safe: one caller, sequential calls (as the
+ *     reference's OptixTracer, tracer.h:67-126).
+ *
+ * Layouts (the reference's, tracer.cpp:16-142, forward.cu:161-215)
+ *   ray_o, ray_d       float [H*W][3]  pixel p = y*W + x
+ *   vertices           float [num_vertices][3]
+ *   cumsum, npts       int   [P]       triangle i = vertices cumsum[i] .. cumsum[i]+2
+ *   shs                float [P][M][3]
+ *   opacities, sigma   float [P]
+ *   color              float [3][H*W]  planar
+ *   others             float [7][H*W]  0 depth(=0) 1 alpha 2..4 normal 5 mid-depth(=0) 6 distortion(=0)
+ *   dL_dvertices       float [rows][3] rows = npts[P-1] + cumsum[P-1]
+ */
+#ifndef TRITRACER_HIP_H
+#define TRITRACER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TT_OK 0
+#define TT_ERR_INVALID_ARG 1
+#define TT_ERR_HIP 2
+#define TT_ERR_STATE 3
+#define TT_ERR_UNSUPPORTED 4
+
+/* forward-program variants; the reference selects them by PTX file name
+   (diff_tritracer/__init__.py:233-236 -> pipeline.cpp:186-195) */
+#define TT_VARIANT_FORWARD 0  /* forward.cu  */
+#define TT_VARIANT_REFLECT 1  /* reflect.cu  */
+#define TT_VARIANT_REFRACT 2  /* refract.cu  */
+#define TT_VARIANT_ENVLIGHT 3 /* envlight.cu */
+
+/* acceleration-structure modes: the reference's rebuild_gas codes
+   (diff_tritracer/__init__.py:42, tracer.cpp:181-195) */
+#define TT_BUILD 0
+#define TT_REFIT 1
+#define TT_REUSE 2
+
+/* chunk resume rule between k-closest gathers */
+#define TT_CHUNK_COMPAT 0 /* resume at t >= last_hit_t (inclusive; OptiX closed ray interval,
+                             closestk.h:29 with forward.cu:111) */
+#define TT_CHUNK_EXACT 1  /* resume strictly after the last (t, prim); result independent of k */
+
+typedef struct tt_tracer tt_tracer;
+
+/* Replaces OptixTracer::OptixTracer context/pipeline creation (pipeline.cpp:267-302):
+   binds to the current HIP device. */
+int tt_create(tt_tracer **out);
+/* Replaces OptixTracer::~OptixTracer (pipeline.cpp:304-326). Frees device memory. */
+int tt_destroy(tt_tracer *tr);
+/* Message of the last failed call on this thread ("" if none). */
+const char *tt_last_error(void);
+/* Library version string. */
+const char *tt_version(void);
+
+/* Replaces OptixTracer::build_tri_accel (pipeline.cpp:206-265) and the index-buffer
+   set-up of trace_fwd (tracer.cpp:179-195). mode TT_BUILD builds an LBVH over the P
+   triangles, TT_REFIT refits it to moved vertices (same P and cumsum), TT_REUSE
+   does nothing. Vertices are copied into the tracer's leaf array. */
+int tt_build(tt_tracer *tr, const float *vertices_dev, int num_vertices, const int *cumsum_dev, int P, int mode,
+             void *stream);
+
+typedef struct {
+  int width, height;
+  const float *ray_o_dev, *ray_d_dev;
+  const float *vertices_dev;
+  int num_vertices;
+  const int *npts_dev, *cumsum_dev;
+  int P;
+  const float *shs_dev;
+  int M, sh_degree;
+  const float *opacities_dev, *sigma_dev;
+  const float *bg_dev;    /* [3] */
+  const float *extra_dev; /* variant parameters r, cx, cy, cz[, samples]; may be NULL for forward */
+  int extra_len;          /* floats in extra */
+  int variant;            /* TT_VARIANT_* */
+  int k;                  /* hits per chunk: 8, 16 or 32 (reference CHUNK_SIZE = 16) */
+  int chunk_mode;         /* TT_CHUNK_* */
+  float cutoff;           /* transmittance cutoff (reference forward.cu:134: 1e-4) */
+  float near_t, far_t;    /* 0.2, 1000 (auxiliary.h:29-30) */
+  float *color_dev;       /* [3][R] written */
+  float *others_dev;      /* [7][R] written by TT_VARIANT_FORWARD only (may be NULL otherwise) */
+  float *max_blending_dev;/* [P] atomic max of blending weights; caller zeroes (forward only) */
+  float *density_dev;     /* [P] in/out: set to 1 for blended triangles (forward only) */
+  float *scaling_dev;     /* [P] in/out: filled where <= 0 (forward only) */
+  int *stats_dev;         /* optional [R][2]: consumed hits, blended hits */
+  int hit_log_cap;        /* optional debug log of the first hit_log_cap consumed hits per ray */
+  int *hit_log_prim_dev;  /* [R][cap] */
+  float *hit_log_t_dev;   /* [R][cap] */
+  int *hit_log_code_dev;  /* [R][cap] 0 near skip, 1 alpha skip, 2 blended, 3 terminated */
+  int *hit_log_count_dev; /* [R] total consumed hits */
+} tt_fwd_args;
+
+/* Replaces OptixTracer::trace_fwd's launch (tracer.cpp:105-132) and the raygen programs
+   forward.cu:161-215, reflect.cu:102-142, refract.cu:287-327, envlight.cu:190-210. */
+int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream);
+
+typedef struct {
+  int width, height;
+  const float *ray_o_dev, *ray_d_dev;
+  const float *vertices_dev;
+  int num_vertices;
+  const int *npts_dev, *cumsum_dev;
+  int P;
+  const float *shs_dev;
+  int M, sh_degree;
+  const float *opacities_dev, *sigma_dev;
+  const float *color_dev;      /* forward colour [3][R] */
+  const float *others_dev;     /* forward others [7][R] */
+  const float *dL_dpix_dev;    /* [3][R] */
+  const float *dL_dothers_dev; /* [7][R] */
+  int k, chunk_mode;
+  float cutoff, near_t, far_t;
+  int zero_grads;              /* nonzero: the library zeroes the four gradient buffers first */
+  int dL_dvertices_rows;       /* rows of dL_dvertices (for zeroing) */
+  float *dL_dvertices_dev;     /* [rows][3] accumulated (FP32 atomics) */
+  float *dL_dsigma_dev;        /* [P] */
+  float *dL_dopacity_dev;      /* [P] */
+  float *dL_dsh_dev;           /* [P][M][3] */
+} tt_bwd_args;
+
+/* Replaces OptixTracer::trace_bwd's launch (tracer.cpp:199-222) and backward.cu:191-391:
+   re-traverses every ray with the forward's LBVH and scatters per-triangle gradients. */
+int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream);
+
+/* Number of triangles of the current acceleration structure (-1 if none). */
+int tt_num_primitives(const tt_tracer *tr);
+
+/* Timing of the last tt_build on its stream in ms (hipEvents), or -1. */
+float tt_last_build_ms(const tt_tracer *tr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRITRACER_HIP_H */
