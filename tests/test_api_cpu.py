@@ -1,0 +1,159 @@
+"""CPU-side checks of the drop-in boundary (no device work):
+  - libtritracer_hip.so loads and exports every entry point include/tritracer_hip.h declares
+  - the C ABI rejects bad arguments with status codes and messages before touching a device
+  - diff_tritracer._C binds the reference's names (ext.cpp:11-27)
+  - the autograd wiring of _Tracer / Tracer follows the reference's __init__.py:54-283
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import diff_tritracer as D
+from diff_tritracer import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "tritracer_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|float|const char \*)\s*\*?\s*(tt_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = _declared()
+    assert set(names) == set(capi.EXPORTED)
+    lib = C.CDLL(capi.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert b"gfx950" in capi.lib().tt_version()
+
+
+def test_c_abi_argument_errors_without_device():
+    L = capi.lib()
+    assert L.tt_build(None, None, 0, None, 0, 0, None) == 1  # TT_ERR_INVALID_ARG
+    assert "NULL" in capi.last_error()
+    a = capi.FwdArgs()
+    a.k = 16
+    assert L.tt_trace_fwd(None, C.byref(a), None) == 1
+    b = capi.BwdArgs()
+    assert L.tt_trace_bwd(None, C.byref(b), None) == 1
+    assert L.tt_create(None) == 1
+    assert L.tt_destroy(None) == 0
+    assert L.tt_num_primitives(None) == -1
+
+
+def test_extension_binds_reference_names():
+    for name in ("OptixTracer", "FwdTraceResult", "BwdTraceResult"):
+        assert hasattr(D._C, name)
+    for m in ("build_tri_accel", "trace_forward", "trace_backward", "set_options", "get_options"):
+        assert hasattr(D._C.OptixTracer, m)
+    assert D.TracerSettings._fields == ("bg", "scale_modifier", "sh_degree", "debug", "rebuild_gas", "extra_params")
+
+
+def test_unknown_program_names_raise():
+    with pytest.raises(RuntimeError, match="Unknown forward program"):
+        D._C.OptixTracer(D.PACKAGE_DIR, "bogus.ptx", "backward.ptx")
+    with pytest.raises(RuntimeError, match="Unknown backward program"):
+        D._C.OptixTracer(D.PACKAGE_DIR, "forward.ptx", "forward.ptx")
+
+
+class _FakeResult:
+    def __init__(self, data):
+        self.data = data
+
+
+class _FakeTracer:
+    """Stands in for _C.OptixTracer to check the autograd plumbing on the CPU."""
+
+    def __init__(self, fail=False):
+        self.fail = fail
+        self.fwd_args = None
+        self.bwd_args = None
+
+    def trace_forward(self, *args):
+        if self.fail:
+            raise RuntimeError("forced failure")
+        self.fwd_args = args
+        ray_o, P = args[0], args[6]
+        H, W = ray_o.shape[:2]
+        color = torch.full((3, H, W), 0.5)
+        others = torch.zeros(7, H, W)
+        return _FakeResult((color, others, torch.zeros(1), args[11], args[12], torch.zeros(P)))
+
+    def trace_backward(self, *args):
+        self.bwd_args = args
+        verts, sigma, P, shs, opac = args[3], args[4], args[9], args[10], args[12]
+        return _FakeResult((torch.ones(verts.numel() // 3, 3), torch.full((P, 1), 2.0), torch.full_like(opac, 3.0),
+                            torch.full_like(shs, 4.0)))
+
+
+def _inputs(P=5, H=2, W=3):
+    verts = torch.zeros(3 * P * 3, requires_grad=True)
+    sigma = torch.ones(P, 1, requires_grad=True)
+    opac = torch.ones(P, 1, requires_grad=True)
+    shs = torch.zeros(P, 16, 3, requires_grad=True)
+    st = D.TracerSettings(bg=torch.zeros(3), scale_modifier=1.0, sh_degree=3, debug=False, rebuild_gas=0,
+                          extra_params=torch.zeros(0))
+    return dict(ray_o=torch.zeros(H, W, 3), ray_d=torch.zeros(H, W, 3), verts=verts, sigma=sigma,
+                npts=torch.full((P,), 3, dtype=torch.int32), cumsum=torch.arange(P, dtype=torch.int32) * 3, P=P,
+                opac=opac, scaling=torch.zeros(P), density=torch.zeros(P), shs=shs, st=st)
+
+
+def test_autograd_wiring_and_gradient_layouts():
+    x = _inputs()
+    fake = _FakeTracer()
+    out = D._Tracer.apply(fake, x["ray_o"], x["ray_d"], x["verts"], x["sigma"], x["npts"], x["cumsum"], x["P"],
+                          x["shs"], torch.empty(0), x["opac"], x["scaling"], x["density"], x["st"])
+    color, gpu_time, scaling, density, others, maxb = out
+    assert color.shape == (3, 2, 3) and others.shape == (7, 2, 3)
+    # forward argument order of trace_forward (reference __init__.py:75-93)
+    fa = fake.fwd_args
+    assert fa[6] == x["P"] and fa[8] == 3 and fa[15] == 0 and fa[16] is False
+    (color.sum() + others.sum()).backward()
+    # backward argument order (reference __init__.py:162-180) and gradient layouts (:205-227)
+    ba = fake.bwd_args
+    assert ba[9] == x["P"] and ba[11] == 3 and ba[13].shape == (3, 2, 3) and ba[14].shape == (7, 2, 3)
+    assert torch.all(x["verts"].grad == 1.0) and x["verts"].grad.shape == x["verts"].shape
+    assert torch.all(x["sigma"].grad == 2.0) and x["sigma"].grad.shape == (5, 1)
+    assert torch.all(x["opac"].grad == 3.0) and torch.all(x["shs"].grad == 4.0)
+
+
+def test_tracer_requires_exactly_one_colour_source():
+    x = _inputs()
+    tr = D.Tracer.__new__(D.Tracer)
+    torch.nn.Module.__init__(tr)
+    tr.tracer = _FakeTracer()
+    args = (x["ray_o"], x["ray_d"], x["verts"], x["sigma"], x["npts"], x["cumsum"], x["P"], x["opac"], x["scaling"],
+            x["density"])
+    with pytest.raises(Exception, match="exactly|excatly"):
+        tr(*args, None, None, x["st"])
+    with pytest.raises(Exception, match="exactly|excatly"):
+        tr(*args, x["shs"], torch.zeros(5, 3), x["st"])
+    tr(*args, x["shs"], None, x["st"])
+
+
+def test_debug_snapshot_on_failure(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    x = _inputs()
+    st = x["st"]._replace(debug=True)
+    with pytest.raises(RuntimeError, match="forced failure"):
+        D._Tracer.apply(_FakeTracer(fail=True), x["ray_o"], x["ray_d"], x["verts"], x["sigma"], x["npts"],
+                        x["cumsum"], x["P"], x["shs"], torch.empty(0), x["opac"], x["scaling"], x["density"], st)
+    snap = torch.load(tmp_path / "snapshot_fw.dump", weights_only=True)
+    assert len(snap) == 17 and snap[6] == x["P"]
+
+
+def test_scene_generators_deterministic():
+    from diff_tritracer import scenes
+
+    a, ro, rd = scenes.small_config(0, 32, 32)
+    b, ro2, rd2 = scenes.small_config(0, 32, 32)
+    assert np.array_equal(a.verts, b.verts) and np.array_equal(ro, ro2)
+    assert a.verts.shape == (30000, 3) and a.shs.shape == (10000, 16, 3)
+    np.testing.assert_allclose(np.linalg.norm(rd.reshape(-1, 3), axis=1), 1.0, atol=1e-6)
+    c = scenes.clustered_scene(2000, 8, seed=1)
+    assert c.P == 2000 and np.all(np.abs(c.verts) < 2.5)
