@@ -70,8 +70,10 @@ def main():
     ap.add_argument("--triangles", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (calibrates rocprof traffic counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -84,6 +86,15 @@ def main():
     from diff_tritracer import capi, scenes
 
     dev = torch.device("cuda:0")
+    if args.copy_mib:
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty(n, device=dev)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n, "launches": 3}))
+        return
     t0 = time.perf_counter()
     scene, ray_o, ray_d = scenes.scale_config(seed=1, P=args.triangles, width=args.width, height=args.height)
     gen_s = time.perf_counter() - t0
@@ -103,6 +114,7 @@ def main():
     dens = torch.zeros(P, device=dev)
     scal = torch.zeros(P, device=dev)
     stats = torch.zeros(R, 2, dtype=torch.int32, device=dev)
+    trav = torch.zeros(R, 2, dtype=torch.int32, device=dev)
     g = torch.Generator(device="cpu").manual_seed(5)
     dpix = torch.randn(3, R, generator=g).to(dev)
     doth = torch.zeros(7, R, device=dev)
@@ -128,32 +140,41 @@ def main():
     fa = capi.fwd_args(W, H, ro, rd, verts, npts, cumsum, shs, 3, opac, sigma, bg, color, others, maxb, dens, scal,
                        k=args.k, cutoff=args.cutoff)
     fa_stats = capi.fwd_args(W, H, ro, rd, verts, npts, cumsum, shs, 3, opac, sigma, bg, color, others, maxb, dens,
-                             scal, k=args.k, cutoff=args.cutoff, stats=stats)
+                             scal, k=args.k, cutoff=args.cutoff, stats=stats, trav_stats=trav)
     ba = capi.bwd_args(W, H, ro, rd, verts, npts, cumsum, shs, 3, opac, sigma, color, others, dpix, doth, dverts,
-                       dsig, dop, dsh, k=args.k, cutoff=args.cutoff, zero_grads=True)
+                       dsig, dop, dsh, k=args.k, cutoff=args.cutoff, zero_grads=False)
+    grads = (dverts, dsig, dop, dsh)
 
     # counting pass for the algorithmic bytes
     h.fwd(fa_stats, sp)
     torch.cuda.synchronize()
     st = stats.to(torch.int64).sum(0).cpu().numpy()
     Hc, Hb = int(st[0]), int(st[1])
-    print(f"[bench] hits consumed {Hc / R:.2f}/ray, blended {Hb / R:.2f}/ray", file=sys.stderr, flush=True)
+    tv = trav.to(torch.int64).sum(0).cpu().numpy()
+    print(f"[bench] hits consumed {Hc / R:.2f}/ray, blended {Hb / R:.2f}/ray; BVH nodes visited {tv[0] / R:.1f}/ray, "
+          f"triangles tested {tv[1] / R:.1f}/ray (forward)", file=sys.stderr, flush=True)
 
     def step(ev=None):
+        # forward, gradient zeroing (part of the step: the reference allocates zeroed
+        # gradients every call), backward; events bracket each kernel on the launch stream
         if ev is not None:
             ev[0].record(stream)
         h.fwd(fa, sp)
         if ev is not None:
             ev[1].record(stream)
-        h.bwd(ba, sp)
+        for t in grads:
+            t.zero_()
         if ev is not None:
             ev[2].record(stream)
+        h.bwd(ba, sp)
+        if ev is not None:
+            ev[3].record(stream)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
 
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
@@ -165,18 +186,20 @@ def main():
     wall = time.perf_counter() - t_wall
     total_ms = start.elapsed_time(stop)
     fwd_ms = [e[0].elapsed_time(e[1]) for e in evs]
-    bwd_ms = [e[1].elapsed_time(e[2]) for e in evs]
+    zero_ms = [e[1].elapsed_time(e[2]) for e in evs]
+    bwd_ms = [e[2].elapsed_time(e[3]) for e in evs]
     ms_per_step = total_ms / args.steps
     mrays = R / (ms_per_step * 1e-3) / 1e6
-    print(f"[bench] {ms_per_step:.3f} ms/step (fwd {np.mean(fwd_ms):.3f}, bwd+zero {np.mean(bwd_ms):.3f}); "
+    print(f"[bench] {ms_per_step:.3f} ms/step (fwd {np.mean(fwd_ms):.3f}, zero {np.mean(zero_ms):.3f}, "
+          f"bwd {np.mean(bwd_ms):.3f}); "
           f"wall {wall:.2f} s", file=sys.stderr, flush=True)
 
     bf, bb = algorithmic_bytes(R, Hc, Hb)
     f_avg, b_avg = float(np.mean(fwd_ms)), float(np.mean(bwd_ms))
     if b_avg >= f_avg:
-        dom, dom_bytes, dom_ms = "k_backward<16> (with gradient zeroing)", bb, b_avg
+        dom, dom_bytes, dom_ms = f"k_backward<{args.k}>", bb, b_avg
     else:
-        dom, dom_bytes, dom_ms = "k_forward<16,0>", bf, f_avg
+        dom, dom_bytes, dom_ms = f"k_forward<{args.k},0>", bf, f_avg
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=dom,
@@ -208,8 +231,9 @@ def main():
                         f"k={args.k}, cutoff {args.cutoff:g}, N(0,1) loss gradient",
             "triangles": P, "width": W, "height": H, "k": args.k, "cutoff": args.cutoff, "chunk_mode": "compat",
             "lbvh_build_ms": round(build_ms, 3), "lbvh_refit_ms": round(refit_ms, 3),
-            "fwd_ms": round(f_avg, 4), "bwd_ms": round(b_avg, 4),
+            "fwd_ms": round(f_avg, 4), "bwd_ms": round(b_avg, 4), "zero_grads_ms": round(float(np.mean(zero_ms)), 4),
             "hits_consumed_per_ray": round(Hc / R, 3), "hits_blended_per_ray": round(Hb / R, 3),
+            "nodes_visited_per_ray": round(float(tv[0]) / R, 2), "triangles_tested_per_ray": round(float(tv[1]) / R, 2),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -9,13 +9,16 @@
 //   3. LSD radix sort, 4 x 8-bit passes: LDS digit histograms, one-block
 //      exclusive scan, stable LDS-ranked scatter (ballot peer masks, wave64)
 //   4. Karras hierarchy (index tie-break for equal codes), parent links
-//   5. bottom-up child boxes with per-node atomic arrival counters; the same
-//      kernel is the refit (topology kept, leaves re-gathered from new vertices)
+//   5. node heights bottom-up (one 64-bit atomic OR per node, no fences), nodes
+//      bucketed by height, then child boxes level by level, one launch per height
+//      (the kernel boundary orders the levels; per-XCD L2s need no fences).
+//      Refit = re-gather leaves + the level launches (topology and levels kept).
 // Child boxes live in the parent node, so traversal reads one 64 B node to
 // test both children.
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "tt_internal.h"
 
@@ -242,21 +245,18 @@ __global__ void k_single(int4 *links, int *leaf_parent, uint32_t *vals) {
   vals[1] = 0xFFFFFFFFu;
 }
 
-__global__ void k_zero_int(int *p, int n) {
+__global__ void k_zero_u64(unsigned long long *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+  if (i < n) p[i] = 0ull;
+}
+__global__ void k_zero_u32(uint32_t *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
-__device__ __forceinline__ float ld_acq(const float *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Leaves (re)gathered in Morton order, then boxes propagated to the root: the
-// second child to arrive at a node unions both child boxes and moves up.
-__global__ __launch_bounds__(256) void k_bottom_up(const float *__restrict__ verts, const int *__restrict__ cumsum,
-                                                   const uint32_t *__restrict__ order, int n_leaves,
-                                                   float4 *leaves, float4 *nodes, const int *__restrict__ leaf_parent,
-                                                   int *flags) {
+// Leaves (re)gathered from the vertex buffer in Morton order.
+__global__ __launch_bounds__(256) void k_leaves(const float *__restrict__ verts, const int *__restrict__ cumsum,
+                                                const uint32_t *__restrict__ order, int n_leaves, float4 *leaves) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_leaves) return;
   uint32_t prim = order[s];
@@ -270,42 +270,159 @@ __global__ __launch_bounds__(256) void k_bottom_up(const float *__restrict__ ver
   leaves[3 * (size_t)s + 0] = make_float4(A[0], A[1], A[2], __uint_as_float(prim));
   leaves[3 * (size_t)s + 1] = make_float4(B[0], B[1], B[2], 0.0f);
   leaves[3 * (size_t)s + 2] = make_float4(C[0], C[1], C[2], 0.0f);
-  float lo[3], hi[3];
-  for (int a = 0; a < 3; a++) {
-    lo[a] = fminf(A[a], fminf(B[a], C[a]));
-    hi[a] = fmaxf(A[a], fmaxf(B[a], C[a]));
-    float pad = 1e-5f * (1.0f + fmaxf(fabsf(lo[a]), fabsf(hi[a])));
-    lo[a] -= pad;
-    hi[a] += pad;
-  }
+}
+
+// Node heights (leaf 0, internal 1 + max over children), bottom-up. Both children
+// report into one 64-bit word of the parent (one 32-bit half each) with a single
+// atomic OR: the second arrival sees the sibling's value in the returned word, so
+// no other memory has to be made visible across XCDs and no fence is needed.
+__global__ __launch_bounds__(256) void k_heights(const float4 *__restrict__ nodes, const int *__restrict__ leaf_parent,
+                                                 int n_leaves, unsigned long long *hword, int *height,
+                                                 uint32_t *maxh) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_leaves) return;
   int child = ~s;
   int p = leaf_parent[s];
+  uint32_t h = 1;
   while (true) {
-    float4 *nd = nodes + 4 * (size_t)p;
-    const int *li = reinterpret_cast<const int *>(nd + 3);
-    int slot = (li[0] == child) ? 0 : 1;
-    float *b = reinterpret_cast<float *>(nd + slot);
-    b[0] = lo[0], b[1] = hi[0], b[2] = lo[1], b[3] = hi[1];
-    float *bz = reinterpret_cast<float *>(nd + 2) + 2 * slot;
-    bz[0] = lo[2], bz[1] = hi[2];
-    __threadfence();
-    int old = atomicAdd(flags + p, 1);
-    if (old == 0) return;
-    __threadfence();
-    const float *f0 = reinterpret_cast<const float *>(nd);
-    const float *f1 = reinterpret_cast<const float *>(nd + 1);
-    const float *f2 = reinterpret_cast<const float *>(nd + 2);
-    lo[0] = fminf(ld_acq(f0 + 0), ld_acq(f1 + 0));
-    hi[0] = fmaxf(ld_acq(f0 + 1), ld_acq(f1 + 1));
-    lo[1] = fminf(ld_acq(f0 + 2), ld_acq(f1 + 2));
-    hi[1] = fmaxf(ld_acq(f0 + 3), ld_acq(f1 + 3));
-    lo[2] = fminf(ld_acq(f2 + 0), ld_acq(f2 + 2));
-    hi[2] = fmaxf(ld_acq(f2 + 1), ld_acq(f2 + 3));
-    int parent = li[2];
-    if (parent < 0) return;
+    const int *li = reinterpret_cast<const int *>(nodes + 4 * (size_t)p + 3);
+    const int slot = (li[0] == child) ? 0 : 1;
+    const unsigned long long old = atomicOr(hword + p, (unsigned long long)h << (32 * slot));
+    const uint32_t other = (uint32_t)(old >> (32 * (1 - slot)));
+    if (other == 0u) return;
+    const uint32_t hp = max(h, other);
+    height[p] = (int)hp;
+    const int parent = li[2];
+    if (parent < 0) {
+      atomicMax(maxh, hp);
+      return;
+    }
     child = p;
     p = parent;
+    h = hp + 1;
   }
+}
+
+// Height histogram: per-block LDS counts, one global add per (block, height).
+__global__ __launch_bounds__(256) void k_level_hist(const int *__restrict__ height, int ni, uint32_t *hist) {
+  __shared__ uint32_t h[kMaxLevels];
+  for (int i = threadIdx.x; i < kMaxLevels; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < ni) atomicAdd(&h[height[n]], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kMaxLevels; i += blockDim.x)
+    if (h[i]) atomicAdd(hist + i, h[i]);
+}
+
+// Nodes into height order: each block reserves one range per height with a single
+// global atomic, then places its nodes by their LDS rank inside that range.
+__global__ __launch_bounds__(256) void k_level_scatter(const int *__restrict__ height, int ni,
+                                                       const uint32_t *__restrict__ offs, uint32_t *cursor,
+                                                       int *level_nodes) {
+  __shared__ uint32_t cnt[kMaxLevels];
+  __shared__ uint32_t base[kMaxLevels];
+  for (int i = threadIdx.x; i < kMaxLevels; i += blockDim.x) cnt[i] = 0u;
+  __syncthreads();
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int h = 0;
+  uint32_t rank = 0;
+  if (n < ni) {
+    h = height[n];
+    rank = atomicAdd(&cnt[h], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kMaxLevels; i += blockDim.x)
+    if (cnt[i]) base[i] = offs[i] + atomicAdd(cursor + i, cnt[i]);
+  __syncthreads();
+  if (n < ni) level_nodes[base[h] + rank] = n;
+}
+
+// One height level: write both child boxes of every node of the level into the
+// node. Children are leaves (box from the leaf triangle, padded) or nodes of lower
+// height, completed by an earlier launch.
+__global__ __launch_bounds__(256) void k_level_boxes(const int *__restrict__ level_nodes, int count,
+                                                     const float4 *__restrict__ leaves, float4 *nodes) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int n = level_nodes[i];
+  float4 *nd = nodes + 4 * (size_t)n;
+  const int4 li = *reinterpret_cast<const int4 *>(nd + 3);
+  float lo[2][3], hi[2][3];
+  for (int slot = 0; slot < 2; slot++) {
+    const int c = slot ? li.y : li.x;
+    if (c < 0) {
+      const int s = ~c;
+      const float4 A = leaves[3 * (size_t)s], B = leaves[3 * (size_t)s + 1], C = leaves[3 * (size_t)s + 2];
+      const float av[3] = {A.x, A.y, A.z}, bv[3] = {B.x, B.y, B.z}, cv[3] = {C.x, C.y, C.z};
+      for (int a = 0; a < 3; a++) {
+        float l = fminf(av[a], fminf(bv[a], cv[a])), h = fmaxf(av[a], fmaxf(bv[a], cv[a]));
+        const float pad = 1e-5f * (1.0f + fmaxf(fabsf(l), fabsf(h)));
+        lo[slot][a] = l - pad;
+        hi[slot][a] = h + pad;
+      }
+    } else {
+      const float4 f0 = nodes[4 * (size_t)c], f1 = nodes[4 * (size_t)c + 1], f2 = nodes[4 * (size_t)c + 2];
+      lo[slot][0] = fminf(f0.x, f1.x);
+      hi[slot][0] = fmaxf(f0.y, f1.y);
+      lo[slot][1] = fminf(f0.z, f1.z);
+      hi[slot][1] = fmaxf(f0.w, f1.w);
+      lo[slot][2] = fminf(f2.x, f2.z);
+      hi[slot][2] = fmaxf(f2.y, f2.w);
+    }
+  }
+  nd[0] = make_float4(lo[0][0], hi[0][0], lo[0][1], hi[0][1]);
+  nd[1] = make_float4(lo[1][0], hi[1][0], lo[1][1], hi[1][1]);
+  nd[2] = make_float4(lo[0][2], hi[0][2], lo[1][2], hi[1][2]);
+}
+
+// 4-wide nodes: node4[n] holds n's children, each internal child replaced by its
+// two children. The traversal visits binary depths 0, 2, 4, ... only.
+__global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ nodes, int ni, float4 *nodes4) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ni) return;
+  float lo[3][4], hi[3][4];
+  int ref[4];
+  int e = 0;
+  auto slot_box = [&](int node, int slot, float l[3], float h[3]) {
+    const float4 a = nodes[4 * (size_t)node + slot], z = nodes[4 * (size_t)node + 2];
+    l[0] = a.x, h[0] = a.y, l[1] = a.z, h[1] = a.w;
+    l[2] = slot ? z.z : z.x;
+    h[2] = slot ? z.w : z.y;
+  };
+  auto put = [&](int r, const float l[3], const float h[3]) {
+    for (int a = 0; a < 3; a++) lo[a][e] = l[a], hi[a][e] = h[a];
+    ref[e] = r;
+    e++;
+  };
+  const int4 li = *reinterpret_cast<const int4 *>(nodes + 4 * (size_t)n + 3);
+  for (int slot = 0; slot < 2; slot++) {
+    const int c = slot ? li.y : li.x;
+    float l[3], h[3];
+    if (c < 0) {
+      slot_box(n, slot, l, h);
+      put(c, l, h);
+    } else {
+      const int4 lc = *reinterpret_cast<const int4 *>(nodes + 4 * (size_t)c + 3);
+      slot_box(c, 0, l, h);
+      put(lc.x, l, h);
+      slot_box(c, 1, l, h);
+      put(lc.y, l, h);
+    }
+  }
+  for (; e < 4; e++) {
+    for (int a = 0; a < 3; a++) lo[a][e] = 1.0f, hi[a][e] = 0.0f;
+    ref[e] = kEmptyChild;
+  }
+  float4 *o = nodes4 + 8 * (size_t)n;
+  o[0] = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+  o[1] = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+  o[2] = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+  o[3] = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+  o[4] = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+  o[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+  *reinterpret_cast<int4 *>(o + 6) = make_int4(ref[0], ref[1], ref[2], ref[3]);
+  o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 template <class T>
@@ -322,8 +439,8 @@ hipError_t grow(T *&ptr, size_t &cap, size_t need) {
 }  // namespace
 
 void bvh_free(Bvh &b) {
-  void *ps[] = {b.nodes, b.leaves, b.leaf_parent, b.flags, b.keys[0], b.keys[1], b.vals[0], b.vals[1],
-                b.hist, b.bounds, b.centroids};
+  void *ps[] = {b.nodes, b.nodes4, b.leaves, b.leaf_parent, b.keys[0], b.keys[1], b.vals[0], b.vals[1], b.hist,
+                b.bounds, b.centroids, b.hword, b.height, b.level_nodes, b.level_meta};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   b = Bvh();
@@ -335,75 +452,116 @@ void bvh_free(Bvh &b) {
     if (e_ != hipSuccess) return std::string(#x " failed: ") + hipGetErrorString(e_);     \
   } while (0)
 
+static std::string refit_levels(Bvh &b, const float *verts, const int *cumsum, hipStream_t stream) {
+  const int nl = b.n_leaves;
+  hipLaunchKernelGGL(k_leaves, dim3((nl + 255) / 256), dim3(256), 0, stream, verts, cumsum, b.vals[0], nl, b.leaves);
+  for (int h = 1; h <= b.height_max; h++) {
+    const int off = b.level_off[h], cnt = b.level_off[h + 1] - b.level_off[h];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_level_boxes, dim3((cnt + 255) / 256), dim3(256), 0, stream, b.level_nodes + off, cnt,
+                       b.leaves, b.nodes);
+  }
+  hipLaunchKernelGGL(k_collapse4, dim3((b.n_internal + 255) / 256), dim3(256), 0, stream, b.nodes, b.n_internal,
+                     b.nodes4);
+  TT_TRY(hipGetLastError());
+  return "";
+}
+
 std::string bvh_build(Bvh &b, const float *verts, int num_vertices, const int *cumsum, int P, int mode,
                       hipStream_t stream) {
   (void)num_vertices;
   if (mode == 1) {
     if (b.P < 0) return "refit requested but no acceleration structure has been built";
     if (b.P != P) return "refit requires the same number of triangles as the last build";
+    if (P == 0) return "";
+    return refit_levels(b, verts, cumsum, stream);
   }
   if (P == 0) {
     b.P = 0;
     b.n_leaves = 0;
     b.n_internal = 0;
+    b.height_max = 0;
     return "";
   }
   const int nl = P == 1 ? 2 : P;
   const int ni = nl - 1;
-  if (mode == 0) {
-    size_t need = (size_t)nl;
-    if (b.cap_leaves < need || !b.nodes) {
-      void *ps[] = {b.nodes, b.leaves, b.leaf_parent, b.flags, b.keys[0], b.keys[1], b.vals[0], b.vals[1], b.centroids};
-      for (void *p : ps)
-        if (p) (void)hipFree(p);
-      b.nodes = nullptr, b.leaves = nullptr, b.leaf_parent = nullptr, b.flags = nullptr;
-      b.keys[0] = b.keys[1] = b.vals[0] = b.vals[1] = nullptr;
-      b.centroids = nullptr;
-      b.cap_leaves = 0;
-      TT_TRY(hipMalloc(&b.nodes, sizeof(float4) * 4 * need));
-      TT_TRY(hipMalloc(&b.leaves, sizeof(float4) * 3 * need));
-      TT_TRY(hipMalloc(&b.leaf_parent, sizeof(int) * need));
-      TT_TRY(hipMalloc(&b.flags, sizeof(int) * need));
-      TT_TRY(hipMalloc(&b.keys[0], sizeof(uint32_t) * need));
-      TT_TRY(hipMalloc(&b.keys[1], sizeof(uint32_t) * need));
-      TT_TRY(hipMalloc(&b.vals[0], sizeof(uint32_t) * need));
-      TT_TRY(hipMalloc(&b.vals[1], sizeof(uint32_t) * need));
-      TT_TRY(hipMalloc(&b.centroids, sizeof(float4) * need));
-      b.cap_leaves = need;
-    }
-    if (!b.bounds) TT_TRY(hipMalloc(&b.bounds, sizeof(uint32_t) * 8));
-    if (P == 1) {
-      hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, stream, reinterpret_cast<int4 *>(b.nodes), b.leaf_parent,
-                         b.vals[0]);
-    } else {
-      const int nblk = (P + 255) / 256;
-      hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, stream, b.bounds);
-      hipLaunchKernelGGL(k_centroids, dim3(nblk), dim3(256), 0, stream, verts, cumsum, P, b.centroids, b.bounds);
-      hipLaunchKernelGGL(k_morton, dim3(nblk), dim3(256), 0, stream, b.centroids, P, b.bounds, b.keys[0], b.vals[0]);
-      const int sblk = (P + kSortTile - 1) / kSortTile;
-      TT_TRY(grow(b.hist, b.cap_hist, (size_t)256 * sblk));
-      int cur = 0;
-      for (int shift = 0; shift < 32; shift += 8) {
-        hipLaunchKernelGGL(k_hist, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], P, shift, b.hist, sblk);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, b.hist, 256 * sblk);
-        hipLaunchKernelGGL(k_scatter, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], b.vals[cur],
-                           b.keys[cur ^ 1], b.vals[cur ^ 1], P, shift, b.hist, sblk);
-        cur ^= 1;
-      }
-      // four passes: sorted data is back in buffer 0
-      hipLaunchKernelGGL(k_karras, dim3((ni + 255) / 256), dim3(256), 0, stream, b.keys[0], P,
-                         reinterpret_cast<int4 *>(b.nodes), b.leaf_parent);
-    }
-    b.P = P;
-    b.n_leaves = nl;
-    b.n_internal = ni;
-    b.cumsum_seen = cumsum;
+  const size_t need = (size_t)nl;
+  if (b.cap_leaves < need || !b.nodes) {
+    void *ps[] = {b.nodes, b.nodes4, b.leaves, b.leaf_parent, b.keys[0], b.keys[1], b.vals[0], b.vals[1],
+                  b.centroids, b.hword, b.height, b.level_nodes};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    b.nodes = nullptr, b.nodes4 = nullptr, b.leaves = nullptr, b.leaf_parent = nullptr;
+    b.keys[0] = b.keys[1] = b.vals[0] = b.vals[1] = nullptr;
+    b.centroids = nullptr, b.hword = nullptr, b.height = nullptr, b.level_nodes = nullptr;
+    b.cap_leaves = 0;
+    TT_TRY(hipMalloc(&b.nodes, sizeof(float4) * 4 * need));
+    TT_TRY(hipMalloc(&b.nodes4, sizeof(float4) * 8 * need));
+    TT_TRY(hipMalloc(&b.leaves, sizeof(float4) * 3 * need));
+    TT_TRY(hipMalloc(&b.leaf_parent, sizeof(int) * need));
+    TT_TRY(hipMalloc(&b.keys[0], sizeof(uint32_t) * need));
+    TT_TRY(hipMalloc(&b.keys[1], sizeof(uint32_t) * need));
+    TT_TRY(hipMalloc(&b.vals[0], sizeof(uint32_t) * need));
+    TT_TRY(hipMalloc(&b.vals[1], sizeof(uint32_t) * need));
+    TT_TRY(hipMalloc(&b.centroids, sizeof(float4) * need));
+    TT_TRY(hipMalloc(&b.hword, sizeof(unsigned long long) * need));
+    TT_TRY(hipMalloc(&b.height, sizeof(int) * need));
+    TT_TRY(hipMalloc(&b.level_nodes, sizeof(int) * need));
+    b.cap_leaves = need;
   }
-  hipLaunchKernelGGL(k_zero_int, dim3((ni + 255) / 256), dim3(256), 0, stream, b.flags, ni);
-  hipLaunchKernelGGL(k_bottom_up, dim3((nl + 255) / 256), dim3(256), 0, stream, verts, cumsum, b.vals[0], nl,
-                     b.leaves, b.nodes, b.leaf_parent, b.flags);
-  TT_TRY(hipGetLastError());
-  return "";
+  if (!b.bounds) TT_TRY(hipMalloc(&b.bounds, sizeof(uint32_t) * 8));
+  if (!b.level_meta) TT_TRY(hipMalloc(&b.level_meta, sizeof(uint32_t) * 3 * kMaxLevels));
+  uint32_t *hist = b.level_meta, *offs = b.level_meta + kMaxLevels, *cursor = b.level_meta + 2 * kMaxLevels;
+  if (P == 1) {
+    hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, stream, reinterpret_cast<int4 *>(b.nodes), b.leaf_parent,
+                       b.vals[0]);
+  } else {
+    const int nblk = (P + 255) / 256;
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, stream, b.bounds);
+    hipLaunchKernelGGL(k_centroids, dim3(nblk), dim3(256), 0, stream, verts, cumsum, P, b.centroids, b.bounds);
+    hipLaunchKernelGGL(k_morton, dim3(nblk), dim3(256), 0, stream, b.centroids, P, b.bounds, b.keys[0], b.vals[0]);
+    const int sblk = (P + kSortTile - 1) / kSortTile;
+    TT_TRY(grow(b.hist, b.cap_hist, (size_t)256 * sblk));
+    int cur = 0;
+    for (int shift = 0; shift < 32; shift += 8) {
+      hipLaunchKernelGGL(k_hist, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], P, shift, b.hist, sblk);
+      hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, b.hist, 256 * sblk);
+      hipLaunchKernelGGL(k_scatter, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], b.vals[cur],
+                         b.keys[cur ^ 1], b.vals[cur ^ 1], P, shift, b.hist, sblk);
+      cur ^= 1;
+    }
+    // four passes: sorted data is back in buffer 0
+    hipLaunchKernelGGL(k_karras, dim3((ni + 255) / 256), dim3(256), 0, stream, b.keys[0], P,
+                       reinterpret_cast<int4 *>(b.nodes), b.leaf_parent);
+  }
+  // heights and levels (the level sizes come back to the host once per build)
+  hipLaunchKernelGGL(k_zero_u64, dim3((ni + 255) / 256), dim3(256), 0, stream, b.hword, ni);
+  hipLaunchKernelGGL(k_zero_u32, dim3((3 * kMaxLevels + 255) / 256), dim3(256), 0, stream, b.level_meta,
+                     3 * kMaxLevels);
+  hipLaunchKernelGGL(k_heights, dim3((nl + 255) / 256), dim3(256), 0, stream, b.nodes, b.leaf_parent, nl, b.hword,
+                     b.height, cursor + kMaxLevels - 1);
+  hipLaunchKernelGGL(k_level_hist, dim3((ni + 255) / 256), dim3(256), 0, stream, b.height, ni, hist);
+  std::vector<uint32_t> h_hist(kMaxLevels);
+  TT_TRY(hipMemcpyAsync(h_hist.data(), hist, sizeof(uint32_t) * kMaxLevels, hipMemcpyDeviceToHost, stream));
+  uint32_t hmax = 0;
+  TT_TRY(hipMemcpyAsync(&hmax, cursor + kMaxLevels - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  TT_TRY(hipStreamSynchronize(stream));
+  if (hmax == 0 || hmax >= (uint32_t)kMaxLevels - 2) return "LBVH height out of range";
+  b.height_max = (int)hmax;
+  b.level_off.assign(kMaxLevels + 1, 0);
+  for (int h = 0; h < kMaxLevels; h++) b.level_off[h + 1] = b.level_off[h] + (int)h_hist[h];
+  if (b.level_off[hmax + 1] != ni) return "LBVH level bucketing lost nodes";
+  std::vector<uint32_t> h_offs(b.level_off.begin(), b.level_off.begin() + kMaxLevels);
+  TT_TRY(hipMemcpyAsync(offs, h_offs.data(), sizeof(uint32_t) * kMaxLevels, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, cursor, kMaxLevels - 1);
+  hipLaunchKernelGGL(k_level_scatter, dim3((ni + 255) / 256), dim3(256), 0, stream, b.height, ni, offs, cursor,
+                     b.level_nodes);
+  TT_TRY(hipStreamSynchronize(stream));  // h_offs must outlive the copy
+  b.P = P;
+  b.n_leaves = nl;
+  b.n_internal = ni;
+  b.cumsum_seen = cumsum;
+  return refit_levels(b, verts, cumsum, stream);
 }
 
 }  // namespace tt

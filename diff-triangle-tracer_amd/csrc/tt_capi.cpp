@@ -15,6 +15,8 @@
 struct tt_tracer {
   int device = 0;
   tt::Bvh bvh;
+  int *ov_stack = nullptr;  // traversal stack spill (entries below the LDS part)
+  size_t ov_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool build_timed = false;
 };
@@ -60,6 +62,27 @@ int check_common(const tt_tracer *tr, int W, int H, const float *ro, const float
   return TT_OK;
 }
 
+// Stack spill for a launch over W x H rays: a 4-wide walk defers at most 3 children
+// at each of at most ceil(height/2) nodes on its path; entries beyond the LDS part spill.
+int prepare_spill(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
+  const int depth = 3 * ((tr->bvh.height_max + 1) / 2) + 1 - tt::kLdsStack;
+  p.ov_depth = depth > 0 ? depth : 0;
+  p.ov_stack = tr->ov_stack;
+  if (p.ov_depth == 0 || tr->bvh.n_leaves == 0) return TT_OK;
+  const size_t need = tt::trace_lanes(W, H) * (size_t)p.ov_depth;
+  if (need > tr->ov_cap) {
+    if (tr->ov_stack) (void)hipFree(tr->ov_stack);
+    tr->ov_stack = nullptr;
+    tr->ov_cap = 0;
+    hipError_t e = hipMalloc(&tr->ov_stack, need * sizeof(int));
+    if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("traversal stack spill allocation failed: ") +
+                                                     hipGetErrorString(e));
+    tr->ov_cap = need;
+  }
+  p.ov_stack = tr->ov_stack;
+  return TT_OK;
+}
+
 void fill_common(tt::TraceParams &p, const tt_tracer *tr, int W, int H, const float *ro, const float *rd,
                  const float *verts, const int *npts, const int *cumsum, int P, const float *shs, int M, int deg,
                  const float *opac, const float *sigma, int chunk_mode, float cutoff, float near_t, float far_t) {
@@ -79,6 +102,7 @@ void fill_common(tt::TraceParams &p, const tt_tracer *tr, int W, int H, const fl
   p.opac = opac;
   p.sigma = sigma;
   p.nodes = tr->bvh.nodes;
+  p.nodes4 = tr->bvh.nodes4;
   p.leaves = tr->bvh.leaves;
   p.n_leaves = P > 0 ? tr->bvh.n_leaves : 0;
   p.chunk_mode = chunk_mode;
@@ -116,6 +140,7 @@ int tt_destroy(tt_tracer *tr) {
   (void)hipGetDevice(&dev);
   (void)hipSetDevice(tr->device);
   tt::bvh_free(tr->bvh);
+  if (tr->ov_stack) (void)hipFree(tr->ov_stack);
   if (tr->ev0) (void)hipEventDestroy(tr->ev0);
   if (tr->ev1) (void)hipEventDestroy(tr->ev1);
   (void)hipSetDevice(dev);
@@ -197,11 +222,13 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.density = a->density_dev;
   p.scaling = a->scaling_dev;
   p.stats = a->stats_dev;
+  p.trav_stats = a->trav_stats_dev;
   p.log_cap = a->hit_log_cap > 0 ? a->hit_log_cap : 0;
   p.log_prim = a->hit_log_prim_dev;
   p.log_t = a->hit_log_t_dev;
   p.log_code = a->hit_log_code_dev;
   p.log_count = a->hit_log_count_dev;
+  if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   return ok();
@@ -243,6 +270,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   p.dsigma = a->dL_dsigma_dev;
   p.dopac = a->dL_dopacity_dev;
   p.dsh = a->dL_dsh_dev;
+  if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   std::string err = tt::launch_backward(p, a->k, s);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   return ok();

@@ -4,8 +4,12 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 namespace tt {
+
+constexpr int kMaxLevels = 256;
+constexpr int kEmptyChild = 0x7fffffff;  // bound on LBVH height bookkeeping (a Karras tree has at most 64)
 
 // LBVH in HBM.
 //   nodes  : 4 x 16 B per internal node
@@ -13,15 +17,25 @@ namespace tt {
 //            [2] (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)   [3] int (child0, child1, parent, 0)
 //            child >= 0: internal node index; child < 0: leaf ~s (sorted position s)
 //   leaves : 3 x 16 B per leaf in Morton order: (A.xyz, prim bits) (B.xyz, 0) (C.xyz, 0)
+//   nodes4 : 8 x 16 B per binary node n, the traversal's 4-wide node: n's children, with
+//            each internal child replaced by its own two children (2 to 4 entries)
+//            [0] lo.x[4] [1] hi.x[4] [2] lo.y[4] [3] hi.y[4] [4] lo.z[4] [5] hi.z[4]
+//            [6] int child[4] (>= 0 binary node, < 0 leaf ~s, kEmptyChild unused) [7] unused
 struct Bvh {
   int P = -1;           // triangles in the structure (-1: none built)
   int n_leaves = 0;     // P, or 2 when P == 1 (second leaf is an unhittable dummy)
   int n_internal = 0;   // n_leaves - 1
   size_t cap_leaves = 0;
   float4 *nodes = nullptr;
+  float4 *nodes4 = nullptr;   // 4-wide view, 8 x 16 B per binary node n: the up-to-4 grandchildren of n
   float4 *leaves = nullptr;
   int *leaf_parent = nullptr;
-  int *flags = nullptr;
+  unsigned long long *hword = nullptr;  // per node: the two children's reported heights
+  int *height = nullptr;                // per internal node: 1 + max child height (leaf = 0)
+  int *level_nodes = nullptr;           // internal nodes ordered by height
+  uint32_t *level_meta = nullptr;       // device: histogram / offsets / cursors of the levels
+  std::vector<int> level_off;           // host: level h occupies level_nodes[level_off[h], level_off[h+1])
+  int height_max = 0;                   // root height = bound on the traversal stack depth
   uint32_t *keys[2] = {nullptr, nullptr};
   uint32_t *vals[2] = {nullptr, nullptr};
   uint32_t *hist = nullptr;     // radix digit counts [256][blocks]
@@ -46,8 +60,10 @@ struct TraceParams {
   const float *shs;
   int M, deg;
   const float *opac, *sigma;
-  const float4 *nodes, *leaves;
+  const float4 *nodes, *leaves, *nodes4;
   int n_leaves;
+  int *ov_stack;  // traversal stack spill: ov_depth entries per launched lane
+  int ov_depth;
   int chunk_mode;
   float cutoff, near_t, far_t;
   // forward
@@ -56,6 +72,7 @@ struct TraceParams {
   int extra_len;
   float *color, *others, *maxb, *density, *scaling;
   int *stats;
+  int *trav_stats;
   int log_cap;
   int *log_prim;
   float *log_t;
@@ -65,6 +82,10 @@ struct TraceParams {
   const float *fcolor, *fothers, *dpix, *dothers;
   float *dverts, *dsigma, *dopac, *dsh;
 };
+
+// Lanes a trace launch uses for an image of W x H (the stack spill needs one slot each).
+size_t trace_lanes(int W, int H);
+constexpr int kLdsStack = 16;  // traversal stack entries per lane kept in LDS; deeper ones spill
 
 // Launch the forward kernel for (k, variant). Returns an error message or "".
 std::string launch_forward(const TraceParams &p, int k, int variant, hipStream_t stream);

@@ -10,12 +10,20 @@
 // Mapping: one ray per lane; each wave64 takes an 8x8 pixel tile, a block of
 // 128 lanes a 16x8 tile. Blocks are remapped so that the tiles one XCD works
 // on form a contiguous band of the image (workgroups are dispatched to the 8
-// XCDs round-robin and each XCD has its own L2). The traversal stack is a
-// per-lane stack in LDS ([slot][lane] layout, bank-conflict free) of 64
-// entries: a Karras LBVH over 30-bit Morton codes with index tie-break has at
-// most 63 internal levels (the common prefix of a node's key range grows by at
-// least one bit per level, over 62 key bits), and the ordered traversal defers
-// at most one sibling per level, so the stack cannot overflow.
+// XCDs round-robin and each XCD has its own L2). Traversal walks the 4-wide
+// view of the LBVH (every second binary level: half the dependent node fetches
+// of a binary walk for the same box tests). The stack keeps its top 16 entries
+// per lane in LDS ([slot][lane] layout, bank-conflict free) and spills deeper
+// entries to a per-lane global area sized from the tree height: each visited
+// 4-wide node defers at most 3 children and a root-to-leaf path visits at most
+// ceil(height/2) of them.
+//
+// Backward gradients are first summed per block in an LDS table keyed by
+// triangle (128 slots x 59 floats, LDS float atomics; linear probing, global
+// atomics on overflow) and scattered once per block: on the 1080p / 1M-splat
+// workload a 128-ray block sees ~11x fewer distinct triangles than blended
+// hits, and the global FP32 atomics (which execute past the per-XCD L2) were
+// the backward's bound.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -28,7 +36,7 @@ namespace {
 
 constexpr int kBlock = 128;   // lanes per block: two 8x8 wave tiles -> 16x8 pixels
 constexpr int kTileW = 16, kTileH = 8;
-constexpr int kStack = 64;    // entries per lane (deferred node index)
+constexpr int kStack = kLdsStack;  // LDS entries per lane (deferred node index)
 
 template <int K>
 struct KBuf {
@@ -66,20 +74,39 @@ TT_D void kpop(KBuf<K> &b) {
   b.p[K - 1] = kInvalid;
 }
 
+// Optional traversal counters (debug/profiling): nodes visited and triangles tested.
+struct TravCount {
+  int nodes = 0, tris = 0;
+};
+
+TT_D void cswap(float &da, int &ca, float &db, int &cb) {
+  const bool sw = db < da;
+  const float td = sw ? db : da;
+  const int tc = sw ? cb : ca;
+  db = sw ? da : db;
+  cb = sw ? ca : cb;
+  da = td;
+  ca = tc;
+}
+
 // One chunk: the K smallest (t, prim) with t <= far and (t, prim) after the key.
-// Ordered traversal: both children of a node are tested from the node itself;
-// hit leaves are intersected at once, the nearer internal child is descended
-// and the farther one deferred on the LDS stack.
-template <int K>
-TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo, KBuf<K> &b, int *s_node) {
+// Ordered traversal of the 4-wide nodes: the up-to-4 child boxes are tested from
+// the node itself, hit leaves are intersected at once, the hit internal children
+// are sorted by entry distance (5-comparator network), the nearest is descended
+// and the others are deferred farthest-first.
+template <int K, bool COUNT>
+TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo, KBuf<K> &b, int *s_node,
+                 TravCount &tc) {
   kinit(b);
   if (P.n_leaves == 0) return;
   const int lane = threadIdx.x;
+  int *ov = P.ov_stack + ((size_t)blockIdx.x * kBlock + lane) * P.ov_depth;
   const float tlo_c = tlo - fabsf(tlo) * 1e-6f - 1e-7f;
   const float far_t = P.far_t;
   int node = 0;
   int sp = 0;
   auto leaf = [&](int s) {
+    if (COUNT) tc.tris++;
     const float4 a = P.leaves[3 * s], bb = P.leaves[3 * s + 1], c = P.leaves[3 * s + 2];
     const uint32_t prim = __float_as_uint(a.w);
     float t;
@@ -88,39 +115,52 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
     if (!after_key(t, prim, tlo, plo)) return;
     kinsert(b, t, prim);
   };
+  auto push = [&](int n) {
+    if (sp < kStack)
+      s_node[sp * kBlock + lane] = n;
+    else
+      ov[sp - kStack] = n;
+    sp++;
+  };
   while (true) {
     if (node >= 0) {
-      const float4 f0 = P.nodes[4 * node], f1 = P.nodes[4 * node + 1], f2 = P.nodes[4 * node + 2];
-      const float4 f3 = P.nodes[4 * node + 3];
-      const int c0 = __float_as_int(f3.x), c1 = __float_as_int(f3.y);
+      if (COUNT) tc.nodes++;
+      const float4 *nd = P.nodes4 + 8 * (size_t)node;
+      const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+      const int4 ref = *reinterpret_cast<const int4 *>(nd + 6);
       const float thi = fminf(far_t, b.t[K - 1]);
       const float thi_c = thi + fabsf(thi) * 1e-6f + 1e-7f;
-      float tn0, tf0, tn1, tf1;
-      slab(r, f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, tn0, tf0);
-      slab(r, f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, tn1, tf1);
-      bool h0 = box_ok(tn0, tf0, tlo_c, thi_c);
-      bool h1 = box_ok(tn1, tf1, tlo_c, thi_c);
-      if (h0 && c0 < 0) {
-        leaf(~c0);
-        h0 = false;
-      }
-      if (h1 && c1 < 0) {
-        leaf(~c1);
-        h1 = false;
-      }
-      if (h0 && h1) {
-        const bool first0 = tn0 <= tn1;
-        s_node[(sp & (kStack - 1)) * kBlock + lane] = first0 ? c1 : c0;
-        sp++;
-        node = first0 ? c0 : c1;
-      } else {
-        node = h0 ? c0 : (h1 ? c1 : -1);
-      }
+      float tn0, tf0, tn1, tf1, tn2, tf2, tn3, tf3;
+      slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn0, tf0);
+      slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn1, tf1);
+      slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn2, tf2);
+      slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn3, tf3);
+      bool h0 = ref.x != kEmptyChild && box_ok(tn0, tf0, tlo_c, thi_c);
+      bool h1 = ref.y != kEmptyChild && box_ok(tn1, tf1, tlo_c, thi_c);
+      bool h2 = ref.z != kEmptyChild && box_ok(tn2, tf2, tlo_c, thi_c);
+      bool h3 = ref.w != kEmptyChild && box_ok(tn3, tf3, tlo_c, thi_c);
+      if (h0 && ref.x < 0) leaf(~ref.x), h0 = false;
+      if (h1 && ref.y < 0) leaf(~ref.y), h1 = false;
+      if (h2 && ref.z < 0) leaf(~ref.z), h2 = false;
+      if (h3 && ref.w < 0) leaf(~ref.w), h3 = false;
+      const float inf = __builtin_huge_valf();
+      float d0 = h0 ? tn0 : inf, d1 = h1 ? tn1 : inf, d2 = h2 ? tn2 : inf, d3 = h3 ? tn3 : inf;
+      int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
+      const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+      cswap(d0, c0, d1, c1);
+      cswap(d2, c2, d3, c3);
+      cswap(d0, c0, d2, c2);
+      cswap(d1, c1, d3, c3);
+      cswap(d1, c1, d2, c2);
+      if (cnt >= 4) push(c3);
+      if (cnt >= 3) push(c2);
+      if (cnt >= 2) push(c1);
+      node = cnt >= 1 ? c0 : -1;
     }
     if (node < 0) {
       if (sp == 0) return;
       sp--;
-      node = s_node[(sp & (kStack - 1)) * kBlock + lane];
+      node = sp < kStack ? s_node[sp * kBlock + lane] : ov[sp - kStack];
     }
   }
 }
@@ -171,15 +211,20 @@ struct SegState {
 
 // The chunk loop of forward.cu:76-155 (FULL: with normal/statistics outputs) and
 // of the variants (colour only).
-template <int K, bool FULL>
+template <int K, bool FULL, bool COUNT>
 TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node) {
   RayPre r = prep_ray(o, d);
   KBuf<K> b;
   float last = 0.0f, tlo = 0.0f;
   long long plo = -1;
   const bool logging = FULL && P.log_cap > 0;
+  TravCount tc;
   while (true) {
-    gather<K>(P, r, tlo, plo, b, s_node);
+    gather<K, COUNT>(P, r, tlo, plo, b, s_node, tc);
+    if (COUNT) {
+      P.trav_stats[2 * (size_t)ray] = tc.nodes;
+      P.trav_stats[2 * (size_t)ray + 1] = tc.tris;
+    }
     float lt = 0.0f;
     uint32_t lp = 0;
     for (int i = 0; i < K; i++) {
@@ -250,8 +295,8 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
   }
 }
 
-template <int K, int VARIANT>
-__global__ __launch_bounds__(kBlock) void k_forward(TraceParams P) {
+template <int K, int VARIANT, bool COUNT = false>
+__global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   __shared__ int s_node[kStack * kBlock];
   int x, y;
   if (!pixel_of(P, x, y)) return;
@@ -263,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_forward(TraceParams P) {
   st.T = 1.0f;
   st.consumed = st.blended = 0;
   if (VARIANT == 0) {
-    trace_segment<K, true>(P, o, d, st, pix, o, s_node);
+    trace_segment<K, true, COUNT>(P, o, d, st, pix, o, s_node);
   } else {
     const float rad = P.extra_len > 0 ? P.extra[0] : 1.0f;
     const v3f cnt = P.extra_len > 3 ? mk(P.extra[1], P.extra[2], P.extra[3]) : mk(0.0f, 0.0f, 0.0f);
@@ -276,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_forward(TraceParams P) {
         o = o + nrm * eps;
         d = reflect3(normalize(d), nrm);
       }
-      trace_segment<K, false>(P, o, d, st, pix, o, s_node);
+      trace_segment<K, false, false>(P, o, d, st, pix, o, s_node);
     } else if (VARIANT == 2) {  // refract.cu:189-213
       float sth;
       if (hit_sphere(rad, cnt, o, d, sth)) {
@@ -292,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_forward(TraceParams P) {
         o = hp + nrm * eps;
         d = nd;
       }
-      trace_segment<K, false>(P, o, d, st, pix, o, s_node);
+      trace_segment<K, false, false>(P, o, d, st, pix, o, s_node);
     } else {  // envlight.cu:116-184
       const int samples = (int)(P.extra_len > 4 ? P.extra[4] : 0.0f);
       float sth;
@@ -311,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_forward(TraceParams P) {
           ss.N[0] = ss.N[1] = ss.N[2] = 0.0f;
           ss.T = 1.0f;
           ss.consumed = ss.blended = 0;
-          trace_segment<K, false>(P, on, dn, ss, pix, on, s_node);
+          trace_segment<K, false, false>(P, on, dn, ss, pix, on, s_node);
           for (int c = 0; c < 3; c++) radc[c] += ss.C[c] + ss.T * P.bg[c];
           st.consumed += ss.consumed;
           st.blended += ss.blended;
@@ -320,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void k_forward(TraceParams P) {
         for (int c = 0; c < 3; c++) st.C[c] += radc[c] * inv;
         st.T = 0.0f;
       } else {
-        trace_segment<K, false>(P, o, d, st, pix, o, s_node);
+        trace_segment<K, false, false>(P, o, d, st, pix, o, s_node);
       }
     }
   }
@@ -401,17 +446,47 @@ TT_D void dne_dp3(v3f p1, v3f p2, v3f p3, float out[9]) {
   for (int i = 0; i < 9; i++) out[i] = m[i] - d[i];
 }
 
-TT_D void atomic_add3(float *p, v3f g) {
-  unsafeAtomicAdd(p, g.x);
-  unsafeAtomicAdd(p + 1, g.y);
-  unsafeAtomicAdd(p + 2, g.z);
+// ---- per-block gradient accumulation
+constexpr int kSlots = 128;  // triangles per block table (power of two)
+constexpr int kFields = 59;  // 9 vertex coordinates, opacity, sigma, 16 x 3 SH coefficients
+constexpr int F_OPAC = 9, F_SIGMA = 10, F_SH = 11;
+
+// Slot of `prim` in the block's table (linear probing, 8 probes), or -1 when full.
+TT_D int claim_slot(int *keys, uint32_t prim) {
+  uint32_t h = (prim * 2654435761u) >> 25;
+  for (int probe = 0; probe < 8; probe++) {
+    const int old = atomicCAS(&keys[h], -1, (int)prim);
+    if (old == -1 || old == (int)prim) return (int)h;
+    h = (h + 1) & (kSlots - 1);
+  }
+  return -1;
 }
+
+// Where one blended hit's gradient goes: the LDS slot of its triangle, or (table
+// full) straight to the global buffers.
+struct GradSink {
+  float *acc;
+  int slot;
+  float *g_verts, *g_opac, *g_sigma, *g_sh;
+  TT_D void add(int f, float v) {
+    if (slot >= 0)
+      atomicAdd(&acc[f * kSlots + slot], v);
+    else if (f < 9)
+      unsafeAtomicAdd(g_verts + f, v);
+    else if (f == F_OPAC)
+      unsafeAtomicAdd(g_opac, v);
+    else if (f == F_SIGMA)
+      unsafeAtomicAdd(g_sigma, v);
+    else
+      unsafeAtomicAdd(g_sh + (f - F_SH), v);
+  }
+};
 
 // computeWindowFuncBwd (backward.cu:104-189): gradient of the max edge distance
 // (|d_k - phi| <= 1e-6) and of the face normal w.r.t. the three vertices, hit
-// point and incenter distance held constant. The per-vertex sums are formed in
-// registers and scattered once (9 atomics instead of up to 27).
-TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, float *dverts_tri) {
+// point and incenter distance held constant. Per-vertex sums are formed in
+// registers before they reach the sink.
+TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, GradSink &sink) {
   v3f g[3] = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
   for (int k = 0; k < 3; k++) {
     if (fabsf(w.ed[k] - w.phi_max) > 1e-6f) continue;
@@ -454,13 +529,15 @@ TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, fl
     g[1] = g[1] + vecmat(dL_dn, s2);
     g[2] = g[2] + vecmat(dL_dn, s3);
   }
-  atomic_add3(dverts_tri, g[0]);
-  atomic_add3(dverts_tri + 3, g[1]);
-  atomic_add3(dverts_tri + 6, g[2]);
+  for (int j = 0; j < 3; j++) {
+    sink.add(3 * j, g[j].x);
+    sink.add(3 * j + 1, g[j].y);
+    sink.add(3 * j + 2, g[j].z);
+  }
 }
 
 // computeColorFromSHBwd (backward.cu:33-102); clamped channels carry no gradient.
-TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3], float *dsh) {
+TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3], GradSink &sink) {
   const v3f dir = divs(dir0, vlen(dir0));
   v3f g = mk(dc[0], dc[1], dc[2]);
   g.x *= c0 ? 0.0f : 1.0f;
@@ -468,10 +545,9 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
   g.z *= c2 ? 0.0f : 1.0f;
   const float x = dir.x, y = dir.y, z = dir.z;
   auto add = [&](int j, v3f v) {
-    float *p = dsh + 3 * j;
-    if (!c0) unsafeAtomicAdd(p, v.x);
-    if (!c1) unsafeAtomicAdd(p + 1, v.y);
-    if (!c2) unsafeAtomicAdd(p + 2, v.z);
+    if (!c0) sink.add(F_SH + 3 * j, v.x);
+    if (!c1) sink.add(F_SH + 3 * j + 1, v.y);
+    if (!c2) sink.add(F_SH + 3 * j + 2, v.z);
   };
   add(0, SH_C0 * g);
   if (deg > 0) {
@@ -499,14 +575,10 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
   }
 }
 
-// backward.cu:191-391: re-traverse the ray with the same chunk logic, recompute
-// the forward state hit by hit and scatter the gradients.
+// backward.cu:191-350: re-traverse one ray with the same chunk logic, recompute
+// the forward state hit by hit and send each blended hit's gradient to the sink.
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
-  __shared__ int s_node[kStack * kBlock];
-  int x, y;
-  if (!pixel_of(P, x, y)) return;
-  const int pix = y * P.W + x;
+TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, float *s_acc) {
   const size_t R = (size_t)P.R;
   const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
   const float final_depth = P.fothers[pix];
@@ -523,8 +595,9 @@ __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
   KBuf<K> b;
   float last = 0.0f, tlo = 0.0f;
   long long plo = -1;
+  TravCount tc;
   while (true) {
-    gather<K>(P, r, tlo, plo, b, s_node);
+    gather<K, false>(P, r, tlo, plo, b, s_node, tc);
     float lt = 0.0f;
     uint32_t lp = 0;
     for (int i = 0; i < K; i++) {
@@ -563,13 +636,20 @@ __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
       dL_dalpha += dL_ddepth * (T * t - (final_depth - D) * inv1a);
       const v3f Nf = mk(N[0], N[1], N[2]);
       dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
-      unsafeAtomicAdd(P.dopac + prim, dL_dalpha * w.Cx);
+      GradSink sink;
+      sink.acc = s_acc;
+      sink.slot = claim_slot(s_keys, prim);
+      sink.g_verts = P.dverts + 3 * (size_t)off;
+      sink.g_opac = P.dopac + prim;
+      sink.g_sigma = P.dsigma + prim;
+      sink.g_sh = P.dsh + (size_t)prim * P.M * 3;
+      sink.add(F_OPAC, dL_dalpha * w.Cx);
       const float dL_dC = opacity * dL_dalpha;
-      if (w.phi_final > 0.0f) unsafeAtomicAdd(P.dsigma + prim, (dL_dC * w.Cx) * log_tt(w.phi_final));
+      if (w.phi_final > 0.0f) sink.add(F_SIGMA, (dL_dC * w.Cx) * log_tt(w.phi_final));
       const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
       const v3f dL_dn3 = dL_dn2 * bw;
-      window_bwd(V, w, dL_dphi, dL_dn3, P.dverts + 3 * (size_t)off);
-      sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, P.dsh + (size_t)prim * P.M * 3);
+      window_bwd(V, w, dL_dphi, dL_dn3, sink);
+      sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, sink);
       T = testT;
     }
     if (P.chunk_mode == 1) {
@@ -584,6 +664,38 @@ __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
   }
 }
 
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
+  __shared__ int s_node[kStack * kBlock];
+  __shared__ int s_keys[kSlots];
+  __shared__ float s_acc[kFields * kSlots];
+  for (int i = threadIdx.x; i < kSlots; i += kBlock) s_keys[i] = -1;
+  for (int i = threadIdx.x; i < kFields * kSlots; i += kBlock) s_acc[i] = 0.0f;
+  __syncthreads();
+  int x, y;
+  if (pixel_of(P, x, y)) backward_ray<K>(P, y * P.W + x, s_node, s_keys, s_acc);
+  __syncthreads();
+  const int nsh3 = 3 * (P.deg + 1) * (P.deg + 1);
+  for (int s = threadIdx.x; s < kSlots; s += kBlock) {
+    const int prim = s_keys[s];
+    if (prim < 0) continue;
+    float *gv = P.dverts + 3 * (size_t)P.cumsum[prim];
+    for (int f = 0; f < 9; f++) {
+      const float v = s_acc[f * kSlots + s];
+      if (v != 0.0f) unsafeAtomicAdd(gv + f, v);
+    }
+    float v = s_acc[F_OPAC * kSlots + s];
+    if (v != 0.0f) unsafeAtomicAdd(P.dopac + prim, v);
+    v = s_acc[F_SIGMA * kSlots + s];
+    if (v != 0.0f) unsafeAtomicAdd(P.dsigma + prim, v);
+    float *gs = P.dsh + (size_t)prim * P.M * 3;
+    for (int j = 0; j < nsh3; j++) {
+      v = s_acc[(F_SH + j) * kSlots + s];
+      if (v != 0.0f) unsafeAtomicAdd(gs + j, v);
+    }
+  }
+}
+
 __global__ void k_fill(float *p, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -594,7 +706,10 @@ hipError_t fwd_k(const TraceParams &p, int variant, hipStream_t s) {
   dim3 g(p.n_tiles), bl(kBlock);
   switch (variant) {
     case 0:
-      hipLaunchKernelGGL((k_forward<K, 0>), g, bl, 0, s, p);
+      if (p.trav_stats)
+        hipLaunchKernelGGL((k_forward<K, 0, true>), g, bl, 0, s, p);
+      else
+        hipLaunchKernelGGL((k_forward<K, 0>), g, bl, 0, s, p);
       break;
     case 1:
       hipLaunchKernelGGL((k_forward<K, 1>), g, bl, 0, s, p);
@@ -610,6 +725,11 @@ hipError_t fwd_k(const TraceParams &p, int variant, hipStream_t s) {
 }
 
 }  // namespace
+
+size_t trace_lanes(int W, int H) {
+  const size_t tx = (size_t)(W + kTileW - 1) / kTileW, ty = (size_t)(H + kTileH - 1) / kTileH;
+  return tx * ty * kBlock;
+}
 
 static void tiles(TraceParams &p) {
   p.tiles_x = (p.W + kTileW - 1) / kTileW;

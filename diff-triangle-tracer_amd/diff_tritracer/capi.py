@@ -39,7 +39,7 @@ class FwdArgs(C.Structure):
         ("variant", C.c_int), ("k", C.c_int), ("chunk_mode", C.c_int),
         ("cutoff", C.c_float), ("near_t", C.c_float), ("far_t", C.c_float),
         ("color_dev", _fp), ("others_dev", _fp), ("max_blending_dev", _fp), ("density_dev", _fp),
-        ("scaling_dev", _fp), ("stats_dev", _fp),
+        ("scaling_dev", _fp), ("stats_dev", _fp), ("trav_stats_dev", _fp),
         ("hit_log_cap", C.c_int), ("hit_log_prim_dev", _fp), ("hit_log_t_dev", _fp), ("hit_log_code_dev", _fp),
         ("hit_log_count_dev", _fp),
     ]
@@ -139,7 +139,7 @@ class TracerHandle:
 
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
              scaling, k=16, cutoff=1e-4, chunk_mode="compat", variant="forward", extra=None, stats=None,
-             log=None, near=0.2, far=1000.0) -> FwdArgs:
+             log=None, near=0.2, far=1000.0, trav_stats=None) -> FwdArgs:
     a = FwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -158,6 +158,7 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.color_dev, a.others_dev = ptr(color), ptr(others)
     a.max_blending_dev, a.density_dev, a.scaling_dev = ptr(maxb), ptr(density), ptr(scaling)
     a.stats_dev = ptr(stats)
+    a.trav_stats_dev = ptr(trav_stats)
     if log is not None:
         prim, t, code, count = log
         a.hit_log_cap = int(prim.shape[-1])

@@ -104,6 +104,7 @@ typedef struct {
   float *density_dev;     /* [P] in/out: set to 1 for blended triangles (forward only) */
   float *scaling_dev;     /* [P] in/out: filled where <= 0 (forward only) */
   int *stats_dev;         /* optional [R][2]: consumed hits, blended hits */
+  int *trav_stats_dev;    /* optional [R][2]: BVH nodes visited, triangles tested (all chunks) */
   int hit_log_cap;        /* optional debug log of the first hit_log_cap consumed hits per ray */
   int *hit_log_prim_dev;  /* [R][cap] */
   float *hit_log_t_dev;   /* [R][cap] */
