@@ -239,14 +239,17 @@ TT_D bool hit_less(float t, uint32_t p, float bt, uint32_t bp) {
 
 // Conservative slab test: boxes are padded at build time and the interval is
 // widened, so a triangle whose computed t lies in [tlo, thi] is never culled.
+// Plane distances as (plane - o) * (1/d): relative error of a few ulp of t itself,
+// covered by the widening below (the fma form (plane*(1/d) - o/d) errs by ~ulp of
+// o/d, which for a distant origin exceeds the box padding).
 TT_D void slab(const RayPre &r, float lx, float hx, float ly, float hy, float lz, float hz, float &tn, float &tf) {
-  float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
-  float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
-  float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
+  const float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
+  const float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
+  const float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
   tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
   tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-  tf = tf + fabsf(tf) * 1e-6f + 1e-7f;
-  tn = tn - fabsf(tn) * 1e-6f - 1e-7f;
+  tf = fmaf(fabsf(tf), 1e-6f, tf) + 1e-7f;
+  tn = fmaf(fabsf(tn), -1e-6f, tn) - 1e-7f;
 }
 TT_D bool box_ok(float tn, float tf, float tlo_c, float thi_c) {
   return tn <= tf && tf >= tlo_c && tn <= thi_c;

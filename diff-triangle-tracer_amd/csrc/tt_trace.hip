@@ -139,10 +139,24 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       bool h1 = ref.y != kEmptyChild && box_ok(tn1, tf1, tlo_c, thi_c);
       bool h2 = ref.z != kEmptyChild && box_ok(tn2, tf2, tlo_c, thi_c);
       bool h3 = ref.w != kEmptyChild && box_ok(tn3, tf3, tlo_c, thi_c);
-      if (h0 && ref.x < 0) leaf(~ref.x), h0 = false;
-      if (h1 && ref.y < 0) leaf(~ref.y), h1 = false;
-      if (h2 && ref.z < 0) leaf(~ref.z), h2 = false;
-      if (h3 && ref.w < 0) leaf(~ref.w), h3 = false;
+      // hit leaves: queued, then tested in one loop so that the intersection code
+      // runs once per queued leaf for the whole wave (not once per slot)
+      int q0 = -1, q1 = -1, q2 = -1, q3 = -1, nq = 0;
+      auto enq = [&](int s) {
+        q3 = nq == 3 ? s : q3;
+        q2 = nq == 2 ? s : q2;
+        q1 = nq == 1 ? s : q1;
+        q0 = nq == 0 ? s : q0;
+        nq++;
+      };
+      if (h0 && ref.x < 0) enq(~ref.x), h0 = false;
+      if (h1 && ref.y < 0) enq(~ref.y), h1 = false;
+      if (h2 && ref.z < 0) enq(~ref.z), h2 = false;
+      if (h3 && ref.w < 0) enq(~ref.w), h3 = false;
+      for (int j = 0; j < nq; j++) {
+        leaf(q0);
+        q0 = q1, q1 = q2, q2 = q3;
+      }
       const float inf = __builtin_huge_valf();
       float d0 = h0 ? tn0 : inf, d1 = h1 ? tn1 : inf, d2 = h2 ? tn2 : inf, d3 = h3 ? tn3 : inf;
       int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
