@@ -7,7 +7,10 @@
 // these codes).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/tritracer_hip.h"
 #include "tt_internal.h"
@@ -17,6 +20,10 @@ struct tt_tracer {
   tt::Bvh bvh;
   int *ov_stack = nullptr;  // traversal stack spill (entries below the LDS part)
   size_t ov_cap = 0;
+  int *heads = nullptr;                    // 8 work-queue heads of the persistent launches
+  std::map<long long, int> grids;          // persistent grid size per (k, block, backward)
+  hipStream_t capture_stream = nullptr;    // private stream used only to record graphs
+  hipGraphExec_t graph = nullptr;          // the recorded multi-camera step
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool build_timed = false;
 };
@@ -62,14 +69,14 @@ int check_common(const tt_tracer *tr, int W, int H, const float *ro, const float
   return TT_OK;
 }
 
-// Stack spill for a launch over W x H rays: a 4-wide walk defers at most 3 children
+// Stack spill for a launch of `lanes` lanes: a 4-wide walk defers at most 3 children
 // at each of at most ceil(height/2) nodes on its path; entries beyond the LDS part spill.
-int prepare_spill(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
+int prepare_spill_lanes(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
   const int depth = 3 * ((tr->bvh.height_max + 1) / 2) + 1 - tt::kLdsStack;
   p.ov_depth = depth > 0 ? depth : 0;
   p.ov_stack = tr->ov_stack;
   if (p.ov_depth == 0 || tr->bvh.n_leaves == 0) return TT_OK;
-  const size_t need = tt::trace_lanes(W, H) * (size_t)p.ov_depth;
+  const size_t need = lanes * (size_t)p.ov_depth;
   if (need > tr->ov_cap) {
     if (tr->ov_stack) (void)hipFree(tr->ov_stack);
     tr->ov_stack = nullptr;
@@ -81,6 +88,9 @@ int prepare_spill(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
   }
   p.ov_stack = tr->ov_stack;
   return TT_OK;
+}
+int prepare_spill(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
+  return prepare_spill_lanes(tr, tt::trace_lanes(W, H), p);
 }
 
 void fill_common(tt::TraceParams &p, const tt_tracer *tr, int W, int H, const float *ro, const float *rd,
@@ -141,6 +151,9 @@ int tt_destroy(tt_tracer *tr) {
   (void)hipSetDevice(tr->device);
   tt::bvh_free(tr->bvh);
   if (tr->ov_stack) (void)hipFree(tr->ov_stack);
+  if (tr->heads) (void)hipFree(tr->heads);
+  if (tr->graph) (void)hipGraphExecDestroy(tr->graph);
+  if (tr->capture_stream) (void)hipStreamDestroy(tr->capture_stream);
   if (tr->ev0) (void)hipEventDestroy(tr->ev0);
   if (tr->ev1) (void)hipEventDestroy(tr->ev1);
   (void)hipSetDevice(dev);
@@ -273,6 +286,213 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   std::string err = tt::launch_backward(p, a->k, s);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
+  return ok();
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- multi-camera
+namespace {
+
+int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiCams &mc, tt::TraceParams &p,
+                int &grid) {
+  TT_ARG(a, "args is NULL");
+  TT_ARG(a->n_cameras >= 1 && a->n_cameras <= TT_MAX_CAMERAS, "n_cameras must be in [1, 32]");
+  TT_ARG(a->width && a->height, "camera size arrays are NULL");
+  TT_ARG(a->block == 64 || a->block == 128 || a->block == 256, "block must be 64, 128 or 256");
+  TT_ARG(a->tiles_per_fetch >= 1, "tiles_per_fetch must be >= 1");
+  long long rays = 0;
+  int tw, th;
+  tt::multi_tile_shape(a->block, tw, th);
+  mc = tt::MultiCams{};
+  mc.n = a->n_cameras;
+  int tiles = 0;
+  for (int c = 0; c < a->n_cameras; c++) {
+    TT_ARG(a->width[c] > 0 && a->height[c] > 0, "camera sizes must be positive");
+    mc.W[c] = a->width[c];
+    mc.H[c] = a->height[c];
+    mc.tiles_x[c] = (a->width[c] + tw - 1) / tw;
+    mc.tile_begin[c] = tiles;
+    mc.ray_off[c] = rays;
+    tiles += mc.tiles_x[c] * ((a->height[c] + th - 1) / th);
+    rays += (long long)a->width[c] * a->height[c];
+    TT_ARG(tiles < (1 << 30), "too many tiles");
+  }
+  mc.tile_begin[a->n_cameras] = tiles;
+  mc.total_tiles = tiles;
+  for (int q = 0; q <= 8; q++) mc.q_begin[q] = (int)((long long)tiles * q / 8);
+  int rc = check_common(tr, 1, 1, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev, a->cumsum_dev, a->P,
+                        a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->k, a->chunk_mode);
+  if (rc != TT_OK) return rc;
+  fill_common(p, tr, 1, 1, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev, a->cumsum_dev, a->P, a->shs_dev,
+              a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->chunk_mode, a->cutoff, a->near_t, a->far_t);
+  p.bg = a->bg_dev;
+  if (!backward) {
+    TT_ARG(a->color_dev && a->others_dev && a->bg_dev, "colour / others / background buffers are NULL");
+    if (a->P > 0) TT_ARG(a->max_blending_dev && a->density_dev && a->scaling_dev, "statistics buffers are NULL");
+    p.color = a->color_dev;
+    p.others = a->others_dev;
+    p.maxb = a->max_blending_dev;
+    p.density = a->density_dev;
+    p.scaling = a->scaling_dev;
+    p.stats = a->stats_dev;
+  } else {
+    TT_ARG(a->color_dev && a->others_dev && a->dL_dpix_dev && a->dL_dothers_dev,
+           "forward outputs / upstream gradients are NULL");
+    if (a->P > 0)
+      TT_ARG(a->dL_dvertices_dev && a->dL_dsigma_dev && a->dL_dopacity_dev && a->dL_dsh_dev, "gradient buffers are NULL");
+    p.fcolor = a->color_dev;
+    p.fothers = a->others_dev;
+    p.dpix = a->dL_dpix_dev;
+    p.dothers = a->dL_dothers_dev;
+    p.dverts = a->dL_dvertices_dev;
+    p.dsigma = a->dL_dsigma_dev;
+    p.dopac = a->dL_dopacity_dev;
+    p.dsh = a->dL_dsh_dev;
+  }
+  const long long key = ((long long)a->k << 20) | ((long long)a->block << 1) | (backward ? 1 : 0);
+  auto it = tr->grids.find(key);
+  if (it == tr->grids.end()) {
+    const int g = tt::multi_grid(a->k, a->block, backward);
+    if (g <= 0) return fail(TT_ERR_HIP, "could not size the persistent grid");
+    it = tr->grids.emplace(key, g).first;
+  }
+  grid = it->second;
+  if (!tr->heads) {
+    hipError_t e = hipMalloc(&tr->heads, 8 * sizeof(int));
+    if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("queue allocation failed: ") + hipGetErrorString(e));
+  }
+  return prepare_spill_lanes(tr, (size_t)grid * a->block, p);
+}
+
+int zero_grads(const tt_multi_args *a, hipStream_t s) {
+  if (!a->zero_grads || a->P == 0) return TT_OK;
+  std::string e = tt::launch_fill(a->dL_dvertices_dev, (size_t)a->dL_dvertices_rows * 3, 0.0f, s);
+  if (e.empty()) e = tt::launch_fill(a->dL_dsigma_dev, (size_t)a->P, 0.0f, s);
+  if (e.empty()) e = tt::launch_fill(a->dL_dopacity_dev, (size_t)a->P, 0.0f, s);
+  if (e.empty()) e = tt::launch_fill(a->dL_dsh_dev, (size_t)a->P * a->M * 3, 0.0f, s);
+  if (!e.empty()) return fail(TT_ERR_HIP, e);
+  return TT_OK;
+}
+
+int multi_issue(tt_tracer *tr, const tt_multi_args *a, int what, hipStream_t s) {
+  tt::MultiCams mc;
+  tt::TraceParams p;
+  int grid = 0, rc;
+  if (what & 1) {
+    if ((rc = multi_setup(tr, a, false, mc, p, grid)) != TT_OK) return rc;
+    std::string e = tt::launch_multi(p, mc, tr->heads, a->k, a->block, a->tiles_per_fetch, grid, false, s);
+    if (!e.empty()) return fail(TT_ERR_HIP, e);
+  }
+  if (what & 2) {
+    if ((rc = multi_setup(tr, a, true, mc, p, grid)) != TT_OK) return rc;
+    if ((rc = zero_grads(a, s)) != TT_OK) return rc;
+    if (a->P > 0) {
+      std::string e = tt::launch_multi(p, mc, tr->heads, a->k, a->block, a->tiles_per_fetch, grid, true, s);
+      if (!e.empty()) return fail(TT_ERR_HIP, e);
+    }
+  }
+  return TT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tt_multi_forward(tt_tracer *tr, const tt_multi_args *a, void *stream) {
+  int rc = multi_issue(tr, a, 1, static_cast<hipStream_t>(stream));
+  return rc == TT_OK ? ok() : rc;
+}
+
+int tt_multi_backward(tt_tracer *tr, const tt_multi_args *a, void *stream) {
+  int rc = multi_issue(tr, a, 2, static_cast<hipStream_t>(stream));
+  return rc == TT_OK ? ok() : rc;
+}
+
+int tt_multi_capture(tt_tracer *tr, const tt_multi_args *a, int what, void *stream) {
+  TT_ARG(tr && a, "tracer / args is NULL");
+  TT_ARG(what >= 1 && what <= 3, "what must be 1 (forward), 2 (backward) or 3 (both)");
+  (void)stream;
+  // all allocation happens here, outside the capture
+  tt::MultiCams mc;
+  tt::TraceParams p;
+  int grid, rc;
+  if ((what & 1) && (rc = multi_setup(tr, a, false, mc, p, grid)) != TT_OK) return rc;
+  if ((what & 2) && (rc = multi_setup(tr, a, true, mc, p, grid)) != TT_OK) return rc;
+  if (!tr->capture_stream) {
+    hipError_t e = hipStreamCreateWithFlags(&tr->capture_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+  }
+  hipError_t e = hipStreamBeginCapture(tr->capture_stream, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("hipStreamBeginCapture failed: ") + hipGetErrorString(e));
+  rc = multi_issue(tr, a, what, tr->capture_stream);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(tr->capture_stream, &g);
+  if (rc != TT_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("hipStreamEndCapture failed: ") + hipGetErrorString(e));
+  if (tr->graph) (void)hipGraphExecDestroy(tr->graph);
+  tr->graph = nullptr;
+  e = hipGraphInstantiate(&tr->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(e));
+  return ok();
+}
+
+int tt_multi_replay(tt_tracer *tr, void *stream) {
+  TT_ARG(tr, "tracer is NULL");
+  if (!tr->graph) return fail(TT_ERR_STATE, "no graph has been captured");
+  hipError_t e = hipGraphLaunch(tr->graph, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("hipGraphLaunch failed: ") + hipGetErrorString(e));
+  return ok();
+}
+
+int tt_multi_autotune(tt_tracer *tr, tt_multi_args *a, const int *ks, int nk, const int *blocks, int nb,
+                      const int *tpf, int nt, int reps, float *results_ms, void *stream) {
+  TT_ARG(tr && a && ks && blocks && tpf && results_ms, "NULL argument");
+  TT_ARG(nk > 0 && nb > 0 && nt > 0 && reps > 0, "empty sweep");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(TT_ERR_HIP, "hipEventCreate failed");
+  const tt_multi_args base = *a;
+  float best = -1.0f;
+  tt_multi_args best_a = base;
+  int rc = TT_OK;
+  for (int i = 0; i < nk && rc == TT_OK; i++)
+    for (int j = 0; j < nb && rc == TT_OK; j++)
+      for (int l = 0; l < nt && rc == TT_OK; l++) {
+        float &out = results_ms[(i * nb + j) * nt + l];
+        out = -1.0f;
+        if (base.chunk_mode != TT_CHUNK_EXACT && ks[i] != base.k) continue;  // k changes results in compat mode
+        tt_multi_args c = base;
+        c.k = ks[i];
+        c.block = blocks[j];
+        c.tiles_per_fetch = tpf[l];
+        if ((rc = multi_issue(tr, &c, 3, s)) != TT_OK) break;  // warm-up (and allocations)
+        std::vector<float> ms;
+        for (int r = 0; r < reps; r++) {
+          (void)hipEventRecord(e0, s);
+          if ((rc = multi_issue(tr, &c, 3, s)) != TT_OK) break;
+          (void)hipEventRecord(e1, s);
+          (void)hipEventSynchronize(e1);
+          float t = 0.0f;
+          (void)hipEventElapsedTime(&t, e0, e1);
+          ms.push_back(t);
+        }
+        if (rc != TT_OK) break;
+        std::sort(ms.begin(), ms.end());
+        out = ms[ms.size() / 2];
+        if (best < 0.0f || out < best) best = out, best_a = c;
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != TT_OK) return rc;
+  a->k = best_a.k;
+  a->block = best_a.block;
+  a->tiles_per_fetch = best_a.tiles_per_fetch;
   return ok();
 }
 

@@ -83,6 +83,24 @@ struct TraceParams {
   float *dverts, *dsigma, *dopac, *dsh;
 };
 
+// Cameras of one persistent multi-camera launch (passed by value as a kernel argument).
+constexpr int kMaxCams = 32;
+struct MultiCams {
+  int n, total_tiles;
+  int W[kMaxCams], H[kMaxCams], tiles_x[kMaxCams];
+  int tile_begin[kMaxCams + 1];  // camera c owns global tiles [tile_begin[c], tile_begin[c+1])
+  long long ray_off[kMaxCams];   // first ray of camera c in the concatenated ray / pixel buffers
+  int q_begin[9];                // 8 work queues: queue q serves global tiles [q_begin[q], q_begin[q+1])
+};
+// Tile shape of a persistent launch with `block` lanes per block (64 -> 8x8, 128 -> 16x8, 256 -> 16x16).
+void multi_tile_shape(int block, int &tw, int &th);
+// Blocks that fill the device for the persistent kernel (k, block, backward?), 0 if unsupported.
+int multi_grid(int k, int block, bool backward);
+// Persistent launch over all cameras; `heads` = 8 device ints (reset inside, graph-safe).
+std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, int k, int block, int tiles_per_fetch,
+                         int grid, bool backward, hipStream_t stream);
+std::string launch_fill_int(int *ptr, int n, int v, hipStream_t stream);
+
 // Lanes a trace launch uses for an image of W x H (the stack spill needs one slot each).
 size_t trace_lanes(int W, int H);
 constexpr int kLdsStack = 16;  // traversal stack entries per lane kept in LDS; deeper ones spill

@@ -1,0 +1,170 @@
+// Persistent multi-camera traversal: every camera of a batch (uneven resolutions)
+// is rendered forward or backward by ONE launch that fills the device with
+// resident blocks; blocks pull tiles from 8 work queues. Queue q serves one
+// contiguous band of the camera-major tile order and is the home queue of the
+// blocks dispatched to XCD q (workgroups go to XCDs round-robin), so a block
+// stays on pixels that share its XCD's L2 until its band is done, then steals
+// from the other queues. The number of tiles per fetch is a tuning knob.
+//
+// A batched multi-view training step (several cameras per optimizer step) has
+// no counterpart in the reference, which traces one camera per call
+// (tracer.cpp:16-142); per-pixel work is the same device code as the one-shot
+// kernels (tt_trace_dev.h), so the results equal per-camera launches.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "tt_trace_dev.h"
+
+namespace tt {
+namespace {
+
+// Thread 0 of a block: claims `G` tiles, home queue first. Returns the first tile
+// (end in `end`) or -1 when every queue is drained.
+TT_D int fetch_tiles(int *heads, const MultiCams &mc, int G, int &end) {
+  const int home = blockIdx.x & 7;
+  for (int i = 0; i < 8; i++) {
+    const int q = (home + i) & 7;
+    const int qb = mc.q_begin[q], qe = mc.q_begin[q + 1];
+    if (qb >= qe) continue;
+    if (__hip_atomic_load(heads + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= qe - qb) continue;
+    const int old = atomicAdd(heads + q, G);
+    if (qb + old < qe) {
+      end = min(qb + old + G, qe);
+      return qb + old;
+    }
+  }
+  return -1;
+}
+
+// The launch parameters seen from camera c = the camera owning global tile `tile`.
+TT_D TraceParams camera_view(const TraceParams &P, const MultiCams &mc, int tile, int &local_tile) {
+  int c = 0;
+  while (c + 1 < mc.n && mc.tile_begin[c + 1] <= tile) c++;
+  TraceParams Q = P;
+  Q.W = mc.W[c];
+  Q.H = mc.H[c];
+  Q.R = mc.W[c] * mc.H[c];
+  Q.tiles_x = mc.tiles_x[c];
+  const long long off = mc.ray_off[c];
+  Q.ray_o = P.ray_o + 3 * off;
+  Q.ray_d = P.ray_d + 3 * off;
+  if (P.color) Q.color = P.color + 3 * off;
+  if (P.others) Q.others = P.others + 7 * off;
+  if (P.stats) Q.stats = P.stats + 2 * off;
+  if (P.fcolor) Q.fcolor = P.fcolor + 3 * off;
+  if (P.fothers) Q.fothers = P.fothers + 7 * off;
+  if (P.dpix) Q.dpix = P.dpix + 3 * off;
+  if (P.dothers) Q.dothers = P.dothers + 7 * off;
+  local_tile = tile - mc.tile_begin[c];
+  return Q;
+}
+
+template <int K, int BLOCK>
+__global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, MultiCams mc, int *heads, int G) {
+  __shared__ int s_node[kStack * BLOCK];
+  __shared__ int s_range[2];
+  while (true) {
+    if (threadIdx.x == 0) {
+      int e = 0;
+      s_range[0] = fetch_tiles(heads, mc, G, e);
+      s_range[1] = e;
+    }
+    __syncthreads();
+    const int b = s_range[0], e = s_range[1];
+    __syncthreads();
+    if (b < 0) break;
+    for (int t = b; t < e; t++) {
+      int lt, x, y;
+      const TraceParams Q = camera_view(P, mc, t, lt);
+      if (tile_pixel<BLOCK>(Q, lt, x, y)) forward_pixel<K, BLOCK, 0, false>(Q, y * Q.W + x, s_node);
+    }
+  }
+}
+
+template <int K, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_backward_multi(TraceParams P, MultiCams mc, int *heads, int G) {
+  __shared__ int s_node[kStack * BLOCK];
+  __shared__ int s_keys[BLOCK];
+  __shared__ float s_acc[kFields * BLOCK];
+  __shared__ int s_range[2];
+  while (true) {
+    if (threadIdx.x == 0) {
+      int e = 0;
+      s_range[0] = fetch_tiles(heads, mc, G, e);
+      s_range[1] = e;
+    }
+    __syncthreads();
+    const int b = s_range[0], e = s_range[1];
+    __syncthreads();
+    if (b < 0) break;
+    for (int t = b; t < e; t++) {
+      int lt, x, y;
+      const TraceParams Q = camera_view(P, mc, t, lt);
+      const bool valid = tile_pixel<BLOCK>(Q, lt, x, y);
+      backward_block<K, BLOCK>(Q, valid, y * Q.W + x, s_node, s_keys, s_acc);
+    }
+  }
+}
+
+template <int K, int BLOCK>
+const void *kernel_of(bool backward) {
+  return backward ? reinterpret_cast<const void *>(&k_backward_multi<K, BLOCK>)
+                  : reinterpret_cast<const void *>(&k_forward_multi<K, BLOCK>);
+}
+
+template <int K>
+const void *kernel_k(int block, bool backward) {
+  if (block == 64) return kernel_of<K, 64>(backward);
+  if (block == 128) return kernel_of<K, 128>(backward);
+  if (block == 256) return kernel_of<K, 256>(backward);
+  return nullptr;
+}
+
+const void *multi_kernel(int k, int block, bool backward) {
+  if (k == 8) return kernel_k<8>(block, backward);
+  if (k == 16) return kernel_k<16>(block, backward);
+  if (k == 32) return kernel_k<32>(block, backward);
+  return nullptr;
+}
+
+}  // namespace
+
+void multi_tile_shape(int block, int &tw, int &th) {
+  tw = block == 64 ? 8 : 16;
+  th = block == 256 ? 16 : 8;
+}
+
+int multi_grid(int k, int block, bool backward) {
+  const void *fn = multi_kernel(k, block, backward);
+  if (!fn) return 0;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0) != hipSuccess) return 0;
+  return cus * (per_cu > 0 ? per_cu : 1);
+}
+
+std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, int k, int block, int G, int grid,
+                         bool backward, hipStream_t stream) {
+  const void *fn = multi_kernel(k, block, backward);
+  if (!fn) return "unsupported (k, block): k in {8, 16, 32}, block in {64, 128, 256}";
+  if (mc.total_tiles == 0) return "";
+  std::string e = launch_fill_int(heads, 8, 0, stream);
+  if (!e.empty()) return e;
+  TraceParams pp = p;
+  MultiCams m = mc;
+  void *args[] = {&pp, &m, &heads, &G};
+  hipError_t err = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+  if (err != hipSuccess) return std::string("persistent launch failed: ") + hipGetErrorString(err);
+  return "";
+}
+
+std::string launch_fill_int(int *ptr, int n, int v, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, stream, ptr, n, v);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("fill launch failed: ") + hipGetErrorString(e);
+  return "";
+}
+
+}  // namespace tt

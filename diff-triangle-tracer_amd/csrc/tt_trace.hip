@@ -1,718 +1,32 @@
-// Traversal kernels: k-closest gather over the LBVH, front-to-back compositing,
-// and the re-traversing backward.
-//
-// Replaces the OptiX programs of the reference:
-//   __raygen__rg forward ........ tritracer/forward.cu:76-215
-//   __raygen__rg backward ....... tritracer/backward.cu:191-391
-//   reflect / refract / envlight  tritracer/reflect.cu, refract.cu, envlight.cu
-//   optixTrace + __anyhit__ah ... tritracer/closestk.h:17-149 (k-buffer insertion)
-//
-// Mapping: one ray per lane; each wave64 takes an 8x8 pixel tile, a block of
-// 128 lanes a 16x8 tile. Blocks are remapped so that the tiles one XCD works
-// on form a contiguous band of the image (workgroups are dispatched to the 8
-// XCDs round-robin and each XCD has its own L2). Traversal walks the 4-wide
-// view of the LBVH (every second binary level: half the dependent node fetches
-// of a binary walk for the same box tests). The stack keeps its top 16 entries
-// per lane in LDS ([slot][lane] layout, bank-conflict free) and spills deeper
-// entries to a per-lane global area sized from the tree height: each visited
-// 4-wide node defers at most 3 children and a root-to-leaf path visits at most
-// ceil(height/2) of them.
-//
-// Backward gradients are first summed per block in an LDS table keyed by
-// triangle (128 slots x 59 floats, LDS float atomics; linear probing, global
-// atomics on overflow) and scattered once per block: on the 1080p / 1M-splat
-// workload a 128-ray block sees ~11x fewer distinct triangles than blended
-// hits, and the global FP32 atomics (which execute past the per-XCD L2) were
-// the backward's bound.
+// One-shot traversal kernels (one block per 16x8 tile of one image) and their
+// launchers. The device code is in tt_trace_dev.h.
 #include <hip/hip_runtime.h>
 
 #include <string>
 
-#include "tt_device.h"
-#include "tt_internal.h"
+#include "tt_trace_dev.h"
 
 namespace tt {
 namespace {
 
-constexpr int kBlock = 128;   // lanes per block: two 8x8 wave tiles -> 16x8 pixels
-constexpr int kTileW = 16, kTileH = 8;
-constexpr int kStack = kLdsStack;  // LDS entries per lane (deferred node index)
-
-template <int K>
-struct KBuf {
-  float t[K];
-  uint32_t p[K];
-};
-
-template <int K>
-TT_D void kinit(KBuf<K> &b) {
-#pragma unroll
-  for (int i = 0; i < K; i++) b.t[i] = kInfDist, b.p[i] = kInvalid;
-}
-
-// Sorted insertion (the anyhit compare-and-swap chain of closestk.h:111-147) with a
-// deterministic (t, prim) order instead of traversal order for equal t.
-template <int K>
-TT_D void kinsert(KBuf<K> &b, float t, uint32_t p) {
-  if (!hit_less(t, p, b.t[K - 1], b.p[K - 1])) return;
-#pragma unroll
-  for (int i = 0; i < K; i++) {
-    bool lt = hit_less(t, p, b.t[i], b.p[i]);
-    float ot = b.t[i];
-    uint32_t op = b.p[i];
-    b.t[i] = lt ? t : ot;
-    b.p[i] = lt ? p : op;
-    t = lt ? ot : t;
-    p = lt ? op : p;
-  }
-}
-template <int K>
-TT_D void kpop(KBuf<K> &b) {
-#pragma unroll
-  for (int i = 0; i < K - 1; i++) b.t[i] = b.t[i + 1], b.p[i] = b.p[i + 1];
-  b.t[K - 1] = kInfDist;
-  b.p[K - 1] = kInvalid;
-}
-
-// Optional traversal counters (debug/profiling): nodes visited and triangles tested.
-struct TravCount {
-  int nodes = 0, tris = 0;
-};
-
-TT_D void cswap(float &da, int &ca, float &db, int &cb) {
-  const bool sw = db < da;
-  const float td = sw ? db : da;
-  const int tc = sw ? cb : ca;
-  db = sw ? da : db;
-  cb = sw ? ca : cb;
-  da = td;
-  ca = tc;
-}
-
-// One chunk: the K smallest (t, prim) with t <= far and (t, prim) after the key.
-// Ordered traversal of the 4-wide nodes: the up-to-4 child boxes are tested from
-// the node itself, hit leaves are intersected at once, the hit internal children
-// are sorted by entry distance (5-comparator network), the nearest is descended
-// and the others are deferred farthest-first.
-template <int K, bool COUNT>
-TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo, KBuf<K> &b, int *s_node,
-                 TravCount &tc) {
-  kinit(b);
-  if (P.n_leaves == 0) return;
-  const int lane = threadIdx.x;
-  int *ov = P.ov_stack + ((size_t)blockIdx.x * kBlock + lane) * P.ov_depth;
-  const float tlo_c = tlo - fabsf(tlo) * 1e-6f - 1e-7f;
-  const float far_t = P.far_t;
-  int node = 0;
-  int sp = 0;
-  auto leaf = [&](int s) {
-    if (COUNT) tc.tris++;
-    const float4 a = P.leaves[3 * s], bb = P.leaves[3 * s + 1], c = P.leaves[3 * s + 2];
-    const uint32_t prim = __float_as_uint(a.w);
-    float t;
-    if (!intersect(r, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), mk(c.x, c.y, c.z), t)) return;
-    if (!(t <= far_t)) return;
-    if (!after_key(t, prim, tlo, plo)) return;
-    kinsert(b, t, prim);
-  };
-  auto push = [&](int n) {
-    if (sp < kStack)
-      s_node[sp * kBlock + lane] = n;
-    else
-      ov[sp - kStack] = n;
-    sp++;
-  };
-  while (true) {
-    if (node >= 0) {
-      if (COUNT) tc.nodes++;
-      const float4 *nd = P.nodes4 + 8 * (size_t)node;
-      const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-      const int4 ref = *reinterpret_cast<const int4 *>(nd + 6);
-      const float thi = fminf(far_t, b.t[K - 1]);
-      const float thi_c = thi + fabsf(thi) * 1e-6f + 1e-7f;
-      float tn0, tf0, tn1, tf1, tn2, tf2, tn3, tf3;
-      slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn0, tf0);
-      slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn1, tf1);
-      slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn2, tf2);
-      slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn3, tf3);
-      bool h0 = ref.x != kEmptyChild && box_ok(tn0, tf0, tlo_c, thi_c);
-      bool h1 = ref.y != kEmptyChild && box_ok(tn1, tf1, tlo_c, thi_c);
-      bool h2 = ref.z != kEmptyChild && box_ok(tn2, tf2, tlo_c, thi_c);
-      bool h3 = ref.w != kEmptyChild && box_ok(tn3, tf3, tlo_c, thi_c);
-      // hit leaves: queued, then tested in one loop so that the intersection code
-      // runs once per queued leaf for the whole wave (not once per slot)
-      int q0 = -1, q1 = -1, q2 = -1, q3 = -1, nq = 0;
-      auto enq = [&](int s) {
-        q3 = nq == 3 ? s : q3;
-        q2 = nq == 2 ? s : q2;
-        q1 = nq == 1 ? s : q1;
-        q0 = nq == 0 ? s : q0;
-        nq++;
-      };
-      if (h0 && ref.x < 0) enq(~ref.x), h0 = false;
-      if (h1 && ref.y < 0) enq(~ref.y), h1 = false;
-      if (h2 && ref.z < 0) enq(~ref.z), h2 = false;
-      if (h3 && ref.w < 0) enq(~ref.w), h3 = false;
-      for (int j = 0; j < nq; j++) {
-        leaf(q0);
-        q0 = q1, q1 = q2, q2 = q3;
-      }
-      const float inf = __builtin_huge_valf();
-      float d0 = h0 ? tn0 : inf, d1 = h1 ? tn1 : inf, d2 = h2 ? tn2 : inf, d3 = h3 ? tn3 : inf;
-      int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
-      const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-      cswap(d0, c0, d1, c1);
-      cswap(d2, c2, d3, c3);
-      cswap(d0, c0, d2, c2);
-      cswap(d1, c1, d3, c3);
-      cswap(d1, c1, d2, c2);
-      if (cnt >= 4) push(c3);
-      if (cnt >= 3) push(c2);
-      if (cnt >= 2) push(c1);
-      node = cnt >= 1 ? c0 : -1;
-    }
-    if (node < 0) {
-      if (sp == 0) return;
-      sp--;
-      node = sp < kStack ? s_node[sp * kBlock + lane] : ov[sp - kStack];
-    }
-  }
-}
-
-// Pixel of this lane: block -> tile with the XCD band remap, wave -> 8x8 quadrant.
-TT_D bool pixel_of(const TraceParams &P, int &x, int &y) {
-  const int nb = P.n_tiles;
-  const int i = blockIdx.x;
-  const int xcd = i & 7, j = i >> 3;
-  const int per = nb >> 3, rem = nb & 7;
-  const int L = xcd * per + min(xcd, rem) + j;
-  const int tx = L % P.tiles_x, ty = L / P.tiles_x;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  x = tx * kTileW + (wave & 1) * 8 + (lane & 7);
-  y = ty * kTileH + (lane >> 3);
-  return x < P.W && y < P.H;
-}
-
-// forward.cu:34-74 computeScaling
-TT_D void compute_scaling(const TraceParams &P, int idx, v3f org) {
-  if (P.scaling[idx] > 0.0f) return;
-  const int off = P.cumsum[idx];
-  const int np = P.npts[idx];
-  v3f cen = mk(0.0f, 0.0f, 0.0f);
-  for (int i = 0; i < np; i++) cen = cen + ld3(P.verts + 3 * (size_t)(off + i));
-  cen = cen * (1.0f / (float)np);
-  v3f tri2org = cen - org;
-  float mx = 0.0f;
-  for (int i = 0; i < np; i++) {
-    v3f a = ld3(P.verts + 3 * (size_t)(off + i)) - org;
-    float len_ab = vlen(a) * vlen(tri2org);
-    float ang = 0.0f;
-    if (!(len_ab < 1e-12f)) {
-      float c = fmaxf(-1.0f, fminf(1.0f, dot(a, tri2org) / len_ab));
-      ang = acosf(c);
-    }
-    mx = fmaxf(ang, mx);
-  }
-  P.scaling[idx] = mx;
-}
-
-struct SegState {
-  float C[3];
-  float T;
-  float N[3];
-  int consumed, blended;
-};
-
-// The chunk loop of forward.cu:76-155 (FULL: with normal/statistics outputs) and
-// of the variants (colour only).
-template <int K, bool FULL, bool COUNT>
-TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node) {
-  RayPre r = prep_ray(o, d);
-  KBuf<K> b;
-  float last = 0.0f, tlo = 0.0f;
-  long long plo = -1;
-  const bool logging = FULL && P.log_cap > 0;
-  TravCount tc;
-  while (true) {
-    gather<K, COUNT>(P, r, tlo, plo, b, s_node, tc);
-    if (COUNT) {
-      P.trav_stats[2 * (size_t)ray] = tc.nodes;
-      P.trav_stats[2 * (size_t)ray + 1] = tc.tris;
-    }
-    float lt = 0.0f;
-    uint32_t lp = 0;
-    for (int i = 0; i < K; i++) {
-      const float t = b.t[0];
-      const uint32_t prim = b.p[0];
-      kpop(b);
-      if (prim == kInvalid) return;
-      lt = t;
-      lp = prim;
-      const int slot = st.consumed;
-      st.consumed++;
-      last = fmaxf((float)((double)last + kStepEps), t);
-      int code;
-      if (t < P.near_t) {
-        code = 0;
-      } else {
-        const int off = P.cumsum[prim];
-        const float *vp = P.verts + 3 * (size_t)off;
-        Win w;
-        window32(ld3(vp), ld3(vp + 3), ld3(vp + 6), o, d, t, P.sigma[prim], w);
-        const float alpha = fminf(0.99f, P.opac[prim] * w.Cx);
-        if (alpha < 1.0f / 255.0f) {
-          code = 1;
-        } else {
-          const float testT = st.T * (1.0f - alpha);
-          if (testT < P.cutoff) {
-            if (logging && slot < P.log_cap) {
-              P.log_prim[(size_t)ray * P.log_cap + slot] = (int)prim;
-              P.log_t[(size_t)ray * P.log_cap + slot] = t;
-              P.log_code[(size_t)ray * P.log_cap + slot] = 3;
-            }
-            return;
-          }
-          const float bw = alpha * st.T;
-          if (FULL) {
-            P.density[prim] = 1.0f;
-            compute_scaling(P, prim, scaling_org);
-            atomicMax(reinterpret_cast<int *>(P.maxb) + prim, __float_as_int(bw));
-            st.N[0] += w.n.x * bw;
-            st.N[1] += w.n.y * bw;
-            st.N[2] += w.n.z * bw;
-          }
-          bool c0, c1, c2;
-          const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
-          st.C[0] += col.x * bw;
-          st.C[1] += col.y * bw;
-          st.C[2] += col.z * bw;
-          st.T = testT;
-          st.blended++;
-          code = 2;
-        }
-      }
-      if (logging && slot < P.log_cap) {
-        P.log_prim[(size_t)ray * P.log_cap + slot] = (int)prim;
-        P.log_t[(size_t)ray * P.log_cap + slot] = t;
-        P.log_code[(size_t)ray * P.log_cap + slot] = code;
-      }
-    }
-    if (P.chunk_mode == 1) {
-      tlo = lt;
-      plo = (long long)lp;
-    } else if (last > tlo) {
-      tlo = last;
-      plo = -1;
-    } else {
-      plo = (long long)lp;
-    }
-  }
-}
+constexpr int kBlock = 128;  // lanes per block: two 8x8 wave tiles -> 16x8 pixels
+constexpr int kTileW = TileShape<kBlock>::W, kTileH = TileShape<kBlock>::H;
 
 template <int K, int VARIANT, bool COUNT = false>
 __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   __shared__ int s_node[kStack * kBlock];
   int x, y;
-  if (!pixel_of(P, x, y)) return;
-  const int pix = y * P.W + x;
-  v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
-  SegState st;
-  st.C[0] = st.C[1] = st.C[2] = 0.0f;
-  st.N[0] = st.N[1] = st.N[2] = 0.0f;
-  st.T = 1.0f;
-  st.consumed = st.blended = 0;
-  if (VARIANT == 0) {
-    trace_segment<K, true, COUNT>(P, o, d, st, pix, o, s_node);
-  } else {
-    const float rad = P.extra_len > 0 ? P.extra[0] : 1.0f;
-    const v3f cnt = P.extra_len > 3 ? mk(P.extra[1], P.extra[2], P.extra[3]) : mk(0.0f, 0.0f, 0.0f);
-    const float eps = (float)kStepEps;
-    if (VARIANT == 1) {  // reflect.cu:25-48
-      float sth;
-      if (hit_sphere(rad, cnt, o, d, sth)) {
-        o = o + d * sth;
-        const v3f nrm = normalize(o - cnt);
-        o = o + nrm * eps;
-        d = reflect3(normalize(d), nrm);
-      }
-      trace_segment<K, false, false>(P, o, d, st, pix, o, s_node);
-    } else if (VARIANT == 2) {  // refract.cu:189-213
-      float sth;
-      if (hit_sphere(rad, cnt, o, d, sth)) {
-        const v3f hp = o + d * sth;
-        const v3f nrm = normalize(hp - cnt);
-        const v3f in = normalize(d);
-        const float ior = 1.5f / 1.0003f;
-        v3f nd = in;
-        if (!refract_quirk(nd, in, nrm, ior)) {
-          const v3f rn = dot(in, nrm) < 0.0f ? nrm : -nrm;
-          nd = reflect3(in, rn);
-        }
-        o = hp + nrm * eps;
-        d = nd;
-      }
-      trace_segment<K, false, false>(P, o, d, st, pix, o, s_node);
-    } else {  // envlight.cu:116-184
-      const int samples = (int)(P.extra_len > 4 ? P.extra[4] : 0.0f);
-      float sth;
-      if (hit_sphere(rad, cnt, o, d, sth)) {
-        o = o + d * sth;
-        const v3f nrm0 = normalize(o - cnt);
-        o = o + nrm0 * eps;
-        float radc[3] = {0.0f, 0.0f, 0.0f};
-        for (int i = 0; i < samples; i++) {
-          const v3f nrm = normalize(o - cnt);
-          uint32_t seed = (((uint32_t)pix + 1u) * 9781u) ^ (((uint32_t)i + 1u) * 6271u);
-          const v3f dn = cos_hemi(seed, nrm);
-          const v3f on = o + dn * eps;
-          SegState ss;
-          ss.C[0] = ss.C[1] = ss.C[2] = 0.0f;
-          ss.N[0] = ss.N[1] = ss.N[2] = 0.0f;
-          ss.T = 1.0f;
-          ss.consumed = ss.blended = 0;
-          trace_segment<K, false, false>(P, on, dn, ss, pix, on, s_node);
-          for (int c = 0; c < 3; c++) radc[c] += ss.C[c] + ss.T * P.bg[c];
-          st.consumed += ss.consumed;
-          st.blended += ss.blended;
-        }
-        const float inv = 1.0f / (float)samples;
-        for (int c = 0; c < 3; c++) st.C[c] += radc[c] * inv;
-        st.T = 0.0f;
-      } else {
-        trace_segment<K, false, false>(P, o, d, st, pix, o, s_node);
-      }
-    }
-  }
-  const size_t R = (size_t)P.R;
-  for (int c = 0; c < 3; c++) P.color[c * R + pix] = st.C[c] + st.T * P.bg[c];
-  if (VARIANT == 0) {
-    P.others[0 * R + pix] = 0.0f;
-    P.others[1 * R + pix] = 1.0f - st.T;
-    P.others[2 * R + pix] = st.N[0];
-    P.others[3 * R + pix] = st.N[1];
-    P.others[4 * R + pix] = st.N[2];
-    P.others[5 * R + pix] = 0.0f;
-    P.others[6 * R + pix] = 0.0f;
-    if (P.log_cap > 0) P.log_count[pix] = st.consumed;
-  }
-  if (P.stats) {
-    P.stats[2 * (size_t)pix] = st.consumed;
-    P.stats[2 * (size_t)pix + 1] = st.blended;
-  }
-}
-
-// ------------------------------------------------------------------ backward
-TT_D v3f vecmat(v3f v, const float A[9]) {
-  return {A[0] * v.x + A[3] * v.y + A[6] * v.z, A[1] * v.x + A[4] * v.y + A[7] * v.z,
-          A[2] * v.x + A[5] * v.y + A[8] * v.z};
-}
-TT_D void skew(v3f v, float S[9]) {
-  S[0] = 0.0f, S[1] = -v.z, S[2] = v.y;
-  S[3] = v.z, S[4] = 0.0f, S[5] = -v.x;
-  S[6] = -v.y, S[7] = v.x, S[8] = 0.0f;
-}
-TT_D void outer(v3f u, v3f v, float M[9]) {
-  M[0] = u.x * v.x, M[1] = u.x * v.y, M[2] = u.x * v.z;
-  M[3] = u.y * v.x, M[4] = u.y * v.y, M[5] = u.y * v.z;
-  M[6] = u.z * v.x, M[7] = u.z * v.y, M[8] = u.z * v.z;
-}
-TT_D void eye_scaled(float s, float M[9]) {
-#pragma unroll
-  for (int i = 0; i < 9; i++) M[i] = ((i % 4) == 0 ? 1.0f : 0.0f) * s;
-}
-// d(v/|v|)/dv = (I - n n^T)/|v| (vecmath.h:2798-2809); scale formed as 1/|v| in double.
-TT_D void dnormvdv(v3f nv, v3f v, float out[9]) {
-  float op[9];
-  outer(nv, nv, op);
-#pragma unroll
-  for (int i = 0; i < 9; i++) out[i] = ((i % 4) == 0 ? 1.0f : 0.0f) - op[i];
-  const float s = (float)(1.0 / (double)vlen(v));
-#pragma unroll
-  for (int i = 0; i < 9; i++) out[i] *= s;
-}
-// Jacobians of ne = ((p1-p3) x (p2-p3)) x (p1-p2) (vecmath.h:2811-2859).
-TT_D void dne_dp1(v3f p1, v3f p2, v3f p3, float out[9]) {
-  v3f b = p2 - p3, c = p1 - p2;
-  v3f axb = cross(p1 - p3, b);
-  float m[9], d[9], s[9];
-  outer(b, c, m);
-  eye_scaled(dot(b, c), d);
-  skew(axb, s);
-#pragma unroll
-  for (int i = 0; i < 9; i++) out[i] = (m[i] - d[i]) + s[i];
-}
-TT_D void dne_dp2(v3f p1, v3f p2, v3f p3, float out[9]) {
-  v3f a = p1 - p3, b = p2 - p3, c = p1 - p2;
-  v3f axb = cross(p1 - p3, b);
-  float d[9], m[9], s[9];
-  eye_scaled(dot(a, c), d);
-  outer(a, c, m);
-  skew(axb, s);
-#pragma unroll
-  for (int i = 0; i < 9; i++) out[i] = (d[i] - m[i]) - s[i];
-}
-TT_D void dne_dp3(v3f p1, v3f p2, v3f p3, float out[9]) {
-  v3f c = p1 - p2;
-  float d[9], m[9];
-  eye_scaled(dot(c, c), d);
-  outer(c, c, m);
-#pragma unroll
-  for (int i = 0; i < 9; i++) out[i] = m[i] - d[i];
-}
-
-// ---- per-block gradient accumulation
-constexpr int kSlots = 128;  // triangles per block table (power of two)
-constexpr int kFields = 59;  // 9 vertex coordinates, opacity, sigma, 16 x 3 SH coefficients
-constexpr int F_OPAC = 9, F_SIGMA = 10, F_SH = 11;
-
-// Slot of `prim` in the block's table (linear probing, 8 probes), or -1 when full.
-TT_D int claim_slot(int *keys, uint32_t prim) {
-  uint32_t h = (prim * 2654435761u) >> 25;
-  for (int probe = 0; probe < 8; probe++) {
-    const int old = atomicCAS(&keys[h], -1, (int)prim);
-    if (old == -1 || old == (int)prim) return (int)h;
-    h = (h + 1) & (kSlots - 1);
-  }
-  return -1;
-}
-
-// Where one blended hit's gradient goes: the LDS slot of its triangle, or (table
-// full) straight to the global buffers.
-struct GradSink {
-  float *acc;
-  int slot;
-  float *g_verts, *g_opac, *g_sigma, *g_sh;
-  TT_D void add(int f, float v) {
-    if (slot >= 0)
-      atomicAdd(&acc[f * kSlots + slot], v);
-    else if (f < 9)
-      unsafeAtomicAdd(g_verts + f, v);
-    else if (f == F_OPAC)
-      unsafeAtomicAdd(g_opac, v);
-    else if (f == F_SIGMA)
-      unsafeAtomicAdd(g_sigma, v);
-    else
-      unsafeAtomicAdd(g_sh + (f - F_SH), v);
-  }
-};
-
-// computeWindowFuncBwd (backward.cu:104-189): gradient of the max edge distance
-// (|d_k - phi| <= 1e-6) and of the face normal w.r.t. the three vertices, hit
-// point and incenter distance held constant. Per-vertex sums are formed in
-// registers before they reach the sink.
-TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, GradSink &sink) {
-  v3f g[3] = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
-  for (int k = 0; k < 3; k++) {
-    if (fabsf(w.ed[k] - w.phi_max) > 1e-6f) continue;
-    const int k1 = k == 2 ? 0 : k + 1, k2 = k == 0 ? 2 : k - 1;
-    const v3f p1 = V[k], p2 = V[k1], p3 = V[k2];
-    const v3f n = cross(p1 - p3, p2 - p3);
-    const v3f ne = cross(n, p1 - p2);
-    const v3f Ne = normalize(ne);
-    float J[9];
-    dnormvdv(Ne, ne, J);
-    const v3f dL_dNe = dL_dphi * w.rel[k];
-    const v3f dL_dne = vecmat(dL_dNe, J);
-    float j1[9], j2[9], j3[9];
-    dne_dp1(p1, p2, p3, j1);
-    dne_dp2(p1, p2, p3, j2);
-    dne_dp3(p1, p2, p3, j3);
-    const v3f g1 = vecmat(dL_dne, j1) - dL_dphi * Ne;
-    const v3f g2 = vecmat(dL_dne, j2);
-    const v3f g3 = vecmat(dL_dne, j3);
-    if (k == 0) {
-      g[0] = g[0] + g1, g[1] = g[1] + g2, g[2] = g[2] + g3;
-    } else if (k == 1) {
-      g[1] = g[1] + g1, g[2] = g[2] + g2, g[0] = g[0] + g3;
-    } else {
-      g[2] = g[2] + g1, g[0] = g[0] + g2, g[1] = g[1] + g3;
-    }
-  }
-  if (dL_dn3.x != 0.0f || dL_dn3.y != 0.0f || dL_dn3.z != 0.0f) {
-    const v3f p1 = V[0], p2 = V[1], p3 = V[2];
-    const v3f n = cross(p1 - p3, p2 - p3);
-    const v3f Nn = normalize(n);
-    float J[9];
-    dnormvdv(Nn, n, J);
-    const v3f dL_dn = vecmat(dL_dn3, J);
-    float s1[9], s2[9], s3[9];
-    skew(p3 - p2, s1);
-    skew(p1 - p3, s2);
-    skew(p2 - p1, s3);
-    g[0] = g[0] + vecmat(dL_dn, s1);
-    g[1] = g[1] + vecmat(dL_dn, s2);
-    g[2] = g[2] + vecmat(dL_dn, s3);
-  }
-  for (int j = 0; j < 3; j++) {
-    sink.add(3 * j, g[j].x);
-    sink.add(3 * j + 1, g[j].y);
-    sink.add(3 * j + 2, g[j].z);
-  }
-}
-
-// computeColorFromSHBwd (backward.cu:33-102); clamped channels carry no gradient.
-TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3], GradSink &sink) {
-  const v3f dir = divs(dir0, vlen(dir0));
-  v3f g = mk(dc[0], dc[1], dc[2]);
-  g.x *= c0 ? 0.0f : 1.0f;
-  g.y *= c1 ? 0.0f : 1.0f;
-  g.z *= c2 ? 0.0f : 1.0f;
-  const float x = dir.x, y = dir.y, z = dir.z;
-  auto add = [&](int j, v3f v) {
-    if (!c0) sink.add(F_SH + 3 * j, v.x);
-    if (!c1) sink.add(F_SH + 3 * j + 1, v.y);
-    if (!c2) sink.add(F_SH + 3 * j + 2, v.z);
-  };
-  add(0, SH_C0 * g);
-  if (deg > 0) {
-    add(1, (-SH_C1 * y) * g);
-    add(2, (SH_C1 * z) * g);
-    add(3, (-SH_C1 * x) * g);
-    if (deg > 1) {
-      const float xx = x * x, yy = y * y, zz = z * z;
-      const float xy = x * y, yz = y * z, xz = x * z;
-      add(4, (SH_C2_0 * xy) * g);
-      add(5, (SH_C2_1 * yz) * g);
-      add(6, (SH_C2_2 * (2.0f * zz - xx - yy)) * g);
-      add(7, (SH_C2_3 * xz) * g);
-      add(8, (SH_C2_4 * (xx - yy)) * g);
-      if (deg > 2) {
-        add(9, ((SH_C3_0 * y) * (3.0f * xx - yy)) * g);
-        add(10, ((SH_C3_1 * xy) * z) * g);
-        add(11, ((SH_C3_2 * y) * (4.0f * zz - xx - yy)) * g);
-        add(12, ((SH_C3_3 * z) * (2.0f * zz - 3.0f * xx - 3.0f * yy)) * g);
-        add(13, ((SH_C3_4 * x) * (4.0f * zz - xx - yy)) * g);
-        add(14, ((SH_C3_5 * z) * (xx - yy)) * g);
-        add(15, ((SH_C3_6 * x) * (xx - 3.0f * yy)) * g);
-      }
-    }
-  }
-}
-
-// backward.cu:191-350: re-traverse one ray with the same chunk logic, recompute
-// the forward state hit by hit and send each blended hit's gradient to the sink.
-template <int K>
-TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, float *s_acc) {
-  const size_t R = (size_t)P.R;
-  const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
-  const float final_depth = P.fothers[pix];
-  const v3f final_normal = mk(P.fothers[2 * R + pix], P.fothers[3 * R + pix], P.fothers[4 * R + pix]);
-  float final_color[3], dL_dpixel[3];
-  for (int c = 0; c < 3; c++) {
-    final_color[c] = P.fcolor[c * R + pix];
-    dL_dpixel[c] = P.dpix[c * R + pix];
-  }
-  const float dL_ddepth = P.dothers[pix];
-  const v3f dL_dn2 = mk(P.dothers[2 * R + pix], P.dothers[3 * R + pix], P.dothers[4 * R + pix]);
-  float C[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f}, D = 0.0f, T = 1.0f;
-  const RayPre r = prep_ray(o, d);
-  KBuf<K> b;
-  float last = 0.0f, tlo = 0.0f;
-  long long plo = -1;
-  TravCount tc;
-  while (true) {
-    gather<K, false>(P, r, tlo, plo, b, s_node, tc);
-    float lt = 0.0f;
-    uint32_t lp = 0;
-    for (int i = 0; i < K; i++) {
-      const float t = b.t[0];
-      const uint32_t prim = b.p[0];
-      kpop(b);
-      if (prim == kInvalid) return;
-      lt = t;
-      lp = prim;
-      last = fmaxf((float)((double)last + kStepEps), t);
-      if (t < P.near_t) continue;
-      const int off = P.cumsum[prim];
-      const float *vp = P.verts + 3 * (size_t)off;
-      const v3f V[3] = {ld3(vp), ld3(vp + 3), ld3(vp + 6)};
-      Win w;
-      window32(V[0], V[1], V[2], o, d, t, P.sigma[prim], w);
-      const float opacity = P.opac[prim];
-      const float alpha = fminf(0.99f, opacity * w.Cx);
-      if (alpha < 1.0f / 255.0f) continue;
-      const float testT = T * (1.0f - alpha);
-      if (testT < P.cutoff) return;
-      const float bw = alpha * T;
-      bool c0, c1, c2;
-      const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
-      C[0] += col.x * bw, C[1] += col.y * bw, C[2] += col.z * bw;
-      N[0] += w.n.x * bw, N[1] += w.n.y * bw, N[2] += w.n.z * bw;
-      D += bw * t;
-      float dL_dcolor[3];
-      const float cc[3] = {col.x, col.y, col.z};
-      float dL_dalpha = 0.0f;
-      const float inv1a = 1.0f / (1.0f - alpha);
-      for (int c = 0; c < 3; c++) {
-        dL_dcolor[c] = dL_dpixel[c] * bw;
-        dL_dalpha += dL_dpixel[c] * (T * cc[c] - (final_color[c] - C[c]) * inv1a);
-      }
-      dL_dalpha += dL_ddepth * (T * t - (final_depth - D) * inv1a);
-      const v3f Nf = mk(N[0], N[1], N[2]);
-      dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
-      GradSink sink;
-      sink.acc = s_acc;
-      sink.slot = claim_slot(s_keys, prim);
-      sink.g_verts = P.dverts + 3 * (size_t)off;
-      sink.g_opac = P.dopac + prim;
-      sink.g_sigma = P.dsigma + prim;
-      sink.g_sh = P.dsh + (size_t)prim * P.M * 3;
-      sink.add(F_OPAC, dL_dalpha * w.Cx);
-      const float dL_dC = opacity * dL_dalpha;
-      if (w.phi_final > 0.0f) sink.add(F_SIGMA, (dL_dC * w.Cx) * log_tt(w.phi_final));
-      const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
-      const v3f dL_dn3 = dL_dn2 * bw;
-      window_bwd(V, w, dL_dphi, dL_dn3, sink);
-      sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, sink);
-      T = testT;
-    }
-    if (P.chunk_mode == 1) {
-      tlo = lt;
-      plo = (long long)lp;
-    } else if (last > tlo) {
-      tlo = last;
-      plo = -1;
-    } else {
-      plo = (long long)lp;
-    }
-  }
+  if (tile_pixel<kBlock>(P, xcd_tile(P.n_tiles), x, y)) forward_pixel<K, kBlock, VARIANT, COUNT>(P, y * P.W + x, s_node);
 }
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
   __shared__ int s_node[kStack * kBlock];
-  __shared__ int s_keys[kSlots];
-  __shared__ float s_acc[kFields * kSlots];
-  for (int i = threadIdx.x; i < kSlots; i += kBlock) s_keys[i] = -1;
-  for (int i = threadIdx.x; i < kFields * kSlots; i += kBlock) s_acc[i] = 0.0f;
-  __syncthreads();
+  __shared__ int s_keys[kBlock];
+  __shared__ float s_acc[kFields * kBlock];
   int x, y;
-  if (pixel_of(P, x, y)) backward_ray<K>(P, y * P.W + x, s_node, s_keys, s_acc);
-  __syncthreads();
-  const int nsh3 = 3 * (P.deg + 1) * (P.deg + 1);
-  for (int s = threadIdx.x; s < kSlots; s += kBlock) {
-    const int prim = s_keys[s];
-    if (prim < 0) continue;
-    float *gv = P.dverts + 3 * (size_t)P.cumsum[prim];
-    for (int f = 0; f < 9; f++) {
-      const float v = s_acc[f * kSlots + s];
-      if (v != 0.0f) unsafeAtomicAdd(gv + f, v);
-    }
-    float v = s_acc[F_OPAC * kSlots + s];
-    if (v != 0.0f) unsafeAtomicAdd(P.dopac + prim, v);
-    v = s_acc[F_SIGMA * kSlots + s];
-    if (v != 0.0f) unsafeAtomicAdd(P.dsigma + prim, v);
-    float *gs = P.dsh + (size_t)prim * P.M * 3;
-    for (int j = 0; j < nsh3; j++) {
-      v = s_acc[(F_SH + j) * kSlots + s];
-      if (v != 0.0f) unsafeAtomicAdd(gs + j, v);
-    }
-  }
-}
-
-__global__ void k_fill(float *p, size_t n, float v) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  const bool valid = tile_pixel<kBlock>(P, xcd_tile(P.n_tiles), x, y);
+  backward_block<K, kBlock>(P, valid, y * P.W + x, s_node, s_keys, s_acc);
 }
 
 template <int K>

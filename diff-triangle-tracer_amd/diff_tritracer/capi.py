@@ -21,7 +21,9 @@ BUILD, REFIT, REUSE = 0, 1, 2
 CHUNK = {"compat": 0, "exact": 1}
 
 EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build", "tt_trace_fwd", "tt_trace_bwd",
-            "tt_num_primitives", "tt_last_build_ms"]
+            "tt_num_primitives", "tt_last_build_ms", "tt_multi_forward", "tt_multi_backward", "tt_multi_capture",
+            "tt_multi_replay", "tt_multi_autotune"]
+MAX_CAMERAS = 32
 
 _vp = C.c_void_p
 _fp = C.c_void_p  # device pointers travel as void*
@@ -61,6 +63,23 @@ class BwdArgs(C.Structure):
     ]
 
 
+class MultiArgs(C.Structure):
+    _fields_ = [
+        ("n_cameras", C.c_int), ("width", C.POINTER(C.c_int)), ("height", C.POINTER(C.c_int)),
+        ("ray_o_dev", _fp), ("ray_d_dev", _fp),
+        ("vertices_dev", _fp), ("num_vertices", C.c_int),
+        ("npts_dev", _fp), ("cumsum_dev", _fp), ("P", C.c_int),
+        ("shs_dev", _fp), ("M", C.c_int), ("sh_degree", C.c_int),
+        ("opacities_dev", _fp), ("sigma_dev", _fp), ("bg_dev", _fp),
+        ("k", C.c_int), ("chunk_mode", C.c_int), ("block", C.c_int), ("tiles_per_fetch", C.c_int),
+        ("cutoff", C.c_float), ("near_t", C.c_float), ("far_t", C.c_float),
+        ("color_dev", _fp), ("others_dev", _fp), ("max_blending_dev", _fp), ("density_dev", _fp),
+        ("scaling_dev", _fp), ("stats_dev", _fp),
+        ("dL_dpix_dev", _fp), ("dL_dothers_dev", _fp), ("zero_grads", C.c_int), ("dL_dvertices_rows", C.c_int),
+        ("dL_dvertices_dev", _fp), ("dL_dsigma_dev", _fp), ("dL_dopacity_dev", _fp), ("dL_dsh_dev", _fp),
+    ]
+
+
 _lib = None
 
 
@@ -81,6 +100,12 @@ def lib():
         L.tt_num_primitives.argtypes = [_vp]
         L.tt_last_build_ms.restype = C.c_float
         L.tt_last_build_ms.argtypes = [_vp]
+        L.tt_multi_forward.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
+        L.tt_multi_backward.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
+        L.tt_multi_capture.argtypes = [_vp, C.POINTER(MultiArgs), C.c_int, _vp]
+        L.tt_multi_replay.argtypes = [_vp, _vp]
+        L.tt_multi_autotune.argtypes = [_vp, C.POINTER(MultiArgs), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                                        C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_float), _vp]
         _lib = L
     return _lib
 
@@ -136,6 +161,35 @@ class TracerHandle:
     def bwd(self, args: BwdArgs, stream=None):
         check(lib().tt_trace_bwd(self.h, C.byref(args), stream), "tt_trace_bwd")
 
+    def multi_fwd(self, args: "MultiArgs", stream=None):
+        check(lib().tt_multi_forward(self.h, C.byref(args), stream), "tt_multi_forward")
+
+    def multi_bwd(self, args: "MultiArgs", stream=None):
+        check(lib().tt_multi_backward(self.h, C.byref(args), stream), "tt_multi_backward")
+
+    def multi_capture(self, args: "MultiArgs", what=3, stream=None):
+        check(lib().tt_multi_capture(self.h, C.byref(args), int(what), stream), "tt_multi_capture")
+
+    def multi_replay(self, stream=None):
+        check(lib().tt_multi_replay(self.h, stream), "tt_multi_replay")
+
+    def multi_autotune(self, args: "MultiArgs", ks=(8, 16, 32), blocks=(64, 128, 256), tpf=(1, 2, 4), reps=3,
+                       stream=None):
+        """Times fwd+bwd for every configuration, stores the fastest in `args`; returns
+        {(k, block, tiles_per_fetch): median ms} (skipped configurations omitted)."""
+        ka, ba, ta = (C.c_int * len(ks))(*ks), (C.c_int * len(blocks))(*blocks), (C.c_int * len(tpf))(*tpf)
+        res = (C.c_float * (len(ks) * len(blocks) * len(tpf)))()
+        check(lib().tt_multi_autotune(self.h, C.byref(args), ka, len(ks), ba, len(blocks), ta, len(tpf), int(reps),
+                                      res, stream), "tt_multi_autotune")
+        out = {}
+        for i, k in enumerate(ks):
+            for j, b in enumerate(blocks):
+                for l, t in enumerate(tpf):
+                    v = res[(i * len(blocks) + j) * len(tpf) + l]
+                    if v >= 0:
+                        out[(k, b, t)] = float(v)
+        return out
+
 
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
              scaling, k=16, cutoff=1e-4, chunk_mode="compat", variant="forward", extra=None, stats=None,
@@ -185,5 +239,38 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.cutoff, a.near_t, a.far_t = float(cutoff), float(near), float(far)
     a.zero_grads = 1 if zero_grads else 0
     a.dL_dvertices_rows = int(dverts.numel() // 3)
+    a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
+    return a
+
+
+def multi_args(widths, heights, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others,
+               maxb, density, scaling, dL_dpix=None, dL_dothers=None, dverts=None, dsigma=None, dopac=None, dsh=None,
+               k=16, cutoff=1e-3, chunk_mode="exact", block=128, tiles_per_fetch=1, zero_grads=True, stats=None,
+               near=0.2, far=1000.0) -> MultiArgs:
+    """Arguments of the batched multi-camera path; per-camera buffers are concatenated
+    in camera order (rays [sum R_c][3], colour [3][R_c] per camera, others [7][R_c])."""
+    a = MultiArgs()
+    n = len(widths)
+    a.n_cameras = n
+    a._w = (C.c_int * n)(*[int(w) for w in widths])
+    a._h = (C.c_int * n)(*[int(h) for h in heights])
+    a.width, a.height = a._w, a._h
+    a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
+    a.vertices_dev, a.num_vertices = ptr(verts), int(verts.numel() // 3)
+    a.npts_dev, a.cumsum_dev, a.P = ptr(npts), ptr(cumsum), int(cumsum.numel())
+    a.shs_dev = ptr(shs)
+    a.M = int(shs.shape[1]) if shs is not None and shs.numel() else 0
+    a.sh_degree = int(sh_degree)
+    a.opacities_dev, a.sigma_dev, a.bg_dev = ptr(opac), ptr(sigma), ptr(bg)
+    a.k = int(k)
+    a.chunk_mode = CHUNK[chunk_mode] if isinstance(chunk_mode, str) else int(chunk_mode)
+    a.block, a.tiles_per_fetch = int(block), int(tiles_per_fetch)
+    a.cutoff, a.near_t, a.far_t = float(cutoff), float(near), float(far)
+    a.color_dev, a.others_dev = ptr(color), ptr(others)
+    a.max_blending_dev, a.density_dev, a.scaling_dev = ptr(maxb), ptr(density), ptr(scaling)
+    a.stats_dev = ptr(stats)
+    a.dL_dpix_dev, a.dL_dothers_dev = ptr(dL_dpix), ptr(dL_dothers)
+    a.zero_grads = 1 if zero_grads else 0
+    a.dL_dvertices_rows = int(dverts.numel() // 3) if dverts is not None else 0
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
     return a

@@ -145,3 +145,31 @@ def scale_config(seed: int = 1, P: int = 1_000_000, width: int = 1920, height: i
     eye = 4.0 * np.array([0.35, 0.3, 1.0]) / np.linalg.norm([0.35, 0.3, 1.0])
     ro, rd = look_at_rays(width, height, eye)
     return scene, ro, rd
+
+
+MULTIVIEW_SIZES = [(640, 480), (800, 600), (1024, 768), (1280, 720), (1280, 720), (1600, 900), (1920, 1080),
+                   (1920, 1080)]
+
+
+def multiview_rays(sizes, seed: int = 2, rmin: float = 3.0, rmax: float = 5.0, fov_deg: float = 60.0):
+    """Cameras of uneven resolution on random poses on a sphere of radius [rmin, rmax],
+    looking at the origin. Returns (widths, heights, ray_o [sum R, 3], ray_d [sum R, 3])
+    with the cameras' rays concatenated in order."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    os_, ds_ = [], []
+    for i, (w, h) in enumerate(sizes):
+        r = rng.uniform(rmin, rmax)
+        eye = orbit_eye(r, seed, i)
+        ro, rd = look_at_rays(w, h, eye, fov_deg=fov_deg)
+        os_.append(ro.reshape(-1, 3))
+        ds_.append(rd.reshape(-1, 3))
+    return ([w for w, _ in sizes], [h for _, h in sizes], np.ascontiguousarray(np.concatenate(os_)),
+            np.ascontiguousarray(np.concatenate(ds_)))
+
+
+def multiview_config(seed: int = 2, P: int = 2_000_000, sizes=None):
+    """Batched multi-view step: 2M clustered splats, 8 cameras of uneven resolution
+    (640x480 ... 1920x1080, 9,004,032 rays) on random poses at radius 3-5."""
+    scene = clustered_scene(P, 64, seed)
+    w, h, ro, rd = multiview_rays(sizes or MULTIVIEW_SIZES, seed)
+    return scene, w, h, ro, rd

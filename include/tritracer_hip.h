@@ -144,6 +144,54 @@ typedef struct {
    re-traverses every ray with the forward's LBVH and scatters per-triangle gradients. */
 int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream);
 
+/* ---- batched multi-camera path (no counterpart in the reference, which traces one
+   camera per call; per-pixel results equal per-camera tt_trace_fwd / tt_trace_bwd) ---- */
+#define TT_MAX_CAMERAS 32
+
+typedef struct {
+  int n_cameras;                /* 1 .. TT_MAX_CAMERAS */
+  const int *width, *height;    /* HOST arrays [n_cameras] */
+  const float *ray_o_dev, *ray_d_dev; /* all cameras' rays, camera c first at ray sum_{j<c} W_j H_j */
+  const float *vertices_dev;
+  int num_vertices;
+  const int *npts_dev, *cumsum_dev;
+  int P;
+  const float *shs_dev;
+  int M, sh_degree;
+  const float *opacities_dev, *sigma_dev;
+  const float *bg_dev;          /* [3] */
+  int k;                        /* 8, 16, 32 */
+  int chunk_mode;               /* TT_CHUNK_EXACT makes results independent of k */
+  int block;                    /* lanes per block: 64 (8x8 tile), 128 (16x8) or 256 (16x16) */
+  int tiles_per_fetch;          /* tiles a block claims from the work queue at once (>= 1) */
+  float cutoff, near_t, far_t;
+  /* forward outputs: per camera [3][R_c] / [7][R_c], concatenated in camera order */
+  float *color_dev, *others_dev;
+  float *max_blending_dev, *density_dev, *scaling_dev; /* [P], shared by all cameras */
+  int *stats_dev;               /* optional [sum R_c][2] */
+  /* backward inputs (same concatenation as color/others) and gradient outputs */
+  const float *dL_dpix_dev, *dL_dothers_dev;
+  int zero_grads;               /* tt_multi_backward zeroes the gradient buffers first */
+  int dL_dvertices_rows;
+  float *dL_dvertices_dev, *dL_dsigma_dev, *dL_dopacity_dev, *dL_dsh_dev;
+} tt_multi_args;
+
+/* Forward of every camera in ONE persistent launch (blocks pull tiles from 8 work queues). */
+int tt_multi_forward(tt_tracer *tr, const tt_multi_args *a, void *stream);
+/* Backward of every camera in ONE persistent launch; gradients of all cameras accumulate. */
+int tt_multi_backward(tt_tracer *tr, const tt_multi_args *a, void *stream);
+/* Record one training step into a hipGraph owned by the tracer: what = 1 forward, 2 backward
+   (with zeroing when a->zero_grads), 3 both. Buffers must stay valid while the graph is used. */
+int tt_multi_capture(tt_tracer *tr, const tt_multi_args *a, int what, void *stream);
+/* Launch the captured graph on `stream`. */
+int tt_multi_replay(tt_tracer *tr, void *stream);
+/* Autotuner: times forward+backward for every (k, block, tiles_per_fetch) in the given lists
+   (k is swept only in TT_CHUNK_EXACT mode, where results do not depend on it), writes the
+   median ms of `reps` runs per configuration into results_ms[nk][nb][nt] (-1 = skipped),
+   and stores the fastest configuration into *a. Synchronises the stream. */
+int tt_multi_autotune(tt_tracer *tr, tt_multi_args *a, const int *ks, int nk, const int *blocks, int nb,
+                      const int *tpf, int nt, int reps, float *results_ms, void *stream);
+
 /* Number of triangles of the current acceleration structure (-1 if none). */
 int tt_num_primitives(const tt_tracer *tr);
 
