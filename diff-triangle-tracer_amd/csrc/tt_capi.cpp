@@ -220,6 +220,10 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
     if (a->hit_log_cap > 0)
       TT_ARG(a->hit_log_prim_dev && a->hit_log_t_dev && a->hit_log_code_dev && a->hit_log_count_dev,
              "hit log buffers are NULL");
+    if (a->interval_in_dev || a->interval_out_dev)
+      TT_ARG(a->variant == TT_VARIANT_FORWARD, "depth intervals are supported by the forward variant only");
+    if (a->interval_in_dev)
+      TT_ARG(a->chunk_mode == TT_CHUNK_EXACT, "interval_in requires TT_CHUNK_EXACT");
   }
   if (R == 0) return ok();
   tt::TraceParams p;
@@ -241,6 +245,9 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.log_t = a->hit_log_t_dev;
   p.log_code = a->hit_log_code_dev;
   p.log_count = a->hit_log_count_dev;
+  p.ival_in = reinterpret_cast<const float2 *>(a->interval_in_dev);
+  p.ival_out = reinterpret_cast<float4 *>(a->interval_out_dev);
+  p.ival_hard = a->interval_in_dev && a->interval_hard_cap ? 1 : 0;
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
   if (!err.empty()) return fail(TT_ERR_HIP, err);
@@ -271,6 +278,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   if (R == 0 || a->P == 0) return ok();
   TT_ARG(a->color_dev && a->others_dev && a->dL_dpix_dev && a->dL_dothers_dev,
          "forward outputs / upstream gradients are NULL");
+  if (a->interval_dev) TT_ARG(a->chunk_mode == TT_CHUNK_EXACT, "interval requires TT_CHUNK_EXACT");
   tt::TraceParams p;
   fill_common(p, tr, a->width, a->height, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev, a->cumsum_dev,
               a->P, a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->chunk_mode, a->cutoff,
@@ -283,6 +291,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   p.dsigma = a->dL_dsigma_dev;
   p.dopac = a->dL_dopacity_dev;
   p.dsh = a->dL_dsh_dev;
+  p.ival_bwd = reinterpret_cast<const float4 *>(a->interval_dev);
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   std::string err = tt::launch_backward(p, a->k, s);
   if (!err.empty()) return fail(TT_ERR_HIP, err);

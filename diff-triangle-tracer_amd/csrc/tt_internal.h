@@ -81,6 +81,12 @@ struct TraceParams {
   // backward
   const float *fcolor, *fothers, *dpix, *dothers;
   float *dverts, *dsigma, *dopac, *dsh;
+  // depth intervals: forward input (start, cap) and output (first, last, term, prim_first);
+  // backward input = the forward's output
+  const float2 *ival_in;
+  int ival_hard;  // 1: hits past the cap are dropped (the proposal is trusted); 0: exact continuation
+  float4 *ival_out;
+  const float4 *ival_bwd;
 };
 
 // Cameras of one persistent multi-camera launch (passed by value as a kernel argument).

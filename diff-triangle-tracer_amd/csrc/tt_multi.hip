@@ -62,7 +62,7 @@ TT_D TraceParams camera_view(const TraceParams &P, const MultiCams &mc, int tile
 
 template <int K, int BLOCK>
 __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, MultiCams mc, int *heads, int G) {
-  __shared__ int s_node[kStack * BLOCK];
+  __shared__ int s_node[kLdsInts * BLOCK];
   __shared__ int s_range[2];
   while (true) {
     if (threadIdx.x == 0) {
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, Multi
 
 template <int K, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_backward_multi(TraceParams P, MultiCams mc, int *heads, int G) {
-  __shared__ int s_node[kStack * BLOCK];
+  __shared__ int s_node[kLdsInts * BLOCK];
   __shared__ int s_keys[BLOCK];
   __shared__ float s_acc[kFields * BLOCK];
   __shared__ int s_range[2];

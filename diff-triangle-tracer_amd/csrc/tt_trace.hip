@@ -14,14 +14,14 @@ constexpr int kTileW = TileShape<kBlock>::W, kTileH = TileShape<kBlock>::H;
 
 template <int K, int VARIANT, bool COUNT = false>
 __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
-  __shared__ int s_node[kStack * kBlock];
+  __shared__ int s_node[kLdsInts * kBlock];
   int x, y;
   if (tile_pixel<kBlock>(P, xcd_tile(P.n_tiles), x, y)) forward_pixel<K, kBlock, VARIANT, COUNT>(P, y * P.W + x, s_node);
 }
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
-  __shared__ int s_node[kStack * kBlock];
+  __shared__ int s_node[kLdsInts * kBlock];
   __shared__ int s_keys[kBlock];
   __shared__ float s_acc[kFields * kBlock];
   int x, y;

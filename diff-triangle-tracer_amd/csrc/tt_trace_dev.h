@@ -35,6 +35,8 @@ namespace tt {
 namespace {
 
 constexpr int kStack = kLdsStack;  // LDS entries per lane (deferred node index)
+constexpr int kLeafQ = 8;          // LDS queue of postponed leaves per lane
+constexpr int kLdsInts = kStack + kLeafQ;  // ints of LDS per lane the gather uses
 
 // Pixel tile of a block: one 8x8 quadrant per wave; 64 lanes -> 8x8, 128 -> 16x8, 256 -> 16x16.
 template <int BLOCK>
@@ -94,32 +96,28 @@ TT_D void cswap(float &da, int &ca, float &db, int &cb) {
   ca = tc;
 }
 
-// One chunk: the K smallest (t, prim) with t <= far and (t, prim) after the key.
+// One chunk: the K smallest (t, prim) with t <= far_t and (t, prim) after the key.
 // Ordered traversal of the 4-wide nodes: the up-to-4 child boxes are tested from
-// the node itself, hit leaves are intersected at once, the hit internal children
-// are sorted by entry distance (5-comparator network), the nearest is descended
-// and the others are deferred farthest-first.
+// the node itself, the hit internal children are sorted by entry distance
+// (5-comparator network), the nearest is descended and the others are deferred
+// farthest-first. Hit leaves are not tested at once: they go to a per-lane queue in
+// LDS and a lane tests one queued leaf per step when at least half of the wave's
+// active lanes have one queued (or its queue is nearly full, or its walk is over),
+// so the intersection code runs with most lanes busy instead of once per leaf-holding
+// lane. The K-buffer keeps the K smallest keys whatever the insertion order, so
+// postponing a leaf only weakens the culling bound for a while.
 template <int K, int BLOCK, bool COUNT>
-TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo, KBuf<K> &b, int *s_node,
-                 TravCount &tc) {
+TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo, float far_t, KBuf<K> &b,
+                 int *s_node, TravCount &tc) {
   kinit(b);
   if (P.n_leaves == 0) return;
   const int lane = threadIdx.x;
   int *ov = P.ov_stack + ((size_t)blockIdx.x * BLOCK + lane) * P.ov_depth;
+  int *s_leaf = s_node + kStack * BLOCK;
   const float tlo_c = tlo - fabsf(tlo) * 1e-6f - 1e-7f;
-  const float far_t = P.far_t;
   int node = 0;
   int sp = 0;
-  auto leaf = [&](int s) {
-    if (COUNT) tc.tris++;
-    const float4 a = P.leaves[3 * s], bb = P.leaves[3 * s + 1], c = P.leaves[3 * s + 2];
-    const uint32_t prim = __float_as_uint(a.w);
-    float t;
-    if (!intersect(r, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), mk(c.x, c.y, c.z), t)) return;
-    if (!(t <= far_t)) return;
-    if (!after_key(t, prim, tlo, plo)) return;
-    kinsert(b, t, prim);
-  };
+  int nq = 0;  // queued leaves of this lane
   auto push = [&](int n) {
     if (sp < kStack)
       s_node[sp * BLOCK + lane] = n;
@@ -128,7 +126,7 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
     sp++;
   };
   while (true) {
-    if (node >= 0) {
+    if (node >= 0 && nq <= kLeafQ - 4) {
       if (COUNT) tc.nodes++;
       const float4 *nd = P.nodes4 + 8 * (size_t)node;
       const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
@@ -144,24 +142,10 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       bool h1 = ref.y != kEmptyChild && box_ok(tn1, tf1, tlo_c, thi_c);
       bool h2 = ref.z != kEmptyChild && box_ok(tn2, tf2, tlo_c, thi_c);
       bool h3 = ref.w != kEmptyChild && box_ok(tn3, tf3, tlo_c, thi_c);
-      // hit leaves: queued, then tested in one loop so that the intersection code
-      // runs once per queued leaf for the whole wave (not once per slot)
-      int q0 = -1, q1 = -1, q2 = -1, q3 = -1, nq = 0;
-      auto enq = [&](int s) {
-        q3 = nq == 3 ? s : q3;
-        q2 = nq == 2 ? s : q2;
-        q1 = nq == 1 ? s : q1;
-        q0 = nq == 0 ? s : q0;
-        nq++;
-      };
-      if (h0 && ref.x < 0) enq(~ref.x), h0 = false;
-      if (h1 && ref.y < 0) enq(~ref.y), h1 = false;
-      if (h2 && ref.z < 0) enq(~ref.z), h2 = false;
-      if (h3 && ref.w < 0) enq(~ref.w), h3 = false;
-      for (int j = 0; j < nq; j++) {
-        leaf(q0);
-        q0 = q1, q1 = q2, q2 = q3;
-      }
+      if (h0 && ref.x < 0) s_leaf[nq++ * BLOCK + lane] = ~ref.x, h0 = false;
+      if (h1 && ref.y < 0) s_leaf[nq++ * BLOCK + lane] = ~ref.y, h1 = false;
+      if (h2 && ref.z < 0) s_leaf[nq++ * BLOCK + lane] = ~ref.z, h2 = false;
+      if (h3 && ref.w < 0) s_leaf[nq++ * BLOCK + lane] = ~ref.w, h3 = false;
       const float inf = __builtin_huge_valf();
       float d0 = h0 ? tn0 : inf, d1 = h1 ? tn1 : inf, d2 = h2 ? tn2 : inf, d3 = h3 ? tn3 : inf;
       int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
@@ -175,12 +159,24 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       if (cnt >= 3) push(c2);
       if (cnt >= 2) push(c1);
       node = cnt >= 1 ? c0 : -1;
+      if (node < 0 && sp > 0) {
+        sp--;
+        node = sp < kStack ? s_node[sp * BLOCK + lane] : ov[sp - kStack];
+      }
     }
-    if (node < 0) {
-      if (sp == 0) return;
-      sp--;
-      node = sp < kStack ? s_node[sp * BLOCK + lane] : ov[sp - kStack];
+    const bool walked = node < 0;
+    const int busy = __popcll(__ballot(nq > 0)), active = __popcll(__ballot(1));
+    if (nq > 0 && (walked || nq > kLeafQ - 4 || 2 * busy >= active)) {
+      const int s = s_leaf[--nq * BLOCK + lane];
+      if (COUNT) tc.tris++;
+      const float4 a = P.leaves[3 * s], bb = P.leaves[3 * s + 1], c = P.leaves[3 * s + 2];
+      const uint32_t prim = __float_as_uint(a.w);
+      float t;
+      if (intersect(r, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), mk(c.x, c.y, c.z), t) && t <= far_t &&
+          after_key(t, prim, tlo, plo))
+        kinsert(b, t, prim);
     }
+    if (walked && nq == 0) return;
   }
 }
 
@@ -232,31 +228,53 @@ struct SegState {
   float T;
   float N[3];
   int consumed, blended;
+  // depth interval of the blended hits: first/last t, t of the terminating hit, first prim
+  float t_first, t_last, t_term;
+  uint32_t p_first;
 };
+
+TT_D void seg_init(SegState &st) {
+  st.C[0] = st.C[1] = st.C[2] = 0.0f;
+  st.N[0] = st.N[1] = st.N[2] = 0.0f;
+  st.T = 1.0f;
+  st.consumed = st.blended = 0;
+  st.t_first = st.t_last = st.t_term = __builtin_huge_valf();
+  st.p_first = kInvalid;
+}
 
 // The chunk loop of forward.cu:76-155 (FULL: with normal/statistics outputs) and
 // of the variants (colour only).
+// Depth interval (exact chunk mode): hits start at t >= tstart; the gathers stop at
+// `cap` until the hits up to it are used up, and go on past it (t > cap) only while
+// the ray is still open. The result is that of a trace starting at tstart for any cap.
+// With P.ival_hard the trace ends at the cap instead.
 template <int K, int BLOCK, bool FULL, bool COUNT>
-TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node) {
+TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node,
+                        float tstart = 0.0f, float cap = kInfDist) {
   RayPre r = prep_ray(o, d);
   KBuf<K> b;
-  float last = 0.0f, tlo = 0.0f;
+  float last = 0.0f, tlo = tstart;
   long long plo = -1;
+  bool capped = cap < P.far_t && cap >= tstart;  // a cap before the start is no cap
   const bool logging = FULL && P.log_cap > 0;
   TravCount tc;
   while (true) {
-    gather<K, BLOCK, COUNT>(P, r, tlo, plo, b, s_node, tc);
+    gather<K, BLOCK, COUNT>(P, r, tlo, plo, capped ? cap : P.far_t, b, s_node, tc);
     if (COUNT) {
       P.trav_stats[2 * (size_t)ray] = tc.nodes;
       P.trav_stats[2 * (size_t)ray + 1] = tc.tris;
     }
     float lt = 0.0f;
     uint32_t lp = 0;
+    bool spent = false;
     for (int i = 0; i < K; i++) {
       const float t = b.t[0];
       const uint32_t prim = b.p[0];
       kpop(b);
-      if (prim == kInvalid) return;
+      if (prim == kInvalid) {
+        spent = true;
+        break;
+      }
       lt = t;
       lp = prim;
       const int slot = st.consumed;
@@ -281,8 +299,11 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
               P.log_t[(size_t)ray * P.log_cap + slot] = t;
               P.log_code[(size_t)ray * P.log_cap + slot] = 3;
             }
+            st.t_term = t;
             return;
           }
+          if (st.blended == 0) st.t_first = t, st.p_first = prim;
+          st.t_last = t;
           const float bw = alpha * st.T;
           if (FULL) {
             P.density[prim] = 1.0f;
@@ -308,6 +329,13 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
         P.log_code[(size_t)ray * P.log_cap + slot] = code;
       }
     }
+    if (spent) {
+      if (!capped || P.ival_hard) return;
+      capped = false;  // everything up to cap is used: continue strictly past it
+      tlo = cap;
+      plo = (long long)kInvalid;
+      continue;
+    }
     if (P.chunk_mode == 1) {
       tlo = lt;
       plo = (long long)lp;
@@ -325,12 +353,17 @@ template <int K, int BLOCK, int VARIANT, bool COUNT>
 TT_D void forward_pixel(const TraceParams &P, int pix, int *s_node) {
   v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
   SegState st;
-  st.C[0] = st.C[1] = st.C[2] = 0.0f;
-  st.N[0] = st.N[1] = st.N[2] = 0.0f;
-  st.T = 1.0f;
-  st.consumed = st.blended = 0;
+  seg_init(st);
   if (VARIANT == 0) {
-    trace_segment<K, BLOCK, true, COUNT>(P, o, d, st, pix, o, s_node);
+    float tstart = 0.0f, cap = kInfDist;
+    if (P.ival_in) {
+      const float2 iv = P.ival_in[pix];
+      tstart = iv.x;
+      cap = iv.y;
+    }
+    trace_segment<K, BLOCK, true, COUNT>(P, o, d, st, pix, o, s_node, tstart, cap);
+    if (P.ival_out)
+      P.ival_out[pix] = make_float4(st.t_first, st.t_last, st.t_term, __uint_as_float(st.p_first));
   } else {
     const float rad = P.extra_len > 0 ? P.extra[0] : 1.0f;
     const v3f cnt = P.extra_len > 3 ? mk(P.extra[1], P.extra[2], P.extra[3]) : mk(0.0f, 0.0f, 0.0f);
@@ -374,10 +407,7 @@ TT_D void forward_pixel(const TraceParams &P, int pix, int *s_node) {
           const v3f dn = cos_hemi(seed, nrm);
           const v3f on = o + dn * eps;
           SegState ss;
-          ss.C[0] = ss.C[1] = ss.C[2] = 0.0f;
-          ss.N[0] = ss.N[1] = ss.N[2] = 0.0f;
-          ss.T = 1.0f;
-          ss.consumed = ss.blended = 0;
+          seg_init(ss);
           trace_segment<K, BLOCK, false, false>(P, on, dn, ss, pix, on, s_node);
           for (int c = 0; c < 3; c++) radc[c] += ss.C[c] + ss.T * P.bg[c];
           st.consumed += ss.consumed;
@@ -617,11 +647,18 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
   float C[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f}, D = 0.0f, T = 1.0f;
   const RayPre r = prep_ray(o, d);
   KBuf<K> b;
-  float last = 0.0f, tlo = 0.0f;
+  float last = 0.0f, tlo = 0.0f, far_t = P.far_t;
   long long plo = -1;
+  if (P.ival_bwd) {  // only the forward's blended hits: from the first one (inclusive) to the last
+    const float4 iv = P.ival_bwd[pix];
+    if (!(iv.x < kInfDist)) return;
+    tlo = iv.x;
+    plo = (long long)__float_as_uint(iv.w) - 1;
+    far_t = iv.y;
+  }
   TravCount tc;
   while (true) {
-    gather<K, BLOCK, false>(P, r, tlo, plo, b, s_node, tc);
+    gather<K, BLOCK, false>(P, r, tlo, plo, far_t, b, s_node, tc);
     float lt = 0.0f;
     uint32_t lp = 0;
     for (int i = 0; i < K; i++) {

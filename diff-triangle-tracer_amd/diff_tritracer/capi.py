@@ -43,7 +43,8 @@ class FwdArgs(C.Structure):
         ("color_dev", _fp), ("others_dev", _fp), ("max_blending_dev", _fp), ("density_dev", _fp),
         ("scaling_dev", _fp), ("stats_dev", _fp), ("trav_stats_dev", _fp),
         ("hit_log_cap", C.c_int), ("hit_log_prim_dev", _fp), ("hit_log_t_dev", _fp), ("hit_log_code_dev", _fp),
-        ("hit_log_count_dev", _fp),
+        ("hit_log_count_dev", _fp), ("interval_in_dev", _fp), ("interval_out_dev", _fp),
+        ("interval_hard_cap", C.c_int),
     ]
 
 
@@ -60,6 +61,7 @@ class BwdArgs(C.Structure):
         ("cutoff", C.c_float), ("near_t", C.c_float), ("far_t", C.c_float),
         ("zero_grads", C.c_int), ("dL_dvertices_rows", C.c_int),
         ("dL_dvertices_dev", _fp), ("dL_dsigma_dev", _fp), ("dL_dopacity_dev", _fp), ("dL_dsh_dev", _fp),
+        ("interval_dev", _fp),
     ]
 
 
@@ -193,7 +195,8 @@ class TracerHandle:
 
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
              scaling, k=16, cutoff=1e-4, chunk_mode="compat", variant="forward", extra=None, stats=None,
-             log=None, near=0.2, far=1000.0, trav_stats=None) -> FwdArgs:
+             log=None, near=0.2, far=1000.0, trav_stats=None, interval_in=None, interval_out=None,
+             hard_cap=False) -> FwdArgs:
     a = FwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -213,6 +216,8 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.max_blending_dev, a.density_dev, a.scaling_dev = ptr(maxb), ptr(density), ptr(scaling)
     a.stats_dev = ptr(stats)
     a.trav_stats_dev = ptr(trav_stats)
+    a.interval_in_dev, a.interval_out_dev = ptr(interval_in), ptr(interval_out)
+    a.interval_hard_cap = 1 if hard_cap else 0
     if log is not None:
         prim, t, code, count = log
         a.hit_log_cap = int(prim.shape[-1])
@@ -223,7 +228,7 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
 
 def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, color, others, dL_dpix,
              dL_dothers, dverts, dsigma, dopac, dsh, k=16, cutoff=1e-4, chunk_mode="compat", zero_grads=True,
-             near=0.2, far=1000.0) -> BwdArgs:
+             near=0.2, far=1000.0, interval=None) -> BwdArgs:
     a = BwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -240,6 +245,7 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.zero_grads = 1 if zero_grads else 0
     a.dL_dvertices_rows = int(dverts.numel() // 3)
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
+    a.interval_dev = ptr(interval)
     return a
 
 

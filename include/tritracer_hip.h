@@ -110,6 +110,17 @@ typedef struct {
   float *hit_log_t_dev;   /* [R][cap] */
   int *hit_log_code_dev;  /* [R][cap] 0 near skip, 1 alpha skip, 2 blended, 3 terminated */
   int *hit_log_count_dev; /* [R] total consumed hits */
+  /* Depth intervals (no counterpart in the reference; TT_VARIANT_FORWARD, TT_CHUNK_EXACT).
+     interval_in: optional [R][2] (start, cap) per ray. The gather starts at t >= start; the
+     first gathers stop at cap and go on past it only while the ray is still open, so the
+     result equals a trace that starts at `start` (start <= near_t: the full trace; a cap
+     below start is ignored).
+     interval_out: optional [R][4] per ray: t of the first and of the last blended hit, t of
+     the hit that ended the ray (inf if none), first blended triangle id (int bits); the
+     first field is inf for a ray with no blended hit. */
+  const float *interval_in_dev;
+  float *interval_out_dev;
+  int interval_hard_cap;         /* nonzero: the trace ends at cap (hits past it are dropped) */
 } tt_fwd_args;
 
 /* Replaces OptixTracer::trace_fwd's launch (tracer.cpp:105-132) and the raygen programs
@@ -138,6 +149,9 @@ typedef struct {
   float *dL_dsigma_dev;        /* [P] */
   float *dL_dopacity_dev;      /* [P] */
   float *dL_dsh_dev;           /* [P][M][3] */
+  /* optional [R][4]: the forward's interval_out (TT_CHUNK_EXACT). The re-traversal then
+     covers only [first blended hit, last blended hit] of each ray: same gradients. */
+  const float *interval_dev;
 } tt_bwd_args;
 
 /* Replaces OptixTracer::trace_bwd's launch (tracer.cpp:199-222) and backward.cu:191-391:

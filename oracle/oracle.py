@@ -44,6 +44,7 @@ class _Opts(C.Structure):
         ("bg", C.c_float * 3),
         ("extra", C.POINTER(C.c_float)),
         ("extra_len", C.c_int),
+        ("tstart", C.POINTER(C.c_float)),
     ]
 
 
@@ -217,7 +218,9 @@ class Scene:
 
 
 def make_opts(k=16, chunk_mode="compat", variant="forward", use_bvh=True, cutoff=1e-4, near=0.2, far=1000.0,
-              bg=(0.0, 0.0, 0.0), extra=None):
+              bg=(0.0, 0.0, 0.0), extra=None, tstart=None):
+    """tstart: optional per-ray start distance [R] (forward variant, exact chunk mode;
+    honoured by forward_f32 and the decision log)."""
     o = _Opts()
     o.k = int(k)
     o.chunk_mode = {"compat": 0, "exact": 1}[chunk_mode] if isinstance(chunk_mode, str) else int(chunk_mode)
@@ -237,6 +240,12 @@ def make_opts(k=16, chunk_mode="compat", variant="forward", use_bvh=True, cutoff
         o.extra = C.POINTER(C.c_float)()
         o.extra_len = 0
     o._keep = keep  # keep the extra buffer alive with the struct
+    if tstart is not None:
+        ts = _f32(tstart).reshape(-1)
+        o.tstart = _f(ts)
+        o._keep_ts = ts
+    else:
+        o.tstart = C.POINTER(C.c_float)()
     return o
 
 

@@ -716,6 +716,7 @@ struct Opts {
   float cutoff, near_t, far_t;
   float bg[3];
   std::vector<float> extra;
+  const float *tstart = nullptr;
 };
 Opts to_opts(const or_opts *o) {
   Opts p;
@@ -728,6 +729,7 @@ Opts to_opts(const or_opts *o) {
   p.far_t = o->far_t;
   for (int i = 0; i < 3; i++) p.bg[i] = o->bg[i];
   if (o->extra && o->extra_len > 0) p.extra.assign(o->extra, o->extra + o->extra_len);
+  p.tstart = o->variant == 0 ? o->tstart : nullptr;
   return p;
 }
 
@@ -788,11 +790,12 @@ struct FwdSinks {
 // The chunk loop of forward.cu:76-155 (FULL) and of the variants (colour only).
 // Returns with C, T, N updated; counts consumed/blended hits; optionally logs hits.
 void trace_segment_f32(const Scene &s, const Opts &op, V3f o, V3f d, float C[3], float &T, float N[3],
-                       int &consumed, int &blended, const FwdSinks *sinks, std::vector<LogHit> *log) {
+                       int &consumed, int &blended, const FwdSinks *sinks, std::vector<LogHit> *log,
+                       float tstart = 0.0f) {
   RayPre r = prep_ray(o, d);
   std::vector<Hit> buf(op.k);
   float last = 0.0f;
-  float tlo = 0.0f;
+  float tlo = tstart;  // per-ray start distance (exact mode): the trace begins at key (tstart, -1)
   long long plo = -1;
   while (true) {
     gather(s, r, tlo, plo, op.k, op.far_t, op.bvh, buf.data());
@@ -1092,7 +1095,8 @@ int or_forward_f32(void *scene, const float *o, const float *d, int R, const or_
       if (env)
         trace_segment_f32(s, op, so, sd, Cs, *Ts, N, consumed, blended, nullptr, nullptr);
       else
-        trace_segment_f32(s, op, so, sd, C, T, N, consumed, blended, op.variant == 0 ? &sk : nullptr, nullptr);
+        trace_segment_f32(s, op, so, sd, C, T, N, consumed, blended, op.variant == 0 ? &sk : nullptr, nullptr,
+                          op.tstart ? op.tstart[pix] : 0.0f);
     };
     pixel_segments(op, pix, ro, rd, seg, C, T);
     for (int c = 0; c < 3; c++) color[(size_t)c * R + pix] = C[c] + T * op.bg[c];
@@ -1238,7 +1242,8 @@ void *or_record(void *scene, const float *o, const float *d, int R, const or_opt
       sg.d = sd;
       sg.weight = wgt;
       float Cl[3] = {0, 0, 0}, Tl = 1.0f, Nl[3] = {0, 0, 0};
-      trace_segment_f32(s, op, so, sd, Cl, Tl, Nl, consumed, blended, nullptr, &sg.hits);
+      trace_segment_f32(s, op, so, sd, Cl, Tl, Nl, consumed, blended, nullptr, &sg.hits,
+                        (!env && op.tstart) ? op.tstart[pix] : 0.0f);
       if (env) {
         for (int c = 0; c < 3; c++) Cs[c] = Cl[c];
         *Ts = Tl;

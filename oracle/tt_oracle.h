@@ -47,6 +47,8 @@ typedef struct {
   float bg[3];
   const float *extra; /* variant parameters: r, cx, cy, cz[, samples] */
   int extra_len;      /* number of floats in extra */
+  const float *tstart; /* optional [R] (forward variant, exact mode; or_forward_f32 and or_record
+                          only): each ray's trace starts at key (tstart[ray], -1) instead of 0 */
 } or_opts;
 
 /* ---- scene ---- */

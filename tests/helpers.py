@@ -38,7 +38,7 @@ class GpuScene:
             self.h.build(self.verts, self.cumsum, self.P, capi.BUILD)
 
     def forward(self, ray_o, ray_d, k=16, cutoff=1e-3, chunk_mode="compat", variant="forward", extra=None,
-                bg=(0.0, 0.0, 0.0), log_cap=0, stats=True):
+                bg=(0.0, 0.0, 0.0), log_cap=0, stats=True, interval_in=None, interval_out=False):
         from diff_tritracer import capi
 
         torch = self.torch
@@ -59,9 +59,11 @@ class GpuScene:
             log = (torch.full((R, log_cap), -1, dtype=torch.int32, device=dev), torch.zeros(R, log_cap, device=dev),
                    torch.full((R, log_cap), -1, dtype=torch.int32, device=dev),
                    torch.zeros(R, dtype=torch.int32, device=dev))
+        ivi = to_dev(np.asarray(interval_in, np.float32), dev) if interval_in is not None else None
+        ivo = torch.full((R, 4), -7.0, device=dev) if interval_out else None
         a = capi.fwd_args(W, H, ro, rd, self.verts, self.npts, self.cumsum, self.shs, 3, self.opac, self.sigma, bgt,
                           color, others, maxb, dens, scal, k=k, cutoff=cutoff, chunk_mode=chunk_mode,
-                          variant=variant, extra=ext, stats=st, log=log)
+                          variant=variant, extra=ext, stats=st, log=log, interval_in=ivi, interval_out=ivo)
         self.h.fwd(a, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         out = dict(color=color.cpu().numpy(), others=others.cpu().numpy(), max_blending=maxb.cpu().numpy(),
@@ -70,9 +72,12 @@ class GpuScene:
             out["stats"] = st.cpu().numpy()
         if log is not None:
             out["log"] = tuple(x.cpu().numpy() for x in log)
+        if ivo is not None:
+            out["interval"] = ivo.cpu().numpy()
         return out
 
-    def backward(self, ray_o, ray_d, color, others, dL_dpix, dL_dothers, k=16, cutoff=1e-3, chunk_mode="compat"):
+    def backward(self, ray_o, ray_d, color, others, dL_dpix, dL_dothers, k=16, cutoff=1e-3, chunk_mode="compat",
+                 interval=None):
         from diff_tritracer import capi
 
         torch = self.torch
@@ -89,7 +94,8 @@ class GpuScene:
         dsh = torch.empty(self.P, self.shs.shape[1], 3, device=dev)
         a = capi.bwd_args(W, H, ro, rd, self.verts, self.npts, self.cumsum, self.shs, 3, self.opac, self.sigma, col,
                           oth, gp, go, dverts, dsig, dop, dsh, k=k, cutoff=cutoff, chunk_mode=chunk_mode,
-                          zero_grads=True)
+                          zero_grads=True,
+                          interval=to_dev(np.asarray(interval, np.float32), dev) if interval is not None else None)
         self.h.bwd(a, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         return dict(dverts=dverts.cpu().numpy(), dsigma=dsig.cpu().numpy(), dopac=dop.cpu().numpy(),

@@ -79,6 +79,32 @@ def test_stacked_exact_mode_is_plain_front_to_back():
         assert out["stats"][0, 0] == P and out["stats"][0, 1] == n
 
 
+def test_stacked_start_distance_drops_the_planes_before_it():
+    # a per-ray start distance (exact mode) composites only the planes at t >= start
+    P = 40
+    zs = 1.0 + 0.1 * np.arange(P)
+    rng = np.random.Generator(np.random.PCG64(2))
+    opac = rng.uniform(0.02, 0.08, P)
+    sh0 = rng.normal(0, 0.5, (P, 3))
+    S, inc = stacked_scene(zs, opac, sh0)
+    o = np.array([[inc[0], inc[1], 0.0]], np.float32)
+    d = np.array([[0.0, 0.0, 1.0]], np.float32)
+    prims, ts = O.gather(S, o[0], d[0], k=P)
+    assert list(prims) == list(range(P))
+    t5 = float(ts[5])  # the start key is inclusive: t == start is kept, the next float is not
+    cases = ((0, 0.0), (0, 0.9), (5, 1.45), (5, t5), (6, float(np.nextafter(np.float32(t5), np.float32(9)))),
+             (17, 2.65), (39, 4.85), (40, 6.0))
+    for first, start in cases:
+        for k in (8, 16):
+            opts = O.make_opts(k=k, cutoff=1e-3, chunk_mode="exact", tstart=[start])
+            out = O.forward_f32(S, o, d, opts)
+            c64, _ = O.Log(S, o, d, opts).forward_f64()
+            C, T, n = closed_form(range(first, P), opac, sh0, 1e-3)
+            np.testing.assert_allclose(out["color"][:, 0], C, atol=2e-6)
+            np.testing.assert_allclose(c64[:, 0], C, atol=1e-9)
+            assert out["stats"][0, 0] == P - first and out["stats"][0, 1] == n
+
+
 def test_stacked_compat_mode_repeats_chunk_boundary_hit():
     # resuming at t >= last_hit_t re-reports the last hit of every full chunk
     P = 40
