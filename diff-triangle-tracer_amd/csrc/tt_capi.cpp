@@ -30,7 +30,13 @@ struct tt_tracer {
 
 namespace {
 thread_local std::string g_err;
+}  // namespace
 
+void tt::set_last_error(const std::string &msg) {
+  g_err = msg;
+}
+
+namespace {
 int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;

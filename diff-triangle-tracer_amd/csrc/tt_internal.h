@@ -107,6 +107,9 @@ std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, 
                          int grid, bool backward, hipStream_t stream);
 std::string launch_fill_int(int *ptr, int n, int v, hipStream_t stream);
 
+// Message returned by tt_last_error() on this thread (shared by every entry point).
+void set_last_error(const std::string &msg);
+
 // Lanes a trace launch uses for an image of W x H (the stack spill needs one slot each).
 size_t trace_lanes(int W, int H);
 constexpr int kLdsStack = 16;  // traversal stack entries per lane kept in LDS; deeper ones spill

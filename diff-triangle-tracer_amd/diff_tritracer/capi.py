@@ -22,7 +22,9 @@ CHUNK = {"compat": 0, "exact": 1}
 
 EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build", "tt_trace_fwd", "tt_trace_bwd",
             "tt_num_primitives", "tt_last_build_ms", "tt_multi_forward", "tt_multi_backward", "tt_multi_capture",
-            "tt_multi_replay", "tt_multi_autotune"]
+            "tt_multi_replay", "tt_multi_autotune", "tt_proposal_default_hparams", "tt_proposal_num_params",
+            "tt_proposal_create", "tt_proposal_destroy", "tt_proposal_set_params", "tt_proposal_get_params",
+            "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train"]
 MAX_CAMERAS = 32
 
 _vp = C.c_void_p
@@ -82,6 +84,13 @@ class MultiArgs(C.Structure):
     ]
 
 
+class ProposalHparams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("enc_scale", "t_scale", "t_skip", "tau_start", "tau_end", "lr", "beta1",
+                                        "beta2", "eps")]
+
+
+PROPOSAL_RAW, PROPOSAL_INTERVAL = 0, 1
+
 _lib = None
 
 
@@ -108,6 +117,15 @@ def lib():
         L.tt_multi_replay.argtypes = [_vp, _vp]
         L.tt_multi_autotune.argtypes = [_vp, C.POINTER(MultiArgs), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                         C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_float), _vp]
+        L.tt_proposal_default_hparams.argtypes = [C.POINTER(ProposalHparams)]
+        L.tt_proposal_default_hparams.restype = None
+        L.tt_proposal_create.argtypes = [C.POINTER(_vp), C.POINTER(ProposalHparams), C.c_ulonglong]
+        L.tt_proposal_destroy.argtypes = [_vp]
+        L.tt_proposal_set_params.argtypes = [_vp, C.POINTER(C.c_float)]
+        L.tt_proposal_get_params.argtypes = [_vp, C.POINTER(C.c_float)]
+        L.tt_proposal_forward.argtypes = [_vp, _fp, _fp, C.c_longlong, C.c_float, C.c_float, C.c_int, _fp, _vp]
+        L.tt_proposal_loss_grad.argtypes = [_vp, _fp, _fp, _fp, C.c_int, _fp, _fp, _vp]
+        L.tt_proposal_train.argtypes = [_vp, _fp, _fp, _fp, C.c_longlong, C.c_int, C.c_int, _fp, _vp]
         _lib = L
     return _lib
 
@@ -126,6 +144,65 @@ def ptr(t) -> int | None:
     if t is None or t.numel() == 0:
         return None
     return t.data_ptr()
+
+
+def default_hparams(**kw) -> ProposalHparams:
+    hp = ProposalHparams()
+    lib().tt_proposal_default_hparams(C.byref(hp))
+    for k, v in kw.items():
+        setattr(hp, k, float(v))
+    return hp
+
+
+class Proposal:
+    """The learned depth-interval proposal (tt_proposal_* of the C ABI)."""
+
+    def __init__(self, seed=0, **hparams):
+        self.hp = default_hparams(**hparams)
+        self.h = _vp()
+        check(lib().tt_proposal_create(C.byref(self.h), C.byref(self.hp), int(seed)), "tt_proposal_create")
+        self.n_params = lib().tt_proposal_num_params()
+
+    def close(self):
+        if self.h:
+            lib().tt_proposal_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        import numpy as np
+
+        out = np.empty(self.n_params, np.float32)
+        check(lib().tt_proposal_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))), "tt_proposal_get_params")
+        return out
+
+    def set_params(self, params):
+        import numpy as np
+
+        a = np.ascontiguousarray(params, np.float32).reshape(-1)
+        if a.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {a.size}")
+        check(lib().tt_proposal_set_params(self.h, a.ctypes.data_as(C.POINTER(C.c_float))), "tt_proposal_set_params")
+
+    def forward(self, ray_o, ray_d, out, mode=PROPOSAL_RAW, margin_start=0.0, margin_end=0.0, stream=None):
+        R = ray_o.numel() // 3
+        check(lib().tt_proposal_forward(self.h, ptr(ray_o), ptr(ray_d), R, float(margin_start), float(margin_end),
+                                        int(mode), ptr(out), stream), "tt_proposal_forward")
+
+    def loss_grad(self, ray_o, ray_d, interval, grad, loss=None, stream=None):
+        B = ray_o.numel() // 3
+        check(lib().tt_proposal_loss_grad(self.h, ptr(ray_o), ptr(ray_d), ptr(interval), B, ptr(grad), ptr(loss),
+                                          stream), "tt_proposal_loss_grad")
+
+    def train(self, ray_o, ray_d, interval, batch, steps, loss_hist=None, stream=None):
+        n = ray_o.numel() // 3
+        check(lib().tt_proposal_train(self.h, ptr(ray_o), ptr(ray_d), ptr(interval), n, int(batch), int(steps),
+                                      ptr(loss_hist), stream), "tt_proposal_train")
 
 
 class TracerHandle:

@@ -206,6 +206,48 @@ int tt_multi_replay(tt_tracer *tr, void *stream);
 int tt_multi_autotune(tt_tracer *tr, tt_multi_args *a, const int *ks, int nk, const int *blocks, int nb,
                       const int *tpf, int nt, int reps, float *results_ms, void *stream);
 
+/* ---- learned depth-interval proposal (no counterpart in the reference) ----
+   A 102-64-64-64-2 MLP (bf16 weights and activations, fp32 accumulation, matrix cores)
+   mapping a ray to (start, end) of the depth range where its blended hits lie; trained
+   on the GPU with Adam on the intervals tt_trace_fwd emits (interval_out), and used to
+   fill tt_fwd_args.interval_in. Input encoding: (o / enc_scale, d) and sin, cos of
+   2^f pi v, f = 0..7, for each of those six values (|o| / enc_scale must stay below 64). */
+typedef struct tt_proposal tt_proposal;
+
+typedef struct {
+  float enc_scale;  /* ray origins are divided by this before encoding (8) */
+  float t_scale;    /* outputs are distances divided by this (8) */
+  float t_skip;     /* target distance of rays without blended hits (16: beyond the scene) */
+  float tau_start;  /* pinball-loss quantile of the start output (0.1: errs early) */
+  float tau_end;    /* pinball-loss quantile of the end output (0.9: errs late) */
+  float lr, beta1, beta2, eps; /* Adam (1e-3, 0.9, 0.999, 1e-8) */
+} tt_proposal_hparams;
+
+#define TT_PROPOSAL_RAW 0      /* outputs [R][2]: the network's two values */
+#define TT_PROPOSAL_INTERVAL 1 /* outputs [R][2]: start = max(0, (y0 - margin_start) t_scale),
+                                  cap = (y1 + margin_end) t_scale (tt_fwd_args.interval_in) */
+
+void tt_proposal_default_hparams(tt_proposal_hparams *hp);
+/* Parameters: W1[64][102] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64] b4[2] (fp32 master copy). */
+int tt_proposal_num_params(void);
+/* He-uniform weights from `seed`, zero biases, on the current device. hp NULL = defaults. */
+int tt_proposal_create(tt_proposal **out, const tt_proposal_hparams *hp, unsigned long long seed);
+int tt_proposal_destroy(tt_proposal *p);
+/* Host copies of the fp32 parameters (synchronise the device); set resets the Adam state. */
+int tt_proposal_set_params(tt_proposal *p, const float *host_params);
+int tt_proposal_get_params(tt_proposal *p, float *host_params);
+/* Network outputs for R rays (ray_o, ray_d [R][3]) into out [R][2] (mode TT_PROPOSAL_*). */
+int tt_proposal_forward(tt_proposal *p, const float *ray_o_dev, const float *ray_d_dev, long long R,
+                        float margin_start, float margin_end, int mode, float *out_dev, void *stream);
+/* Gradient of the mean pinball loss over the B rays given (B % 32 == 0) into grad [n_params],
+   and the loss into *loss_dev (optional). interval: [B][4] from tt_trace_fwd's interval_out. */
+int tt_proposal_loss_grad(tt_proposal *p, const float *ray_o_dev, const float *ray_d_dev, const float *interval_dev,
+                          int B, float *grad_dev, float *loss_dev, void *stream);
+/* `steps` Adam steps, each on `batch` rays (batch % 32 == 0) drawn by hash from the n_rays given;
+   loss_hist [steps] (optional) receives each step's loss. Asynchronous. */
+int tt_proposal_train(tt_proposal *p, const float *ray_o_dev, const float *ray_d_dev, const float *interval_dev,
+                      long long n_rays, int batch, int steps, float *loss_hist_dev, void *stream);
+
 /* Number of triangles of the current acceleration structure (-1 if none). */
 int tt_num_primitives(const tt_tracer *tr);
 
