@@ -1,22 +1,38 @@
 #!/usr/bin/env python
-"""Benchmark: plain fwd+bwd Mrays/s of the HIP triangle-splat tracer at 1080p on
-1M clustered triangle splats (k = 16 hits per chunk, transmittance cutoff 1e-3,
-random N(0,1) image-space loss gradient), on one MI355X.
+"""Benchmark of the HIP triangle-splat tracer on one MI355X.
 
-One step = one forward launch + one backward launch (gradient buffers zeroed
-inside the step) over the whole 1920x1080 frame, inputs resident in HBM. The
-LBVH build and refit are timed separately and reported beside the metric.
+Default workload (the headline): fwd+bwd Mrays/s at 1920x1080 on 1M clustered
+triangle splats (64 clusters, seed 1), k = 16 hits per chunk, transmittance
+cutoff 1e-3, random N(0,1) image-space loss gradient; exact chunk mode.
 
-Prints one JSON line (rank 0). Extra objects:
+  plain        : forward launch + gradient zeroing + re-traversing backward launch
+  proposal     : the learned depth-interval proposal (bf16 MLP on the matrix
+                 cores, trained first on the intervals the tracer emits for 16
+                 other camera poses of the same scene; the measured frame is held
+                 out) -> forward capped at the proposed end with exact continuation
+                 past it (same image as plain, bit for bit) -> backward bounded by
+                 the interval the forward emitted (same gradients)
+  proposal_hard: lossy variant in config: proposed start and a hard cap, with
+                 fallbacks to the full trace; its PSNR against plain is reported
+  value        : Mrays/s of the proposal step; plain and the ablations are in config
+
+One step = one pass over the frame with inputs resident in HBM. The LBVH build,
+refit and the proposal's training are timed separately and reported beside it.
+
+Prints one JSON line. Extra objects:
   roofline      : the dominant kernel's algorithmic bytes per launch / its average
                   launch time (HIP events on the launch stream) against 8 TB/s
-  cpu_baseline  : the scalar C++/OpenMP oracle (same algorithm, FP32) on a
-                  bounded ray sample of the same frame, timed on the host cores
+  cpu_baseline  : the scalar C++/OpenMP oracle (same algorithm, FP32) on the frame,
+                  timed on the host cores
+Other workloads: --workload multiview (2M splats, 8 uneven cameras, one persistent
+hipGraph launch, autotuned), --workload proposal (3M splats, 128 clusters: the
+learned-proposal configuration).
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -25,14 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "diff-triangle-tracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
-
-
-def algorithmic_bytes(R, Hc, Hb):
-    """Bytes per launch the algorithm must move (ray/pixel I/O + per-hit gathers and
-    gradient scatters), BVH node traffic excluded:
-      forward : 64 R + 48 Hc + 204 Hb     backward : 104 R + 48 Hc + 428 Hb
-    (Hc = consumed hits, Hb = blended hits, summed over the frame)."""
-    return 64 * R + 48 * Hc + 204 * Hb, 104 * R + 48 * Hc + 428 * Hb
+TRAIN_POSES, HELD_OUT_EXTRA = 16, 3
 
 
 def cpu_baseline(scene, ray_o, ray_d, stride, cutoff, k):
@@ -60,6 +69,8 @@ def cpu_baseline(scene, ray_o, ray_d, stride, cutoff, k):
                        f"{t_build:.2f} s excluded", seconds=round(t, 3))
 
 
+
+
 def multiview(args, dev):
     """Batched multi-view step: 2M clustered splats, 8 cameras of uneven resolution
     (9,004,032 rays) rendered fwd+bwd by the persistent tile-queue kernels captured in
@@ -70,7 +81,7 @@ def multiview(args, dev):
     from diff_tritracer import capi, scenes
 
     t0 = time.perf_counter()
-    P = args.triangles if args.triangles != 1_000_000 else 2_000_000
+    P = args.triangles
     scene, ws, hs, ray_o, ray_d = scenes.multiview_config(seed=2, P=P)
     Rt = ray_o.shape[0]
     print(f"[bench] multiview: {P} splats, {len(ws)} cameras, {Rt} rays, generated in "
@@ -127,6 +138,278 @@ def multiview(args, dev):
     print(json.dumps(line), flush=True)
 
 
+class Frame:
+    """Scene tensors, the tracer handle and per-frame buffers of one workload."""
+
+    def __init__(self, dev, scene, k, cutoff):
+        import torch
+        from diff_tritracer import capi
+
+        self.torch, self.capi, self.dev = torch, capi, dev
+        T = lambda a: torch.from_numpy(a).to(dev)
+        self.scene = scene
+        self.P = scene.P
+        self.verts, self.npts, self.cumsum = T(scene.verts), T(scene.npts), T(scene.cumsum)
+        self.shs, self.opac, self.sigma = T(scene.shs), T(scene.opac.reshape(-1)), T(scene.sigma.reshape(-1))
+        self.k, self.cutoff = k, cutoff
+        self.bg = torch.zeros(3, device=dev)
+        self.stream = torch.cuda.current_stream()
+        self.sp = self.stream.cuda_stream
+        self.h = capi.TracerHandle()
+
+    def buffers(self, W, H):
+        torch, dev, P = self.torch, self.dev, self.P
+        R = W * H
+        self.W, self.H, self.R = W, H, R
+        self.ro = torch.empty(H, W, 3, device=dev)
+        self.rd = torch.empty(H, W, 3, device=dev)
+        self.color, self.others = torch.zeros(3, R, device=dev), torch.zeros(7, R, device=dev)
+        self.maxb, self.dens, self.scal = (torch.zeros(P, device=dev) for _ in range(3))
+        g = torch.Generator(device="cpu").manual_seed(5)
+        self.dpix = torch.randn(3, R, generator=g).to(dev)
+        self.doth = torch.zeros(7, R, device=dev)
+        self.dverts = torch.zeros(self.verts.shape[0], 3, device=dev)
+        self.dsig, self.dop = torch.zeros(P, device=dev), torch.zeros(P, device=dev)
+        self.dsh = torch.zeros(P, self.shs.shape[1], 3, device=dev)
+        self.ival = torch.zeros(R, 4, device=dev)
+        self.prop = torch.zeros(R, 2, device=dev)
+        self.prop2 = torch.zeros(R, 2, device=dev)
+
+    def camera(self, basis):
+        self.capi.camera_rays(basis, self.W, self.H, self.ro, self.rd, self.sp)
+
+    def fwd_args(self, mode="exact", interval_in=None, interval_out=None, hard=False, stats=None, trav=None,
+                 fb_start=0.0, fb_end=0.0):
+        c = self.capi
+        return c.fwd_args(self.W, self.H, self.ro, self.rd, self.verts, self.npts, self.cumsum, self.shs, 3, self.opac,
+                          self.sigma, self.bg, self.color, self.others, self.maxb, self.dens, self.scal, k=self.k,
+                          cutoff=self.cutoff, chunk_mode=mode, interval_in=interval_in, interval_out=interval_out,
+                          hard_cap=hard, fallback_start=fb_start, fallback_end=fb_end, stats=stats,
+                          trav_stats=trav)
+
+    def bwd_args(self, mode="exact", interval=None):
+        return self.capi.bwd_args(self.W, self.H, self.ro, self.rd, self.verts, self.npts, self.cumsum, self.shs, 3,
+                                  self.opac, self.sigma, self.color, self.others, self.dpix, self.doth, self.dverts,
+                                  self.dsig, self.dop, self.dsh, k=self.k, cutoff=self.cutoff, chunk_mode=mode,
+                                  zero_grads=False, interval=interval)
+
+    def zero_grads(self):
+        for t in (self.dverts, self.dsig, self.dop, self.dsh):
+            t.zero_()
+
+
+def timed(fr, launches, steps, warmup):
+    """Run `launches` (list of (name, fn)) as one step: warmup, then `steps` timed steps
+    with HIP events around every launch on the launch stream. Returns (ms/step, {name: mean ms})."""
+    torch = fr.torch
+    for _ in range(warmup):
+        for _, fn in launches:
+            fn()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(launches) + 1)] for _ in range(steps)]
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(fr.stream)
+    for i in range(steps):
+        evs[i][0].record(fr.stream)
+        for j, (_, fn) in enumerate(launches):
+            fn()
+            evs[i][j + 1].record(fr.stream)
+    stop.record(fr.stream)
+    torch.cuda.synchronize()
+    per = {name: sum(e[j].elapsed_time(e[j + 1]) for e in evs) / steps for j, (name, _) in enumerate(launches)}
+    return start.elapsed_time(stop) / steps, per
+
+
+def psnr(a, b):
+    mse = float(((a - b) ** 2).mean())
+    return float("inf") if mse == 0.0 else 10.0 * math.log10(1.0 / mse)
+
+
+def train_proposal(fr, poses, W, H, steps, batch, seed):
+    """Emit intervals with the plain exact forward for each training pose (rays made on
+    the GPU), train the proposal on them, calibrate the margins on the training rays."""
+    import numpy as np
+    from diff_tritracer import scenes
+
+    torch, capi = fr.torch, fr.capi
+    n = len(poses)
+    R = W * H
+    O = torch.empty(n, H, W, 3, device=fr.dev)
+    D = torch.empty(n, H, W, 3, device=fr.dev)
+    IV = torch.empty(n, R, 4, device=fr.dev)
+    fr.buffers(W, H)
+    t0 = time.perf_counter()
+    for i, eye in enumerate(poses):
+        b = scenes.camera_basis(W, H, eye)
+        capi.camera_rays(b, W, H, O[i], D[i], fr.sp)
+        fr.ro, fr.rd = O[i], D[i]
+        fr.h.fwd(fr.fwd_args("exact", interval_out=IV[i]), fr.sp)
+    torch.cuda.synchronize()
+    t_emit = time.perf_counter() - t0
+    net = capi.Proposal(seed=seed)
+    hist = torch.zeros(steps, device=fr.dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(fr.stream)
+    net.train(O, D, IV, batch=batch, steps=steps, loss_hist=hist, stream=fr.sp)
+    e1.record(fr.stream)
+    torch.cuda.synchronize()
+    t_train = e0.elapsed_time(e1)
+    # margins: smallest widening that covers 99.5% of the training rays' blended ranges
+    y = torch.empty(n * R, 2, device=fr.dev)
+    net.forward(O, D, y, capi.PROPOSAL_RAW, stream=fr.sp)
+    iv = IV.reshape(-1, 4)
+    has = torch.isfinite(iv[:, 0])
+    stop = torch.where(torch.isfinite(iv[:, 2]), iv[:, 2], iv[:, 1])
+    ts = float(net.hp.t_scale)
+    over_s = (y[:, 0] - iv[:, 0] / ts)[has]
+    under_e = (stop / ts - y[:, 1])[has]
+    q = 0.995
+    ms = max(0.0, float(torch.quantile(over_s[:: max(1, over_s.numel() // 4_000_000)].float(), q)))
+    me = max(0.0, float(torch.quantile(under_e[:: max(1, under_e.numel() // 4_000_000)].float(), q)))
+    h = hist.cpu().numpy()
+    info = dict(train_poses=n, train_rays=n * R, train_steps=steps, batch=batch, emit_s=round(t_emit, 3),
+                train_ms=round(t_train, 2), loss_first=round(float(h[:20].mean()), 5),
+                loss_last=round(float(h[-100:].mean()), 5), margin_start=round(ms, 5), margin_end=round(me, 5))
+    del O, D, IV, y
+    return net, ms, me, info
+
+
+def frame_benchmark(args, dev, scene, cam_eye, label):
+    import numpy as np
+    import torch
+    from diff_tritracer import capi, scenes
+
+    W, H = args.width, args.height
+    fr = Frame(dev, scene, args.k, args.cutoff)
+    P = fr.P
+    # LBVH build and refit, timed separately (median of 5 after one warm-up)
+    builds, refits = [], []
+    for _ in range(6):
+        fr.h.build(fr.verts, fr.cumsum, P, capi.BUILD, fr.sp)
+        builds.append(fr.h.last_build_ms())
+        fr.h.build(fr.verts, fr.cumsum, P, capi.REFIT, fr.sp)
+        refits.append(fr.h.last_build_ms())
+    build_ms, refit_ms = float(np.median(builds[1:])), float(np.median(refits[1:]))
+    print(f"[bench] LBVH build {build_ms:.3f} ms, refit {refit_ms:.3f} ms", file=sys.stderr, flush=True)
+
+    # the proposal: 16 training poses on the sphere of radius 3.5-4.5 around the scene
+    rng = np.random.Generator(np.random.PCG64(101))
+    poses = [scenes.orbit_eye(float(rng.uniform(3.5, 4.5)), 101, i) for i in range(TRAIN_POSES)]
+    net, ms, me, pinfo = train_proposal(fr, poses, W, H, args.proposal_steps, args.proposal_batch, seed=3)
+    print(f"[bench] proposal: {pinfo}", file=sys.stderr, flush=True)
+
+    fr.buffers(W, H)
+    fr.camera(scenes.camera_basis(W, H, cam_eye))
+    R = fr.R
+    # counting pass (plain, exact) for the algorithmic bytes
+    stats = torch.zeros(R, 2, dtype=torch.int32, device=dev)
+    trav = torch.zeros(R, 2, dtype=torch.int32, device=dev)
+    fr.h.fwd(fr.fwd_args("exact", stats=stats, trav=trav), fr.sp)
+    torch.cuda.synchronize()
+    st = stats.to(torch.int64).sum(0).cpu().numpy()
+    Hc, Hb = int(st[0]), int(st[1])
+    tv_plain = trav.to(torch.int64).sum(0).cpu().numpy()
+    print(f"[bench] hits consumed {Hc / R:.2f}/ray, blended {Hb / R:.2f}/ray; BVH nodes visited "
+          f"{tv_plain[0] / R:.1f}/ray, triangles tested {tv_plain[1] / R:.1f}/ray", file=sys.stderr, flush=True)
+
+    sp = fr.sp
+    fa_plain = fr.fwd_args("exact")
+    ba_plain = fr.bwd_args("exact")
+    fa_compat = fr.fwd_args("compat")
+    ba_compat = fr.bwd_args("compat")
+    fa_emit = fr.fwd_args("exact", interval_out=fr.ival)
+    fa_soft = fr.fwd_args("exact", interval_in=fr.prop, interval_out=fr.ival)
+    ts = float(net.hp.t_scale)
+    fa_hard = fr.fwd_args("exact", interval_in=fr.prop2, interval_out=fr.ival, hard=True, fb_start=ms * ts,
+                          fb_end=me * ts)
+    ba_bound = fr.bwd_args("exact", interval=fr.ival)
+    # exact: proposed cap only (start 0, continuation past the cap): the image equals plain
+    propose = lambda: net.forward(fr.ro, fr.rd, fr.prop, capi.PROPOSAL_CAP, ms, me, stream=sp)
+    # lossy: proposed start and hard cap, with the fallbacks
+    propose2 = lambda: net.forward(fr.ro, fr.rd, fr.prop2, capi.PROPOSAL_INTERVAL, ms, me, stream=sp)
+
+    runs = {
+        "plain": [("fwd", lambda: fr.h.fwd(fa_plain, sp)), ("zero", fr.zero_grads), ("bwd", lambda: fr.h.bwd(ba_plain, sp))],
+        "plain_compat": [("fwd", lambda: fr.h.fwd(fa_compat, sp)), ("zero", fr.zero_grads),
+                         ("bwd", lambda: fr.h.bwd(ba_compat, sp))],
+        "handoff_only": [("fwd", lambda: fr.h.fwd(fa_emit, sp)), ("zero", fr.zero_grads),
+                         ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
+        "proposal": [("mlp", propose), ("fwd", lambda: fr.h.fwd(fa_soft, sp)), ("zero", fr.zero_grads),
+                     ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
+        "proposal_hard": [("mlp", propose2), ("fwd", lambda: fr.h.fwd(fa_hard, sp)), ("zero", fr.zero_grads),
+                          ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
+    }
+    res, images = {}, {}
+    for name, launches in runs.items():
+        ms_step, per = timed(fr, launches, args.steps, args.warmup)
+        res[name] = dict(ms=ms_step, per=per)
+        images[name] = fr.color.clone()
+        print(f"[bench] {name:14s} {ms_step:8.3f} ms/step " + " ".join(f"{k} {v:.3f}" for k, v in per.items()) +
+              f"  ({R / ms_step / 1e3:.2f} Mrays/s)", file=sys.stderr, flush=True)
+    ref_img = images["plain"].cpu().numpy()
+    exact_equal = bool(np.array_equal(images["proposal"].cpu().numpy().view(np.uint32), ref_img.view(np.uint32)))
+    hard_psnr = psnr(images["proposal_hard"].cpu().numpy(), ref_img)
+    # node visits of the proposal-bounded forward (the network's gain in traversal work)
+    propose()
+    fr.h.fwd(fr.fwd_args("exact", interval_in=fr.prop, stats=stats, trav=trav), sp)
+    torch.cuda.synchronize()
+    tv_prop = trav.to(torch.int64).sum(0).cpu().numpy()
+    # extra held-out poses: equality and hard-mode PSNR
+    rng2 = np.random.Generator(np.random.PCG64(202))
+    held = []
+    for i in range(HELD_OUT_EXTRA):
+        eye = scenes.orbit_eye(float(rng2.uniform(3.5, 4.5)), 202, i)
+        fr.camera(scenes.camera_basis(W, H, eye))
+        fr.h.fwd(fa_plain, sp)
+        a = fr.color.cpu().numpy()
+        propose()
+        fr.h.fwd(fa_soft, sp)
+        b = fr.color.cpu().numpy()
+        propose2()
+        fr.h.fwd(fa_hard, sp)
+        c = fr.color.cpu().numpy()
+        held.append(dict(equal=bool(np.array_equal(a.view(np.uint32), b.view(np.uint32))), hard_psnr_db=round(psnr(c, a), 2)))
+    fr.camera(scenes.camera_basis(W, H, cam_eye))
+
+    mr = lambda name: R / (res[name]["ms"] * 1e-3) / 1e6
+    bf = 64 * R + 48 * Hc + 204 * Hb + 8 * R + 16 * R  # + proposed interval read, emitted interval written
+    bb = 104 * R + 48 * Hc + 428 * Hb + 16 * R         # + emitted interval read
+    per = res["proposal"]["per"]
+    if per["bwd"] >= per["fwd"]:
+        dom, dom_bytes, dom_ms = f"k_backward<{args.k}>", bb, per["bwd"]
+    else:
+        dom, dom_bytes, dom_ms = f"k_forward<{args.k},0>", bf, per["fwd"]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=dom,
+                    bytes_per_launch=int(dom_bytes), avg_launch_ms=round(dom_ms, 4),
+                    step_frac=round((bf + bb) / ((per["fwd"] + per["bwd"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5))
+    mlp_flops = 2 * 14848 * R
+    cfg = {
+        "workload": f"{label}: fwd+bwd, {P} clustered triangle splats, one {W}x{H} camera (held out from the "
+                    f"proposal's {TRAIN_POSES} training poses), k={args.k}, cutoff {args.cutoff:g}, exact chunk mode, "
+                    f"N(0,1) loss gradient",
+        "triangles": P, "width": W, "height": H, "k": args.k, "cutoff": args.cutoff, "chunk_mode": "exact",
+        "plain_mrays": round(mr("plain"), 3), "plain_compat_mrays": round(mr("plain_compat"), 3),
+        "handoff_only_mrays": round(mr("handoff_only"), 3), "proposal_mrays": round(mr("proposal"), 3),
+        "proposal_hard_mrays": round(mr("proposal_hard"), 3),
+        "speedup_vs_plain": round(res["plain"]["ms"] / res["proposal"]["ms"], 4),
+        "speedup_hard_vs_plain": round(res["plain"]["ms"] / res["proposal_hard"]["ms"], 4),
+        "proposal_image_equals_plain": exact_equal, "psnr_loss_db": 0.0 if exact_equal else None,
+        "hard_psnr_vs_plain_db": round(hard_psnr, 2), "held_out_extra": held,
+        "ms": {k: round(v["ms"], 4) for k, v in res.items()},
+        "kernel_ms": {k: {n: round(x, 4) for n, x in v["per"].items()} for k, v in res.items()},
+        "mlp_tflops": round(mlp_flops / (per["mlp"] * 1e-3) / 1e12, 2),
+        "nodes_visited_per_ray_plain": round(float(tv_plain[0]) / R, 2),
+        "nodes_visited_per_ray_proposal": round(float(tv_prop[0]) / R, 2),
+        "hits_consumed_per_ray": round(Hc / R, 3), "hits_blended_per_ray": round(Hb / R, 3),
+        "lbvh_build_ms": round(build_ms, 3), "lbvh_refit_ms": round(refit_ms, 3), "proposal_training": pinfo,
+    }
+    ms_step = res["proposal"]["ms"]
+    out = dict(value=round(mr("proposal"), 4), ms_per_step=round(ms_step, 4), config=cfg, roofline=roofline)
+    return out, fr
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,14 +417,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--cutoff", type=float, default=1e-3)
-    ap.add_argument("--triangles", type=int, default=1_000_000)
+    ap.add_argument("--triangles", type=int, default=0, help="default: 1M (frame), 2M (multiview), 3M (proposal)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--proposal-steps", type=int, default=2000)
+    ap.add_argument("--proposal-batch", type=int, default=16384)
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "multiview"], default="frame",
-                    help="frame: the 1080p / 1M-splat headline; multiview: 2M splats, 8 uneven cameras in one "
-                         "persistent hipGraph launch, autotuned")
+    ap.add_argument("--workload", choices=["frame", "multiview", "proposal"], default="frame")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates rocprof traffic counters)")
     args = ap.parse_args()
@@ -153,7 +436,7 @@ def main():
 
     import numpy as np
     import torch
-    from diff_tritracer import capi, scenes
+    from diff_tritracer import scenes
 
     dev = torch.device("cuda:0")
     if args.copy_mib:
@@ -166,148 +449,44 @@ def main():
         print(json.dumps({"copy_bytes": 4 * n, "launches": 3}))
         return
     if args.workload == "multiview":
+        args.triangles = args.triangles or 2_000_000
         return multiview(args, dev)
     t0 = time.perf_counter()
-    scene, ray_o, ray_d = scenes.scale_config(seed=1, P=args.triangles, width=args.width, height=args.height)
-    gen_s = time.perf_counter() - t0
-    H, W = ray_o.shape[:2]
-    R = H * W
-    P = scene.P
-    print(f"[bench] scene {P} splats, {W}x{H} rays, generated in {gen_s:.1f} s", file=sys.stderr, flush=True)
-
-    T = lambda a, dt=None: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    verts, npts, cumsum = T(scene.verts), T(scene.npts), T(scene.cumsum)
-    shs, opac, sigma = T(scene.shs), T(scene.opac.reshape(-1)), T(scene.sigma.reshape(-1))
-    ro, rd = T(ray_o), T(ray_d)
-    bg = torch.zeros(3, device=dev)
-    color = torch.zeros(3, R, device=dev)
-    others = torch.zeros(7, R, device=dev)
-    maxb = torch.zeros(P, device=dev)
-    dens = torch.zeros(P, device=dev)
-    scal = torch.zeros(P, device=dev)
-    stats = torch.zeros(R, 2, dtype=torch.int32, device=dev)
-    trav = torch.zeros(R, 2, dtype=torch.int32, device=dev)
-    g = torch.Generator(device="cpu").manual_seed(5)
-    dpix = torch.randn(3, R, generator=g).to(dev)
-    doth = torch.zeros(7, R, device=dev)
-    dverts = torch.zeros(verts.shape[0], 3, device=dev)
-    dsig = torch.zeros(P, device=dev)
-    dop = torch.zeros(P, device=dev)
-    dsh = torch.zeros(P, shs.shape[1], 3, device=dev)
-
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
-    h = capi.TracerHandle()
-    # LBVH build and refit, timed separately (median of 5)
-    builds, refits = [], []
-    for i in range(6):
-        h.build(verts, cumsum, P, capi.BUILD, sp)
-        builds.append(h.last_build_ms())
-        h.build(verts, cumsum, P, capi.REFIT, sp)
-        refits.append(h.last_build_ms())
-    build_ms = float(np.median(builds[1:]))
-    refit_ms = float(np.median(refits[1:]))
-    print(f"[bench] LBVH build {build_ms:.3f} ms, refit {refit_ms:.3f} ms", file=sys.stderr, flush=True)
-
-    fa = capi.fwd_args(W, H, ro, rd, verts, npts, cumsum, shs, 3, opac, sigma, bg, color, others, maxb, dens, scal,
-                       k=args.k, cutoff=args.cutoff)
-    fa_stats = capi.fwd_args(W, H, ro, rd, verts, npts, cumsum, shs, 3, opac, sigma, bg, color, others, maxb, dens,
-                             scal, k=args.k, cutoff=args.cutoff, stats=stats, trav_stats=trav)
-    ba = capi.bwd_args(W, H, ro, rd, verts, npts, cumsum, shs, 3, opac, sigma, color, others, dpix, doth, dverts,
-                       dsig, dop, dsh, k=args.k, cutoff=args.cutoff, zero_grads=False)
-    grads = (dverts, dsig, dop, dsh)
-
-    # counting pass for the algorithmic bytes
-    h.fwd(fa_stats, sp)
-    torch.cuda.synchronize()
-    st = stats.to(torch.int64).sum(0).cpu().numpy()
-    Hc, Hb = int(st[0]), int(st[1])
-    tv = trav.to(torch.int64).sum(0).cpu().numpy()
-    print(f"[bench] hits consumed {Hc / R:.2f}/ray, blended {Hb / R:.2f}/ray; BVH nodes visited {tv[0] / R:.1f}/ray, "
-          f"triangles tested {tv[1] / R:.1f}/ray (forward)", file=sys.stderr, flush=True)
-
-    def step(ev=None):
-        # forward, gradient zeroing (part of the step: the reference allocates zeroed
-        # gradients every call), backward; events bracket each kernel on the launch stream
-        if ev is not None:
-            ev[0].record(stream)
-        h.fwd(fa, sp)
-        if ev is not None:
-            ev[1].record(stream)
-        for t in grads:
-            t.zero_()
-        if ev is not None:
-            ev[2].record(stream)
-        h.bwd(ba, sp)
-        if ev is not None:
-            ev[3].record(stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter()
-    start.record(stream)
-    for i in range(args.steps):
-        step(evs[i])
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_wall
-    total_ms = start.elapsed_time(stop)
-    fwd_ms = [e[0].elapsed_time(e[1]) for e in evs]
-    zero_ms = [e[1].elapsed_time(e[2]) for e in evs]
-    bwd_ms = [e[2].elapsed_time(e[3]) for e in evs]
-    ms_per_step = total_ms / args.steps
-    mrays = R / (ms_per_step * 1e-3) / 1e6
-    print(f"[bench] {ms_per_step:.3f} ms/step (fwd {np.mean(fwd_ms):.3f}, zero {np.mean(zero_ms):.3f}, "
-          f"bwd {np.mean(bwd_ms):.3f}); "
-          f"wall {wall:.2f} s", file=sys.stderr, flush=True)
-
-    bf, bb = algorithmic_bytes(R, Hc, Hb)
-    f_avg, b_avg = float(np.mean(fwd_ms)), float(np.mean(bwd_ms))
-    if b_avg >= f_avg:
-        dom, dom_bytes, dom_ms = f"k_backward<{args.k}>", bb, b_avg
+    if args.workload == "proposal":
+        P = args.triangles or 3_000_000
+        scene = scenes.clustered_scene(P, 128, seed=4)
+        label = "learned-proposal configuration (128 clusters, seed 4)"
     else:
-        dom, dom_bytes, dom_ms = f"k_forward<{args.k},0>", bf, f_avg
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=dom,
-                    bytes_per_launch=int(dom_bytes), avg_launch_ms=round(dom_ms, 4),
-                    fwd_bwd_frac=round((bf + bb) / ((f_avg + b_avg) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5))
+        P = args.triangles or 1_000_000
+        scene = scenes.clustered_scene(P, 64, seed=1)
+        label = "1080p frame (64 clusters, seed 1)"
+    eye = 4.0 * np.array([0.35, 0.3, 1.0]) / np.linalg.norm([0.35, 0.3, 1.0])
+    print(f"[bench] scene {P} splats generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    out, fr = frame_benchmark(args, dev, scene, eye, label)
 
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
+        ray_o, ray_d = scenes.look_at_rays(args.width, args.height, eye)
         cpu = cpu_baseline(scene, ray_o, ray_d, args.cpu_stride, args.cutoff, args.k)
         print(f"[bench] cpu baseline {cpu['value']} Mrays/s on {cpu['cores']} threads in {cpu['seconds']} s",
               file=sys.stderr, flush=True)
 
     line = {
-        "metric": "fwd+bwd Mrays/s @1080p, 1M tris (plain tracer; proposal-culled path not built yet)",
-        "value": round(mrays, 4),
+        "metric": "fwd+bwd Mrays/s @1080p, 1M tris; proposal-culled vs plain at equal PSNR",
+        "value": out["value"],
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": out["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded clustered triangle splats, pinhole camera rays)",
-        "config": {
-            "workload": f"plain fwd+bwd, {P} clustered triangle splats (64 clusters, seed 1), one {W}x{H} camera, "
-                        f"k={args.k}, cutoff {args.cutoff:g}, N(0,1) loss gradient",
-            "triangles": P, "width": W, "height": H, "k": args.k, "cutoff": args.cutoff, "chunk_mode": "compat",
-            "lbvh_build_ms": round(build_ms, 3), "lbvh_refit_ms": round(refit_ms, 3),
-            "fwd_ms": round(f_avg, 4), "bwd_ms": round(b_avg, 4), "zero_grads_ms": round(float(np.mean(zero_ms)), 4),
-            "hits_consumed_per_ray": round(Hc / R, 3), "hits_blended_per_ray": round(Hb / R, 3),
-            "nodes_visited_per_ray": round(float(tv[0]) / R, 2), "triangles_tested_per_ray": round(float(tv[1]) / R, 2),
-        },
-        "roofline": roofline,
+        "config": out["config"],
+        "roofline": out["roofline"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -254,6 +254,8 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.ival_in = reinterpret_cast<const float2 *>(a->interval_in_dev);
   p.ival_out = reinterpret_cast<float4 *>(a->interval_out_dev);
   p.ival_hard = a->interval_in_dev && a->interval_hard_cap ? 1 : 0;
+  p.ival_fb_start = a->interval_fallback_start;
+  p.ival_fb_end = a->interval_fallback_end;
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
   if (!err.empty()) return fail(TT_ERR_HIP, err);

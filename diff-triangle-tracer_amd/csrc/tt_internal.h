@@ -85,6 +85,7 @@ struct TraceParams {
   // backward input = the forward's output
   const float2 *ival_in;
   int ival_hard;  // 1: hits past the cap are dropped (the proposal is trusted); 0: exact continuation
+  float ival_fb_start, ival_fb_end;  // hard mode fallbacks (distances), see trace_segment
   float4 *ival_out;
   const float4 *ival_bwd;
 };

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(64) void k_prop_infer(const float *ro, const float 
           out[2 * ray] = y[0][i];
           out[2 * ray + 1] = other;
         } else {
-          out[2 * ray] = fmaxf(0.0f, (y[0][i] - ms) * hp.t_scale);
+          out[2 * ray] = mode == 2 ? 0.0f : fmaxf(0.0f, (y[0][i] - ms) * hp.t_scale);
           out[2 * ray + 1] = (other + me) * hp.t_scale;
         }
       }
@@ -542,7 +542,7 @@ int tt_proposal_forward(tt_proposal *p, const float *ray_o_dev, const float *ray
                         float margin_end, int mode, float *out_dev, void *stream) {
   PARG(p, "proposal is NULL");
   PARG(R >= 0, "R must be >= 0");
-  PARG(mode == TT_PROPOSAL_RAW || mode == TT_PROPOSAL_INTERVAL, "unknown mode");
+  PARG(mode == TT_PROPOSAL_RAW || mode == TT_PROPOSAL_INTERVAL || mode == TT_PROPOSAL_CAP, "unknown mode");
   if (R == 0) return TT_OK;
   PARG(ray_o_dev && ray_d_dev && out_dev, "NULL buffer");
   PARG(R <= (1ll << 31) - 64, "too many rays");

@@ -247,7 +247,11 @@ TT_D void seg_init(SegState &st) {
 // Depth interval (exact chunk mode): hits start at t >= tstart; the gathers stop at
 // `cap` until the hits up to it are used up, and go on past it (t > cap) only while
 // the ray is still open. The result is that of a trace starting at tstart for any cap.
-// With P.ival_hard the trace ends at the cap instead.
+// With P.ival_hard the trace ends at the cap instead, with two fallbacks: a ray whose
+// first hit at or after tstart lies within ival_fb_start of it (or that has no hit
+// up to the cap) is traced again from 0 without cap, before anything is blended;
+// and a ray still open at the cap that has no blended hit yet, or whose last
+// blended hit lies within ival_fb_end of the cap, goes on past it.
 template <int K, int BLOCK, bool FULL, bool COUNT>
 TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node,
                         float tstart = 0.0f, float cap = kInfDist) {
@@ -256,10 +260,21 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
   float last = 0.0f, tlo = tstart;
   long long plo = -1;
   bool capped = cap < P.far_t && cap >= tstart;  // a cap before the start is no cap
+  bool hard = P.ival_hard && capped;
+  bool probe = P.ival_hard && tstart > 0.0f;  // first gather of a hard interval with a start
   const bool logging = FULL && P.log_cap > 0;
   TravCount tc;
   while (true) {
     gather<K, BLOCK, COUNT>(P, r, tlo, plo, capped ? cap : P.far_t, b, s_node, tc);
+    if (probe) {
+      probe = false;
+      if (b.p[0] == kInvalid || b.t[0] < tstart + P.ival_fb_start) {  // the proposal may have missed: full trace
+        tlo = 0.0f;
+        plo = -1;
+        capped = hard = false;
+        continue;
+      }
+    }
     if (COUNT) {
       P.trav_stats[2 * (size_t)ray] = tc.nodes;
       P.trav_stats[2 * (size_t)ray + 1] = tc.tris;
@@ -330,7 +345,7 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
       }
     }
     if (spent) {
-      if (!capped || P.ival_hard) return;
+      if (!capped || (hard && !(st.t_last > cap - P.ival_fb_end))) return;
       capped = false;  // everything up to cap is used: continue strictly past it
       tlo = cap;
       plo = (long long)kInvalid;

@@ -24,7 +24,7 @@ EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build"
             "tt_num_primitives", "tt_last_build_ms", "tt_multi_forward", "tt_multi_backward", "tt_multi_capture",
             "tt_multi_replay", "tt_multi_autotune", "tt_proposal_default_hparams", "tt_proposal_num_params",
             "tt_proposal_create", "tt_proposal_destroy", "tt_proposal_set_params", "tt_proposal_get_params",
-            "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train"]
+            "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays"]
 MAX_CAMERAS = 32
 
 _vp = C.c_void_p
@@ -46,7 +46,7 @@ class FwdArgs(C.Structure):
         ("scaling_dev", _fp), ("stats_dev", _fp), ("trav_stats_dev", _fp),
         ("hit_log_cap", C.c_int), ("hit_log_prim_dev", _fp), ("hit_log_t_dev", _fp), ("hit_log_code_dev", _fp),
         ("hit_log_count_dev", _fp), ("interval_in_dev", _fp), ("interval_out_dev", _fp),
-        ("interval_hard_cap", C.c_int),
+        ("interval_hard_cap", C.c_int), ("interval_fallback_start", C.c_float), ("interval_fallback_end", C.c_float),
     ]
 
 
@@ -89,7 +89,7 @@ class ProposalHparams(C.Structure):
                                         "beta2", "eps")]
 
 
-PROPOSAL_RAW, PROPOSAL_INTERVAL = 0, 1
+PROPOSAL_RAW, PROPOSAL_INTERVAL, PROPOSAL_CAP = 0, 1, 2
 
 _lib = None
 
@@ -117,6 +117,8 @@ def lib():
         L.tt_multi_replay.argtypes = [_vp, _vp]
         L.tt_multi_autotune.argtypes = [_vp, C.POINTER(MultiArgs), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                         C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_float), _vp]
+        _d3 = C.POINTER(C.c_double)
+        L.tt_camera_rays.argtypes = [C.c_int, C.c_int, _d3, _d3, _d3, _d3, C.c_double, C.c_double, _fp, _fp, _vp]
         L.tt_proposal_default_hparams.argtypes = [C.POINTER(ProposalHparams)]
         L.tt_proposal_default_hparams.restype = None
         L.tt_proposal_create.argtypes = [C.POINTER(_vp), C.POINTER(ProposalHparams), C.c_ulonglong]
@@ -144,6 +146,17 @@ def ptr(t) -> int | None:
     if t is None or t.numel() == 0:
         return None
     return t.data_ptr()
+
+
+def camera_rays(basis, width, height, ray_o, ray_d, stream=None):
+    """Pinhole rays on the GPU into ray_o, ray_d ([H, W, 3] float32 tensors) from a
+    scenes.camera_basis() dict; equal to scenes.look_at_rays bit for bit."""
+    def d3(v):
+        return (C.c_double * 3)(*[float(x) for x in v])
+
+    check(lib().tt_camera_rays(int(width), int(height), d3(basis["eye"]), d3(basis["forward"]), d3(basis["right"]),
+                               d3(basis["up"]), float(basis["tan_x"]), float(basis["tan_y"]), ptr(ray_o), ptr(ray_d),
+                               stream), "tt_camera_rays")
 
 
 def default_hparams(**kw) -> ProposalHparams:
@@ -273,7 +286,7 @@ class TracerHandle:
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
              scaling, k=16, cutoff=1e-4, chunk_mode="compat", variant="forward", extra=None, stats=None,
              log=None, near=0.2, far=1000.0, trav_stats=None, interval_in=None, interval_out=None,
-             hard_cap=False) -> FwdArgs:
+             hard_cap=False, fallback_start=0.0, fallback_end=0.0) -> FwdArgs:
     a = FwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -295,6 +308,7 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.trav_stats_dev = ptr(trav_stats)
     a.interval_in_dev, a.interval_out_dev = ptr(interval_in), ptr(interval_out)
     a.interval_hard_cap = 1 if hard_cap else 0
+    a.interval_fallback_start, a.interval_fallback_end = float(fallback_start), float(fallback_end)
     if log is not None:
         prim, t, code, count = log
         a.hit_log_cap = int(prim.shape[-1])

@@ -101,10 +101,9 @@ def clustered_scene(P: int = 1_000_000, clusters: int = 64, seed: int = 1, edge_
     return _attrs(rng, P, verts.astype(np.float32))
 
 
-def look_at_rays(width: int, height: int, eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
-                 fov_deg: float = 60.0):
-    """Pinhole camera: ray_o, ray_d as float32 [H, W, 3]; unit directions through pixel
-    centres; fov_deg is the horizontal field of view."""
+def camera_basis(width: int, height: int, eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg: float = 60.0):
+    """Look-at pinhole basis in double: eye, forward, right, up (unit), tan of the half
+    field of view in x (fov_deg is horizontal) and y."""
     eye = np.asarray(eye, np.float64)
     fwd = np.asarray(target, np.float64) - eye
     fwd /= np.linalg.norm(fwd)
@@ -115,11 +114,20 @@ def look_at_rays(width: int, height: int, eye, target=(0.0, 0.0, 0.0), up=(0.0, 
     upv = np.cross(right, fwd)
     tx = math.tan(math.radians(fov_deg) / 2)
     ty = tx * height / width
-    xs = (2.0 * (np.arange(width) + 0.5) / width - 1.0) * tx
-    ys = (1.0 - 2.0 * (np.arange(height) + 0.5) / height) * ty
+    return dict(eye=eye, forward=fwd, right=right, up=upv, tan_x=tx, tan_y=ty)
+
+
+def look_at_rays(width: int, height: int, eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
+                 fov_deg: float = 60.0):
+    """Pinhole camera: ray_o, ray_d as float32 [H, W, 3]; unit directions through pixel
+    centres; fov_deg is the horizontal field of view."""
+    b = camera_basis(width, height, eye, target, up, fov_deg)
+    fwd, right, upv = b["forward"], b["right"], b["up"]
+    xs = (2.0 * (np.arange(width) + 0.5) / width - 1.0) * b["tan_x"]
+    ys = (1.0 - 2.0 * (np.arange(height) + 0.5) / height) * b["tan_y"]
     d = fwd[None, None, :] + xs[None, :, None] * right[None, None, :] + ys[:, None, None] * upv[None, None, :]
-    d /= np.linalg.norm(d, axis=-1, keepdims=True)
-    o = np.broadcast_to(eye, d.shape)
+    d /= np.sqrt((d[..., 0:1] * d[..., 0:1] + d[..., 1:2] * d[..., 1:2]) + d[..., 2:3] * d[..., 2:3])
+    o = np.broadcast_to(b["eye"], d.shape)
     return np.ascontiguousarray(o, np.float32), np.ascontiguousarray(d, np.float32)
 
 

@@ -120,7 +120,13 @@ typedef struct {
      first field is inf for a ray with no blended hit. */
   const float *interval_in_dev;
   float *interval_out_dev;
-  int interval_hard_cap;         /* nonzero: the trace ends at cap (hits past it are dropped) */
+  int interval_hard_cap;         /* nonzero: the trace ends at cap (hits past it are dropped), except:
+                                    a ray whose first hit at or after start lies within
+                                    interval_fallback_start of it, or that has no hit up to cap, is
+                                    traced again in full; a ray still open at cap that has no blended
+                                    hit yet or whose last blended hit lies within
+                                    interval_fallback_end of cap goes on past it */
+  float interval_fallback_start, interval_fallback_end;
 } tt_fwd_args;
 
 /* Replaces OptixTracer::trace_fwd's launch (tracer.cpp:105-132) and the raygen programs
@@ -226,6 +232,7 @@ typedef struct {
 #define TT_PROPOSAL_RAW 0      /* outputs [R][2]: the network's two values */
 #define TT_PROPOSAL_INTERVAL 1 /* outputs [R][2]: start = max(0, (y0 - margin_start) t_scale),
                                   cap = (y1 + margin_end) t_scale (tt_fwd_args.interval_in) */
+#define TT_PROPOSAL_CAP 2      /* outputs [R][2]: start = 0, cap as above (exact with a soft cap) */
 
 void tt_proposal_default_hparams(tt_proposal_hparams *hp);
 /* Parameters: W1[64][102] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64] b4[2] (fp32 master copy). */
@@ -247,6 +254,13 @@ int tt_proposal_loss_grad(tt_proposal *p, const float *ray_o_dev, const float *r
    loss_hist [steps] (optional) receives each step's loss. Asynchronous. */
 int tt_proposal_train(tt_proposal *p, const float *ray_o_dev, const float *ray_d_dev, const float *interval_dev,
                       long long n_rays, int batch, int steps, float *loss_hist_dev, void *stream);
+
+/* Pinhole camera rays into ray_o, ray_d [height][width][3] (the step before tt_trace_fwd; the
+   reference takes these tensors from its Python caller). Pixel (x, y) looks along
+   forward + xs right + ys up, xs = (2 (x + 0.5) / width - 1) tan_x, ys = (1 - 2 (y + 0.5) / height) tan_y,
+   normalised in double and rounded once to float. eye/forward/right/up: double[3]. */
+int tt_camera_rays(int width, int height, const double *eye, const double *forward, const double *right,
+                   const double *up, double tan_x, double tan_y, float *ray_o_dev, float *ray_d_dev, void *stream);
 
 /* Number of triangles of the current acceleration structure (-1 if none). */
 int tt_num_primitives(const tt_tracer *tr);

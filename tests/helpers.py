@@ -38,7 +38,8 @@ class GpuScene:
             self.h.build(self.verts, self.cumsum, self.P, capi.BUILD)
 
     def forward(self, ray_o, ray_d, k=16, cutoff=1e-3, chunk_mode="compat", variant="forward", extra=None,
-                bg=(0.0, 0.0, 0.0), log_cap=0, stats=True, interval_in=None, interval_out=False):
+                bg=(0.0, 0.0, 0.0), log_cap=0, stats=True, interval_in=None, interval_out=False, hard_cap=False,
+                fallback=(0.0, 0.0)):
         from diff_tritracer import capi
 
         torch = self.torch
@@ -63,7 +64,8 @@ class GpuScene:
         ivo = torch.full((R, 4), -7.0, device=dev) if interval_out else None
         a = capi.fwd_args(W, H, ro, rd, self.verts, self.npts, self.cumsum, self.shs, 3, self.opac, self.sigma, bgt,
                           color, others, maxb, dens, scal, k=k, cutoff=cutoff, chunk_mode=chunk_mode,
-                          variant=variant, extra=ext, stats=st, log=log, interval_in=ivi, interval_out=ivo)
+                          variant=variant, extra=ext, stats=st, log=log, interval_in=ivi, interval_out=ivo,
+                          hard_cap=hard_cap, fallback_start=fallback[0], fallback_end=fallback[1])
         self.h.fwd(a, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         out = dict(color=color.cpu().numpy(), others=others.cpu().numpy(), max_blending=maxb.cpu().numpy(),

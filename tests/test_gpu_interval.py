@@ -108,6 +108,36 @@ def test_start_equals_oracle_started_there(small, k):
     assert np.array_equal(out["color"][:, sel], plain["color"][:, sel])
 
 
+def test_hard_cap_with_the_emitted_interval_reproduces_plain(small):
+    sc, ro, rd, g, S, plain, log = small
+    R = ro.shape[0] * ro.shape[1]
+    iv = plain["interval"]
+    has = np.isfinite(iv[:, 0])
+    stop = np.where(np.isfinite(iv[:, 2]), iv[:, 2], iv[:, 1])
+    # rays without blended hits: an empty interval past everything (the probe finds no hit -> full trace)
+    ivi = np.stack([np.where(has, iv[:, 0], 900.0), np.where(has, stop, 901.0)], 1).astype(np.float32)
+    _say("hard cap on the exact blended range")
+    out = g.forward(ro, rd, k=K, cutoff=CUT, chunk_mode="exact", interval_in=ivi, hard_cap=True)
+    assert np.array_equal(out["color"], plain["color"]) and np.array_equal(out["others"], plain["others"])
+    # a start fallback wider than the scene sends every ray back to the full trace
+    rng = np.random.Generator(np.random.PCG64(12))
+    bad = np.stack([rng.uniform(0.0, 6.0, R), rng.uniform(0.0, 6.0, R)], 1).astype(np.float32)
+    out = g.forward(ro, rd, k=K, cutoff=CUT, chunk_mode="exact", interval_in=bad, hard_cap=True,
+                    fallback=(1000.0, 0.0))
+    full = bad[:, 0] > 0.0
+    assert np.array_equal(out["color"][:, full], plain["color"][:, full])
+    # a hard cap inside the blended range of an open ray drops the rest of it; a ray
+    # with no blended hit up to its cap goes on past it
+    mid = 0.5 * (iv[:, 0] + iv[:, 1])
+    open_ = has & ~np.isfinite(iv[:, 2]) & (iv[:, 1] > mid) & (mid > iv[:, 0])
+    ivc = np.stack([np.zeros(R), np.where(has, mid, 0.3)], 1).astype(np.float32)
+    out = g.forward(ro, rd, k=K, cutoff=CUT, chunk_mode="exact", interval_in=ivc, hard_cap=True)
+    _say(f"open rays cut inside their blended range: {open_.sum()}")
+    assert open_.sum() > 50
+    assert np.all(out["others"][1, open_] < plain["others"][1, open_])
+    assert np.array_equal(out["color"][:, ~has], plain["color"][:, ~has])
+
+
 def test_backward_bounded_by_forward_interval(small):
     sc, ro, rd, g, S, plain, log = small
     R = ro.shape[0] * ro.shape[1]
