@@ -211,11 +211,17 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
                         a->cumsum_dev, a->P, a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->k,
                         a->chunk_mode);
   if (rc != TT_OK) return rc;
-  TT_ARG(a->variant >= TT_VARIANT_FORWARD && a->variant <= TT_VARIANT_ENVLIGHT, "unknown variant");
+  TT_ARG(a->variant >= TT_VARIANT_FORWARD && a->variant <= TT_VARIANT_BOUNCE, "unknown variant");
   const long long R = (long long)a->width * a->height;
   if (R > 0) {
     TT_ARG(a->color_dev && a->bg_dev, "colour / background buffers are NULL");
-    if (a->variant == TT_VARIANT_FORWARD) {
+    if (a->variant == TT_VARIANT_BOUNCE) {
+      TT_ARG(a->envmap_dev && a->env_width > 0 && a->env_height > 0, "bounce variant needs an environment map");
+      TT_ARG((long long)a->env_width * a->env_height < (1ll << 29), "environment map too large");
+      TT_ARG(a->bounce_aux_dev, "bounce variant needs bounce_aux");
+      if (a->P > 0) TT_ARG(a->reflectivity_dev && a->ior_dev, "bounce variant needs reflectivity and ior");
+    }
+    if (a->variant == TT_VARIANT_FORWARD || a->variant == TT_VARIANT_BOUNCE) {
       TT_ARG(a->others_dev, "others buffer is NULL");
       if (a->P > 0) TT_ARG(a->max_blending_dev && a->density_dev && a->scaling_dev, "statistics buffers are NULL");
     }
@@ -256,6 +262,14 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.ival_hard = a->interval_in_dev && a->interval_hard_cap ? 1 : 0;
   p.ival_fb_start = a->interval_fallback_start;
   p.ival_fb_end = a->interval_fallback_end;
+  if (a->variant == TT_VARIANT_BOUNCE) {
+    p.refl = a->reflectivity_dev;
+    p.ior = a->ior_dev;
+    p.env = a->envmap_dev;
+    p.env_w = a->env_width;
+    p.env_h = a->env_height;
+    p.aux = a->bounce_aux_dev;
+  }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
   if (!err.empty()) return fail(TT_ERR_HIP, err);
@@ -280,8 +294,25 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
       if (e.empty()) e = tt::launch_fill(a->dL_dsigma_dev, (size_t)a->P, 0.0f, s);
       if (e.empty()) e = tt::launch_fill(a->dL_dopacity_dev, (size_t)a->P, 0.0f, s);
       if (e.empty()) e = tt::launch_fill(a->dL_dsh_dev, (size_t)a->P * a->M * 3, 0.0f, s);
+      if (e.empty() && a->variant == TT_VARIANT_BOUNCE && a->dL_dreflectivity_dev)
+        e = tt::launch_fill(a->dL_dreflectivity_dev, (size_t)a->P, 0.0f, s);
       if (!e.empty()) return fail(TT_ERR_HIP, e);
     }
+  }
+  TT_ARG(a->variant == TT_VARIANT_FORWARD || a->variant == TT_VARIANT_BOUNCE,
+         "the backward supports TT_VARIANT_FORWARD and TT_VARIANT_BOUNCE");
+  if (a->variant == TT_VARIANT_BOUNCE) {
+    TT_ARG(a->envmap_dev && a->env_width > 0 && a->env_height > 0 && a->dL_denvmap_dev,
+           "bounce backward needs the environment map and its gradient buffer");
+    TT_ARG((long long)a->env_width * a->env_height < (1ll << 29), "environment map too large");
+    TT_ARG(a->interval_dev == nullptr, "interval is not supported by the bounce backward");
+    if (a->zero_grads) {
+      std::string e = tt::launch_fill(a->dL_denvmap_dev, (size_t)a->env_width * a->env_height * 3, 0.0f, s);
+      if (!e.empty()) return fail(TT_ERR_HIP, e);
+    }
+    if (R > 0 && a->P > 0)
+      TT_ARG(a->bounce_aux_dev && a->reflectivity_dev && a->dL_dreflectivity_dev,
+             "bounce backward needs bounce_aux, reflectivity and its gradient buffer");
   }
   if (R == 0 || a->P == 0) return ok();
   TT_ARG(a->color_dev && a->others_dev && a->dL_dpix_dev && a->dL_dothers_dev,
@@ -300,8 +331,17 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   p.dopac = a->dL_dopacity_dev;
   p.dsh = a->dL_dsh_dev;
   p.ival_bwd = reinterpret_cast<const float4 *>(a->interval_dev);
+  if (a->variant == TT_VARIANT_BOUNCE) {
+    p.refl = a->reflectivity_dev;
+    p.env = a->envmap_dev;
+    p.env_w = a->env_width;
+    p.env_h = a->env_height;
+    p.aux = const_cast<float *>(a->bounce_aux_dev);
+    p.drefl = a->dL_dreflectivity_dev;
+    p.denv = a->dL_denvmap_dev;
+  }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
-  std::string err = tt::launch_backward(p, a->k, s);
+  std::string err = tt::launch_backward(p, a->k, a->variant, s);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   return ok();
 }

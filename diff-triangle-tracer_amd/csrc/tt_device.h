@@ -359,6 +359,82 @@ TT_D bool hit_sphere(float rad, v3f cnt, v3f org, v3f dir, float &t) {
 TT_D v3f reflect3(v3f i, v3f n) {
   return i - (2.0f * n) * dot(n, i);
 }
+
+// ---- bounce variant (project-defined; no counterpart in the reference)
+// Equirect environment [H][W][3]: direction -> phi = atan2(x, -z), theta = acos(y) in
+// double after normalising in double, u = (phi / 2pi + 0.5) W, v = theta / pi H;
+// bilinear over texel centres, wrapping in u, clamped in v.
+struct EnvTaps {
+  int idx[4];
+  float w[4];
+};
+TT_D EnvTaps env_taps(v3f d, int W, int H) {
+  constexpr double kInvPi = 0.31830988618379067154, kInv2Pi = 0.15915494309189533577;
+  EnvTaps e;
+  double dx = d.x, dy = d.y, dz = d.z;
+  const double n = sqrt((dx * dx + dy * dy) + dz * dz);
+  if (!(n > 0.0)) {
+    for (int j = 0; j < 4; j++) e.idx[j] = 0, e.w[j] = j == 0 ? 1.0f : 0.0f;
+    return e;
+  }
+  dx /= n, dy /= n, dz /= n;
+  const double phi = atan2(dx, -dz);
+  const double theta = acos(fmin(1.0, fmax(-1.0, dy)));
+  const double u = (phi * kInv2Pi + 0.5) * (double)W - 0.5;
+  const double v = theta * kInvPi * (double)H - 0.5;
+  const double x0 = floor(u), y0 = floor(v);
+  const double fx = u - x0, fy = v - y0;
+  const int ix = (int)x0, iy = (int)y0;
+  const int xa = ((ix % W) + W) % W, xb = (xa + 1) % W;
+  const int ya = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy), yb = iy + 1 < 0 ? 0 : (iy + 1 > H - 1 ? H - 1 : iy + 1);
+  e.idx[0] = ya * W + xa, e.idx[1] = ya * W + xb, e.idx[2] = yb * W + xa, e.idx[3] = yb * W + xb;
+  e.w[0] = (float)((1.0 - fx) * (1.0 - fy)), e.w[1] = (float)(fx * (1.0 - fy));
+  e.w[2] = (float)((1.0 - fx) * fy), e.w[3] = (float)(fx * fy);
+  return e;
+}
+TT_D void env_value(const float *env, int W, int H, v3f d, float out[3]) {
+  const EnvTaps e = env_taps(d, W, H);
+  for (int c = 0; c < 3; c++)
+    out[c] = ((e.w[0] * env[3 * e.idx[0] + c] + e.w[1] * env[3 * e.idx[1] + c]) + e.w[2] * env[3 * e.idx[2] + c]) +
+             e.w[3] * env[3 * e.idx[3] + c];
+}
+
+// Secondary rays from a primary segment's sums: expected normal N/|N| turned to face
+// the ray, expected depth D/A, expected IOR eta/A (A = 1 - T); reflected ray from
+// x + n eps; refracted ray (Snell, entering with ratio 1/eta; total internal
+// reflection -> the reflected direction) from x - n eps.
+struct Bounce {
+  bool valid;
+  v3f orr, dr, ot, dt;
+};
+TT_D Bounce bounce_rays(v3f o, v3f d, float T, const float N[3], float D, float eta_sum) {
+  Bounce g;
+  g.valid = false;
+  const float A = 1.0f - T;
+  const float nn = (N[0] * N[0] + N[1] * N[1]) + N[2] * N[2];
+  if (!(A > 1e-6f) || !(nn > 1e-24f)) return g;
+  v3f n = normalize(mk(N[0], N[1], N[2]));
+  if (dot(n, d) > 0.0f) n = -n;
+  const float eps = (float)kStepEps;
+  const float tbar = D / A;
+  const v3f x = o + d * tbar;
+  const float eta = eta_sum / A;
+  const v3f r = reflect3(d, n);
+  const float cosi = -dot(d, n);
+  const float ratio = 1.0f / eta;
+  const float k = 1.0f - (ratio * ratio) * (1.0f - cosi * cosi);
+  g.valid = true;
+  g.orr = x + n * eps;
+  g.dr = r;
+  if (k < 0.0f) {
+    g.ot = g.orr;
+    g.dt = r;
+  } else {
+    g.ot = x - n * eps;
+    g.dt = d * ratio + n * (ratio * cosi - sqrtf(k));
+  }
+  return g;
+}
 // refract.cu:166-187, with its formula as written.
 TT_D bool refract_quirk(v3f &out, v3f in, v3f n, float eta_org) {
   float cosi = dot(in, n);

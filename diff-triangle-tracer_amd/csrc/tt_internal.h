@@ -88,7 +88,16 @@ struct TraceParams {
   float ival_fb_start, ival_fb_end;  // hard mode fallbacks (distances), see trace_segment
   float4 *ival_out;
   const float4 *ival_bwd;
+  // bounce variant: per-triangle reflectivity / IOR, environment map [env_h][env_w][3],
+  // per-pixel record for the backward (kBounceAux floats), and their gradients
+  const float *refl, *ior, *env;
+  int env_w, env_h;
+  float *aux;
+  float *drefl, *denv;
 };
+// Per-pixel record of the bounce variant: L_r[3] L_t[3] R Q o_r[3] d_r[3] o_t[3] d_t[3]
+// (d_r = 0: no secondary rays).
+constexpr int kBounceAux = 20;
 
 // Cameras of one persistent multi-camera launch (passed by value as a kernel argument).
 constexpr int kMaxCams = 32;
@@ -117,7 +126,7 @@ constexpr int kLdsStack = 16;  // traversal stack entries per lane kept in LDS; 
 
 // Launch the forward kernel for (k, variant). Returns an error message or "".
 std::string launch_forward(const TraceParams &p, int k, int variant, hipStream_t stream);
-std::string launch_backward(const TraceParams &p, int k, hipStream_t stream);
+std::string launch_backward(const TraceParams &p, int k, int variant, hipStream_t stream);
 std::string launch_fill(float *ptr, size_t n, float v, hipStream_t stream);
 
 }  // namespace tt

@@ -12,21 +12,22 @@ namespace {
 constexpr int kBlock = 128;  // lanes per block: two 8x8 wave tiles -> 16x8 pixels
 constexpr int kTileW = TileShape<kBlock>::W, kTileH = TileShape<kBlock>::H;
 
-template <int K, int VARIANT, bool COUNT = false>
+template <int K, int VARIANT, bool COUNT = false, bool IVAL = false>
 __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
   int x, y;
-  if (tile_pixel<kBlock>(P, xcd_tile(P.n_tiles), x, y)) forward_pixel<K, kBlock, VARIANT, COUNT>(P, y * P.W + x, s_node);
+  if (tile_pixel<kBlock>(P, xcd_tile(P.n_tiles), x, y))
+    forward_pixel<K, kBlock, VARIANT, COUNT, IVAL>(P, y * P.W + x, s_node);
 }
 
-template <int K>
+template <int K, int VARIANT = 0>
 __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
   __shared__ int s_keys[kBlock];
   __shared__ float s_acc[kFields * kBlock];
   int x, y;
   const bool valid = tile_pixel<kBlock>(P, xcd_tile(P.n_tiles), x, y);
-  backward_block<K, kBlock>(P, valid, y * P.W + x, s_node, s_keys, s_acc);
+  backward_block<K, kBlock, VARIANT>(P, valid, y * P.W + x, s_node, s_keys, s_acc);
 }
 
 template <int K>
@@ -35,7 +36,9 @@ hipError_t fwd_k(const TraceParams &p, int variant, hipStream_t s) {
   switch (variant) {
     case 0:
       if (p.trav_stats)
-        hipLaunchKernelGGL((k_forward<K, 0, true>), g, bl, 0, s, p);
+        hipLaunchKernelGGL((k_forward<K, 0, true, true>), g, bl, 0, s, p);
+      else if (p.ival_in || p.ival_out)
+        hipLaunchKernelGGL((k_forward<K, 0, false, true>), g, bl, 0, s, p);
       else
         hipLaunchKernelGGL((k_forward<K, 0>), g, bl, 0, s, p);
       break;
@@ -45,8 +48,11 @@ hipError_t fwd_k(const TraceParams &p, int variant, hipStream_t s) {
     case 2:
       hipLaunchKernelGGL((k_forward<K, 2>), g, bl, 0, s, p);
       break;
-    default:
+    case 3:
       hipLaunchKernelGGL((k_forward<K, 3>), g, bl, 0, s, p);
+      break;
+    default:
+      hipLaunchKernelGGL((k_forward<K, 4>), g, bl, 0, s, p);
       break;
   }
   return hipGetLastError();
@@ -81,19 +87,21 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
   return "";
 }
 
-std::string launch_backward(const TraceParams &p0, int k, hipStream_t stream) {
+std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream_t stream) {
   TraceParams p = p0;
   tiles(p);
   if (p.n_tiles == 0) return "";
   dim3 g(p.n_tiles), bl(kBlock);
-  if (k == 8)
-    hipLaunchKernelGGL(k_backward<8>, g, bl, 0, stream, p);
-  else if (k == 16)
-    hipLaunchKernelGGL(k_backward<16>, g, bl, 0, stream, p);
-  else if (k == 32)
-    hipLaunchKernelGGL(k_backward<32>, g, bl, 0, stream, p);
-  else
-    return "k must be 8, 16 or 32";
+  if (k != 8 && k != 16 && k != 32) return "k must be 8, 16 or 32";
+  if (variant == 4) {
+    if (k == 8) hipLaunchKernelGGL((k_backward<8, 4>), g, bl, 0, stream, p);
+    if (k == 16) hipLaunchKernelGGL((k_backward<16, 4>), g, bl, 0, stream, p);
+    if (k == 32) hipLaunchKernelGGL((k_backward<32, 4>), g, bl, 0, stream, p);
+  } else {
+    if (k == 8) hipLaunchKernelGGL(k_backward<8>, g, bl, 0, stream, p);
+    if (k == 16) hipLaunchKernelGGL(k_backward<16>, g, bl, 0, stream, p);
+    if (k == 32) hipLaunchKernelGGL(k_backward<32>, g, bl, 0, stream, p);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("backward launch failed: ") + hipGetErrorString(e);
   return "";

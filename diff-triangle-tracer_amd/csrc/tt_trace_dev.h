@@ -231,6 +231,8 @@ struct SegState {
   // depth interval of the blended hits: first/last t, t of the terminating hit, first prim
   float t_first, t_last, t_term;
   uint32_t p_first;
+  // bounce variant: sums over blended hits of w rho, w (1 - rho), w eta, w t (w = alpha T)
+  float R, Q, eta, D;
 };
 
 TT_D void seg_init(SegState &st) {
@@ -240,6 +242,7 @@ TT_D void seg_init(SegState &st) {
   st.consumed = st.blended = 0;
   st.t_first = st.t_last = st.t_term = __builtin_huge_valf();
   st.p_first = kInvalid;
+  st.R = st.Q = st.eta = st.D = 0.0f;
 }
 
 // The chunk loop of forward.cu:76-155 (FULL: with normal/statistics outputs) and
@@ -252,16 +255,17 @@ TT_D void seg_init(SegState &st) {
 // up to the cap) is traced again from 0 without cap, before anything is blended;
 // and a ray still open at the cap that has no blended hit yet, or whose last
 // blended hit lies within ival_fb_end of the cap, goes on past it.
-template <int K, int BLOCK, bool FULL, bool COUNT>
+// IVAL: the interval features (start / cap, emitted interval) are compiled in.
+template <int K, int BLOCK, bool FULL, bool COUNT, bool BOUNCE = false, bool IVAL = false>
 TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node,
                         float tstart = 0.0f, float cap = kInfDist) {
   RayPre r = prep_ray(o, d);
   KBuf<K> b;
-  float last = 0.0f, tlo = tstart;
+  float last = 0.0f, tlo = IVAL ? tstart : 0.0f;
   long long plo = -1;
-  bool capped = cap < P.far_t && cap >= tstart;  // a cap before the start is no cap
-  bool hard = P.ival_hard && capped;
-  bool probe = P.ival_hard && tstart > 0.0f;  // first gather of a hard interval with a start
+  bool capped = IVAL && cap < P.far_t && cap >= tstart;  // a cap before the start is no cap
+  bool hard = IVAL && P.ival_hard && capped;
+  bool probe = IVAL && P.ival_hard && tstart > 0.0f;  // first gather of a hard interval with a start
   const bool logging = FULL && P.log_cap > 0;
   TravCount tc;
   while (true) {
@@ -314,11 +318,13 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
               P.log_t[(size_t)ray * P.log_cap + slot] = t;
               P.log_code[(size_t)ray * P.log_cap + slot] = 3;
             }
-            st.t_term = t;
+            if (IVAL) st.t_term = t;
             return;
           }
-          if (st.blended == 0) st.t_first = t, st.p_first = prim;
-          st.t_last = t;
+          if (IVAL) {
+            if (st.blended == 0) st.t_first = t, st.p_first = prim;
+            st.t_last = t;
+          }
           const float bw = alpha * st.T;
           if (FULL) {
             P.density[prim] = 1.0f;
@@ -333,6 +339,13 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
           st.C[0] += col.x * bw;
           st.C[1] += col.y * bw;
           st.C[2] += col.z * bw;
+          if (BOUNCE) {
+            const float rho = P.refl[prim];
+            st.R += bw * rho;
+            st.Q += bw * (1.0f - rho);
+            st.eta += bw * P.ior[prim];
+            st.D += bw * t;
+          }
           st.T = testT;
           st.blended++;
           code = 2;
@@ -364,21 +377,51 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
 }
 
 // One pixel of the forward programs (forward.cu:161-215 and the variants' raygens).
-template <int K, int BLOCK, int VARIANT, bool COUNT>
+template <int K, int BLOCK, int VARIANT, bool COUNT, bool IVAL = false>
 TT_D void forward_pixel(const TraceParams &P, int pix, int *s_node) {
   v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
   SegState st;
   seg_init(st);
   if (VARIANT == 0) {
     float tstart = 0.0f, cap = kInfDist;
-    if (P.ival_in) {
+    if (IVAL && P.ival_in) {
       const float2 iv = P.ival_in[pix];
       tstart = iv.x;
       cap = iv.y;
     }
-    trace_segment<K, BLOCK, true, COUNT>(P, o, d, st, pix, o, s_node, tstart, cap);
-    if (P.ival_out)
+    trace_segment<K, BLOCK, true, COUNT, false, IVAL>(P, o, d, st, pix, o, s_node, tstart, cap);
+    if (IVAL && P.ival_out)
       P.ival_out[pix] = make_float4(st.t_first, st.t_last, st.t_term, __uint_as_float(st.p_first));
+  } else if (VARIANT == 4) {
+    trace_segment<K, BLOCK, true, COUNT, true>(P, o, d, st, pix, o, s_node);
+    const Bounce g = bounce_rays(o, d, st.T, st.N, st.D, st.eta);
+    float Lr[3] = {0.0f, 0.0f, 0.0f}, Lt[3] = {0.0f, 0.0f, 0.0f};
+    if (g.valid) {
+      float E[3];
+      SegState sr;
+      seg_init(sr);
+      trace_segment<K, BLOCK, false, false>(P, g.orr, g.dr, sr, pix, g.orr, s_node);
+      env_value(P.env, P.env_w, P.env_h, g.dr, E);
+      for (int c = 0; c < 3; c++) Lr[c] = sr.C[c] + sr.T * E[c];
+      SegState stt;
+      seg_init(stt);
+      trace_segment<K, BLOCK, false, false>(P, g.ot, g.dt, stt, pix, g.ot, s_node);
+      env_value(P.env, P.env_w, P.env_h, g.dt, E);
+      for (int c = 0; c < 3; c++) Lt[c] = stt.C[c] + stt.T * E[c];
+      st.consumed += sr.consumed + stt.consumed;
+      st.blended += sr.blended + stt.blended;
+    }
+    float E0[3];
+    env_value(P.env, P.env_w, P.env_h, d, E0);
+    const size_t R = (size_t)P.R;
+    for (int c = 0; c < 3; c++) P.color[c * R + pix] = ((st.C[c] + st.R * Lr[c]) + st.Q * Lt[c]) + st.T * E0[c];
+    float *a = P.aux + (size_t)kBounceAux * pix;
+    const v3f zero = mk(0.0f, 0.0f, 0.0f);
+    const v3f orr = g.valid ? g.orr : zero, dr = g.valid ? g.dr : zero, ot = g.valid ? g.ot : zero,
+              dt = g.valid ? g.dt : zero;
+    const float rec[kBounceAux] = {Lr[0], Lr[1], Lr[2], Lt[0], Lt[1], Lt[2], st.R, st.Q, orr.x, orr.y,
+                                   orr.z, dr.x, dr.y, dr.z, ot.x, ot.y, ot.z, dt.x, dt.y, dt.z};
+    for (int i = 0; i < kBounceAux; i++) a[i] = rec[i];
   } else {
     const float rad = P.extra_len > 0 ? P.extra[0] : 1.0f;
     const v3f cnt = P.extra_len > 3 ? mk(P.extra[1], P.extra[2], P.extra[3]) : mk(0.0f, 0.0f, 0.0f);
@@ -437,8 +480,9 @@ TT_D void forward_pixel(const TraceParams &P, int pix, int *s_node) {
     }
   }
   const size_t R = (size_t)P.R;
-  for (int c = 0; c < 3; c++) P.color[c * R + pix] = st.C[c] + st.T * P.bg[c];
-  if (VARIANT == 0) {
+  if (VARIANT != 4)
+    for (int c = 0; c < 3; c++) P.color[c * R + pix] = st.C[c] + st.T * P.bg[c];
+  if (VARIANT == 0 || VARIANT == 4) {
     P.others[0 * R + pix] = 0.0f;
     P.others[1 * R + pix] = 1.0f - st.T;
     P.others[2 * R + pix] = st.N[0];
@@ -644,33 +688,21 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
   }
 }
 
-// backward.cu:191-350: re-traverse one ray with the same chunk logic, recompute
-// the forward state hit by hit and send each blended hit's gradient to the sink.
-template <int K, int BLOCK>
-TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, float *s_acc) {
-  const size_t R = (size_t)P.R;
-  const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
-  const float final_depth = P.fothers[pix];
-  const v3f final_normal = mk(P.fothers[2 * R + pix], P.fothers[3 * R + pix], P.fothers[4 * R + pix]);
-  float final_color[3], dL_dpixel[3];
-  for (int c = 0; c < 3; c++) {
-    final_color[c] = P.fcolor[c * R + pix];
-    dL_dpixel[c] = P.dpix[c * R + pix];
-  }
-  const float dL_ddepth = P.dothers[pix];
-  const v3f dL_dn2 = mk(P.dothers[2 * R + pix], P.dothers[3 * R + pix], P.dothers[4 * R + pix]);
+// backward.cu:191-350 for one traced segment: re-traverse it with the same chunk
+// logic, recompute the forward state hit by hit and send each blended hit's gradient
+// to the sink. fin = the segment's output colour (composite incl. background), g = the
+// upstream gradient on it. MODE 0: a primary ray (depth / normal terms); 1: the bounce
+// variant's primary (per-hit colours extended by rho L_r + (1 - rho) L_t, with the
+// reflectivity gradient); 2: a bounce secondary ray (colour only). Returns the
+// segment's final transmittance.
+template <int K, int BLOCK, int MODE>
+TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], const float g[3], float dL_ddepth,
+                        v3f dL_dn2, float final_depth, v3f final_normal, const float *Lr, const float *Lt, float tlo,
+                        long long plo, float far_t, int *s_node, int *s_keys, float *s_acc) {
   float C[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f}, D = 0.0f, T = 1.0f;
   const RayPre r = prep_ray(o, d);
   KBuf<K> b;
-  float last = 0.0f, tlo = 0.0f, far_t = P.far_t;
-  long long plo = -1;
-  if (P.ival_bwd) {  // only the forward's blended hits: from the first one (inclusive) to the last
-    const float4 iv = P.ival_bwd[pix];
-    if (!(iv.x < kInfDist)) return;
-    tlo = iv.x;
-    plo = (long long)__float_as_uint(iv.w) - 1;
-    far_t = iv.y;
-  }
+  float last = 0.0f;
   TravCount tc;
   while (true) {
     gather<K, BLOCK, false>(P, r, tlo, plo, far_t, b, s_node, tc);
@@ -680,7 +712,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
       const float t = b.t[0];
       const uint32_t prim = b.p[0];
       kpop(b);
-      if (prim == kInvalid) return;
+      if (prim == kInvalid) return T;
       lt = t;
       lp = prim;
       last = fmaxf((float)((double)last + kStepEps), t);
@@ -694,24 +726,37 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
       const float alpha = fminf(0.99f, opacity * w.Cx);
       if (alpha < 1.0f / 255.0f) continue;
       const float testT = T * (1.0f - alpha);
-      if (testT < P.cutoff) return;
+      if (testT < P.cutoff) return T;
       const float bw = alpha * T;
       bool c0, c1, c2;
       const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
-      C[0] += col.x * bw, C[1] += col.y * bw, C[2] += col.z * bw;
-      N[0] += w.n.x * bw, N[1] += w.n.y * bw, N[2] += w.n.z * bw;
-      D += bw * t;
+      float cc[3] = {col.x, col.y, col.z};
+      if (MODE == 1) {
+        const float rho = P.refl[prim];
+        float dr = 0.0f;
+        for (int c = 0; c < 3; c++) {
+          dr += g[c] * (Lr[c] - Lt[c]);
+          cc[c] += rho * Lr[c] + (1.0f - rho) * Lt[c];
+        }
+        unsafeAtomicAdd(P.drefl + prim, bw * dr);
+      }
+      C[0] += cc[0] * bw, C[1] += cc[1] * bw, C[2] += cc[2] * bw;
+      if (MODE != 2) {
+        N[0] += w.n.x * bw, N[1] += w.n.y * bw, N[2] += w.n.z * bw;
+        D += bw * t;
+      }
       float dL_dcolor[3];
-      const float cc[3] = {col.x, col.y, col.z};
       float dL_dalpha = 0.0f;
       const float inv1a = 1.0f / (1.0f - alpha);
       for (int c = 0; c < 3; c++) {
-        dL_dcolor[c] = dL_dpixel[c] * bw;
-        dL_dalpha += dL_dpixel[c] * (T * cc[c] - (final_color[c] - C[c]) * inv1a);
+        dL_dcolor[c] = g[c] * bw;
+        dL_dalpha += g[c] * (T * cc[c] - (fin[c] - C[c]) * inv1a);
       }
-      dL_dalpha += dL_ddepth * (T * t - (final_depth - D) * inv1a);
-      const v3f Nf = mk(N[0], N[1], N[2]);
-      dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
+      if (MODE != 2) {
+        dL_dalpha += dL_ddepth * (T * t - (final_depth - D) * inv1a);
+        const v3f Nf = mk(N[0], N[1], N[2]);
+        dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
+      }
       GradSink sink;
       sink.acc = s_acc;
       sink.stride = BLOCK;
@@ -724,7 +769,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
       const float dL_dC = opacity * dL_dalpha;
       if (w.phi_final > 0.0f) sink.add(F_SIGMA, (dL_dC * w.Cx) * log_tt(w.phi_final));
       const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
-      const v3f dL_dn3 = dL_dn2 * bw;
+      const v3f dL_dn3 = MODE != 2 ? dL_dn2 * bw : mk(0.0f, 0.0f, 0.0f);
       window_bwd(V, w, dL_dphi, dL_dn3, sink);
       sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, sink);
       T = testT;
@@ -741,15 +786,78 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
   }
 }
 
+// Environment-map gradient of one lookup: the upstream gradient g spread over the four
+// bilinear taps of direction d.
+TT_D void env_scatter(const TraceParams &P, v3f d, const float g[3]) {
+  const EnvTaps e = env_taps(d, P.env_w, P.env_h);
+  for (int j = 0; j < 4; j++)
+    if (e.w[j] != 0.0f)
+      for (int c = 0; c < 3; c++)
+        if (g[c] != 0.0f) unsafeAtomicAdd(P.denv + 3 * e.idx[j] + c, e.w[j] * g[c]);
+}
+
+// One pixel of the backward (backward.cu:191-391). VARIANT 4: the bounce variant's
+// primary, its environment term and its two secondary rays (upstream R g and Q g).
+template <int K, int BLOCK, int VARIANT>
+TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, float *s_acc) {
+  const size_t R = (size_t)P.R;
+  const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
+  const float final_depth = P.fothers[pix];
+  const v3f final_normal = mk(P.fothers[2 * R + pix], P.fothers[3 * R + pix], P.fothers[4 * R + pix]);
+  float final_color[3], dL_dpixel[3];
+  for (int c = 0; c < 3; c++) {
+    final_color[c] = P.fcolor[c * R + pix];
+    dL_dpixel[c] = P.dpix[c * R + pix];
+  }
+  const float dL_ddepth = P.dothers[pix];
+  const v3f dL_dn2 = mk(P.dothers[2 * R + pix], P.dothers[3 * R + pix], P.dothers[4 * R + pix]);
+  if (VARIANT == 0) {
+    float tlo = 0.0f, far_t = P.far_t;
+    long long plo = -1;
+    if (P.ival_bwd) {  // only the forward's blended hits: from the first one (inclusive) to the last
+      const float4 iv = P.ival_bwd[pix];
+      if (!(iv.x < kInfDist)) return;
+      tlo = iv.x;
+      plo = (long long)__float_as_uint(iv.w) - 1;
+      far_t = iv.y;
+    }
+    backward_seg<K, BLOCK, 0>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal, nullptr,
+                              nullptr, tlo, plo, far_t, s_node, s_keys, s_acc);
+  } else {
+    const float *a = P.aux + (size_t)kBounceAux * pix;
+    float Lr[3], Lt[3];
+    for (int c = 0; c < 3; c++) Lr[c] = a[c], Lt[c] = a[3 + c];
+    const float Rw = a[6], Qw = a[7];
+    const float Tp = backward_seg<K, BLOCK, 1>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
+                                               final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_keys, s_acc);
+    const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
+    env_scatter(P, d, ge);
+    const v3f dr = mk(a[11], a[12], a[13]);
+    if (dr.x != 0.0f || dr.y != 0.0f || dr.z != 0.0f) {
+      const v3f zero = mk(0.0f, 0.0f, 0.0f);
+      for (int s = 0; s < 2; s++) {
+        const float wgt = s == 0 ? Rw : Qw;
+        const v3f so = s == 0 ? mk(a[8], a[9], a[10]) : mk(a[14], a[15], a[16]);
+        const v3f sd = s == 0 ? dr : mk(a[17], a[18], a[19]);
+        const float gs[3] = {wgt * dL_dpixel[0], wgt * dL_dpixel[1], wgt * dL_dpixel[2]};
+        const float Ts = backward_seg<K, BLOCK, 2>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
+                                                   nullptr, 0.0f, -1, P.far_t, s_node, s_keys, s_acc);
+        const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
+        env_scatter(P, sd, gse);
+      }
+    }
+  }
+}
+
 // One block's share of the backward: clear the LDS table, run the lane's ray (if
 // any), then scatter the table to the gradient buffers. Every lane of the block
 // must call this (it contains block barriers).
-template <int K, int BLOCK>
+template <int K, int BLOCK, int VARIANT = 0>
 TT_D void backward_block(const TraceParams &P, bool valid, int pix, int *s_node, int *s_keys, float *s_acc) {
   for (int i = threadIdx.x; i < BLOCK; i += BLOCK) s_keys[i] = -1;
   for (int i = threadIdx.x; i < kFields * BLOCK; i += BLOCK) s_acc[i] = 0.0f;
   __syncthreads();
-  if (valid) backward_ray<K, BLOCK>(P, pix, s_node, s_keys, s_acc);
+  if (valid) backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_keys, s_acc);
   __syncthreads();
   const int nsh3 = 3 * (P.deg + 1) * (P.deg + 1);
   const int s = threadIdx.x;

@@ -16,7 +16,7 @@ from . import PACKAGE_DIR
 LIB_PATH = os.path.join(PACKAGE_DIR, "libtritracer_hip.so")
 
 TT_OK = 0
-VARIANTS = {"forward": 0, "reflect": 1, "refract": 2, "envlight": 3}
+VARIANTS = {"forward": 0, "reflect": 1, "refract": 2, "envlight": 3, "bounce": 4}
 BUILD, REFIT, REUSE = 0, 1, 2
 CHUNK = {"compat": 0, "exact": 1}
 
@@ -47,6 +47,8 @@ class FwdArgs(C.Structure):
         ("hit_log_cap", C.c_int), ("hit_log_prim_dev", _fp), ("hit_log_t_dev", _fp), ("hit_log_code_dev", _fp),
         ("hit_log_count_dev", _fp), ("interval_in_dev", _fp), ("interval_out_dev", _fp),
         ("interval_hard_cap", C.c_int), ("interval_fallback_start", C.c_float), ("interval_fallback_end", C.c_float),
+        ("reflectivity_dev", _fp), ("ior_dev", _fp), ("envmap_dev", _fp), ("env_width", C.c_int),
+        ("env_height", C.c_int), ("bounce_aux_dev", _fp),
     ]
 
 
@@ -63,7 +65,9 @@ class BwdArgs(C.Structure):
         ("cutoff", C.c_float), ("near_t", C.c_float), ("far_t", C.c_float),
         ("zero_grads", C.c_int), ("dL_dvertices_rows", C.c_int),
         ("dL_dvertices_dev", _fp), ("dL_dsigma_dev", _fp), ("dL_dopacity_dev", _fp), ("dL_dsh_dev", _fp),
-        ("interval_dev", _fp),
+        ("interval_dev", _fp), ("variant", C.c_int), ("reflectivity_dev", _fp), ("envmap_dev", _fp),
+        ("env_width", C.c_int), ("env_height", C.c_int), ("bounce_aux_dev", _fp), ("dL_dreflectivity_dev", _fp),
+        ("dL_denvmap_dev", _fp),
     ]
 
 
@@ -286,7 +290,8 @@ class TracerHandle:
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
              scaling, k=16, cutoff=1e-4, chunk_mode="compat", variant="forward", extra=None, stats=None,
              log=None, near=0.2, far=1000.0, trav_stats=None, interval_in=None, interval_out=None,
-             hard_cap=False, fallback_start=0.0, fallback_end=0.0) -> FwdArgs:
+             hard_cap=False, fallback_start=0.0, fallback_end=0.0, bounce=None) -> FwdArgs:
+    """bounce: dict(reflectivity [P], ior [P], envmap [H, W, 3], aux [R, 20]) for variant "bounce"."""
     a = FwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -309,6 +314,10 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.interval_in_dev, a.interval_out_dev = ptr(interval_in), ptr(interval_out)
     a.interval_hard_cap = 1 if hard_cap else 0
     a.interval_fallback_start, a.interval_fallback_end = float(fallback_start), float(fallback_end)
+    if bounce is not None:
+        a.reflectivity_dev, a.ior_dev = ptr(bounce["reflectivity"]), ptr(bounce["ior"])
+        a.envmap_dev, a.bounce_aux_dev = ptr(bounce["envmap"]), ptr(bounce["aux"])
+        a.env_height, a.env_width = int(bounce["envmap"].shape[0]), int(bounce["envmap"].shape[1])
     if log is not None:
         prim, t, code, count = log
         a.hit_log_cap = int(prim.shape[-1])
@@ -319,7 +328,9 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
 
 def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, color, others, dL_dpix,
              dL_dothers, dverts, dsigma, dopac, dsh, k=16, cutoff=1e-4, chunk_mode="compat", zero_grads=True,
-             near=0.2, far=1000.0, interval=None) -> BwdArgs:
+             near=0.2, far=1000.0, interval=None, bounce=None) -> BwdArgs:
+    """bounce: dict(reflectivity, envmap, aux, dreflectivity [P], denvmap [H, W, 3]) selects the
+    bounce variant's backward."""
     a = BwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -337,6 +348,12 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.dL_dvertices_rows = int(dverts.numel() // 3)
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
     a.interval_dev = ptr(interval)
+    if bounce is not None:
+        a.variant = VARIANTS["bounce"]
+        a.reflectivity_dev, a.envmap_dev = ptr(bounce["reflectivity"]), ptr(bounce["envmap"])
+        a.env_height, a.env_width = int(bounce["envmap"].shape[0]), int(bounce["envmap"].shape[1])
+        a.bounce_aux_dev = ptr(bounce["aux"])
+        a.dL_dreflectivity_dev, a.dL_denvmap_dev = ptr(bounce["dreflectivity"]), ptr(bounce["denvmap"])
     return a
 
 

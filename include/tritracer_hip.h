@@ -49,6 +49,9 @@ extern "C" {
 #define TT_VARIANT_REFLECT 1  /* reflect.cu  */
 #define TT_VARIANT_REFRACT 2  /* refract.cu  */
 #define TT_VARIANT_ENVLIGHT 3 /* envlight.cu */
+/* no counterpart in the reference: per-triangle reflectivity / IOR, one reflected and one
+   refracted ray per pixel from the primary's expected surface, equirect environment light */
+#define TT_VARIANT_BOUNCE 4
 
 /* acceleration-structure modes: the reference's rebuild_gas codes
    (diff_tritracer/__init__.py:42, tracer.cpp:181-195) */
@@ -127,6 +130,16 @@ typedef struct {
                                     hit yet or whose last blended hit lies within
                                     interval_fallback_end of cap goes on past it */
   float interval_fallback_start, interval_fallback_end;
+  /* TT_VARIANT_BOUNCE inputs. Per pixel, with the primary's blending weights w = alpha T:
+     colour = C + R L_r + Q L_t + T E(d), R = sum w rho, Q = sum w (1 - rho), L = C' + T' E(d')
+     of the reflected / refracted ray, E = bilinear equirect lookup; the secondary rays start
+     at the expected surface (depth sum w t / (1 - T), normal sum w n turned to face the
+     ray, IOR sum w eta / (1 - T), Snell with ratio 1 / IOR). others as TT_VARIANT_FORWARD. */
+  const float *reflectivity_dev; /* [P] in [0, 1] */
+  const float *ior_dev;          /* [P] */
+  const float *envmap_dev;       /* [env_height][env_width][3] */
+  int env_width, env_height;
+  float *bounce_aux_dev;         /* [R][20] written: the record tt_trace_bwd reads */
 } tt_fwd_args;
 
 /* Replaces OptixTracer::trace_fwd's launch (tracer.cpp:105-132) and the raygen programs
@@ -158,6 +171,15 @@ typedef struct {
   /* optional [R][4]: the forward's interval_out (TT_CHUNK_EXACT). The re-traversal then
      covers only [first blended hit, last blended hit] of each ray: same gradients. */
   const float *interval_dev;
+  /* TT_VARIANT_BOUNCE: the forward's inputs and bounce_aux record; gradients also to the
+     reflectivity and the environment map (accumulated, zeroed with zero_grads). The
+     secondary rays' geometry and the IOR are held fixed (no gradient through the bounce). */
+  int variant;                   /* TT_VARIANT_FORWARD (0) or TT_VARIANT_BOUNCE */
+  const float *reflectivity_dev, *envmap_dev;
+  int env_width, env_height;
+  const float *bounce_aux_dev;
+  float *dL_dreflectivity_dev;   /* [P] */
+  float *dL_denvmap_dev;         /* [env_height][env_width][3] */
 } tt_bwd_args;
 
 /* Replaces OptixTracer::trace_bwd's launch (tracer.cpp:199-222) and backward.cu:191-391:

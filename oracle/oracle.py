@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-VARIANTS = {"forward": 0, "reflect": 1, "refract": 2, "envlight": 3}
+VARIANTS = {"forward": 0, "reflect": 1, "refract": 2, "envlight": 3, "bounce": 4}
 CODE_NEAR, CODE_ALPHA, CODE_BLEND, CODE_TERM = 0, 1, 2, 3
 
 
@@ -45,6 +45,11 @@ class _Opts(C.Structure):
         ("extra", C.POINTER(C.c_float)),
         ("extra_len", C.c_int),
         ("tstart", C.POINTER(C.c_float)),
+        ("refl", C.POINTER(C.c_float)),
+        ("ior", C.POINTER(C.c_float)),
+        ("env", C.POINTER(C.c_float)),
+        ("env_w", C.c_int),
+        ("env_h", C.c_int),
     ]
 
 
@@ -97,6 +102,14 @@ def lib():
         L.or_backward_f64.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.or_loss_f64.restype = C.c_double
         L.or_loss_f64.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.or_forward_f64_bounce.restype = C.c_int
+        L.or_forward_f64_bounce.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_loss_f64_bounce.restype = C.c_double
+        L.or_loss_f64_bounce.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_backward_f64_bounce.restype = C.c_int
+        L.or_backward_f64_bounce.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_env_taps.argtypes = [fp, C.c_int, C.c_int, ip, dp]
+        L.or_log_segment_rays.argtypes = [vp, fp, fp, ip]
         _lib = L
     return _lib
 
@@ -218,9 +231,10 @@ class Scene:
 
 
 def make_opts(k=16, chunk_mode="compat", variant="forward", use_bvh=True, cutoff=1e-4, near=0.2, far=1000.0,
-              bg=(0.0, 0.0, 0.0), extra=None, tstart=None):
+              bg=(0.0, 0.0, 0.0), extra=None, tstart=None, refl=None, ior=None, env=None):
     """tstart: optional per-ray start distance [R] (forward variant, exact chunk mode;
-    honoured by forward_f32 and the decision log)."""
+    honoured by forward_f32 and the decision log). refl, ior [P] and env [H][W][3]: the
+    bounce variant's per-triangle reflectivity, index of refraction and environment map."""
     o = _Opts()
     o.k = int(k)
     o.chunk_mode = {"compat": 0, "exact": 1}[chunk_mode] if isinstance(chunk_mode, str) else int(chunk_mode)
@@ -246,6 +260,16 @@ def make_opts(k=16, chunk_mode="compat", variant="forward", use_bvh=True, cutoff
         o._keep_ts = ts
     else:
         o.tstart = C.POINTER(C.c_float)()
+    o._keep_b = []
+    for name, arr in (("refl", refl), ("ior", ior), ("env", env)):
+        if arr is not None:
+            a = _f32(arr)
+            setattr(o, name, _f(a.reshape(-1)))
+            o._keep_b.append(a)
+        else:
+            setattr(o, name, C.POINTER(C.c_float)())
+    if env is not None:
+        o.env_h, o.env_w = int(np.shape(env)[0]), int(np.shape(env)[1])
     return o
 
 
@@ -256,6 +280,14 @@ def gather(scene: Scene, o, d, tlo=0.0, plo=-1, k=16, far=1000.0, use_bvh=True):
     n = lib().or_gather(scene.h, _f(o), _f(d), float(tlo), int(plo), int(k), float(far), 1 if use_bvh else 0,
                         _i(prim), _f(t))
     return prim.view(np.uint32)[:n].astype(np.int64), t[:n]
+
+
+def env_taps(d, env_w, env_h):
+    """Texel indices (y * env_w + x) and bilinear weights of the environment lookup of d."""
+    idx = np.zeros(4, np.int32)
+    w = np.zeros(4, np.float64)
+    lib().or_env_taps(_f(_f32(d).reshape(3)), int(env_w), int(env_h), _i(idx), _d(w))
+    return idx, w
 
 
 def forward_f32(scene: Scene, ray_o, ray_d, opts, scaling=None, density=None):
@@ -320,6 +352,15 @@ class Log:
                             _i(codes))
         return off, spix, prims[:n], ts[:n], codes[:n]
 
+    def segment_rays(self):
+        """Per segment: origin [3], direction [3] (float32) and role (0 primary, 1 reflected, 2 refracted)."""
+        ns = int(lib().or_log_segments(self.h))
+        o = np.zeros((ns, 3), np.float32)
+        d = np.zeros((ns, 3), np.float32)
+        role = np.zeros(ns, np.int32)
+        lib().or_log_segment_rays(self.h, _f(o), _f(d), _i(role))
+        return o, d, role
+
     def params64(self, verts=None, opac=None, sigma=None, shs=None):
         s = self.scene
         v = np.ascontiguousarray(s.verts if verts is None else verts, np.float64).reshape(-1, 3)
@@ -348,6 +389,38 @@ class Log:
         if rc != 0:
             raise RuntimeError("f64 backward is defined for the forward variant only")
         return dict(dverts=dv, dsigma=dsig, dopac=dop, dsh=dsh)
+
+    def _b64(self, refl, env):
+        return (np.ascontiguousarray(refl, np.float64).reshape(-1),
+                np.ascontiguousarray(env, np.float64).reshape(-1))
+
+    def forward_f64_bounce(self, refl, env, **params):
+        v, o, g, h = self.params64(**params)
+        r, e = self._b64(refl, env)
+        color = np.zeros((3, self.R), np.float64)
+        others = np.zeros((7, self.R), np.float64)
+        if lib().or_forward_f64_bounce(self.h, _d(v), _d(o), _d(g), _d(h), _d(r), _d(e), _d(color), _d(others)):
+            raise RuntimeError("the log is not of the bounce variant")
+        return color, others
+
+    def loss_f64_bounce(self, refl, env, dL_dpix, dL_dothers, **params):
+        v, o, g, h = self.params64(**params)
+        r, e = self._b64(refl, env)
+        gp = np.ascontiguousarray(dL_dpix, np.float64).reshape(3, self.R)
+        go = np.ascontiguousarray(dL_dothers, np.float64).reshape(7, self.R)
+        return float(lib().or_loss_f64_bounce(self.h, _d(v), _d(o), _d(g), _d(h), _d(r), _d(e), _d(gp), _d(go)))
+
+    def backward_f64_bounce(self, refl, env, dL_dpix, dL_dothers, **params):
+        v, o, g, h = self.params64(**params)
+        r, e = self._b64(refl, env)
+        gp = np.ascontiguousarray(dL_dpix, np.float64).reshape(3, self.R)
+        go = np.ascontiguousarray(dL_dothers, np.float64).reshape(7, self.R)
+        dv, dsig, dop, dsh = np.zeros_like(v), np.zeros_like(g), np.zeros_like(o), np.zeros_like(h)
+        dr, de = np.zeros_like(r), np.zeros_like(e)
+        if lib().or_backward_f64_bounce(self.h, _d(v), _d(o), _d(g), _d(h), _d(r), _d(e), _d(gp), _d(go), _d(dv),
+                                        _d(dsig), _d(dop), _d(dsh), _d(dr), _d(de)):
+            raise RuntimeError("the log is not of the bounce variant")
+        return dict(dverts=dv, dsigma=dsig, dopac=dop, dsh=dsh, drefl=dr, denv=de.reshape(np.shape(env)))
 
     def loss_f64(self, dL_dpix, dL_dothers, **params):
         v, o, g, h = self.params64(**params)

@@ -717,6 +717,8 @@ struct Opts {
   float bg[3];
   std::vector<float> extra;
   const float *tstart = nullptr;
+  const float *refl = nullptr, *ior = nullptr, *env = nullptr;
+  int env_w = 0, env_h = 0;
 };
 Opts to_opts(const or_opts *o) {
   Opts p;
@@ -730,6 +732,11 @@ Opts to_opts(const or_opts *o) {
   for (int i = 0; i < 3; i++) p.bg[i] = o->bg[i];
   if (o->extra && o->extra_len > 0) p.extra.assign(o->extra, o->extra + o->extra_len);
   p.tstart = o->variant == 0 ? o->tstart : nullptr;
+  p.refl = o->refl;
+  p.ior = o->ior;
+  p.env = o->env;
+  p.env_w = o->env_w;
+  p.env_h = o->env_h;
   return p;
 }
 
@@ -744,6 +751,13 @@ struct Segment {
   V3f o, d;
   double weight;  // 1, or 1/samples for envlight hemisphere samples
   std::vector<LogHit> hits;
+  int role = 0;   // bounce variant: 0 primary, 1 reflected, 2 refracted
+};
+
+// Bounce variant: sums over a segment's blended hits of w*rho, w*(1-rho), w*eta, w*t
+// (w = alpha * T, the blending weight).
+struct BounceAcc {
+  float R = 0.0f, Q = 0.0f, eta = 0.0f, D = 0.0f;
 };
 
 inline void atomic_max_f(float *p, float v) {
@@ -791,7 +805,7 @@ struct FwdSinks {
 // Returns with C, T, N updated; counts consumed/blended hits; optionally logs hits.
 void trace_segment_f32(const Scene &s, const Opts &op, V3f o, V3f d, float C[3], float &T, float N[3],
                        int &consumed, int &blended, const FwdSinks *sinks, std::vector<LogHit> *log,
-                       float tstart = 0.0f) {
+                       float tstart = 0.0f, BounceAcc *acc = nullptr) {
   RayPre r = prep_ray(o, d);
   std::vector<Hit> buf(op.k);
   float last = 0.0f;
@@ -828,6 +842,8 @@ void trace_segment_f32(const Scene &s, const Opts &op, V3f o, V3f d, float C[3],
         sinks->density[prim] = 1.0f;
         compute_scaling(s, prim, sinks->scaling_org, sinks->scaling);
         atomic_max_f(sinks->maxb + prim, bw);
+      }
+      if (sinks || acc) {
         N[0] += w.n.x * bw;
         N[1] += w.n.y * bw;
         N[2] += w.n.z * bw;
@@ -836,6 +852,13 @@ void trace_segment_f32(const Scene &s, const Opts &op, V3f o, V3f d, float C[3],
       C[0] += col.x * bw;
       C[1] += col.y * bw;
       C[2] += col.z * bw;
+      if (acc) {
+        const float rho = op.refl[prim];
+        acc->R += bw * rho;
+        acc->Q += bw * (1.0f - rho);
+        acc->eta += bw * op.ior[prim];
+        acc->D += bw * t;
+      }
       T = testT;
       blended++;
       if (log) {
@@ -916,6 +939,87 @@ inline V3f cos_hemi(uint32_t &seed, V3f nrm) {
   V3f bb = normalize(cross(nrm, tt));
   V3f wd = dl.x * tt + dl.y * bb + dl.z * nrm;
   return normalize(wd);
+}
+
+// ---- bounce variant (no counterpart in the reference; a project-defined model)
+// Equirect environment [H][W][3]: direction -> phi = atan2(x, -z), theta = acos(y) (in
+// double, after normalising in double), u = (phi / 2pi + 0.5) W, v = theta / pi H,
+// bilinear over texel centres, wrapping in u and clamped in v.
+constexpr double kInvPi = 0.31830988618379067154, kInv2Pi = 0.15915494309189533577;
+struct EnvTaps {
+  int idx[4];
+  double w[4];
+};
+inline EnvTaps env_taps(float fx_, float fy_, float fz_, int W, int H) {
+  EnvTaps e;
+  double dx = fx_, dy = fy_, dz = fz_;
+  const double n = std::sqrt((dx * dx + dy * dy) + dz * dz);
+  if (!(n > 0.0)) {
+    for (int j = 0; j < 4; j++) e.idx[j] = 0, e.w[j] = j == 0 ? 1.0 : 0.0;
+    return e;
+  }
+  dx /= n, dy /= n, dz /= n;
+  const double phi = std::atan2(dx, -dz);
+  const double theta = std::acos(std::fmin(1.0, std::fmax(-1.0, dy)));
+  const double u = (phi * kInv2Pi + 0.5) * (double)W - 0.5;
+  const double v = theta * kInvPi * (double)H - 0.5;
+  const double x0 = std::floor(u), y0 = std::floor(v);
+  const double fx = u - x0, fy = v - y0;
+  const int ix = (int)x0, iy = (int)y0;
+  const int xa = ((ix % W) + W) % W, xb = (xa + 1) % W;
+  const int ya = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy), yb = iy + 1 < 0 ? 0 : (iy + 1 > H - 1 ? H - 1 : iy + 1);
+  e.idx[0] = ya * W + xa, e.idx[1] = ya * W + xb, e.idx[2] = yb * W + xa, e.idx[3] = yb * W + xb;
+  e.w[0] = (1.0 - fx) * (1.0 - fy), e.w[1] = fx * (1.0 - fy), e.w[2] = (1.0 - fx) * fy, e.w[3] = fx * fy;
+  return e;
+}
+inline void env_f32(const Opts &op, V3f d, float out[3]) {
+  const EnvTaps e = env_taps(d.x, d.y, d.z, op.env_w, op.env_h);
+  const float w0 = (float)e.w[0], w1 = (float)e.w[1], w2 = (float)e.w[2], w3 = (float)e.w[3];
+  for (int c = 0; c < 3; c++)
+    out[c] = ((w0 * op.env[3 * e.idx[0] + c] + w1 * op.env[3 * e.idx[1] + c]) + w2 * op.env[3 * e.idx[2] + c]) +
+             w3 * op.env[3 * e.idx[3] + c];
+}
+inline void env_f64(const double *env, int W, int H, V3f d, double out[3]) {
+  const EnvTaps e = env_taps(d.x, d.y, d.z, W, H);
+  for (int c = 0; c < 3; c++) {
+    out[c] = 0.0;
+    for (int j = 0; j < 4; j++) out[c] += e.w[j] * env[3 * e.idx[j] + c];
+  }
+}
+
+// The two secondary rays of a pixel from its primary segment: expected normal
+// N / |N| turned to face the ray, expected depth D / A (A = 1 - T), expected IOR
+// eta / A; reflected ray from x + n eps, refracted ray (Snell, entering with ratio
+// 1/eta; total internal reflection -> the reflected direction) from x - n eps.
+struct BounceGeom {
+  bool valid = false;
+  V3f orr, dr, ot, dt;
+};
+inline BounceGeom bounce_geometry(V3f o, V3f d, float T, const float N[3], const BounceAcc &acc) {
+  BounceGeom g;
+  const float A = 1.0f - T;
+  const float nn = (N[0] * N[0] + N[1] * N[1]) + N[2] * N[2];
+  if (!(A > 1e-6f) || !(nn > 1e-24f)) return g;
+  V3f n = normalize(mk(N[0], N[1], N[2]));
+  if (dot(n, d) > 0.0f) n = -n;
+  const float tbar = acc.D / A;
+  const V3f x = o + d * tbar;
+  const float eta = acc.eta / A;
+  const V3f r = reflect3(d, n);
+  const float cosi = -dot(d, n);
+  const float ratio = 1.0f / eta;
+  const float k = 1.0f - (ratio * ratio) * (1.0f - cosi * cosi);
+  g.valid = true;
+  g.orr = x + n * kEps32;
+  g.dr = r;
+  if (k < 0.0f) {
+    g.ot = g.orr;
+    g.dt = r;
+  } else {
+    g.ot = x - n * kEps32;
+    g.dt = d * ratio + n * (ratio * cosi - std::sqrt(k));
+  }
+  return g;
 }
 
 // Per pixel: the segments (traced rays) a variant launches, mirroring the raygens.
@@ -1098,9 +1202,29 @@ int or_forward_f32(void *scene, const float *o, const float *d, int R, const or_
         trace_segment_f32(s, op, so, sd, C, T, N, consumed, blended, op.variant == 0 ? &sk : nullptr, nullptr,
                           op.tstart ? op.tstart[pix] : 0.0f);
     };
-    pixel_segments(op, pix, ro, rd, seg, C, T);
-    for (int c = 0; c < 3; c++) color[(size_t)c * R + pix] = C[c] + T * op.bg[c];
-    if (op.variant == 0 && others) {
+    if (op.variant == 4) {
+      BounceAcc acc;
+      trace_segment_f32(s, op, ro, rd, C, T, N, consumed, blended, &sk, nullptr, 0.0f, &acc);
+      const BounceGeom g = bounce_geometry(ro, rd, T, N, acc);
+      float Lr[3] = {0, 0, 0}, Lt[3] = {0, 0, 0};
+      if (g.valid) {
+        float Cr[3] = {0, 0, 0}, Tr = 1.0f, Nr[3] = {0, 0, 0}, Er[3];
+        trace_segment_f32(s, op, g.orr, g.dr, Cr, Tr, Nr, consumed, blended, nullptr, nullptr);
+        env_f32(op, g.dr, Er);
+        float Ct[3] = {0, 0, 0}, Tt = 1.0f, Nt[3] = {0, 0, 0}, Et[3];
+        trace_segment_f32(s, op, g.ot, g.dt, Ct, Tt, Nt, consumed, blended, nullptr, nullptr);
+        env_f32(op, g.dt, Et);
+        for (int c = 0; c < 3; c++) Lr[c] = Cr[c] + Tr * Er[c], Lt[c] = Ct[c] + Tt * Et[c];
+      }
+      float E[3];
+      env_f32(op, rd, E);
+      for (int c = 0; c < 3; c++)
+        color[(size_t)c * R + pix] = ((C[c] + acc.R * Lr[c]) + acc.Q * Lt[c]) + T * E[c];
+    } else {
+      pixel_segments(op, pix, ro, rd, seg, C, T);
+      for (int c = 0; c < 3; c++) color[(size_t)c * R + pix] = C[c] + T * op.bg[c];
+    }
+    if ((op.variant == 0 || op.variant == 4) && others) {
       others[0 * (size_t)R + pix] = 0.0f;
       others[1 * (size_t)R + pix] = 1.0f - T;
       for (int c = 0; c < 3; c++) others[(size_t)(2 + c) * R + pix] = N[c];
@@ -1251,6 +1375,26 @@ void *or_record(void *scene, const float *o, const float *d, int R, const or_opt
       }
       per[pix].push_back(std::move(sg));
     };
+    if (op.variant == 4) {
+      Segment sg;
+      sg.pix = pix, sg.o = ro, sg.d = rd, sg.weight = 1.0, sg.role = 0;
+      float Nl[3] = {0, 0, 0};
+      BounceAcc acc;
+      trace_segment_f32(s, op, ro, rd, C, T, Nl, consumed, blended, nullptr, &sg.hits, 0.0f, &acc);
+      per[pix].push_back(std::move(sg));
+      const BounceGeom g = bounce_geometry(ro, rd, T, Nl, acc);
+      if (g.valid) {
+        for (int role = 1; role <= 2; role++) {
+          Segment sb;
+          sb.pix = pix, sb.o = role == 1 ? g.orr : g.ot, sb.d = role == 1 ? g.dr : g.dt, sb.weight = 1.0;
+          sb.role = role;
+          float Cl[3] = {0, 0, 0}, Tl = 1.0f, Nb[3] = {0, 0, 0};
+          trace_segment_f32(s, op, sb.o, sb.d, Cl, Tl, Nb, consumed, blended, nullptr, &sb.hits);
+          per[pix].push_back(std::move(sb));
+        }
+      }
+      continue;
+    }
     pixel_segments(op, pix, ro, rd, seg, C, T);
   }
   lg->pix_segs.resize(R);
@@ -1293,8 +1437,10 @@ struct P64 {
   const double *verts, *opac, *sigma, *shs;
 };
 
-// Replay of one segment in double with frozen decisions, t and incenter distance.
-void replay_segment_f64(const Log &lg, const Segment &sg, const P64 &p, double C[3], double &T, double N[3]) {
+// Replay of one segment in double with frozen decisions, t and incenter distance
+// (refl: also the sums of w*rho and w*(1-rho) of the bounce variant).
+void replay_segment_f64(const Log &lg, const Segment &sg, const P64 &p, double C[3], double &T, double N[3],
+                        const double *refl = nullptr, double *Rw = nullptr, double *Qw = nullptr) {
   const Scene &s = *lg.s;
   V3d o = mk<double>(sg.o.x, sg.o.y, sg.o.z), d = mk<double>(sg.d.x, sg.d.y, sg.d.z);
   for (const LogHit &h : sg.hits) {
@@ -1312,8 +1458,128 @@ void replay_segment_f64(const Log &lg, const Segment &sg, const P64 &p, double C
     N[0] += w.n.x * bw, N[1] += w.n.y * bw, N[2] += w.n.z * bw;
     V3d col = sh_color<double>(p.shs + (size_t)prim * s.M * 3, s.deg, d, nullptr);
     C[0] += col.x * bw, C[1] += col.y * bw, C[2] += col.z * bw;
+    if (refl) *Rw += bw * refl[prim], *Qw += bw * (1.0 - refl[prim]);
     T = T * (1.0 - alpha);
   }
+}
+
+struct B64 {
+  const double *refl, *env;
+  int W, H;
+};
+
+// Bounce variant in double: per pixel the secondary composites L = C + T E(dir), then
+// colour = C_p + R L_r + Q L_t + T_p E(d) (R, Q: sums of w rho, w (1 - rho)).
+void forward_f64_bounce(const Log &lg, const P64 &p, const B64 &b, double *color, double *others,
+                        double *Lbuf = nullptr) {
+  const int R = lg.R;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int pix = 0; pix < R; pix++) {
+    double C[3] = {0, 0, 0}, N[3] = {0, 0, 0}, T = 1.0, Rw = 0.0, Qw = 0.0;
+    double L[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    const Segment *prim = nullptr;
+    for (int si : lg.pix_segs[pix]) {
+      const Segment &sg = lg.segs[si];
+      if (sg.role == 0) {
+        prim = &sg;
+        replay_segment_f64(lg, sg, p, C, T, N, b.refl, &Rw, &Qw);
+      } else {
+        double Cs[3] = {0, 0, 0}, Ns[3] = {0, 0, 0}, Ts = 1.0, E[3];
+        replay_segment_f64(lg, sg, p, Cs, Ts, Ns);
+        env_f64(b.env, b.W, b.H, sg.d, E);
+        for (int c = 0; c < 3; c++) L[sg.role - 1][c] = Cs[c] + Ts * E[c];
+      }
+    }
+    double E0[3] = {0, 0, 0};
+    if (prim) env_f64(b.env, b.W, b.H, prim->d, E0);
+    for (int c = 0; c < 3; c++) color[(size_t)c * R + pix] = C[c] + Rw * L[0][c] + Qw * L[1][c] + T * E0[c];
+    if (Lbuf)
+      for (int c = 0; c < 3; c++) Lbuf[6 * (size_t)pix + c] = L[0][c], Lbuf[6 * (size_t)pix + 3 + c] = L[1][c];
+    if (others) {
+      others[pix] = 0.0;
+      others[(size_t)R + pix] = 1.0 - T;
+      for (int c = 0; c < 3; c++) others[(size_t)(2 + c) * R + pix] = N[c];
+      others[5 * (size_t)R + pix] = 0.0;
+      others[6 * (size_t)R + pix] = 0.0;
+    }
+  }
+}
+
+struct G64 {
+  double *dverts, *dsigma, *dopac, *dsh, *drefl;
+};
+
+// Backward of one replayed segment in double (backward.cu:191-350 per hit) for an
+// upstream gradient g on the segment's output colour `fin` (composite incl. its
+// background), optional depth / normal terms, and (bounce primary) per-hit colours
+// extended by rho L_r + (1 - rho) L_t with their reflectivity gradient. Returns the
+// segment's final transmittance.
+double backward_segment_f64(const Log &lg, const Segment &sg, const P64 &p, const double g[3], const double fin[3],
+                            double dL_ddepth, V3d dL_dn2, double final_depth, V3d final_normal, const double *refl,
+                            const double *Lr, const double *Lt, const G64 &out) {
+  const Scene &s = *lg.s;
+  V3d o = mk<double>(sg.o.x, sg.o.y, sg.o.z), d = mk<double>(sg.d.x, sg.d.y, sg.d.z);
+  double C[3] = {0, 0, 0}, N[3] = {0, 0, 0}, D = 0.0, T = 1.0;
+  for (const LogHit &h : sg.hits) {
+    if (h.code == 3) break;
+    if (h.code != 2) continue;
+    int prim = (int)h.prim;
+    int off = s.cumsum[prim];
+    V3d V[3] = {ld3d<double>(p.verts + 3 * (size_t)off), ld3d<double>(p.verts + 3 * (size_t)(off + 1)),
+                ld3d<double>(p.verts + 3 * (size_t)(off + 2))};
+    double t = (double)h.t;
+    V3d hit = o + t * d;
+    Win<double> w;
+    window<double>(V, hit, p.sigma[prim], w, &h.dist);
+    double opacity = p.opac[prim];
+    double alpha = std::fmin(0.99, opacity * w.Cx);
+    double bw = alpha * T;
+    bool cl[3];
+    V3d col = sh_color<double>(p.shs + (size_t)prim * s.M * 3, s.deg, d, cl);
+    double cc[3] = {col.x, col.y, col.z};
+    if (Lr) {
+      const double rho = refl[prim];
+      double dr = 0.0;
+      for (int c = 0; c < 3; c++) {
+        dr += g[c] * (Lr[c] - Lt[c]);
+        cc[c] += rho * Lr[c] + (1.0 - rho) * Lt[c];
+      }
+      out.drefl[prim] += bw * dr;
+    }
+    C[0] += cc[0] * bw, C[1] += cc[1] * bw, C[2] += cc[2] * bw;
+    N[0] += w.n.x * bw, N[1] += w.n.y * bw, N[2] += w.n.z * bw;
+    D += bw * t;
+    double dL_dcolor[3];
+    double dL_dalpha = 0.0, inv1a = 1.0 / (1.0 - alpha);
+    for (int c = 0; c < 3; c++) {
+      dL_dcolor[c] = g[c] * bw;
+      dL_dalpha += g[c] * (T * cc[c] - (fin[c] - C[c]) * inv1a);
+    }
+    dL_dalpha += dL_ddepth * (T * t - (final_depth - D) * inv1a);
+    V3d Nf = mk(N[0], N[1], N[2]);
+    dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
+    out.dopac[prim] += dL_dalpha * w.Cx;
+    double dL_dC = opacity * dL_dalpha;
+    if (w.phi_final > 0.0) out.dsigma[prim] += dL_dC * w.Cx * std::log(w.phi_final);
+    double dL_dphi = p.sigma[prim] / w.phi_max * dL_dC * w.Cx;
+    V3d dL_dn3 = dL_dn2 * bw;
+    window_bwd<double>(V, w.rel, dL_dphi, w.phi_max, w.ed, dL_dn3, [&](int j, V3d gv) {
+      double *q = out.dverts + 3 * (size_t)(off + j);
+      q[0] += gv.x, q[1] += gv.y, q[2] += gv.z;
+    });
+    sh_bwd<double>(s.deg, d, cl, dL_dcolor, [&](int j, V3d gv) {
+      double *q = out.dsh + ((size_t)prim * s.M + j) * 3;
+      q[0] += gv.x, q[1] += gv.y, q[2] += gv.z;
+    });
+    T = T * (1.0 - alpha);
+  }
+  return T;
+}
+
+void env_scatter_f64(double *denv, int W, int H, V3f d, const double g[3]) {
+  const EnvTaps e = env_taps(d.x, d.y, d.z, W, H);
+  for (int j = 0; j < 4; j++)
+    for (int c = 0; c < 3; c++) denv[3 * e.idx[j] + c] += e.w[j] * g[c];
 }
 
 void forward_f64(const Log &lg, const P64 &p, double *color, double *others) {
@@ -1440,6 +1706,94 @@ int or_backward_f64(void *log, const double *verts, const double *opac, const do
     }
   }
   return 0;
+}
+
+int or_forward_f64_bounce(void *log, const double *verts, const double *opac, const double *sigma,
+                          const double *shs, const double *refl, const double *env, double *color, double *others) {
+  const Log &lg = *static_cast<Log *>(log);
+  if (lg.op.variant != 4) return -1;
+  forward_f64_bounce(lg, P64{verts, opac, sigma, shs}, B64{refl, env, lg.op.env_w, lg.op.env_h}, color, others);
+  return 0;
+}
+
+double or_loss_f64_bounce(void *log, const double *verts, const double *opac, const double *sigma,
+                          const double *shs, const double *refl, const double *env, const double *dL_dpix,
+                          const double *dL_dothers) {
+  const Log &lg = *static_cast<Log *>(log);
+  const int R = lg.R;
+  std::vector<double> color((size_t)3 * R), others((size_t)7 * R);
+  forward_f64_bounce(lg, P64{verts, opac, sigma, shs}, B64{refl, env, lg.op.env_w, lg.op.env_h}, color.data(),
+                     others.data());
+  double L = 0.0;
+  for (size_t i = 0; i < (size_t)3 * R; i++) L += dL_dpix[i] * color[i];
+  if (dL_dothers)
+    for (size_t i = 0; i < (size_t)7 * R; i++) L += dL_dothers[i] * others[i];
+  return L;
+}
+
+// Bounce variant backward in double over the frozen log: the primary segment with the
+// extended colours, its environment term, then each secondary segment with upstream
+// R g (reflected) or Q g (refracted) and its environment term.
+int or_backward_f64_bounce(void *log, const double *verts, const double *opac, const double *sigma,
+                           const double *shs, const double *refl, const double *env, const double *dL_dpix,
+                           const double *dL_dothers, double *dverts, double *dsigma, double *dopac, double *dsh,
+                           double *drefl, double *denv) {
+  const Log &lg = *static_cast<Log *>(log);
+  if (lg.op.variant != 4) return -1;
+  const int R = lg.R, W = lg.op.env_w, H = lg.op.env_h;
+  P64 p{verts, opac, sigma, shs};
+  B64 b{refl, env, W, H};
+  std::vector<double> color((size_t)3 * R), others((size_t)7 * R), L((size_t)6 * R);
+  forward_f64_bounce(lg, p, b, color.data(), others.data(), L.data());
+  G64 out{dverts, dsigma, dopac, dsh, drefl};
+  for (int pix = 0; pix < R; pix++) {
+    const Segment *prim = nullptr;
+    for (int si : lg.pix_segs[pix])
+      if (lg.segs[si].role == 0) prim = &lg.segs[si];
+    if (!prim) continue;
+    double g[3], fin[3];
+    for (int c = 0; c < 3; c++) g[c] = dL_dpix[(size_t)c * R + pix], fin[c] = color[(size_t)c * R + pix];
+    double dL_ddepth = dL_dothers ? dL_dothers[pix] : 0.0;
+    V3d dL_dn2 = dL_dothers ? mk(dL_dothers[2 * (size_t)R + pix], dL_dothers[3 * (size_t)R + pix],
+                                 dL_dothers[4 * (size_t)R + pix])
+                            : mk(0.0, 0.0, 0.0);
+    V3d final_normal = mk(others[2 * (size_t)R + pix], others[3 * (size_t)R + pix], others[4 * (size_t)R + pix]);
+    const double *Lr = &L[6 * (size_t)pix], *Lt = &L[6 * (size_t)pix + 3];
+    // R and Q of the primary (the same sums the forward formed)
+    double Cq[3] = {0, 0, 0}, Nq[3] = {0, 0, 0}, Tq = 1.0, Rw = 0.0, Qw = 0.0;
+    replay_segment_f64(lg, *prim, p, Cq, Tq, Nq, refl, &Rw, &Qw);
+    const double Tp = backward_segment_f64(lg, *prim, p, g, fin, dL_ddepth, dL_dn2, others[pix], final_normal, refl,
+                                           Lr, Lt, out);
+    double ge[3] = {Tp * g[0], Tp * g[1], Tp * g[2]};
+    env_scatter_f64(denv, W, H, prim->d, ge);
+    for (int si : lg.pix_segs[pix]) {
+      const Segment &sg = lg.segs[si];
+      if (sg.role == 0) continue;
+      const double wgt = sg.role == 1 ? Rw : Qw;
+      const double *Ls = sg.role == 1 ? Lr : Lt;
+      double gs[3] = {wgt * g[0], wgt * g[1], wgt * g[2]};
+      const double Ts = backward_segment_f64(lg, sg, p, gs, Ls, 0.0, mk(0.0, 0.0, 0.0), 0.0, mk(0.0, 0.0, 0.0),
+                                             nullptr, nullptr, nullptr, out);
+      double gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
+      env_scatter_f64(denv, W, H, sg.d, gse);
+    }
+  }
+  return 0;
+}
+
+void or_log_segment_rays(void *log, float *o, float *d, int *role) {
+  const Log &lg = *static_cast<Log *>(log);
+  for (size_t i = 0; i < lg.segs.size(); i++) {
+    const Segment &sg = lg.segs[i];
+    o[3 * i] = sg.o.x, o[3 * i + 1] = sg.o.y, o[3 * i + 2] = sg.o.z;
+    d[3 * i] = sg.d.x, d[3 * i + 1] = sg.d.y, d[3 * i + 2] = sg.d.z;
+    role[i] = sg.role;
+  }
+}
+
+void or_env_taps(const float *d, int env_w, int env_h, int *idx, double *w) {
+  const EnvTaps e = env_taps(d[0], d[1], d[2], env_w, env_h);
+  for (int j = 0; j < 4; j++) idx[j] = e.idx[j], w[j] = e.w[j];
 }
 
 }  // extern "C"

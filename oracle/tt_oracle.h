@@ -49,6 +49,10 @@ typedef struct {
   int extra_len;      /* number of floats in extra */
   const float *tstart; /* optional [R] (forward variant, exact mode; or_forward_f32 and or_record
                           only): each ray's trace starts at key (tstart[ray], -1) instead of 0 */
+  /* variant 4 (bounce): per-triangle reflectivity [P] and index of refraction [P], equirect
+     environment map [env_h][env_w][3] */
+  const float *refl, *ior, *env;
+  int env_w, env_h;
 } or_opts;
 
 /* ---- scene ---- */
@@ -99,6 +103,21 @@ int or_backward_f64(void *log, const double *verts, const double *opac, const do
                     double *dopac, double *dsh);
 double or_loss_f64(void *log, const double *verts, const double *opac, const double *sigma, const double *shs,
                    const double *dL_dpix, const double *dL_dothers);
+/* variant 4 (bounce) replays: reflectivity [P] and environment map [env_h][env_w][3] in double;
+   gradients also to them (the bounce geometry and the IOR are held fixed) */
+int or_forward_f64_bounce(void *log, const double *verts, const double *opac, const double *sigma,
+                          const double *shs, const double *refl, const double *env, double *color, double *others);
+double or_loss_f64_bounce(void *log, const double *verts, const double *opac, const double *sigma,
+                          const double *shs, const double *refl, const double *env, const double *dL_dpix,
+                          const double *dL_dothers);
+int or_backward_f64_bounce(void *log, const double *verts, const double *opac, const double *sigma,
+                           const double *shs, const double *refl, const double *env, const double *dL_dpix,
+                           const double *dL_dothers, double *dverts, double *dsigma, double *dopac, double *dsh,
+                           double *drefl, double *denv);
+/* per segment of the log: ray origin [3], direction [3], role (0 primary, 1 reflected, 2 refracted) */
+void or_log_segment_rays(void *log, float *o, float *d, int *role);
+/* environment lookup: 4 texel indices (y * env_w + x) and bilinear weights of direction d */
+void or_env_taps(const float *d, int env_w, int env_h, int *idx, double *w);
 
 #ifdef __cplusplus
 }
