@@ -173,6 +173,64 @@ def multiview(args, dev):
     print(json.dumps(line), flush=True)
 
 
+def bounce(args, dev):
+    """Secondary-ray variant: 1M clustered splats with per-triangle reflectivity ~ Beta(2, 5)
+    and IOR ~ U[1, 1.6], one reflected and one refracted ray per pixel, equirect environment
+    1024x512x3 with lognormal radiance, 1920x1080 camera; fwd+bwd with gradients to the
+    triangles, the reflectivity and the texels."""
+    import numpy as np
+    import torch
+    from diff_tritracer import capi, scenes
+
+    W, H = args.width, args.height
+    scene = scenes.clustered_scene(args.triangles, 64, seed=3)
+    rng = np.random.Generator(np.random.PCG64(3))
+    refl = rng.beta(2.0, 5.0, scene.P).astype(np.float32)
+    ior = rng.uniform(1.0, 1.6, scene.P).astype(np.float32)
+    env = np.exp(rng.normal(0.0, 0.5, (512, 1024, 3))).astype(np.float32)
+    fr = Frame(dev, scene, args.k, args.cutoff)
+    fr.h.build(fr.verts, fr.cumsum, fr.P, capi.BUILD, fr.sp)
+    fr.buffers(W, H)
+    eye = 4.0 * np.array([0.35, 0.3, 1.0]) / np.linalg.norm([0.35, 0.3, 1.0])
+    fr.camera(scenes.camera_basis(W, H, eye))
+    T = lambda a: torch.from_numpy(a).to(dev)
+    B = dict(reflectivity=T(refl), ior=T(ior), envmap=T(env), aux=torch.zeros(fr.R, 20, device=dev),
+             dreflectivity=torch.zeros(fr.P, device=dev), denvmap=torch.zeros(512, 1024, 3, device=dev))
+    stats = torch.zeros(fr.R, 2, dtype=torch.int32, device=dev)
+    c = capi
+    fa = c.fwd_args(W, H, fr.ro, fr.rd, fr.verts, fr.npts, fr.cumsum, fr.shs, 3, fr.opac, fr.sigma, fr.bg, fr.color,
+                    fr.others, fr.maxb, fr.dens, fr.scal, k=args.k, cutoff=args.cutoff, chunk_mode="exact",
+                    variant="bounce", bounce=B)
+    ba = c.bwd_args(W, H, fr.ro, fr.rd, fr.verts, fr.npts, fr.cumsum, fr.shs, 3, fr.opac, fr.sigma, fr.color,
+                    fr.others, fr.dpix, fr.doth, fr.dverts, fr.dsig, fr.dop, fr.dsh, k=args.k, cutoff=args.cutoff,
+                    chunk_mode="exact", zero_grads=True, bounce=B)
+    fs = c.fwd_args(W, H, fr.ro, fr.rd, fr.verts, fr.npts, fr.cumsum, fr.shs, 3, fr.opac, fr.sigma, fr.bg, fr.color,
+                    fr.others, fr.maxb, fr.dens, fr.scal, k=args.k, cutoff=args.cutoff, chunk_mode="exact",
+                    variant="bounce", bounce=B, stats=stats)
+    fr.h.fwd(fs, fr.sp)
+    torch.cuda.synchronize()
+    st = stats.to(torch.int64).sum(0).cpu().numpy()
+    sp = fr.sp
+    ms, per = timed(fr, [("fwd", lambda: fr.h.fwd(fa, sp)), ("bwd", lambda: fr.h.bwd(ba, sp))], args.steps,
+                    args.warmup)
+    R = fr.R
+    print(f"[bench] bounce {ms:.3f} ms/step " + " ".join(f"{k} {v:.3f}" for k, v in per.items()), file=sys.stderr,
+          flush=True)
+    line = {
+        "metric": "bounce variant fwd+bwd Mrays/s (primary + reflected + refracted ray per pixel)",
+        "value": round(R / (ms * 1e-3) / 1e6, 4), "unit": "Mrays/s (primary rays)", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (clustered splats, Beta(2,5) reflectivity, U[1,1.6] IOR, lognormal 1024x512 environment)",
+        "config": {"workload": f"bounce variant fwd+bwd, {scene.P} clustered splats (64 clusters, seed 3), one "
+                               f"{W}x{H} camera, k={args.k}, cutoff {args.cutoff:g}, exact chunk mode",
+                   "fwd_ms": round(per["fwd"], 4), "bwd_ms": round(per["bwd"], 4),
+                   "hits_consumed_per_pixel": round(float(st[0]) / R, 3),
+                   "hits_blended_per_pixel": round(float(st[1]) / R, 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 class Frame:
     """Scene tensors, the tracer handle and per-frame buffers of one workload."""
 
@@ -459,7 +517,7 @@ def main():
     ap.add_argument("--proposal-batch", type=int, default=16384)
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "multiview", "proposal"], default="frame")
+    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce"], default="frame")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates rocprof traffic counters)")
     args = ap.parse_args()
@@ -486,6 +544,9 @@ def main():
     if args.workload == "multiview":
         args.triangles = args.triangles or 2_000_000
         return multiview(args, dev)
+    if args.workload == "bounce":
+        args.triangles = args.triangles or 1_000_000
+        return bounce(args, dev)
     t0 = time.perf_counter()
     if args.workload == "proposal":
         P = args.triangles or 3_000_000

@@ -26,6 +26,12 @@ struct tt_tracer {
   hipGraphExec_t graph = nullptr;          // the recorded multi-camera step
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool build_timed = false;
+  // tile schedule measured by the last one-shot forward (per-tile wall time -> order),
+  // used by a backward over an image of the same size
+  float *tile_cost = nullptr;
+  int *tile_order = nullptr;
+  int tile_cap = 0;
+  int sched_W = -1, sched_H = -1;
 };
 
 namespace {
@@ -162,6 +168,8 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->capture_stream) (void)hipStreamDestroy(tr->capture_stream);
   if (tr->ev0) (void)hipEventDestroy(tr->ev0);
   if (tr->ev1) (void)hipEventDestroy(tr->ev1);
+  if (tr->tile_cost) (void)hipFree(tr->tile_cost);
+  if (tr->tile_order) (void)hipFree(tr->tile_order);
   (void)hipSetDevice(dev);
   delete tr;
   return ok();
@@ -271,8 +279,25 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
     p.aux = a->bounce_aux_dev;
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
+  const int nt = tt::trace_tiles(a->width, a->height);
+  if (nt > tr->tile_cap) {
+    if (tr->tile_cost) (void)hipFree(tr->tile_cost);
+    if (tr->tile_order) (void)hipFree(tr->tile_order);
+    tr->tile_cost = nullptr;
+    tr->tile_order = nullptr;
+    tr->tile_cap = 0;
+    if (hipMalloc(&tr->tile_cost, (size_t)nt * sizeof(float)) != hipSuccess ||
+        hipMalloc(&tr->tile_order, (size_t)nt * sizeof(int)) != hipSuccess)
+      return fail(TT_ERR_HIP, "hipMalloc of the tile schedule failed");
+    tr->tile_cap = nt;
+    tr->sched_W = tr->sched_H = -1;
+  }
+  p.tile_cost = tr->tile_cost;
+  p.tile_order = tr->tile_order;
   std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
   if (!err.empty()) return fail(TT_ERR_HIP, err);
+  tr->sched_W = a->width;
+  tr->sched_H = a->height;
   return ok();
 }
 
@@ -341,6 +366,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
     p.denv = a->dL_denvmap_dev;
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
+  if (tr->sched_W == a->width && tr->sched_H == a->height) p.tile_sched = tr->tile_order;
   std::string err = tt::launch_backward(p, a->k, a->variant, s);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   return ok();

@@ -94,6 +94,12 @@ struct TraceParams {
   int env_w, env_h;
   float *aux;
   float *drefl, *denv;
+  // tile schedule: the one-shot forward records each tile's wall time into tile_cost and
+  // then orders the tiles by it (descending) into tile_order; a backward given tile_order
+  // takes its tiles in that order (any permutation gives the same per-pixel results)
+  float *tile_cost;
+  int *tile_order;
+  const int *tile_sched;
 };
 // Per-pixel record of the bounce variant: L_r[3] L_t[3] R Q o_r[3] d_r[3] o_t[3] d_t[3]
 // (d_r = 0: no secondary rays).
@@ -120,6 +126,8 @@ std::string launch_fill_int(int *ptr, int n, int v, hipStream_t stream);
 // Message returned by tt_last_error() on this thread (shared by every entry point).
 void set_last_error(const std::string &msg);
 
+// Tiles (blocks) of a one-shot trace launch for an image of W x H.
+int trace_tiles(int W, int H);
 // Lanes a trace launch uses for an image of W x H (the stack spill needs one slot each).
 size_t trace_lanes(int W, int H);
 constexpr int kLdsStack = 16;  // traversal stack entries per lane kept in LDS; deeper ones spill

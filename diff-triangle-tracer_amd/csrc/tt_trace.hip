@@ -15,9 +15,47 @@ constexpr int kTileW = TileShape<kBlock>::W, kTileH = TileShape<kBlock>::H;
 template <int K, int VARIANT, bool COUNT = false, bool IVAL = false>
 __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
+  const long long c0 = P.tile_cost ? wall_clock64() : 0;
+  const int tile = xcd_tile(P.n_tiles);
   int x, y;
-  if (tile_pixel<kBlock>(P, xcd_tile(P.n_tiles), x, y))
-    forward_pixel<K, kBlock, VARIANT, COUNT, IVAL>(P, y * P.W + x, s_node);
+  if (tile_pixel<kBlock>(P, tile, x, y)) forward_pixel<K, kBlock, VARIANT, COUNT, IVAL>(P, y * P.W + x, s_node);
+  if (P.tile_cost) {  // the tile's wall time (100 MHz ticks): the backward's schedule key
+    __syncthreads();
+    if (threadIdx.x == 0) P.tile_cost[tile] = (float)(wall_clock64() - c0);
+  }
+}
+
+// Tiles ordered by cost, most expensive first (counting sort over 1024 buckets of
+// sqrt(cost) in one block). The backward then starts its heaviest tiles first, so the
+// end of the launch is not left to a few long blocks.
+__global__ __launch_bounds__(1024) void k_tile_order(const float *cost, int n, int *order) {
+  __shared__ unsigned int hist[1024];
+  __shared__ float s_max[16];
+  const int t = threadIdx.x;
+  float m = 0.0f;
+  for (int i = t; i < n; i += 1024) m = fmaxf(m, cost[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((t & 63) == 0) s_max[t >> 6] = m;
+  hist[t] = 0u;
+  __syncthreads();
+  float mx = 0.0f;
+  for (int w = 0; w < 16; w++) mx = fmaxf(mx, s_max[w]);
+  const float scale = mx > 0.0f ? 1023.0f / sqrtf(mx) : 0.0f;
+  auto bucket = [&](int i) { return 1023 - min(1023, (int)(sqrtf(fmaxf(cost[i], 0.0f)) * scale)); };
+  for (int i = t; i < n; i += 1024) atomicAdd(&hist[bucket(i)], 1u);
+  __syncthreads();
+  // exclusive scan of the 1024 bucket counts (Hillis-Steele, inclusive, then shift)
+  unsigned int v = hist[t];
+  for (int off = 1; off < 1024; off <<= 1) {
+    __syncthreads();
+    const unsigned int add = t >= off ? hist[t - off] : 0u;
+    __syncthreads();
+    hist[t] += add;
+  }
+  __syncthreads();
+  hist[t] -= v;
+  __syncthreads();
+  for (int i = t; i < n; i += 1024) order[atomicAdd(&hist[bucket(i)], 1u)] = i;
 }
 
 template <int K, int VARIANT = 0>
@@ -26,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
   __shared__ int s_keys[kBlock];
   __shared__ float s_acc[kFields * kBlock];
   int x, y;
-  const bool valid = tile_pixel<kBlock>(P, xcd_tile(P.n_tiles), x, y);
+  const bool valid = tile_pixel<kBlock>(P, P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles), x, y);
   backward_block<K, kBlock, VARIANT>(P, valid, y * P.W + x, s_node, s_keys, s_acc);
 }
 
@@ -84,7 +122,16 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
   else
     return "k must be 8, 16 or 32";
   if (e != hipSuccess) return std::string("forward launch failed: ") + hipGetErrorString(e);
+  if (p.tile_cost && p.tile_order) {
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, p.tile_cost, p.n_tiles, p.tile_order);
+    e = hipGetLastError();
+    if (e != hipSuccess) return std::string("tile order launch failed: ") + hipGetErrorString(e);
+  }
   return "";
+}
+
+int trace_tiles(int W, int H) {
+  return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH);
 }
 
 std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream_t stream) {
