@@ -21,13 +21,10 @@
 #include <vector>
 
 #include "tt_internal.h"
+#include "tt_sort.h"
 
 namespace tt {
 namespace {
-
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;
 
 __device__ __forceinline__ uint32_t f2ord(float f) {
   uint32_t u = __float_as_uint(f);
@@ -106,85 +103,6 @@ __global__ __launch_bounds__(256) void k_morton(const float4 *__restrict__ cen, 
   }
   keys[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
   vals[i] = (uint32_t)i;
-}
-
-// ---------------------------------------------------------------- radix sort
-__global__ __launch_bounds__(kSortThreads) void k_hist(const uint32_t *__restrict__ keys, int n, int shift,
-                                                        uint32_t *__restrict__ hist, int nblocks) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  int base = blockIdx.x * kSortTile;
-  int end = min(base + kSortTile, n);
-  for (int i = base + threadIdx.x; i < end; i += kSortThreads) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
-  __syncthreads();
-  hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-}
-
-// Exclusive scan of `n` counts in place, one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t *data, int n) {
-  __shared__ uint32_t s[1024];
-  int per = (n + 1023) / 1024;
-  int b = threadIdx.x * per;
-  int e = min(b + per, n);
-  uint32_t sum = 0;
-  for (int i = b; i < e; i++) sum += data[i];
-  s[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t run = s[threadIdx.x] - sum;
-  for (int i = b; i < e; i++) {
-    uint32_t v = data[i];
-    data[i] = run;
-    run += v;
-  }
-}
-
-__global__ __launch_bounds__(kSortThreads) void k_scatter(const uint32_t *__restrict__ kin,
-                                                           const uint32_t *__restrict__ vin, uint32_t *kout,
-                                                           uint32_t *vout, int n, int shift,
-                                                           const uint32_t *__restrict__ offs, int nblocks) {
-  constexpr int W = kSortThreads / 64;
-  __shared__ uint32_t base[256];
-  __shared__ uint32_t cnt[256];
-  __shared__ uint32_t wcnt[W][256];
-  int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  base[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
-  cnt[tid] = 0;
-  for (int w = 0; w < W; w++) wcnt[w][tid] = 0;
-  __syncthreads();
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  for (int r = 0; r < kSortItems; r++) {
-    int i = blockIdx.x * kSortTile + r * kSortThreads + tid;
-    bool valid = i < n;
-    uint32_t key = valid ? kin[i] : 0u;
-    uint32_t val = valid ? vin[i] : 0u;
-    uint32_t dg = (key >> shift) & 255u;
-    unsigned long long peers = __ballot(valid);
-    for (int bit = 0; bit < 8; bit++) {
-      unsigned long long bb = __ballot(valid && ((dg >> bit) & 1u));
-      peers &= ((dg >> bit) & 1u) ? bb : ~bb;
-    }
-    int rank = __popcll(peers & lt_mask);
-    if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = base[dg] + cnt[dg] + (uint32_t)rank;
-      for (int w = 0; w < wave; w++) pos += wcnt[w][dg];
-      kout[pos] = key;
-      vout[pos] = val;
-    }
-    __syncthreads();
-    uint32_t add = 0;
-    for (int w = 0; w < W; w++) add += wcnt[w][tid], wcnt[w][tid] = 0;
-    cnt[tid] += add;
-    __syncthreads();
-  }
 }
 
 // ---------------------------------------------------------------- hierarchy
@@ -524,10 +442,11 @@ std::string bvh_build(Bvh &b, const float *verts, int num_vertices, const int *c
     TT_TRY(grow(b.hist, b.cap_hist, (size_t)256 * sblk));
     int cur = 0;
     for (int shift = 0; shift < 32; shift += 8) {
-      hipLaunchKernelGGL(k_hist, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], P, shift, b.hist, sblk);
+      hipLaunchKernelGGL(k_hist, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], P, nullptr, shift, b.hist,
+                         sblk);
       hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, b.hist, 256 * sblk);
       hipLaunchKernelGGL(k_scatter, dim3(sblk), dim3(kSortThreads), 0, stream, b.keys[cur], b.vals[cur],
-                         b.keys[cur ^ 1], b.vals[cur ^ 1], P, shift, b.hist, sblk);
+                         b.keys[cur ^ 1], b.vals[cur ^ 1], P, nullptr, shift, b.hist, sblk);
       cur ^= 1;
     }
     // four passes: sorted data is back in buffer 0

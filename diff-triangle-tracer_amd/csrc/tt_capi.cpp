@@ -32,6 +32,9 @@ struct tt_tracer {
   int *tile_order = nullptr;
   int tile_cap = 0;
   int sched_W = -1, sched_H = -1;
+  // backward gradient records: 2048 rows of 80 B per block of the one-shot backward
+  float4 *rec_rows = nullptr;
+  size_t rec_blocks = 0;
 };
 
 namespace {
@@ -105,6 +108,28 @@ int prepare_spill(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
   return prepare_spill_lanes(tr, tt::trace_lanes(W, H), p);
 }
 
+void free_records(tt_tracer *tr) {
+  if (tr->rec_rows) (void)hipFree(tr->rec_rows);
+  tr->rec_rows = nullptr;
+  tr->rec_blocks = 0;
+}
+
+// Gradient-record rows for a backward over an image of W x H: 2048 rows of 80 B for each
+// of its blocks (2.6 GB at 1920x1080).
+int prepare_records(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
+  const size_t blocks = (size_t)tt::trace_tiles(W, H);
+  if (blocks > tr->rec_blocks) {
+    free_records(tr);
+    if (hipMalloc(&tr->rec_rows, blocks * 2048 * 5 * sizeof(float4)) != hipSuccess) {
+      tr->rec_rows = nullptr;
+      return fail(TT_ERR_HIP, "hipMalloc of the gradient records failed");
+    }
+    tr->rec_blocks = blocks;
+  }
+  p.rec_rows = tr->rec_rows;
+  return TT_OK;
+}
+
 void fill_common(tt::TraceParams &p, const tt_tracer *tr, int W, int H, const float *ro, const float *rd,
                  const float *verts, const int *npts, const int *cumsum, int P, const float *shs, int M, int deg,
                  const float *opac, const float *sigma, int chunk_mode, float cutoff, float near_t, float far_t) {
@@ -170,6 +195,7 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->ev1) (void)hipEventDestroy(tr->ev1);
   if (tr->tile_cost) (void)hipFree(tr->tile_cost);
   if (tr->tile_order) (void)hipFree(tr->tile_order);
+  free_records(tr);
   (void)hipSetDevice(dev);
   delete tr;
   return ok();
@@ -367,6 +393,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   if (tr->sched_W == a->width && tr->sched_H == a->height) p.tile_sched = tr->tile_order;
+  if ((rc = prepare_records(tr, a->width, a->height, p)) != TT_OK) return rc;
   std::string err = tt::launch_backward(p, a->k, a->variant, s);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   return ok();

@@ -100,6 +100,9 @@ struct TraceParams {
   float *tile_cost;
   int *tile_order;
   const int *tile_sched;
+  // backward gradient records, kRecBlock 80-byte rows per block (one per blended hit),
+  // aggregated per block; null: the per-block LDS gradient table instead
+  float4 *rec_rows;
 };
 // Per-pixel record of the bounce variant: L_r[3] L_t[3] R Q o_r[3] d_r[3] o_t[3] d_t[3]
 // (d_r = 0: no secondary rays).

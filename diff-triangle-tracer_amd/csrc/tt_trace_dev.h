@@ -595,11 +595,37 @@ struct GradSink {
   }
 };
 
+// Register sinks: the 11 per-hit scalars (9 vertex coordinates, opacity, sigma) of a
+// gradient record, and the 48 SH sums of the record reduce.
+struct HitSink {
+  float v[11];
+  TT_D void add(int f, float x) { v[f] += x; }
+};
+struct ShSink {
+  float v[48];
+  TT_D void add(int f, float x) { v[f - F_SH] += x; }
+};
+// Straight to the global gradient buffers (records that found no room).
+struct GlobalSink {
+  float *g_verts, *g_opac, *g_sigma, *g_sh;
+  TT_D void add(int f, float v) {
+    if (f < 9)
+      unsafeAtomicAdd(g_verts + f, v);
+    else if (f == F_OPAC)
+      unsafeAtomicAdd(g_opac, v);
+    else if (f == F_SIGMA)
+      unsafeAtomicAdd(g_sigma, v);
+    else
+      unsafeAtomicAdd(g_sh + (f - F_SH), v);
+  }
+};
+
 // computeWindowFuncBwd (backward.cu:104-189): gradient of the max edge distance
 // (|d_k - phi| <= 1e-6) and of the face normal w.r.t. the three vertices, hit
 // point and incenter distance held constant. Per-vertex sums are formed in
 // registers before they reach the sink.
-TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, GradSink &sink) {
+template <class Sink>
+TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, Sink &sink) {
   v3f g[3] = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
   for (int k = 0; k < 3; k++) {
     if (fabsf(w.ed[k] - w.phi_max) > 1e-6f) continue;
@@ -650,7 +676,8 @@ TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, Gr
 }
 
 // computeColorFromSHBwd (backward.cu:33-102); clamped channels carry no gradient.
-TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3], GradSink &sink) {
+template <class Sink>
+TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3], Sink &sink) {
   const v3f dir = divs(dir0, vlen(dir0));
   v3f g = mk(dc[0], dc[1], dc[2]);
   g.x *= c0 ? 0.0f : 1.0f;
@@ -688,6 +715,65 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
   }
 }
 
+// Gradient record of one blended hit, 5 x float4 = 80 B:
+//   (prim, clamp bits, d.x, d.y) (d.z, dL/dc[3]) (vertex 0..3) (vertex 4..7) (vertex 8, opacity, sigma, 0)
+// The SH gradient (48 values) is rebuilt from the direction, dL/dcolour and the clamp
+// bits when the block aggregates its records. Each block writes up to kRecBlock records
+// into its own region; the index comes from one LDS atomic per wave (s_rec[0]) and the
+// triangle id is kept in LDS (s_rec[1 + i]). Past the region the hit's gradient goes to
+// the global buffers with atomics instead.
+constexpr int kRecBlock = 2048;
+constexpr int kAggSlots = 256;
+TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const float dc[3], bool c0, bool c1,
+                      bool c2, const HitSink &h, int *s_rec) {
+  const unsigned long long m = __ballot(1);
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(&s_rec[0], __popcll(m));
+  base = __shfl(base, leader);
+  const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
+  if (idx < kRecBlock) {
+    float4 *row = P.rec_rows + 5 * ((size_t)blockIdx.x * kRecBlock + idx);
+    const int cl = (c0 ? 1 : 0) | (c1 ? 2 : 0) | (c2 ? 4 : 0);
+    row[0] = make_float4(__uint_as_float(prim), __int_as_float(cl), d.x, d.y);
+    row[1] = make_float4(d.z, dc[0], dc[1], dc[2]);
+    row[2] = make_float4(h.v[0], h.v[1], h.v[2], h.v[3]);
+    row[3] = make_float4(h.v[4], h.v[5], h.v[6], h.v[7]);
+    row[4] = make_float4(h.v[8], h.v[9], h.v[10], 0.0f);
+    s_rec[1 + idx] = (int)prim;
+  } else {
+    GlobalSink g{P.dverts + 3 * (size_t)off, P.dopac + prim, P.dsigma + prim, P.dsh + (size_t)prim * P.M * 3};
+    for (int f = 0; f < 11; f++) g.add(f, h.v[f]);
+    sh_bwd(P.deg, d, c0, c1, c2, dc, g);
+  }
+}
+
+// Adds one record (row r of this block's region) to register sums.
+TT_D void add_record(const TraceParams &P, int r, float acc[11], ShSink &sh) {
+  const float4 *row = P.rec_rows + 5 * ((size_t)blockIdx.x * kRecBlock + r);
+  const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4];
+  const int cl = __float_as_int(r0.y);
+  const float dc[3] = {r1.y, r1.z, r1.w};
+  acc[0] += r2.x, acc[1] += r2.y, acc[2] += r2.z, acc[3] += r2.w;
+  acc[4] += r3.x, acc[5] += r3.y, acc[6] += r3.z, acc[7] += r3.w;
+  acc[8] += r4.x, acc[9] += r4.y, acc[10] += r4.z;
+  sh_bwd(P.deg, mk(r0.z, r0.w, r1.x), (cl & 1) != 0, (cl & 2) != 0, (cl & 4) != 0, dc, sh);
+}
+
+// Flush of one triangle's sums to the global gradient buffers (FP32 atomics).
+TT_D void flush_sums(const TraceParams &P, uint32_t prim, const float acc[11], const ShSink &sh) {
+  float *gv = P.dverts + 3 * (size_t)P.cumsum[prim];
+  for (int f = 0; f < 9; f++)
+    if (acc[f] != 0.0f) unsafeAtomicAdd(gv + f, acc[f]);
+  if (acc[F_OPAC] != 0.0f) unsafeAtomicAdd(P.dopac + prim, acc[F_OPAC]);
+  if (acc[F_SIGMA] != 0.0f) unsafeAtomicAdd(P.dsigma + prim, acc[F_SIGMA]);
+  float *gs = P.dsh + (size_t)prim * P.M * 3;
+  const int nsh3 = 3 * (P.deg + 1) * (P.deg + 1);
+  for (int f = 0; f < 48; f++)
+    if (f < nsh3 && sh.v[f] != 0.0f) unsafeAtomicAdd(gs + f, sh.v[f]);
+}
+
 // backward.cu:191-350 for one traced segment: re-traverse it with the same chunk
 // logic, recompute the forward state hit by hit and send each blended hit's gradient
 // to the sink. fin = the segment's output colour (composite incl. background), g = the
@@ -695,7 +781,7 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
 // variant's primary (per-hit colours extended by rho L_r + (1 - rho) L_t, with the
 // reflectivity gradient); 2: a bounce secondary ray (colour only). Returns the
 // segment's final transmittance.
-template <int K, int BLOCK, int MODE>
+template <int K, int BLOCK, int MODE, bool REC>
 TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], const float g[3], float dL_ddepth,
                         v3f dL_dn2, float final_depth, v3f final_normal, const float *Lr, const float *Lt, float tlo,
                         long long plo, float far_t, int *s_node, int *s_keys, float *s_acc) {
@@ -757,21 +843,30 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
         const v3f Nf = mk(N[0], N[1], N[2]);
         dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
       }
-      GradSink sink;
-      sink.acc = s_acc;
-      sink.stride = BLOCK;
-      sink.slot = claim_slot<BLOCK>(s_keys, prim);
-      sink.g_verts = P.dverts + 3 * (size_t)off;
-      sink.g_opac = P.dopac + prim;
-      sink.g_sigma = P.dsigma + prim;
-      sink.g_sh = P.dsh + (size_t)prim * P.M * 3;
-      sink.add(F_OPAC, dL_dalpha * w.Cx);
       const float dL_dC = opacity * dL_dalpha;
-      if (w.phi_final > 0.0f) sink.add(F_SIGMA, (dL_dC * w.Cx) * log_tt(w.phi_final));
       const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
       const v3f dL_dn3 = MODE != 2 ? dL_dn2 * bw : mk(0.0f, 0.0f, 0.0f);
-      window_bwd(V, w, dL_dphi, dL_dn3, sink);
-      sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, sink);
+      if (REC) {
+        HitSink hs;
+        for (int f = 0; f < 11; f++) hs.v[f] = 0.0f;
+        hs.v[F_OPAC] = dL_dalpha * w.Cx;
+        if (w.phi_final > 0.0f) hs.v[F_SIGMA] = (dL_dC * w.Cx) * log_tt(w.phi_final);
+        window_bwd(V, w, dL_dphi, dL_dn3, hs);
+        emit_record(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_keys);
+      } else {
+        GradSink sink;
+        sink.acc = s_acc;
+        sink.stride = BLOCK;
+        sink.slot = claim_slot<BLOCK>(s_keys, prim);
+        sink.g_verts = P.dverts + 3 * (size_t)off;
+        sink.g_opac = P.dopac + prim;
+        sink.g_sigma = P.dsigma + prim;
+        sink.g_sh = P.dsh + (size_t)prim * P.M * 3;
+        sink.add(F_OPAC, dL_dalpha * w.Cx);
+        if (w.phi_final > 0.0f) sink.add(F_SIGMA, (dL_dC * w.Cx) * log_tt(w.phi_final));
+        window_bwd(V, w, dL_dphi, dL_dn3, sink);
+        sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, sink);
+      }
       T = testT;
     }
     if (P.chunk_mode == 1) {
@@ -798,7 +893,7 @@ TT_D void env_scatter(const TraceParams &P, v3f d, const float g[3]) {
 
 // One pixel of the backward (backward.cu:191-391). VARIANT 4: the bounce variant's
 // primary, its environment term and its two secondary rays (upstream R g and Q g).
-template <int K, int BLOCK, int VARIANT>
+template <int K, int BLOCK, int VARIANT, bool REC = false>
 TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, float *s_acc) {
   const size_t R = (size_t)P.R;
   const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
@@ -821,15 +916,15 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
       plo = (long long)__float_as_uint(iv.w) - 1;
       far_t = iv.y;
     }
-    backward_seg<K, BLOCK, 0>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal, nullptr,
-                              nullptr, tlo, plo, far_t, s_node, s_keys, s_acc);
+    backward_seg<K, BLOCK, 0, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal,
+                                   nullptr, nullptr, tlo, plo, far_t, s_node, s_keys, s_acc);
   } else {
     const float *a = P.aux + (size_t)kBounceAux * pix;
     float Lr[3], Lt[3];
     for (int c = 0; c < 3; c++) Lr[c] = a[c], Lt[c] = a[3 + c];
     const float Rw = a[6], Qw = a[7];
-    const float Tp = backward_seg<K, BLOCK, 1>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
-                                               final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_keys, s_acc);
+    const float Tp = backward_seg<K, BLOCK, 1, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
+                                                    final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_keys, s_acc);
     const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
     env_scatter(P, d, ge);
     const v3f dr = mk(a[11], a[12], a[13]);
@@ -840,8 +935,8 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
         const v3f so = s == 0 ? mk(a[8], a[9], a[10]) : mk(a[14], a[15], a[16]);
         const v3f sd = s == 0 ? dr : mk(a[17], a[18], a[19]);
         const float gs[3] = {wgt * dL_dpixel[0], wgt * dL_dpixel[1], wgt * dL_dpixel[2]};
-        const float Ts = backward_seg<K, BLOCK, 2>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
-                                                   nullptr, 0.0f, -1, P.far_t, s_node, s_keys, s_acc);
+        const float Ts = backward_seg<K, BLOCK, 2, REC>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero,
+                                                        nullptr, nullptr, 0.0f, -1, P.far_t, s_node, s_keys, s_acc);
         const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
         env_scatter(P, sd, gse);
       }
@@ -879,6 +974,68 @@ TT_D void backward_block(const TraceParams &P, bool valid, int pix, int *s_node,
     }
   }
   __syncthreads();
+}
+
+// The record path of one block's backward: the lanes trace their rays writing gradient
+// records (no LDS atomics on the gradient fields), then the block groups its records by
+// triangle in LDS (hash of the triangle id -> slot, counting sort by slot) and one
+// thread per triangle sums its records -- rebuilding the SH gradient from each record's
+// direction and dL/dcolour -- and flushes the sums with one FP32 atomic per field. Every
+// lane of the block must call this (block barriers). LDS: s_node (traversal stack and
+// leaf queue, reused for the grouping), s_rec (1 + kRecBlock ints), s_tab (4 x kAggSlots).
+template <int K, int BLOCK, int VARIANT>
+TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int *s_node, int *s_rec, int *s_tab) {
+  static_assert(kLdsInts * BLOCK * 4 >= 2 * kRecBlock * 2, "the grouping reuses the traversal LDS");
+  int *t_keys = s_tab, *t_cnt = s_tab + kAggSlots, *t_off = s_tab + 2 * kAggSlots, *t_fill = s_tab + 3 * kAggSlots;
+  if (threadIdx.x == 0) s_rec[0] = 0;
+  for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cnt[i] = 0, t_fill[i] = 0;
+  __syncthreads();
+  if (valid) backward_ray<K, BLOCK, VARIANT, true>(P, pix, s_node, s_rec, nullptr);
+  __syncthreads();
+  const int n = min(s_rec[0], kRecBlock);
+  short *slot_of = reinterpret_cast<short *>(s_node), *list = slot_of + kRecBlock;
+  for (int i = threadIdx.x; i < n; i += BLOCK) {
+    const int sl = claim_slot<kAggSlots>(t_keys, (uint32_t)s_rec[1 + i]);
+    slot_of[i] = (short)sl;
+    if (sl >= 0) atomicAdd(&t_cnt[sl], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {  // exclusive scan of the slot counts: 4 per lane + a wave scan
+    const int l = threadIdx.x;
+    int c[4], sum = 0;
+    for (int j = 0; j < 4; j++) c[j] = t_cnt[4 * l + j], sum += c[j];
+    int inc = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(inc, off);
+      if (l >= off) inc += v;
+    }
+    int run = inc - sum;
+    for (int j = 0; j < 4; j++) t_off[4 * l + j] = run, run += c[j];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += BLOCK) {
+    const int sl = slot_of[i];
+    if (sl >= 0) {
+      list[t_off[sl] + atomicAdd(&t_fill[sl], 1)] = (short)i;
+    } else {  // no slot left: this record goes out on its own
+      float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      ShSink sh;
+      for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
+      add_record(P, i, acc, sh);
+      flush_sums(P, (uint32_t)s_rec[1 + i], acc, sh);
+    }
+  }
+  __syncthreads();
+  for (int sl = threadIdx.x; sl < kAggSlots; sl += BLOCK) {
+    const int prim = t_keys[sl];
+    if (prim < 0) continue;
+    float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    ShSink sh;
+    for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
+    const int b = t_off[sl], e = b + t_cnt[sl];
+    for (int j = b; j < e; j++) add_record(P, list[j], acc, sh);
+    flush_sums(P, (uint32_t)prim, acc, sh);
+  }
 }
 
 __global__ void k_fill(float *p, size_t n, float v) {
