@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
 // Backward through gradient records aggregated per block (no LDS atomics on the
 // gradient fields; 24 KB of LDS per block instead of 42 KB).
 template <int K, int VARIANT>
-__global__ __launch_bounds__(kBlock) void k_backward_rec(TraceParams P) {
+__global__ __launch_bounds__(kBlock, 3) void k_backward_rec(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
   __shared__ int s_rec[1 + kRecBlock];
   __shared__ int s_tab[4 * kAggSlots];
