@@ -34,7 +34,7 @@ struct tt_tracer {
   int sched_W = -1, sched_H = -1;
   // backward gradient records: 2048 rows of 80 B per block of the one-shot backward
   float4 *rec_rows = nullptr;
-  size_t rec_blocks = 0;
+  size_t rec_blocks = 0;  // rows allocated
 };
 
 namespace {
@@ -114,17 +114,17 @@ void free_records(tt_tracer *tr) {
   tr->rec_blocks = 0;
 }
 
-// Gradient-record rows for a backward over an image of W x H: 2048 rows of 80 B for each
-// of its blocks (2.6 GB at 1920x1080).
-int prepare_records(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
-  const size_t blocks = (size_t)tt::trace_tiles(W, H);
-  if (blocks > tr->rec_blocks) {
+// Gradient-record rows for a backward: 16 rows of 80 B per launched lane (2.6 GB for a
+// 1920x1080 one-shot backward).
+int prepare_records(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
+  const size_t rows = lanes * 16;
+  if (rows > tr->rec_blocks) {
     free_records(tr);
-    if (hipMalloc(&tr->rec_rows, blocks * 2048 * 5 * sizeof(float4)) != hipSuccess) {
+    if (hipMalloc(&tr->rec_rows, rows * 5 * sizeof(float4)) != hipSuccess) {
       tr->rec_rows = nullptr;
       return fail(TT_ERR_HIP, "hipMalloc of the gradient records failed");
     }
-    tr->rec_blocks = blocks;
+    tr->rec_blocks = rows;
   }
   p.rec_rows = tr->rec_rows;
   return TT_OK;
@@ -393,7 +393,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   if (tr->sched_W == a->width && tr->sched_H == a->height) p.tile_sched = tr->tile_order;
-  if ((rc = prepare_records(tr, a->width, a->height, p)) != TT_OK) return rc;
+  if ((rc = prepare_records(tr, tt::trace_lanes(a->width, a->height), p)) != TT_OK) return rc;
   std::string err = tt::launch_backward(p, a->k, a->variant, s);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   return ok();
@@ -471,6 +471,10 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
   if (!tr->heads) {
     hipError_t e = hipMalloc(&tr->heads, 8 * sizeof(int));
     if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("queue allocation failed: ") + hipGetErrorString(e));
+  }
+  if (backward) {
+    const int rc = prepare_records(tr, (size_t)grid * a->block, p);
+    if (rc != TT_OK) return rc;
   }
   return prepare_spill_lanes(tr, (size_t)grid * a->block, p);
 }

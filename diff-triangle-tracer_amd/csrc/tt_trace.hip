@@ -73,8 +73,8 @@ __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
 template <int K, int VARIANT>
 __global__ __launch_bounds__(kBlock, 3) void k_backward_rec(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
-  __shared__ int s_rec[1 + kRecBlock];
-  __shared__ int s_tab[4 * kAggSlots];
+  __shared__ int s_rec[1 + rec_per_block<kBlock>()];
+  __shared__ int s_tab[4 * agg_slots<kBlock>()];
   int x, y;
   const bool valid = tile_pixel<kBlock>(P, P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles), x, y);
   backward_block_rec<K, kBlock, VARIANT>(P, valid, y * P.W + x, s_node, s_rec, s_tab);

@@ -565,7 +565,8 @@ constexpr int F_OPAC = 9, F_SIGMA = 10, F_SH = 11;
 // probing, 8 probes), or -1 when full.
 template <int SLOTS>
 TT_D int claim_slot(int *keys, uint32_t prim) {
-  constexpr int bits = SLOTS == 64 ? 6 : (SLOTS == 128 ? 7 : 8);
+  static_assert(SLOTS == 64 || SLOTS == 128 || SLOTS == 256 || SLOTS == 512, "table size");
+  constexpr int bits = SLOTS == 64 ? 6 : (SLOTS == 128 ? 7 : (SLOTS == 256 ? 8 : 9));
   uint32_t h = (prim * 2654435761u) >> (32 - bits);
   for (int probe = 0; probe < 8; probe++) {
     const int old = atomicCAS(&keys[h], -1, (int)prim);
@@ -718,14 +719,22 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
 // Gradient record of one blended hit, 5 x float4 = 80 B:
 //   (prim, clamp bits, d.x, d.y) (d.z, dL/dc[3]) (vertex 0..3) (vertex 4..7) (vertex 8, opacity, sigma, 0)
 // The SH gradient (48 values) is rebuilt from the direction, dL/dcolour and the clamp
-// bits when the block aggregates its records. Each block writes up to kRecBlock records
-// into its own region; the index comes from one LDS atomic per wave (s_rec[0]) and the
+// bits when the block aggregates its records. Each block writes up to 16 records per
+// lane into its own region; the index comes from one LDS atomic per wave (s_rec[0]) and the
 // triangle id is kept in LDS (s_rec[1 + i]). Past the region the hit's gradient goes to
 // the global buffers with atomics instead.
-constexpr int kRecBlock = 2048;
-constexpr int kAggSlots = 256;
+template <int BLOCK>
+constexpr int rec_per_block() {  // record rows per block: 16 per lane
+  return 16 * BLOCK;
+}
+template <int BLOCK>
+constexpr int agg_slots() {  // triangles a block groups its records into
+  return 2 * BLOCK;
+}
+template <int BLOCK>
 TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const float dc[3], bool c0, bool c1,
                       bool c2, const HitSink &h, int *s_rec) {
+  constexpr int kRecBlock = rec_per_block<BLOCK>();
   const unsigned long long m = __ballot(1);
   const int lane = threadIdx.x & 63;
   const int leader = __ffsll((unsigned long long)m) - 1;
@@ -750,8 +759,9 @@ TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const
 }
 
 // Adds one record (row r of this block's region) to register sums.
+template <int BLOCK>
 TT_D void add_record(const TraceParams &P, int r, float acc[11], ShSink &sh) {
-  const float4 *row = P.rec_rows + 5 * ((size_t)blockIdx.x * kRecBlock + r);
+  const float4 *row = P.rec_rows + 5 * ((size_t)blockIdx.x * rec_per_block<BLOCK>() + r);
   const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4];
   const int cl = __float_as_int(r0.y);
   const float dc[3] = {r1.y, r1.z, r1.w};
@@ -852,7 +862,7 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
         hs.v[F_OPAC] = dL_dalpha * w.Cx;
         if (w.phi_final > 0.0f) hs.v[F_SIGMA] = (dL_dC * w.Cx) * log_tt(w.phi_final);
         window_bwd(V, w, dL_dphi, dL_dn3, hs);
-        emit_record(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_keys);
+        emit_record<BLOCK>(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_keys);
       } else {
         GradSink sink;
         sink.acc = s_acc;
@@ -982,9 +992,11 @@ TT_D void backward_block(const TraceParams &P, bool valid, int pix, int *s_node,
 // thread per triangle sums its records -- rebuilding the SH gradient from each record's
 // direction and dL/dcolour -- and flushes the sums with one FP32 atomic per field. Every
 // lane of the block must call this (block barriers). LDS: s_node (traversal stack and
-// leaf queue, reused for the grouping), s_rec (1 + kRecBlock ints), s_tab (4 x kAggSlots).
+// leaf queue, reused for the grouping), s_rec (1 + rec_per_block ints), s_tab
+// (4 x agg_slots ints).
 template <int K, int BLOCK, int VARIANT>
 TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int *s_node, int *s_rec, int *s_tab) {
+  constexpr int kRecBlock = rec_per_block<BLOCK>(), kAggSlots = agg_slots<BLOCK>();
   static_assert(kLdsInts * BLOCK * 4 >= 2 * kRecBlock * 2, "the grouping reuses the traversal LDS");
   int *t_keys = s_tab, *t_cnt = s_tab + kAggSlots, *t_off = s_tab + 2 * kAggSlots, *t_fill = s_tab + 3 * kAggSlots;
   if (threadIdx.x == 0) s_rec[0] = 0;
@@ -1000,17 +1012,18 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int *s_n
     if (sl >= 0) atomicAdd(&t_cnt[sl], 1);
   }
   __syncthreads();
-  if (threadIdx.x < 64) {  // exclusive scan of the slot counts: 4 per lane + a wave scan
+  if (threadIdx.x < 64) {  // exclusive scan of the slot counts: kAggSlots / 64 per lane + a wave scan
+    constexpr int per = kAggSlots / 64;
     const int l = threadIdx.x;
-    int c[4], sum = 0;
-    for (int j = 0; j < 4; j++) c[j] = t_cnt[4 * l + j], sum += c[j];
+    int c[per], sum = 0;
+    for (int j = 0; j < per; j++) c[j] = t_cnt[per * l + j], sum += c[j];
     int inc = sum;
     for (int off = 1; off < 64; off <<= 1) {
       const int v = __shfl_up(inc, off);
       if (l >= off) inc += v;
     }
     int run = inc - sum;
-    for (int j = 0; j < 4; j++) t_off[4 * l + j] = run, run += c[j];
+    for (int j = 0; j < per; j++) t_off[per * l + j] = run, run += c[j];
   }
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += BLOCK) {
@@ -1021,7 +1034,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int *s_n
       float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       ShSink sh;
       for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
-      add_record(P, i, acc, sh);
+      add_record<BLOCK>(P, i, acc, sh);
       flush_sums(P, (uint32_t)s_rec[1 + i], acc, sh);
     }
   }
@@ -1033,7 +1046,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int *s_n
     ShSink sh;
     for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
     const int b = t_off[sl], e = b + t_cnt[sl];
-    for (int j = b; j < e; j++) add_record(P, list[j], acc, sh);
+    for (int j = b; j < e; j++) add_record<BLOCK>(P, list[j], acc, sh);
     flush_sums(P, (uint32_t)prim, acc, sh);
   }
 }
