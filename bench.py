@@ -41,6 +41,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "diff-triangle-tracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernels on the default workload (1M splats, 1920x1080,
+# k=16), 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE from rocprofv3 counter runs of this script
+# (profiles/trace_fwd_bwd_1M_1080p.md; the factors come from a 256 MiB copy).
+MEASURED_TRAFFIC = {"k_backward_rec<16,0>": 18.085e9, "k_forward<16,0,false,true>": 2.994e9}
 TRAIN_POSES, HELD_OUT_EXTRA = 16, 3
 
 
@@ -469,12 +473,14 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     bb = 104 * R + 48 * Hc + 428 * Hb + 16 * R         # + emitted interval read
     per = res["proposal"]["per"]
     if per["bwd"] >= per["fwd"]:
-        dom, dom_bytes, dom_ms = f"k_backward<{args.k}>", bb, per["bwd"]
+        dom, dom_bytes, dom_ms = f"k_backward_rec<{args.k},0>", bb, per["bwd"]
     else:
-        dom, dom_bytes, dom_ms = f"k_forward<{args.k},0>", bf, per["fwd"]
+        dom, dom_bytes, dom_ms = f"k_forward<{args.k},0,false,true>", bf, per["fwd"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    default = P == 1_000_000 and (W, H) == (1920, 1080) and label.startswith("1080p")
+    traffic = MEASURED_TRAFFIC.get(dom) if default else None
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=dom,
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic, kernel=dom,
                     bytes_per_launch=int(dom_bytes), avg_launch_ms=round(dom_ms, 4),
                     step_frac=round((bf + bb) / ((per["fwd"] + per["bwd"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5))
     mlp_flops = 2 * 14848 * R
