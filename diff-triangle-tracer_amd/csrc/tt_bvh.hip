@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ key
   int *li = reinterpret_cast<int *>(links + 4 * (size_t)i + 3);
   li[0] = left;
   li[1] = right;
-  li[3] = 0;
+  li[3] = max(i, j) - min(i, j) + 1;  // leaves under the node
   if (i == 0) li[2] = -1;
   if (left < 0)
     leaf_parent[~left] = i;
@@ -156,7 +156,7 @@ __global__ void k_single(int4 *links, int *leaf_parent, uint32_t *vals) {
   li[0] = ~0;
   li[1] = ~1;
   li[2] = -1;
-  li[3] = 0;
+  li[3] = 2;
   leaf_parent[0] = 0;
   leaf_parent[1] = 0;
   vals[0] = 0u;
@@ -308,9 +308,11 @@ __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ no
     l[2] = slot ? z.z : z.x;
     h[2] = slot ? z.w : z.y;
   };
+  int cnt[4] = {0, 0, 0, 0};  // leaves under each child
   auto put = [&](int r, const float l[3], const float h[3]) {
     for (int a = 0; a < 3; a++) lo[a][e] = l[a], hi[a][e] = h[a];
     ref[e] = r;
+    cnt[e] = r < 0 ? 1 : reinterpret_cast<const int4 *>(nodes + 4 * (size_t)r + 3)->w;
     e++;
   };
   const int4 li = *reinterpret_cast<const int4 *>(nodes + 4 * (size_t)n + 3);
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ no
   o[4] = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
   o[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
   *reinterpret_cast<int4 *>(o + 6) = make_int4(ref[0], ref[1], ref[2], ref[3]);
-  o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  *reinterpret_cast<int4 *>(o + 7) = make_int4(cnt[0], cnt[1], cnt[2], cnt[3]);
 }
 
 template <class T>

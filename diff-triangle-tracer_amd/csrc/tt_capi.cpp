@@ -30,6 +30,7 @@ struct tt_tracer {
   // used by a backward over an image of the same size
   float *tile_cost = nullptr;
   int *tile_order = nullptr;
+  int *probe_order = nullptr;
   int tile_cap = 0;
   int sched_W = -1, sched_H = -1;
   // backward gradient records: 2048 rows of 80 B per block of the one-shot backward
@@ -195,6 +196,7 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->ev1) (void)hipEventDestroy(tr->ev1);
   if (tr->tile_cost) (void)hipFree(tr->tile_cost);
   if (tr->tile_order) (void)hipFree(tr->tile_order);
+  if (tr->probe_order) (void)hipFree(tr->probe_order);
   free_records(tr);
   (void)hipSetDevice(dev);
   delete tr;
@@ -309,17 +311,21 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   if (nt > tr->tile_cap) {
     if (tr->tile_cost) (void)hipFree(tr->tile_cost);
     if (tr->tile_order) (void)hipFree(tr->tile_order);
+  if (tr->probe_order) (void)hipFree(tr->probe_order);
     tr->tile_cost = nullptr;
     tr->tile_order = nullptr;
+    tr->probe_order = nullptr;
     tr->tile_cap = 0;
     if (hipMalloc(&tr->tile_cost, (size_t)nt * sizeof(float)) != hipSuccess ||
-        hipMalloc(&tr->tile_order, (size_t)nt * sizeof(int)) != hipSuccess)
+        hipMalloc(&tr->tile_order, (size_t)nt * sizeof(int)) != hipSuccess ||
+        hipMalloc(&tr->probe_order, (size_t)nt * sizeof(int)) != hipSuccess)
       return fail(TT_ERR_HIP, "hipMalloc of the tile schedule failed");
     tr->tile_cap = nt;
     tr->sched_W = tr->sched_H = -1;
   }
   p.tile_cost = tr->tile_cost;
   p.tile_order = tr->tile_order;
+  p.probe_order = tr->probe_order;
   std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   tr->sched_W = a->width;

@@ -100,6 +100,9 @@ struct TraceParams {
   float *tile_cost;
   int *tile_order;
   const int *tile_sched;
+  // the forward's own schedule: tiles ordered by a BVH probe of each tile's centre ray
+  // (estimated cost, descending); null: tiles in XCD band order
+  int *probe_order;
   // backward gradient records, kRecBlock 80-byte rows per block (one per blended hit),
   // aggregated per block; null: the per-block LDS gradient table instead
   float4 *rec_rows;

@@ -16,13 +16,54 @@ template <int K, int VARIANT, bool COUNT = false, bool IVAL = false>
 __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
   const long long c0 = P.tile_cost ? wall_clock64() : 0;
-  const int tile = xcd_tile(P.n_tiles);
+  const int tile = P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles);
   int x, y;
   if (tile_pixel<kBlock>(P, tile, x, y)) forward_pixel<K, kBlock, VARIANT, COUNT, IVAL>(P, y * P.W + x, s_node);
   if (P.tile_cost) {  // the tile's wall time (100 MHz ticks): the backward's schedule key
     __syncthreads();
     if (threadIdx.x == 0) P.tile_cost[tile] = (float)(wall_clock64() - c0);
   }
+}
+
+// Cost estimate of every tile before the forward: the tile's centre ray walks the
+// 4-wide BVH down to subtrees of at most kProbeCut leaves and sums the leaf counts of
+// the subtrees and leaves whose boxes its line crosses -- how much geometry the tile's
+// rays pass, at a few dozen node visits per tile (a few percent of a full trace).
+// One lane per tile; the LDS stack drops entries past kProbeStack (the estimate then
+// undercounts; it only orders tiles).
+constexpr int kProbeCut = 32, kProbeStack = 48, kProbeBlock = 64;
+__global__ __launch_bounds__(kProbeBlock) void k_tile_probe(TraceParams P, float *cost) {
+  __shared__ int s_st[kProbeStack * kProbeBlock];
+  const int tile = blockIdx.x * kProbeBlock + threadIdx.x;
+  if (tile >= P.n_tiles) return;
+  const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
+  const int x = min(tx * kTileW + kTileW / 2, P.W - 1), y = min(ty * kTileH + kTileH / 2, P.H - 1);
+  const size_t pix = (size_t)y * P.W + x;
+  const RayPre r = prep_ray(ld3(P.ray_o + 3 * pix), ld3(P.ray_d + 3 * pix));
+  const int lane = threadIdx.x;
+  float est = 0.0f;
+  int sp = 0, node = P.n_leaves > 0 ? 0 : -1;
+  const float inf = __builtin_huge_valf();
+  while (node >= 0) {
+    const float4 *nd = P.nodes4 + 8 * (size_t)node;
+    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+    const int4 ref = *reinterpret_cast<const int4 *>(nd + 6), cnt = *reinterpret_cast<const int4 *>(nd + 7);
+    float tn[4], tf[4];
+    slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn[0], tf[0]);
+    slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn[1], tf[1]);
+    slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn[2], tf[2]);
+    slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn[3], tf[3]);
+    const int rf[4] = {ref.x, ref.y, ref.z, ref.w}, ct[4] = {cnt.x, cnt.y, cnt.z, cnt.w};
+    for (int c = 0; c < 4; c++) {
+      if (rf[c] == kEmptyChild || !box_ok(tn[c], tf[c], 0.0f, inf)) continue;
+      if (rf[c] < 0 || ct[c] <= kProbeCut)
+        est += (float)ct[c];
+      else if (sp < kProbeStack)
+        s_st[sp++ * kProbeBlock + lane] = rf[c];
+    }
+    node = sp > 0 ? s_st[--sp * kProbeBlock + lane] : -1;
+  }
+  cost[tile] = est;
 }
 
 // Tiles ordered by cost, most expensive first (counting sort over 1024 buckets of
@@ -125,6 +166,16 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
   tiles(p);
   if (p.n_tiles == 0) return "";
   hipError_t e;
+  if (k != 8 && k != 16 && k != 32) return "k must be 8, 16 or 32";
+  p.tile_sched = nullptr;
+  if (p.tile_cost && p.probe_order) {  // the forward's schedule: estimated tile costs
+    hipLaunchKernelGGL(k_tile_probe, dim3((p.n_tiles + kProbeBlock - 1) / kProbeBlock), dim3(kProbeBlock), 0, stream,
+                       p, p.tile_cost);
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, p.tile_cost, p.n_tiles, p.probe_order);
+    e = hipGetLastError();
+    if (e != hipSuccess) return std::string("tile probe launch failed: ") + hipGetErrorString(e);
+    p.tile_sched = p.probe_order;
+  }
   if (k == 8)
     e = fwd_k<8>(p, variant, stream);
   else if (k == 16)
@@ -134,7 +185,7 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
   else
     return "k must be 8, 16 or 32";
   if (e != hipSuccess) return std::string("forward launch failed: ") + hipGetErrorString(e);
-  if (p.tile_cost && p.tile_order) {
+  if (p.tile_cost && p.tile_order) {  // the backward's schedule: measured tile times
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, p.tile_cost, p.n_tiles, p.tile_order);
     e = hipGetLastError();
     if (e != hipSuccess) return std::string("tile order launch failed: ") + hipGetErrorString(e);
