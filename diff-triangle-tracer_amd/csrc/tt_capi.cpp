@@ -31,6 +31,8 @@ struct tt_tracer {
   float *tile_cost = nullptr;
   int *tile_order = nullptr;
   int *probe_order = nullptr;
+  int *tile_hits = nullptr;  // blended hits per tile in the last forward
+  int *rec_off = nullptr;    // their exclusive prefix sums: the backward's record regions
   int tile_cap = 0;
   int sched_W = -1, sched_H = -1;
   // backward gradient records: 2048 rows of 80 B per block of the one-shot backward
@@ -119,6 +121,7 @@ void free_records(tt_tracer *tr) {
 // 1920x1080 one-shot backward).
 int prepare_records(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
   const size_t rows = lanes * 16;
+  if (rows > (size_t)0x7fffffff) return fail(TT_ERR_INVALID_ARG, "too many rays for one backward launch");
   if (rows > tr->rec_blocks) {
     free_records(tr);
     if (hipMalloc(&tr->rec_rows, rows * 5 * sizeof(float4)) != hipSuccess) {
@@ -128,6 +131,7 @@ int prepare_records(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
     tr->rec_blocks = rows;
   }
   p.rec_rows = tr->rec_rows;
+  p.rec_cap = (int)rows;
   return TT_OK;
 }
 
@@ -197,6 +201,8 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->tile_cost) (void)hipFree(tr->tile_cost);
   if (tr->tile_order) (void)hipFree(tr->tile_order);
   if (tr->probe_order) (void)hipFree(tr->probe_order);
+  if (tr->tile_hits) (void)hipFree(tr->tile_hits);
+  if (tr->rec_off) (void)hipFree(tr->rec_off);
   free_records(tr);
   (void)hipSetDevice(dev);
   delete tr;
@@ -312,13 +318,19 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
     if (tr->tile_cost) (void)hipFree(tr->tile_cost);
     if (tr->tile_order) (void)hipFree(tr->tile_order);
   if (tr->probe_order) (void)hipFree(tr->probe_order);
+  if (tr->tile_hits) (void)hipFree(tr->tile_hits);
+  if (tr->rec_off) (void)hipFree(tr->rec_off);
     tr->tile_cost = nullptr;
     tr->tile_order = nullptr;
     tr->probe_order = nullptr;
+    tr->tile_hits = nullptr;
+    tr->rec_off = nullptr;
     tr->tile_cap = 0;
     if (hipMalloc(&tr->tile_cost, (size_t)nt * sizeof(float)) != hipSuccess ||
         hipMalloc(&tr->tile_order, (size_t)nt * sizeof(int)) != hipSuccess ||
-        hipMalloc(&tr->probe_order, (size_t)nt * sizeof(int)) != hipSuccess)
+        hipMalloc(&tr->probe_order, (size_t)nt * sizeof(int)) != hipSuccess ||
+        hipMalloc(&tr->tile_hits, (size_t)nt * sizeof(int)) != hipSuccess ||
+        hipMalloc(&tr->rec_off, (size_t)nt * sizeof(int)) != hipSuccess)
       return fail(TT_ERR_HIP, "hipMalloc of the tile schedule failed");
     tr->tile_cap = nt;
     tr->sched_W = tr->sched_H = -1;
@@ -326,6 +338,8 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.tile_cost = tr->tile_cost;
   p.tile_order = tr->tile_order;
   p.probe_order = tr->probe_order;
+  p.tile_hits = tr->tile_hits;
+  p.rec_off_out = tr->rec_off;
   std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   tr->sched_W = a->width;
@@ -398,8 +412,12 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
     p.denv = a->dL_denvmap_dev;
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
-  if (tr->sched_W == a->width && tr->sched_H == a->height) p.tile_sched = tr->tile_order;
   if ((rc = prepare_records(tr, tt::trace_lanes(a->width, a->height), p)) != TT_OK) return rc;
+  if (tr->sched_W == a->width && tr->sched_H == a->height) {  // the last forward's schedule and record regions
+    p.tile_sched = tr->tile_order;
+    p.tile_hits = tr->tile_hits;
+    p.rec_off = tr->rec_off;
+  }
   std::string err = tt::launch_backward(p, a->k, a->variant, s);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   return ok();

@@ -103,9 +103,16 @@ struct TraceParams {
   // the forward's own schedule: tiles ordered by a BVH probe of each tile's centre ray
   // (estimated cost, descending); null: tiles in XCD band order
   int *probe_order;
-  // backward gradient records, kRecBlock 80-byte rows per block (one per blended hit),
-  // aggregated per block; null: the per-block LDS gradient table instead
+  // backward gradient records (80-byte rows, one per blended hit), aggregated per block;
+  // null: the per-block LDS gradient table instead. rec_cap = rows allocated.
   float4 *rec_rows;
+  int rec_cap;
+  // per-tile record regions: the forward counts each tile's blended hits into tile_hits
+  // (when non-null) and the exclusive prefix sums go to rec_off; a backward given rec_off
+  // gives tile t the rows [rec_off[t], rec_off[t] + tile_hits[t])
+  int *tile_hits;
+  const int *rec_off;
+  int *rec_off_out;
 };
 // Per-pixel record of the bounce variant: L_r[3] L_t[3] R Q o_r[3] d_r[3] o_t[3] d_t[3]
 // (d_r = 0: no secondary rays).

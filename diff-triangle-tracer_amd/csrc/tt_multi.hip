@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, Multi
 template <int K, int BLOCK>
 __global__ __launch_bounds__(BLOCK, 3) void k_backward_multi(TraceParams P, MultiCams mc, int *heads, int G) {
   __shared__ int s_node[kLdsInts * BLOCK];
-  __shared__ int s_rec[1 + rec_per_block<BLOCK>()];
+  __shared__ int s_rec[kRecLds];
   __shared__ int s_tab[4 * agg_slots<BLOCK>()];
   __shared__ int s_range[2];
   while (true) {
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_backward_multi(TraceParams P, Mult
       int lt, x, y;
       const TraceParams Q = camera_view(P, mc, t, lt);
       const bool valid = tile_pixel<BLOCK>(Q, lt, x, y);
-      backward_block_rec<K, BLOCK, 0>(Q, valid, y * Q.W + x, s_node, s_rec, s_tab);
+      backward_block_rec<K, BLOCK, 0>(Q, valid, y * Q.W + x, lt, s_node, s_rec, s_tab);
     }
   }
 }

@@ -17,11 +17,45 @@ __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
   const long long c0 = P.tile_cost ? wall_clock64() : 0;
   const int tile = P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles);
-  int x, y;
-  if (tile_pixel<kBlock>(P, tile, x, y)) forward_pixel<K, kBlock, VARIANT, COUNT, IVAL>(P, y * P.W + x, s_node);
-  if (P.tile_cost) {  // the tile's wall time (100 MHz ticks): the backward's schedule key
+  int x, y, blended = 0;
+  if (tile_pixel<kBlock>(P, tile, x, y))
+    blended = forward_pixel<K, kBlock, VARIANT, COUNT, IVAL>(P, y * P.W + x, s_node);
+  if (P.tile_cost) {  // the tile's wall time (100 MHz ticks, the backward's schedule key) and blended hits
+    for (int off = 32; off > 0; off >>= 1) blended += __shfl_xor(blended, off);
     __syncthreads();
-    if (threadIdx.x == 0) P.tile_cost[tile] = (float)(wall_clock64() - c0);
+    if ((threadIdx.x & 63) == 0) s_node[threadIdx.x >> 6] = blended;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      P.tile_cost[tile] = (float)(wall_clock64() - c0);
+      if (P.tile_hits) {
+        int sum = 0;
+        for (int w = 0; w < kBlock / 64; w++) sum += s_node[w];
+        P.tile_hits[tile] = sum;
+      }
+    }
+  }
+}
+
+// Exclusive prefix sums of the tiles' blended-hit counts (one block): the backward's
+// per-tile record regions.
+__global__ __launch_bounds__(1024) void k_tile_offsets(const int *hits, int n, int *off) {
+  __shared__ long long s_part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024, b = min(n, t * per), e = min(n, b + per);
+  long long sum = 0;
+  for (int i = b; i < e; i++) sum += hits[i];
+  s_part[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {  // Hillis-Steele inclusive scan
+    const long long v = t >= d ? s_part[t - d] : 0;
+    __syncthreads();
+    s_part[t] += v;
+    __syncthreads();
+  }
+  long long run = s_part[t] - sum;
+  for (int i = b; i < e; i++) {
+    off[i] = (int)min(run, (long long)0x7fffffff);
+    run += hits[i];
   }
 }
 
@@ -110,15 +144,16 @@ __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
 }
 
 // Backward through gradient records aggregated per block (no LDS atomics on the
-// gradient fields; 24 KB of LDS per block instead of 42 KB).
+// gradient fields; 16 KB of LDS per block instead of 42 KB).
 template <int K, int VARIANT>
 __global__ __launch_bounds__(kBlock, 3) void k_backward_rec(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
-  __shared__ int s_rec[1 + rec_per_block<kBlock>()];
+  __shared__ int s_rec[kRecLds];
   __shared__ int s_tab[4 * agg_slots<kBlock>()];
   int x, y;
-  const bool valid = tile_pixel<kBlock>(P, P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles), x, y);
-  backward_block_rec<K, kBlock, VARIANT>(P, valid, y * P.W + x, s_node, s_rec, s_tab);
+  const int tile = P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles);
+  const bool valid = tile_pixel<kBlock>(P, tile, x, y);
+  backward_block_rec<K, kBlock, VARIANT>(P, valid, y * P.W + x, tile, s_node, s_rec, s_tab);
 }
 
 template <int K>
@@ -186,6 +221,8 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
     return "k must be 8, 16 or 32";
   if (e != hipSuccess) return std::string("forward launch failed: ") + hipGetErrorString(e);
   if (p.tile_cost && p.tile_order) {  // the backward's schedule: measured tile times
+    if (p.tile_hits && p.rec_off_out)
+      hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, stream, p.tile_hits, p.n_tiles, p.rec_off_out);
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, p.tile_cost, p.n_tiles, p.tile_order);
     e = hipGetLastError();
     if (e != hipSuccess) return std::string("tile order launch failed: ") + hipGetErrorString(e);

@@ -377,8 +377,9 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
 }
 
 // One pixel of the forward programs (forward.cu:161-215 and the variants' raygens).
+// Returns the pixel's blended hits over all its traced segments.
 template <int K, int BLOCK, int VARIANT, bool COUNT, bool IVAL = false>
-TT_D void forward_pixel(const TraceParams &P, int pix, int *s_node) {
+TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
   v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
   SegState st;
   seg_init(st);
@@ -496,6 +497,7 @@ TT_D void forward_pixel(const TraceParams &P, int pix, int *s_node) {
     P.stats[2 * (size_t)pix] = st.consumed;
     P.stats[2 * (size_t)pix + 1] = st.blended;
   }
+  return st.blended;
 }
 
 // ------------------------------------------------------------------ backward
@@ -719,22 +721,28 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
 // Gradient record of one blended hit, 5 x float4 = 80 B:
 //   (prim, clamp bits, d.x, d.y) (d.z, dL/dc[3]) (vertex 0..3) (vertex 4..7) (vertex 8, opacity, sigma, 0)
 // The SH gradient (48 values) is rebuilt from the direction, dL/dcolour and the clamp
-// bits when the block aggregates its records. Each block writes up to 16 records per
-// lane into its own region; the index comes from one LDS atomic per wave (s_rec[0]) and the
-// triangle id is kept in LDS (s_rec[1 + i]). Past the region the hit's gradient goes to
-// the global buffers with atomics instead.
+// bits when the block aggregates its records. A block writes its records into its own
+// region of P.rec_rows, rows [s_rec[1], s_rec[1] + s_rec[2]); the row index comes from one
+// LDS atomic per wave (counter s_rec[0]). A one-shot backward that follows a forward of the
+// same image sizes each tile's region to that tile's blended-hit count in the forward
+// (P.tile_hits, prefix sums in P.rec_off); otherwise a block has rec_per_block rows. Past
+// its region a hit's gradient goes to the global buffers with atomics instead.
+constexpr int kRecLds = 4;  // s_rec: counter, region start, region rows, spare
 template <int BLOCK>
-constexpr int rec_per_block() {  // record rows per block: 16 per lane
+constexpr int rec_per_block() {  // rows of a fixed region: 16 per lane
   return 16 * BLOCK;
 }
 template <int BLOCK>
-constexpr int agg_slots() {  // triangles a block groups its records into
+constexpr int rec_batch() {  // records grouped per pass (two shorts each in the traversal LDS)
+  return 16 * BLOCK;
+}
+template <int BLOCK>
+constexpr int agg_slots() {  // triangles a block groups one batch of records into
   return 2 * BLOCK;
 }
 template <int BLOCK>
 TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const float dc[3], bool c0, bool c1,
                       bool c2, const HitSink &h, int *s_rec) {
-  constexpr int kRecBlock = rec_per_block<BLOCK>();
   const unsigned long long m = __ballot(1);
   const int lane = threadIdx.x & 63;
   const int leader = __ffsll((unsigned long long)m) - 1;
@@ -742,15 +750,14 @@ TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const
   if (lane == leader) base = atomicAdd(&s_rec[0], __popcll(m));
   base = __shfl(base, leader);
   const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
-  if (idx < kRecBlock) {
-    float4 *row = P.rec_rows + 5 * ((size_t)blockIdx.x * kRecBlock + idx);
+  if (idx < s_rec[2]) {
+    float4 *row = P.rec_rows + 5 * ((size_t)s_rec[1] + idx);
     const int cl = (c0 ? 1 : 0) | (c1 ? 2 : 0) | (c2 ? 4 : 0);
     row[0] = make_float4(__uint_as_float(prim), __int_as_float(cl), d.x, d.y);
     row[1] = make_float4(d.z, dc[0], dc[1], dc[2]);
     row[2] = make_float4(h.v[0], h.v[1], h.v[2], h.v[3]);
     row[3] = make_float4(h.v[4], h.v[5], h.v[6], h.v[7]);
     row[4] = make_float4(h.v[8], h.v[9], h.v[10], 0.0f);
-    s_rec[1 + idx] = (int)prim;
   } else {
     GlobalSink g{P.dverts + 3 * (size_t)off, P.dopac + prim, P.dsigma + prim, P.dsh + (size_t)prim * P.M * 3};
     for (int f = 0; f < 11; f++) g.add(f, h.v[f]);
@@ -758,10 +765,9 @@ TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const
   }
 }
 
-// Adds one record (row r of this block's region) to register sums.
-template <int BLOCK>
-TT_D void add_record(const TraceParams &P, int r, float acc[11], ShSink &sh) {
-  const float4 *row = P.rec_rows + 5 * ((size_t)blockIdx.x * rec_per_block<BLOCK>() + r);
+// Adds record row r (an absolute row of P.rec_rows) to register sums; returns its triangle.
+TT_D uint32_t add_record(const TraceParams &P, size_t r, float acc[11], ShSink &sh) {
+  const float4 *row = P.rec_rows + 5 * r;
   const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4];
   const int cl = __float_as_int(r0.y);
   const float dc[3] = {r1.y, r1.z, r1.w};
@@ -769,6 +775,7 @@ TT_D void add_record(const TraceParams &P, int r, float acc[11], ShSink &sh) {
   acc[4] += r3.x, acc[5] += r3.y, acc[6] += r3.z, acc[7] += r3.w;
   acc[8] += r4.x, acc[9] += r4.y, acc[10] += r4.z;
   sh_bwd(P.deg, mk(r0.z, r0.w, r1.x), (cl & 1) != 0, (cl & 2) != 0, (cl & 4) != 0, dc, sh);
+  return __float_as_uint(r0.x);
 }
 
 // Flush of one triangle's sums to the global gradient buffers (FP32 atomics).
@@ -986,69 +993,92 @@ TT_D void backward_block(const TraceParams &P, bool valid, int pix, int *s_node,
   __syncthreads();
 }
 
-// The record path of one block's backward: the lanes trace their rays writing gradient
-// records (no LDS atomics on the gradient fields), then the block groups its records by
-// triangle in LDS (hash of the triangle id -> slot, counting sort by slot) and one
-// thread per triangle sums its records -- rebuilding the SH gradient from each record's
-// direction and dL/dcolour -- and flushes the sums with one FP32 atomic per field. Every
-// lane of the block must call this (block barriers). LDS: s_node (traversal stack and
-// leaf queue, reused for the grouping), s_rec (1 + rec_per_block ints), s_tab
-// (4 x agg_slots ints).
+// The record path of one block's backward (tile `tile`): the lanes trace their rays
+// writing gradient records (no LDS atomics on the gradient fields); then, in batches of
+// rec_batch records, the block groups its records by triangle in LDS (hash of the
+// triangle id -> slot, counting sort by slot) and one thread per triangle sums its
+// records -- rebuilding the SH gradient from each record's direction and dL/dcolour --
+// and flushes the sums with one FP32 atomic per field. Every lane of the block must call
+// this (block barriers; it ends with one, so the LDS can be reused by the caller). LDS:
+// s_node (traversal stack and leaf queue, reused for the grouping), s_rec (kRecLds ints),
+// s_tab (4 x agg_slots ints).
 template <int K, int BLOCK, int VARIANT>
-TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int *s_node, int *s_rec, int *s_tab) {
-  constexpr int kRecBlock = rec_per_block<BLOCK>(), kAggSlots = agg_slots<BLOCK>();
-  static_assert(kLdsInts * BLOCK * 4 >= 2 * kRecBlock * 2, "the grouping reuses the traversal LDS");
+TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile, int *s_node, int *s_rec,
+                             int *s_tab) {
+  constexpr int kBatch = rec_batch<BLOCK>(), kAggSlots = agg_slots<BLOCK>();
+  static_assert(kLdsInts * BLOCK * 4 >= 2 * kBatch * 2, "the grouping reuses the traversal LDS");
   int *t_keys = s_tab, *t_cnt = s_tab + kAggSlots, *t_off = s_tab + 2 * kAggSlots, *t_fill = s_tab + 3 * kAggSlots;
-  if (threadIdx.x == 0) s_rec[0] = 0;
+  if (threadIdx.x == 0) {
+    int start = blockIdx.x * rec_per_block<BLOCK>(), rows = rec_per_block<BLOCK>();
+    if (P.rec_off) {
+      start = P.rec_off[tile];
+      rows = max(0, min(P.tile_hits[tile], P.rec_cap - start));
+    }
+    s_rec[0] = 0;
+    s_rec[1] = start;
+    s_rec[2] = rows;
+  }
   for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cnt[i] = 0, t_fill[i] = 0;
   __syncthreads();
   if (valid) backward_ray<K, BLOCK, VARIANT, true>(P, pix, s_node, s_rec, nullptr);
   __syncthreads();
-  const int n = min(s_rec[0], kRecBlock);
-  short *slot_of = reinterpret_cast<short *>(s_node), *list = slot_of + kRecBlock;
-  for (int i = threadIdx.x; i < n; i += BLOCK) {
-    const int sl = claim_slot<kAggSlots>(t_keys, (uint32_t)s_rec[1 + i]);
-    slot_of[i] = (short)sl;
-    if (sl >= 0) atomicAdd(&t_cnt[sl], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x < 64) {  // exclusive scan of the slot counts: kAggSlots / 64 per lane + a wave scan
-    constexpr int per = kAggSlots / 64;
-    const int l = threadIdx.x;
-    int c[per], sum = 0;
-    for (int j = 0; j < per; j++) c[j] = t_cnt[per * l + j], sum += c[j];
-    int inc = sum;
-    for (int off = 1; off < 64; off <<= 1) {
-      const int v = __shfl_up(inc, off);
-      if (l >= off) inc += v;
+  const int n_all = min(s_rec[0], s_rec[2]);
+  const size_t start = (size_t)s_rec[1];
+  short *slot_of = reinterpret_cast<short *>(s_node), *list = slot_of + kBatch;
+  for (int b0 = 0; b0 < n_all; b0 += kBatch) {
+    const int n = min(kBatch, n_all - b0);
+    const size_t first = start + b0;
+    if (b0 > 0) {
+      for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cnt[i] = 0, t_fill[i] = 0;
+      __syncthreads();
     }
-    int run = inc - sum;
-    for (int j = 0; j < per; j++) t_off[per * l + j] = run, run += c[j];
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += BLOCK) {
-    const int sl = slot_of[i];
-    if (sl >= 0) {
-      list[t_off[sl] + atomicAdd(&t_fill[sl], 1)] = (short)i;
-    } else {  // no slot left: this record goes out on its own
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+      const uint32_t prim = __float_as_uint(P.rec_rows[5 * (first + i)].x);
+      const int sl = claim_slot<kAggSlots>(t_keys, prim);
+      slot_of[i] = (short)sl;
+      if (sl >= 0) atomicAdd(&t_cnt[sl], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the slot counts: kAggSlots / 64 per lane + a wave scan
+      constexpr int per = kAggSlots / 64;
+      const int l = threadIdx.x;
+      int c[per], sum = 0;
+      for (int j = 0; j < per; j++) c[j] = t_cnt[per * l + j], sum += c[j];
+      int inc = sum;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(inc, off);
+        if (l >= off) inc += v;
+      }
+      int run = inc - sum;
+      for (int j = 0; j < per; j++) t_off[per * l + j] = run, run += c[j];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+      const int sl = slot_of[i];
+      if (sl >= 0) {
+        list[t_off[sl] + atomicAdd(&t_fill[sl], 1)] = (short)i;
+      } else {  // no slot left: this record goes out on its own
+        float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        ShSink sh;
+        for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
+        const uint32_t prim = add_record(P, first + i, acc, sh);
+        flush_sums(P, prim, acc, sh);
+      }
+    }
+    __syncthreads();
+    for (int sl = threadIdx.x; sl < kAggSlots; sl += BLOCK) {
+      const int prim = t_keys[sl];
+      if (prim < 0) continue;
       float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       ShSink sh;
       for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
-      add_record<BLOCK>(P, i, acc, sh);
-      flush_sums(P, (uint32_t)s_rec[1 + i], acc, sh);
+      const int b = t_off[sl], e = b + t_cnt[sl];
+      for (int j = b; j < e; j++) add_record(P, first + list[j], acc, sh);
+      flush_sums(P, (uint32_t)prim, acc, sh);
     }
+    __syncthreads();
   }
   __syncthreads();
-  for (int sl = threadIdx.x; sl < kAggSlots; sl += BLOCK) {
-    const int prim = t_keys[sl];
-    if (prim < 0) continue;
-    float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    ShSink sh;
-    for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
-    const int b = t_off[sl], e = b + t_cnt[sl];
-    for (int j = b; j < e; j++) add_record<BLOCK>(P, list[j], acc, sh);
-    flush_sums(P, (uint32_t)prim, acc, sh);
-  }
 }
 
 __global__ void k_fill(float *p, size_t n, float v) {
