@@ -149,7 +149,7 @@ template <int K, int VARIANT>
 __global__ __launch_bounds__(kBlock, 3) void k_backward_rec(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
   __shared__ int s_rec[kRecLds];
-  __shared__ int s_tab[4 * agg_slots<kBlock>()];
+  __shared__ int s_tab[rec_tab_ints<kBlock>()];
   int x, y;
   const int tile = P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles);
   const bool valid = tile_pixel<kBlock>(P, tile, x, y);

@@ -740,6 +740,11 @@ template <int BLOCK>
 constexpr int agg_slots() {  // triangles a block groups one batch of records into
   return 2 * BLOCK;
 }
+constexpr int kStageSlots = 32;  // triangles whose sums are staged in LDS per coalesced flush
+template <int BLOCK>
+constexpr int rec_tab_ints() {  // s_tab of backward_block_rec: grouping table + flush stage
+  return 4 * agg_slots<BLOCK>() + 64 * kStageSlots;
+}
 template <int BLOCK>
 TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const float dc[3], bool c0, bool c1,
                       bool c2, const HitSink &h, int *s_rec) {
@@ -776,6 +781,37 @@ TT_D uint32_t add_record(const TraceParams &P, size_t r, float acc[11], ShSink &
   acc[8] += r4.x, acc[9] += r4.y, acc[10] += r4.z;
   sh_bwd(P.deg, mk(r0.z, r0.w, r1.x), (cl & 1) != 0, (cl & 2) != 0, (cl & 4) != 0, dc, sh);
   return __float_as_uint(r0.x);
+}
+
+// Stage row of one triangle's sums (64 floats): SH 0..47, vertices 48..56, opacity 57,
+// sigma 58, triangle id 63 (-1: no triangle).
+TT_D void stage_sums(float *row, int prim, const float acc[11], const ShSink &sh) {
+  for (int f = 0; f < 48; f++) row[f] = sh.v[f];
+  for (int f = 0; f < 11; f++) row[48 + f] = acc[f];
+  row[63] = __int_as_float(prim);
+}
+
+// Flush of kStageSlots staged rows by the whole block: lane = (row, field), so a wave's
+// atomic instruction covers one triangle's consecutive SH / vertex fields.
+template <int BLOCK>
+TT_D void flush_stage(const TraceParams &P, const float *stage) {
+  const int nsh3 = 3 * (P.deg + 1) * (P.deg + 1);
+  for (int p = threadIdx.x; p < 64 * kStageSlots; p += BLOCK) {
+    const int f = p & 63;
+    const int prim = __float_as_int(stage[p | 63]);
+    if (prim < 0 || f >= 59) continue;
+    const float v = stage[p];
+    if (v == 0.0f) continue;
+    if (f < 48) {
+      if (f < nsh3) unsafeAtomicAdd(P.dsh + (size_t)prim * P.M * 3 + f, v);
+    } else if (f < 57) {
+      unsafeAtomicAdd(P.dverts + 3 * (size_t)P.cumsum[prim] + (f - 48), v);
+    } else if (f == 57) {
+      unsafeAtomicAdd(P.dopac + prim, v);
+    } else {
+      unsafeAtomicAdd(P.dsigma + prim, v);
+    }
+  }
 }
 
 // Flush of one triangle's sums to the global gradient buffers (FP32 atomics).
@@ -999,15 +1035,18 @@ TT_D void backward_block(const TraceParams &P, bool valid, int pix, int *s_node,
 // triangle id -> slot, counting sort by slot) and one thread per triangle sums its
 // records -- rebuilding the SH gradient from each record's direction and dL/dcolour --
 // and flushes the sums with one FP32 atomic per field. Every lane of the block must call
-// this (block barriers; it ends with one, so the LDS can be reused by the caller). LDS:
-// s_node (traversal stack and leaf queue, reused for the grouping), s_rec (kRecLds ints),
-// s_tab (4 x agg_slots ints).
+// this (block barriers; it ends with one, so the LDS can be reused by the caller). The
+// sums of kStageSlots triangles at a time are staged in LDS and flushed by the whole block
+// with one lane per (triangle, field), so that one atomic instruction covers a triangle's
+// consecutive gradient fields. LDS: s_node (traversal stack and leaf queue, reused for the
+// grouping), s_rec (kRecLds ints), s_tab (rec_tab_ints ints: table + stage).
 template <int K, int BLOCK, int VARIANT>
 TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile, int *s_node, int *s_rec,
                              int *s_tab) {
   constexpr int kBatch = rec_batch<BLOCK>(), kAggSlots = agg_slots<BLOCK>();
   static_assert(kLdsInts * BLOCK * 4 >= 2 * kBatch * 2, "the grouping reuses the traversal LDS");
   int *t_keys = s_tab, *t_cnt = s_tab + kAggSlots, *t_off = s_tab + 2 * kAggSlots, *t_fill = s_tab + 3 * kAggSlots;
+  float *stage = reinterpret_cast<float *>(s_tab + 4 * kAggSlots);
   if (threadIdx.x == 0) {
     int start = blockIdx.x * rec_per_block<BLOCK>(), rows = rec_per_block<BLOCK>();
     if (P.rec_off) {
@@ -1066,17 +1105,24 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
       }
     }
     __syncthreads();
-    for (int sl = threadIdx.x; sl < kAggSlots; sl += BLOCK) {
+    for (int round = 0; round < kAggSlots / BLOCK; round++) {
+      const int sl = round * BLOCK + threadIdx.x;
       const int prim = t_keys[sl];
-      if (prim < 0) continue;
       float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       ShSink sh;
       for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
-      const int b = t_off[sl], e = b + t_cnt[sl];
-      for (int j = b; j < e; j++) add_record(P, first + list[j], acc, sh);
-      flush_sums(P, (uint32_t)prim, acc, sh);
+      if (prim >= 0) {
+        const int b = t_off[sl], e = b + t_cnt[sl];
+#pragma unroll 2
+        for (int j = b; j < e; j++) add_record(P, first + list[j], acc, sh);
+      }
+      for (int q = 0; q < BLOCK / kStageSlots; q++) {
+        if (threadIdx.x / kStageSlots == q) stage_sums(stage + 64 * (threadIdx.x % kStageSlots), prim, acc, sh);
+        __syncthreads();
+        flush_stage<BLOCK>(P, stage);
+        __syncthreads();
+      }
     }
-    __syncthreads();
   }
   __syncthreads();
 }
