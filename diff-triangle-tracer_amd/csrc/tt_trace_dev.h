@@ -727,7 +727,7 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
 // same image sizes each tile's region to that tile's blended-hit count in the forward
 // (P.tile_hits, prefix sums in P.rec_off); otherwise a block has rec_per_block rows. Past
 // its region a hit's gradient goes to the global buffers with atomics instead.
-constexpr int kRecLds = 4;  // s_rec: counter, region start, region rows, spare
+constexpr int kRecLds = 4;  // s_rec: counter, region start, region rows, work items of the aggregation
 template <int BLOCK>
 constexpr int rec_per_block() {  // rows of a fixed region: 16 per lane
   return 16 * BLOCK;
@@ -741,9 +741,10 @@ constexpr int agg_slots() {  // triangles a block groups one batch of records in
   return 2 * BLOCK;
 }
 constexpr int kStageSlots = 32;  // triangles whose sums are staged in LDS per coalesced flush
+constexpr int kSegRecords = 16;  // records one thread sums at most (a triangle's longer runs are split)
 template <int BLOCK>
 constexpr int rec_tab_ints() {  // s_tab of backward_block_rec: grouping table + flush stage
-  return 4 * agg_slots<BLOCK>() + 64 * kStageSlots;
+  return 5 * agg_slots<BLOCK>() + 64 * kStageSlots;
 }
 template <int BLOCK>
 TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const float dc[3], bool c0, bool c1,
@@ -1046,7 +1047,8 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   constexpr int kBatch = rec_batch<BLOCK>(), kAggSlots = agg_slots<BLOCK>();
   static_assert(kLdsInts * BLOCK * 4 >= 2 * kBatch * 2, "the grouping reuses the traversal LDS");
   int *t_keys = s_tab, *t_cnt = s_tab + kAggSlots, *t_off = s_tab + 2 * kAggSlots, *t_fill = s_tab + 3 * kAggSlots;
-  float *stage = reinterpret_cast<float *>(s_tab + 4 * kAggSlots);
+  int *t_item = s_tab + 4 * kAggSlots;  // per slot: its first work item (segment of <= kSegRecords records)
+  float *stage = reinterpret_cast<float *>(s_tab + 5 * kAggSlots);
   if (threadIdx.x == 0) {
     int start = blockIdx.x * rec_per_block<BLOCK>(), rows = rec_per_block<BLOCK>();
     if (P.rec_off) {
@@ -1081,15 +1083,25 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
     if (threadIdx.x < 64) {  // exclusive scan of the slot counts: kAggSlots / 64 per lane + a wave scan
       constexpr int per = kAggSlots / 64;
       const int l = threadIdx.x;
-      int c[per], sum = 0;
-      for (int j = 0; j < per; j++) c[j] = t_cnt[per * l + j], sum += c[j];
-      int inc = sum;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(inc, off);
-        if (l >= off) inc += v;
+      int c[per], sum = 0, items = 0;
+      for (int j = 0; j < per; j++) {
+        c[j] = t_cnt[per * l + j];
+        sum += c[j];
+        items += (c[j] + kSegRecords - 1) / kSegRecords;
       }
-      int run = inc - sum;
-      for (int j = 0; j < per; j++) t_off[per * l + j] = run, run += c[j];
+      int inc = sum, inc_items = items;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(inc, off), w = __shfl_up(inc_items, off);
+        if (l >= off) inc += v, inc_items += w;
+      }
+      int run = inc - sum, run_items = inc_items - items;
+      for (int j = 0; j < per; j++) {
+        t_off[per * l + j] = run;
+        t_item[per * l + j] = run_items;
+        run += c[j];
+        run_items += (c[j] + kSegRecords - 1) / kSegRecords;
+      }
+      if (l == 63) s_rec[3] = inc_items;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += BLOCK) {
@@ -1105,14 +1117,22 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
       }
     }
     __syncthreads();
-    for (int round = 0; round < kAggSlots / BLOCK; round++) {
-      const int sl = round * BLOCK + threadIdx.x;
-      const int prim = t_keys[sl];
+    const int n_items = s_rec[3];
+    for (int i0 = 0; i0 < n_items; i0 += BLOCK) {  // work item = (slot, segment of <= kSegRecords records)
+      const int it = i0 + threadIdx.x;
+      int prim = -1;
       float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       ShSink sh;
       for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
-      if (prim >= 0) {
-        const int b = t_off[sl], e = b + t_cnt[sl];
+      if (it < n_items) {
+        int lo = 0, hi = kAggSlots;  // the last slot whose first item is <= it
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (t_item[mid] <= it) lo = mid;
+          else hi = mid;
+        }
+        prim = t_keys[lo];
+        const int b = t_off[lo] + (it - t_item[lo]) * kSegRecords, e = min(t_off[lo] + t_cnt[lo], b + kSegRecords);
 #pragma unroll 2
         for (int j = b; j < e; j++) add_record(P, first + list[j], acc, sh);
       }
