@@ -5,7 +5,10 @@ Default workload (the headline): fwd+bwd Mrays/s at 1920x1080 on 1M clustered
 triangle splats (64 clusters, seed 1), k = 16 hits per chunk, transmittance
 cutoff 1e-3, random N(0,1) image-space loss gradient; exact chunk mode.
 
-  plain        : forward launch + gradient zeroing + re-traversing backward launch
+  plain        : forward launch (also storing each pixel's blended hits, up to
+                 --hitlist-cap) + gradient zeroing + backward launch reading those
+                 hits instead of re-traversing (same gradients)
+  plain_retrace: the same with the reference's re-traversing backward
   proposal     : the learned depth-interval proposal (bf16 MLP on the matrix
                  cores, trained first on the intervals the tracer emits for 16
                  other camera poses of the same scene; the measured frame is held
@@ -44,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernels on the default workload (1M splats, 1920x1080,
 # k=16), 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE from rocprofv3 counter runs of this script
 # (profiles/trace_fwd_bwd_1M_1080p.md; the factors come from a 256 MiB copy).
-MEASURED_TRAFFIC = {"k_backward_rec<16,0>": 18.085e9, "k_forward<16,0,false,true>": 2.994e9}
+MEASURED_TRAFFIC = {}
 TRAIN_POSES, HELD_OUT_EXTRA = 16, 3
 
 
@@ -238,7 +241,7 @@ def bounce(args, dev):
 class Frame:
     """Scene tensors, the tracer handle and per-frame buffers of one workload."""
 
-    def __init__(self, dev, scene, k, cutoff):
+    def __init__(self, dev, scene, k, cutoff, hitlist_cap=0):
         import torch
         from diff_tritracer import capi
 
@@ -249,6 +252,7 @@ class Frame:
         self.verts, self.npts, self.cumsum = T(scene.verts), T(scene.npts), T(scene.cumsum)
         self.shs, self.opac, self.sigma = T(scene.shs), T(scene.opac.reshape(-1)), T(scene.sigma.reshape(-1))
         self.k, self.cutoff = k, cutoff
+        self.hitlist_cap = hitlist_cap
         self.bg = torch.zeros(3, device=dev)
         self.stream = torch.cuda.current_stream()
         self.sp = self.stream.cuda_stream
@@ -271,24 +275,25 @@ class Frame:
         self.ival = torch.zeros(R, 4, device=dev)
         self.prop = torch.zeros(R, 2, device=dev)
         self.prop2 = torch.zeros(R, 2, device=dev)
+        self.hl = self.capi.hitlist_buffers(R, self.hitlist_cap, dev) if self.hitlist_cap > 0 else None
 
     def camera(self, basis):
         self.capi.camera_rays(basis, self.W, self.H, self.ro, self.rd, self.sp)
 
     def fwd_args(self, mode="exact", interval_in=None, interval_out=None, hard=False, stats=None, trav=None,
-                 fb_start=0.0, fb_end=0.0):
+                 fb_start=0.0, fb_end=0.0, hitlist=True):
         c = self.capi
         return c.fwd_args(self.W, self.H, self.ro, self.rd, self.verts, self.npts, self.cumsum, self.shs, 3, self.opac,
                           self.sigma, self.bg, self.color, self.others, self.maxb, self.dens, self.scal, k=self.k,
                           cutoff=self.cutoff, chunk_mode=mode, interval_in=interval_in, interval_out=interval_out,
                           hard_cap=hard, fallback_start=fb_start, fallback_end=fb_end, stats=stats,
-                          trav_stats=trav)
+                          trav_stats=trav, hitlist=self.hl if hitlist else None)
 
-    def bwd_args(self, mode="exact", interval=None):
+    def bwd_args(self, mode="exact", interval=None, hitlist=True):
         return self.capi.bwd_args(self.W, self.H, self.ro, self.rd, self.verts, self.npts, self.cumsum, self.shs, 3,
                                   self.opac, self.sigma, self.color, self.others, self.dpix, self.doth, self.dverts,
                                   self.dsig, self.dop, self.dsh, k=self.k, cutoff=self.cutoff, chunk_mode=mode,
-                                  zero_grads=False, interval=interval)
+                                  zero_grads=False, interval=interval, hitlist=self.hl if hitlist else None)
 
     def zero_grads(self):
         for t in (self.dverts, self.dsig, self.dop, self.dsh):
@@ -377,7 +382,7 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     from diff_tritracer import capi, scenes
 
     W, H = args.width, args.height
-    fr = Frame(dev, scene, args.k, args.cutoff)
+    fr = Frame(dev, scene, args.k, args.cutoff, args.hitlist_cap)
     P = fr.P
     # LBVH build and refit, timed separately (median of 5 after one warm-up)
     builds, refits = [], []
@@ -412,6 +417,8 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     sp = fr.sp
     fa_plain = fr.fwd_args("exact")
     ba_plain = fr.bwd_args("exact")
+    fa_retrace = fr.fwd_args("exact", hitlist=False)
+    ba_retrace = fr.bwd_args("exact", hitlist=False)
     fa_compat = fr.fwd_args("compat")
     ba_compat = fr.bwd_args("compat")
     fa_emit = fr.fwd_args("exact", interval_out=fr.ival)
@@ -427,6 +434,8 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
 
     runs = {
         "plain": [("fwd", lambda: fr.h.fwd(fa_plain, sp)), ("zero", fr.zero_grads), ("bwd", lambda: fr.h.bwd(ba_plain, sp))],
+        "plain_retrace": [("fwd", lambda: fr.h.fwd(fa_retrace, sp)), ("zero", fr.zero_grads),
+                          ("bwd", lambda: fr.h.bwd(ba_retrace, sp))],
         "plain_compat": [("fwd", lambda: fr.h.fwd(fa_compat, sp)), ("zero", fr.zero_grads),
                          ("bwd", lambda: fr.h.bwd(ba_compat, sp))],
         "handoff_only": [("fwd", lambda: fr.h.fwd(fa_emit, sp)), ("zero", fr.zero_grads),
@@ -471,6 +480,9 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     mr = lambda name: R / (res[name]["ms"] * 1e-3) / 1e6
     bf = 64 * R + 48 * Hc + 204 * Hb + 8 * R + 16 * R  # + proposed interval read, emitted interval written
     bb = 104 * R + 48 * Hc + 428 * Hb + 16 * R         # + emitted interval read
+    if fr.hl is not None:  # hit list: the forward writes 8 B per blended hit + a count, the backward reads them
+        bf += 8 * Hb + 4 * R
+        bb += 8 * Hb + 4 * R - 48 * Hc - 16 * R
     per = res["proposal"]["per"]
     if per["bwd"] >= per["fwd"]:
         dom, dom_bytes, dom_ms = f"k_backward_rec<{args.k},0>", bb, per["bwd"]
@@ -489,7 +501,9 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
                     f"proposal's {TRAIN_POSES} training poses), k={args.k}, cutoff {args.cutoff:g}, exact chunk mode, "
                     f"N(0,1) loss gradient",
         "triangles": P, "width": W, "height": H, "k": args.k, "cutoff": args.cutoff, "chunk_mode": "exact",
-        "plain_mrays": round(mr("plain"), 3), "plain_compat_mrays": round(mr("plain_compat"), 3),
+        "hitlist_cap": args.hitlist_cap,
+        "plain_mrays": round(mr("plain"), 3), "plain_retrace_mrays": round(mr("plain_retrace"), 3),
+        "plain_compat_mrays": round(mr("plain_compat"), 3),
         "handoff_only_mrays": round(mr("handoff_only"), 3), "proposal_mrays": round(mr("proposal"), 3),
         "proposal_hard_mrays": round(mr("proposal_hard"), 3),
         "speedup_vs_plain": round(res["plain"]["ms"] / res["proposal"]["ms"], 4),
@@ -522,6 +536,8 @@ def main():
     ap.add_argument("--proposal-steps", type=int, default=2000)
     ap.add_argument("--proposal-batch", type=int, default=16384)
     ap.add_argument("--cpu-stride", type=int, default=1)
+    ap.add_argument("--hitlist-cap", type=int, default=64,
+                    help="blended hits per pixel the forward stores for the backward (0: the backward re-traverses)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce"], default="frame")
     ap.add_argument("--copy-mib", type=int, default=0,

@@ -278,6 +278,11 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
       TT_ARG(a->variant == TT_VARIANT_FORWARD, "depth intervals are supported by the forward variant only");
     if (a->interval_in_dev)
       TT_ARG(a->chunk_mode == TT_CHUNK_EXACT, "interval_in requires TT_CHUNK_EXACT");
+    if (a->hitlist_dev || a->hitlist_count_dev) {
+      TT_ARG(a->variant == TT_VARIANT_FORWARD, "the hit list is supported by the forward variant only");
+      TT_ARG(a->hitlist_dev && a->hitlist_count_dev && a->hitlist_cap > 0,
+             "the hit list needs hitlist, hitlist_count and hitlist_cap > 0");
+    }
   }
   if (R == 0) return ok();
   tt::TraceParams p;
@@ -304,6 +309,11 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.ival_hard = a->interval_in_dev && a->interval_hard_cap ? 1 : 0;
   p.ival_fb_start = a->interval_fallback_start;
   p.ival_fb_end = a->interval_fallback_end;
+  if (a->hitlist_dev) {
+    p.hl = reinterpret_cast<uint2 *>(a->hitlist_dev);
+    p.hl_count = a->hitlist_count_dev;
+    p.hl_cap = a->hitlist_cap;
+  }
   if (a->variant == TT_VARIANT_BOUNCE) {
     p.refl = a->reflectivity_dev;
     p.ior = a->ior_dev;
@@ -389,6 +399,11 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   TT_ARG(a->color_dev && a->others_dev && a->dL_dpix_dev && a->dL_dothers_dev,
          "forward outputs / upstream gradients are NULL");
   if (a->interval_dev) TT_ARG(a->chunk_mode == TT_CHUNK_EXACT, "interval requires TT_CHUNK_EXACT");
+  if (a->hitlist_dev || a->hitlist_count_dev) {
+    TT_ARG(a->variant == TT_VARIANT_FORWARD, "the hit list is supported by the forward variant only");
+    TT_ARG(a->hitlist_dev && a->hitlist_count_dev && a->hitlist_cap > 0,
+           "the hit list needs hitlist, hitlist_count and hitlist_cap > 0");
+  }
   tt::TraceParams p;
   fill_common(p, tr, a->width, a->height, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev, a->cumsum_dev,
               a->P, a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->chunk_mode, a->cutoff,
@@ -402,6 +417,10 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   p.dopac = a->dL_dopacity_dev;
   p.dsh = a->dL_dsh_dev;
   p.ival_bwd = reinterpret_cast<const float4 *>(a->interval_dev);
+  if (a->hitlist_dev) {
+    p.hl_in = reinterpret_cast<const uint2 *>(a->hitlist_dev);
+    p.hl_count_in = a->hitlist_count_dev;
+  }
   if (a->variant == TT_VARIANT_BOUNCE) {
     p.refl = a->reflectivity_dev;
     p.env = a->envmap_dev;

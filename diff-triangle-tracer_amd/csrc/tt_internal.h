@@ -113,6 +113,15 @@ struct TraceParams {
   int *tile_hits;
   const int *rec_off;
   int *rec_off_out;
+  // stored blended hits of the primary rays (variant 0, one-shot launches): the forward
+  // writes hit i of pixel p as (prim, t bits) to hl[i * R + p] for i < hl_cap and the
+  // count to hl_count[p] (-1: more than hl_cap); a backward given hl_in reads them
+  // instead of re-traversing (pixels with count -1 are re-traversed)
+  uint2 *hl;
+  int *hl_count;
+  int hl_cap;
+  const uint2 *hl_in;
+  const int *hl_count_in;
 };
 // Per-pixel record of the bounce variant: L_r[3] L_t[3] R Q o_r[3] d_r[3] o_t[3] d_t[3]
 // (d_r = 0: no secondary rays).

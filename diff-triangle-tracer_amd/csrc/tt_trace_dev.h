@@ -347,6 +347,8 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
             st.D += bw * t;
           }
           st.T = testT;
+          if (FULL && P.hl && st.blended < P.hl_cap)
+            P.hl[(size_t)st.blended * P.R + ray] = make_uint2(prim, __float_as_uint(t));
           st.blended++;
           code = 2;
         }
@@ -393,6 +395,7 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
     trace_segment<K, BLOCK, true, COUNT, false, IVAL>(P, o, d, st, pix, o, s_node, tstart, cap);
     if (IVAL && P.ival_out)
       P.ival_out[pix] = make_float4(st.t_first, st.t_last, st.t_term, __uint_as_float(st.p_first));
+    if (P.hl_count) P.hl_count[pix] = st.blended <= P.hl_cap ? st.blended : -1;
   } else if (VARIANT == 4) {
     trace_segment<K, BLOCK, true, COUNT, true>(P, o, d, st, pix, o, s_node);
     const Bounce g = bounce_rays(o, d, st.T, st.N, st.D, st.eta);
@@ -829,17 +832,95 @@ TT_D void flush_sums(const TraceParams &P, uint32_t prim, const float acc[11], c
 }
 
 // backward.cu:191-350 for one traced segment: re-traverse it with the same chunk
-// logic, recompute the forward state hit by hit and send each blended hit's gradient
-// to the sink. fin = the segment's output colour (composite incl. background), g = the
-// upstream gradient on it. MODE 0: a primary ray (depth / normal terms); 1: the bounce
-// variant's primary (per-hit colours extended by rho L_r + (1 - rho) L_t, with the
-// reflectivity gradient); 2: a bounce secondary ray (colour only). Returns the
-// segment's final transmittance.
+// logic (or, given hl, read the forward's stored blended hits of the ray: hl[i * stride]
+// for i < n_hl), recompute the forward state hit by hit and send each blended hit's
+// gradient to the sink. fin = the segment's output colour (composite incl. background),
+// g = the upstream gradient on it. MODE 0: a primary ray (depth / normal terms); 1: the
+// bounce variant's primary (per-hit colours extended by rho L_r + (1 - rho) L_t, with the
+// reflectivity gradient); 2: a bounce secondary ray (colour only). Returns the segment's
+// final transmittance.
 template <int K, int BLOCK, int MODE, bool REC>
 TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], const float g[3], float dL_ddepth,
                         v3f dL_dn2, float final_depth, v3f final_normal, const float *Lr, const float *Lt, float tlo,
-                        long long plo, float far_t, int *s_node, int *s_keys, float *s_acc) {
+                        long long plo, float far_t, int *s_node, int *s_keys, float *s_acc,
+                        const uint2 *hl = nullptr, int n_hl = 0, size_t stride = 0) {
   float C[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f}, D = 0.0f, T = 1.0f;
+  // one hit at or past the near plane; false: the ray terminates before blending it
+  auto blend = [&](float t, uint32_t prim) -> bool {
+    const int off = P.cumsum[prim];
+    const float *vp = P.verts + 3 * (size_t)off;
+    const v3f V[3] = {ld3(vp), ld3(vp + 3), ld3(vp + 6)};
+    Win w;
+    window32(V[0], V[1], V[2], o, d, t, P.sigma[prim], w);
+    const float opacity = P.opac[prim];
+    const float alpha = fminf(0.99f, opacity * w.Cx);
+    if (alpha < 1.0f / 255.0f) return true;
+    const float testT = T * (1.0f - alpha);
+    if (testT < P.cutoff) return false;
+    const float bw = alpha * T;
+    bool c0, c1, c2;
+    const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
+    float cc[3] = {col.x, col.y, col.z};
+    if (MODE == 1) {
+      const float rho = P.refl[prim];
+      float dr = 0.0f;
+      for (int c = 0; c < 3; c++) {
+        dr += g[c] * (Lr[c] - Lt[c]);
+        cc[c] += rho * Lr[c] + (1.0f - rho) * Lt[c];
+      }
+      unsafeAtomicAdd(P.drefl + prim, bw * dr);
+    }
+    C[0] += cc[0] * bw, C[1] += cc[1] * bw, C[2] += cc[2] * bw;
+    if (MODE != 2) {
+      N[0] += w.n.x * bw, N[1] += w.n.y * bw, N[2] += w.n.z * bw;
+      D += bw * t;
+    }
+    float dL_dcolor[3];
+    float dL_dalpha = 0.0f;
+    const float inv1a = 1.0f / (1.0f - alpha);
+    for (int c = 0; c < 3; c++) {
+      dL_dcolor[c] = g[c] * bw;
+      dL_dalpha += g[c] * (T * cc[c] - (fin[c] - C[c]) * inv1a);
+    }
+    if (MODE != 2) {
+      dL_dalpha += dL_ddepth * (T * t - (final_depth - D) * inv1a);
+      const v3f Nf = mk(N[0], N[1], N[2]);
+      dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
+    }
+    const float dL_dC = opacity * dL_dalpha;
+    const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
+    const v3f dL_dn3 = MODE != 2 ? dL_dn2 * bw : mk(0.0f, 0.0f, 0.0f);
+    if (REC) {
+      HitSink hs;
+      for (int f = 0; f < 11; f++) hs.v[f] = 0.0f;
+      hs.v[F_OPAC] = dL_dalpha * w.Cx;
+      if (w.phi_final > 0.0f) hs.v[F_SIGMA] = (dL_dC * w.Cx) * log_tt(w.phi_final);
+      window_bwd(V, w, dL_dphi, dL_dn3, hs);
+      emit_record<BLOCK>(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_keys);
+    } else {
+      GradSink sink;
+      sink.acc = s_acc;
+      sink.stride = BLOCK;
+      sink.slot = claim_slot<BLOCK>(s_keys, prim);
+      sink.g_verts = P.dverts + 3 * (size_t)off;
+      sink.g_opac = P.dopac + prim;
+      sink.g_sigma = P.dsigma + prim;
+      sink.g_sh = P.dsh + (size_t)prim * P.M * 3;
+      sink.add(F_OPAC, dL_dalpha * w.Cx);
+      if (w.phi_final > 0.0f) sink.add(F_SIGMA, (dL_dC * w.Cx) * log_tt(w.phi_final));
+      window_bwd(V, w, dL_dphi, dL_dn3, sink);
+      sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, sink);
+    }
+    T = testT;
+    return true;
+  };
+  if (hl) {
+    for (int i = 0; i < n_hl; i++) {
+      const uint2 h = hl[(size_t)i * stride];
+      if (!blend(__uint_as_float(h.y), h.x)) break;
+    }
+    return T;
+  }
   const RayPre r = prep_ray(o, d);
   KBuf<K> b;
   float last = 0.0f;
@@ -857,71 +938,7 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
       lp = prim;
       last = fmaxf((float)((double)last + kStepEps), t);
       if (t < P.near_t) continue;
-      const int off = P.cumsum[prim];
-      const float *vp = P.verts + 3 * (size_t)off;
-      const v3f V[3] = {ld3(vp), ld3(vp + 3), ld3(vp + 6)};
-      Win w;
-      window32(V[0], V[1], V[2], o, d, t, P.sigma[prim], w);
-      const float opacity = P.opac[prim];
-      const float alpha = fminf(0.99f, opacity * w.Cx);
-      if (alpha < 1.0f / 255.0f) continue;
-      const float testT = T * (1.0f - alpha);
-      if (testT < P.cutoff) return T;
-      const float bw = alpha * T;
-      bool c0, c1, c2;
-      const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
-      float cc[3] = {col.x, col.y, col.z};
-      if (MODE == 1) {
-        const float rho = P.refl[prim];
-        float dr = 0.0f;
-        for (int c = 0; c < 3; c++) {
-          dr += g[c] * (Lr[c] - Lt[c]);
-          cc[c] += rho * Lr[c] + (1.0f - rho) * Lt[c];
-        }
-        unsafeAtomicAdd(P.drefl + prim, bw * dr);
-      }
-      C[0] += cc[0] * bw, C[1] += cc[1] * bw, C[2] += cc[2] * bw;
-      if (MODE != 2) {
-        N[0] += w.n.x * bw, N[1] += w.n.y * bw, N[2] += w.n.z * bw;
-        D += bw * t;
-      }
-      float dL_dcolor[3];
-      float dL_dalpha = 0.0f;
-      const float inv1a = 1.0f / (1.0f - alpha);
-      for (int c = 0; c < 3; c++) {
-        dL_dcolor[c] = g[c] * bw;
-        dL_dalpha += g[c] * (T * cc[c] - (fin[c] - C[c]) * inv1a);
-      }
-      if (MODE != 2) {
-        dL_dalpha += dL_ddepth * (T * t - (final_depth - D) * inv1a);
-        const v3f Nf = mk(N[0], N[1], N[2]);
-        dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
-      }
-      const float dL_dC = opacity * dL_dalpha;
-      const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
-      const v3f dL_dn3 = MODE != 2 ? dL_dn2 * bw : mk(0.0f, 0.0f, 0.0f);
-      if (REC) {
-        HitSink hs;
-        for (int f = 0; f < 11; f++) hs.v[f] = 0.0f;
-        hs.v[F_OPAC] = dL_dalpha * w.Cx;
-        if (w.phi_final > 0.0f) hs.v[F_SIGMA] = (dL_dC * w.Cx) * log_tt(w.phi_final);
-        window_bwd(V, w, dL_dphi, dL_dn3, hs);
-        emit_record<BLOCK>(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_keys);
-      } else {
-        GradSink sink;
-        sink.acc = s_acc;
-        sink.stride = BLOCK;
-        sink.slot = claim_slot<BLOCK>(s_keys, prim);
-        sink.g_verts = P.dverts + 3 * (size_t)off;
-        sink.g_opac = P.dopac + prim;
-        sink.g_sigma = P.dsigma + prim;
-        sink.g_sh = P.dsh + (size_t)prim * P.M * 3;
-        sink.add(F_OPAC, dL_dalpha * w.Cx);
-        if (w.phi_final > 0.0f) sink.add(F_SIGMA, (dL_dC * w.Cx) * log_tt(w.phi_final));
-        window_bwd(V, w, dL_dphi, dL_dn3, sink);
-        sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, sink);
-      }
-      T = testT;
+      if (!blend(t, prim)) return T;
     }
     if (P.chunk_mode == 1) {
       tlo = lt;
@@ -963,7 +980,13 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
   if (VARIANT == 0) {
     float tlo = 0.0f, far_t = P.far_t;
     long long plo = -1;
-    if (P.ival_bwd) {  // only the forward's blended hits: from the first one (inclusive) to the last
+    const uint2 *hl = nullptr;
+    int n_hl = 0;
+    if (P.hl_in) {  // the forward's stored blended hits (count -1: more than were stored)
+      n_hl = P.hl_count_in[pix];
+      if (n_hl >= 0) hl = P.hl_in + pix;
+    }
+    if (!hl && P.ival_bwd) {  // only the forward's blended hits: from the first one (inclusive) to the last
       const float4 iv = P.ival_bwd[pix];
       if (!(iv.x < kInfDist)) return;
       tlo = iv.x;
@@ -971,7 +994,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
       far_t = iv.y;
     }
     backward_seg<K, BLOCK, 0, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal,
-                                   nullptr, nullptr, tlo, plo, far_t, s_node, s_keys, s_acc);
+                                   nullptr, nullptr, tlo, plo, far_t, s_node, s_keys, s_acc, hl, n_hl, R);
   } else {
     const float *a = P.aux + (size_t)kBounceAux * pix;
     float Lr[3], Lt[3];

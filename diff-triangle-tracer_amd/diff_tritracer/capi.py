@@ -49,6 +49,7 @@ class FwdArgs(C.Structure):
         ("interval_hard_cap", C.c_int), ("interval_fallback_start", C.c_float), ("interval_fallback_end", C.c_float),
         ("reflectivity_dev", _fp), ("ior_dev", _fp), ("envmap_dev", _fp), ("env_width", C.c_int),
         ("env_height", C.c_int), ("bounce_aux_dev", _fp),
+        ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int),
     ]
 
 
@@ -68,6 +69,7 @@ class BwdArgs(C.Structure):
         ("interval_dev", _fp), ("variant", C.c_int), ("reflectivity_dev", _fp), ("envmap_dev", _fp),
         ("env_width", C.c_int), ("env_height", C.c_int), ("bounce_aux_dev", _fp), ("dL_dreflectivity_dev", _fp),
         ("dL_denvmap_dev", _fp),
+        ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int),
     ]
 
 
@@ -287,11 +289,19 @@ class TracerHandle:
         return out
 
 
+def hitlist_buffers(R, cap, device):
+    """Hit list of a forward for its backward: (hits int32 [cap, R, 2], counts int32 [R])."""
+    import torch
+    return (torch.empty(int(cap), int(R), 2, dtype=torch.int32, device=device),
+            torch.empty(int(R), dtype=torch.int32, device=device))
+
+
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
              scaling, k=16, cutoff=1e-4, chunk_mode="compat", variant="forward", extra=None, stats=None,
              log=None, near=0.2, far=1000.0, trav_stats=None, interval_in=None, interval_out=None,
-             hard_cap=False, fallback_start=0.0, fallback_end=0.0, bounce=None) -> FwdArgs:
-    """bounce: dict(reflectivity [P], ior [P], envmap [H, W, 3], aux [R, 20]) for variant "bounce"."""
+             hard_cap=False, fallback_start=0.0, fallback_end=0.0, bounce=None, hitlist=None) -> FwdArgs:
+    """bounce: dict(reflectivity [P], ior [P], envmap [H, W, 3], aux [R, 20]) for variant "bounce".
+    hitlist: (hits int32 [cap, R, 2], counts int32 [R]) written for the backward (see hitlist_buffers)."""
     a = FwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -318,6 +328,9 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
         a.reflectivity_dev, a.ior_dev = ptr(bounce["reflectivity"]), ptr(bounce["ior"])
         a.envmap_dev, a.bounce_aux_dev = ptr(bounce["envmap"]), ptr(bounce["aux"])
         a.env_height, a.env_width = int(bounce["envmap"].shape[0]), int(bounce["envmap"].shape[1])
+    if hitlist is not None:
+        hits, counts = hitlist
+        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[0])
     if log is not None:
         prim, t, code, count = log
         a.hit_log_cap = int(prim.shape[-1])
@@ -328,9 +341,9 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
 
 def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, color, others, dL_dpix,
              dL_dothers, dverts, dsigma, dopac, dsh, k=16, cutoff=1e-4, chunk_mode="compat", zero_grads=True,
-             near=0.2, far=1000.0, interval=None, bounce=None) -> BwdArgs:
+             near=0.2, far=1000.0, interval=None, bounce=None, hitlist=None) -> BwdArgs:
     """bounce: dict(reflectivity, envmap, aux, dreflectivity [P], denvmap [H, W, 3]) selects the
-    bounce variant's backward."""
+    bounce variant's backward. hitlist: the forward's (hits, counts): read instead of re-traversing."""
     a = BwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -348,6 +361,9 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.dL_dvertices_rows = int(dverts.numel() // 3)
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
     a.interval_dev = ptr(interval)
+    if hitlist is not None:
+        hits, counts = hitlist
+        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[0])
     if bounce is not None:
         a.variant = VARIANTS["bounce"]
         a.reflectivity_dev, a.envmap_dev = ptr(bounce["reflectivity"]), ptr(bounce["envmap"])

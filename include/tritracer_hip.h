@@ -140,6 +140,15 @@ typedef struct {
   const float *envmap_dev;       /* [env_height][env_width][3] */
   int env_width, env_height;
   float *bounce_aux_dev;         /* [R][20] written: the record tt_trace_bwd reads */
+  /* Stored blended hits (no counterpart in the reference, whose backward always re-traverses;
+     TT_VARIANT_FORWARD). Optional: the forward writes blended hit i of pixel p as
+     (triangle id, t bits) to hitlist[i * R + p] for i < hitlist_cap and the number of blended
+     hits to hitlist_count[p], or -1 when the pixel blended more than hitlist_cap. Handing
+     both to tt_trace_bwd replaces its re-traversal by a read of the same hits (same
+     gradients); pixels with count -1 are re-traversed. */
+  unsigned int *hitlist_dev;     /* [hitlist_cap][R][2] written */
+  int *hitlist_count_dev;        /* [R] written */
+  int hitlist_cap;
 } tt_fwd_args;
 
 /* Replaces OptixTracer::trace_fwd's launch (tracer.cpp:105-132) and the raygen programs
@@ -180,6 +189,11 @@ typedef struct {
   const float *bounce_aux_dev;
   float *dL_dreflectivity_dev;   /* [P] */
   float *dL_denvmap_dev;         /* [env_height][env_width][3] */
+  /* optional: the hit list and counts a tt_trace_fwd of the same rays and scene wrote
+     (TT_VARIANT_FORWARD; takes precedence over interval_dev) */
+  const unsigned int *hitlist_dev;
+  const int *hitlist_count_dev;
+  int hitlist_cap;
 } tt_bwd_args;
 
 /* Replaces OptixTracer::trace_bwd's launch (tracer.cpp:199-222) and backward.cu:191-391:

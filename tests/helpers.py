@@ -39,7 +39,7 @@ class GpuScene:
 
     def forward(self, ray_o, ray_d, k=16, cutoff=1e-3, chunk_mode="compat", variant="forward", extra=None,
                 bg=(0.0, 0.0, 0.0), log_cap=0, stats=True, interval_in=None, interval_out=False, hard_cap=False,
-                fallback=(0.0, 0.0)):
+                fallback=(0.0, 0.0), hitlist_cap=0):
         from diff_tritracer import capi
 
         torch = self.torch
@@ -62,10 +62,15 @@ class GpuScene:
                    torch.zeros(R, dtype=torch.int32, device=dev))
         ivi = to_dev(np.asarray(interval_in, np.float32), dev) if interval_in is not None else None
         ivo = torch.full((R, 4), -7.0, device=dev) if interval_out else None
+        hl = None
+        if hitlist_cap:
+            hl = capi.hitlist_buffers(R, hitlist_cap, dev)
+            hl[0].fill_(-7)
+            hl[1].fill_(-7)
         a = capi.fwd_args(W, H, ro, rd, self.verts, self.npts, self.cumsum, self.shs, 3, self.opac, self.sigma, bgt,
                           color, others, maxb, dens, scal, k=k, cutoff=cutoff, chunk_mode=chunk_mode,
                           variant=variant, extra=ext, stats=st, log=log, interval_in=ivi, interval_out=ivo,
-                          hard_cap=hard_cap, fallback_start=fallback[0], fallback_end=fallback[1])
+                          hard_cap=hard_cap, fallback_start=fallback[0], fallback_end=fallback[1], hitlist=hl)
         self.h.fwd(a, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         out = dict(color=color.cpu().numpy(), others=others.cpu().numpy(), max_blending=maxb.cpu().numpy(),
@@ -76,10 +81,13 @@ class GpuScene:
             out["log"] = tuple(x.cpu().numpy() for x in log)
         if ivo is not None:
             out["interval"] = ivo.cpu().numpy()
+        if hl is not None:
+            out["_hitlist"] = hl
+            out["hitlist"] = (hl[0].cpu().numpy(), hl[1].cpu().numpy())
         return out
 
     def backward(self, ray_o, ray_d, color, others, dL_dpix, dL_dothers, k=16, cutoff=1e-3, chunk_mode="compat",
-                 interval=None):
+                 interval=None, hitlist=None):
         from diff_tritracer import capi
 
         torch = self.torch
@@ -97,7 +105,8 @@ class GpuScene:
         a = capi.bwd_args(W, H, ro, rd, self.verts, self.npts, self.cumsum, self.shs, 3, self.opac, self.sigma, col,
                           oth, gp, go, dverts, dsig, dop, dsh, k=k, cutoff=cutoff, chunk_mode=chunk_mode,
                           zero_grads=True,
-                          interval=to_dev(np.asarray(interval, np.float32), dev) if interval is not None else None)
+                          interval=to_dev(np.asarray(interval, np.float32), dev) if interval is not None else None,
+                          hitlist=hitlist)
         self.h.bwd(a, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         return dict(dverts=dverts.cpu().numpy(), dsigma=dsig.cpu().numpy(), dopac=dop.cpu().numpy(),
