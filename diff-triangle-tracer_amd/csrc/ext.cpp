@@ -9,7 +9,9 @@
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   pybind11::class_<FwdTraceResult>(m, "FwdTraceResult")
       .def(pybind11::init<>())
-      .def_readonly("data", &FwdTraceResult::data);
+      .def_readonly("data", &FwdTraceResult::data)
+      .def_readonly("hits", &FwdTraceResult::hits)
+      .def_readonly("hit_counts", &FwdTraceResult::hit_counts);
 
   pybind11::class_<BwdTraceResult>(m, "BwdTraceResult")
       .def(pybind11::init<>())
@@ -20,9 +22,15 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
       .def(pybind11::init<const std::string &, const std::string &, const std::string &>())
       .def("build_tri_accel", &OptixTracer::build_tri_accel)
       .def("trace_forward", &OptixTracer::trace_fwd)
-      .def("trace_backward", &OptixTracer::trace_bwd)
+      .def("trace_backward", &OptixTracer::trace_bwd, pybind11::arg("ray_o"), pybind11::arg("ray_d"),
+           pybind11::arg("background"), pybind11::arg("triangles_points"), pybind11::arg("sigma"),
+           pybind11::arg("num_points_per_triangle"), pybind11::arg("cumsum_of_points_per_triangle"),
+           pybind11::arg("colors"), pybind11::arg("others"), pybind11::arg("number_of_points"), pybind11::arg("shs"),
+           pybind11::arg("degree"), pybind11::arg("opacities"), pybind11::arg("dL_dpixels"),
+           pybind11::arg("dL_dothers"), pybind11::arg("scaling"), pybind11::arg("debug"),
+           pybind11::arg("hits") = pybind11::none(), pybind11::arg("hit_counts") = pybind11::none())
       .def("set_options", &OptixTracer::set_options, pybind11::arg("k") = 16, pybind11::arg("cutoff") = 1e-4,
-           pybind11::arg("chunk_mode") = 0)
+           pybind11::arg("chunk_mode") = 0, pybind11::arg("hit_cache") = 32)
       .def("get_options", &OptixTracer::get_options)
       .def_property_readonly("variant", &OptixTracer::variant_name);
 

@@ -70,7 +70,7 @@ OptixTracer::~OptixTracer() {
 
 OptixTracer::OptixTracer(OptixTracer &&o) noexcept
     : fwdState(std::move(o.fwdState)), bwdState(std::move(o.bwdState)), tr_(o.tr_), variant_(o.variant_), k_(o.k_),
-      cutoff_(o.cutoff_), chunk_mode_(o.chunk_mode_), device_(o.device_) {
+      cutoff_(o.cutoff_), chunk_mode_(o.chunk_mode_), hit_cache_(o.hit_cache_), device_(o.device_) {
   o.tr_ = nullptr;
 }
 
@@ -84,23 +84,26 @@ OptixTracer &OptixTracer::operator=(OptixTracer &&o) noexcept {
     k_ = o.k_;
     cutoff_ = o.cutoff_;
     chunk_mode_ = o.chunk_mode_;
+    hit_cache_ = o.hit_cache_;
     device_ = o.device_;
     o.tr_ = nullptr;
   }
   return *this;
 }
 
-void OptixTracer::set_options(int k, double cutoff, int chunk_mode) {
+void OptixTracer::set_options(int k, double cutoff, int chunk_mode, int hit_cache) {
   if (k != 8 && k != 16 && k != 32) throw Exception("k must be 8, 16 or 32");
   if (!(cutoff >= 0.0 && cutoff < 1.0)) throw Exception("cutoff must be in [0, 1)");
   if (chunk_mode != TT_CHUNK_COMPAT && chunk_mode != TT_CHUNK_EXACT) throw Exception("chunk_mode must be 0 or 1");
+  if (hit_cache < 0 || hit_cache > 1024) throw Exception("hit_cache must be in [0, 1024]");
   k_ = k;
   cutoff_ = (float)cutoff;
   chunk_mode_ = chunk_mode;
+  hit_cache_ = hit_cache;
 }
 
-std::tuple<int, double, int> OptixTracer::get_options() const {
-  return std::make_tuple(k_, (double)cutoff_, chunk_mode_);
+std::tuple<int, double, int, int> OptixTracer::get_options() const {
+  return std::make_tuple(k_, (double)cutoff_, chunk_mode_, hit_cache_);
 }
 
 std::string OptixTracer::variant_name() const {
@@ -206,6 +209,15 @@ FwdTraceResult OptixTracer::trace_fwd(const Tensor &ray_o, const Tensor &ray_d, 
   a.max_blending_dev = max_blending.data_ptr<float>();
   a.density_dev = dens.data_ptr<float>();
   a.scaling_dev = scal.data_ptr<float>();
+  Tensor hits, hit_counts;
+  if (hit_cache_ > 0 && variant_ == TT_VARIANT_FORWARD && (long long)H * W > 0) {
+    auto iopts = torch::TensorOptions().dtype(torch::kInt32).device(torch::kCUDA, dev);
+    hits = torch::empty({hit_cache_, (long long)H * W, 2}, iopts);
+    hit_counts = torch::empty({(long long)H * W}, iopts);
+    a.hitlist_dev = reinterpret_cast<unsigned int *>(hits.data_ptr<int>());
+    a.hitlist_count_dev = hit_counts.data_ptr<int>();
+    a.hitlist_cap = hit_cache_;
+  }
 
   hipEvent_t start = nullptr, stop = nullptr;
   float elapsed_ms = 0.0f;
@@ -230,6 +242,8 @@ FwdTraceResult OptixTracer::trace_fwd(const Tensor &ray_o, const Tensor &ray_d, 
   if (debug) gpu_time.fill_(elapsed_ms);
   FwdTraceResult res;
   res.data = std::make_tuple(out_color, out_others, gpu_time, scaling, density_factor, max_blending);
+  res.hits = hits;
+  res.hit_counts = hit_counts;
   return res;
 }
 
@@ -239,7 +253,8 @@ BwdTraceResult OptixTracer::trace_bwd(const Tensor &ray_o, const Tensor &ray_d, 
                                       const torch::Tensor &cumsum_of_points_per_triangle, const torch::Tensor &colors,
                                       const torch::Tensor &others, const int number_of_points, const Tensor &shs,
                                       const int sh_degree, const Tensor &opacities, const Tensor &dL_dpixels,
-                                      const Tensor &dL_dothers, const Tensor &scaling, const bool debug) {
+                                      const Tensor &dL_dothers, const Tensor &scaling, const bool debug,
+                                      const c10::optional<Tensor> &hits, const c10::optional<Tensor> &hit_counts) {
   (void)background;
   (void)scaling;
   (void)debug;
@@ -306,6 +321,16 @@ BwdTraceResult OptixTracer::trace_bwd(const Tensor &ray_o, const Tensor &ray_d, 
   a.dL_dsigma_dev = dL_dsigma.data_ptr<float>();
   a.dL_dopacity_dev = dL_dopacity.data_ptr<float>();
   a.dL_dsh_dev = dL_dsh.data_ptr<float>();
+  if (hits.has_value() && hits->defined() && hit_counts.has_value() && hit_counts->defined()) {
+    const Tensor &hl = *hits, &hc = *hit_counts;
+    if (hl.dim() != 3 || hl.size(1) != (long long)H * W || hl.size(2) != 2 || hl.scalar_type() != torch::kInt32 ||
+        !hl.is_contiguous() || hc.numel() != (long long)H * W || hc.scalar_type() != torch::kInt32 ||
+        !hc.is_contiguous() || hl.device() != ro.device() || hc.device() != ro.device())
+      throw Exception("hits / hit_counts must be the forward's int32 [cap, H*W, 2] / [H*W] tensors");
+    a.hitlist_dev = reinterpret_cast<const unsigned int *>(hl.data_ptr<int>());
+    a.hitlist_count_dev = hc.data_ptr<int>();
+    a.hitlist_cap = (int)hl.size(0);
+  }
   check_tt(tt_trace_bwd(tr_, &a, cur_stream(dev)), "trace_backward");
   BwdTraceResult res;
   res.data = std::make_tuple(dL_dtriangle, dL_dsigma, dL_dopacity, dL_dsh);

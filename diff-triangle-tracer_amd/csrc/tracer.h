@@ -32,6 +32,9 @@ struct TracerState {
 
 struct FwdTraceResult {
   std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> data;
+  // The forward's stored blended hits (int32 [cap, H*W, 2]) and counts (int32 [H*W]) for
+  // trace_bwd; undefined when the hit cache is off or the program is not "forward".
+  torch::Tensor hits, hit_counts;
 };
 
 struct BwdTraceResult {
@@ -64,12 +67,16 @@ class OptixTracer {
                            const torch::Tensor &cumsum_of_points_per_triangle, const torch::Tensor &colors,
                            const torch::Tensor &others, const int number_of_points, const Tensor &shs,
                            const int degree, const Tensor &opacities, const Tensor &dL_dpixels,
-                           const Tensor &dL_dothers, const Tensor &scaling, const bool debug);
+                           const Tensor &dL_dothers, const Tensor &scaling, const bool debug,
+                           const c10::optional<Tensor> &hits = c10::nullopt,
+                           const c10::optional<Tensor> &hit_counts = c10::nullopt);
 
   // Options this tracer adds (the reference hard-codes CHUNK_SIZE 16 and a 1e-4
-  // transmittance cutoff, config.h:10 and forward.cu:134).
-  void set_options(int k, double cutoff, int chunk_mode);
-  std::tuple<int, double, int> get_options() const;
+  // transmittance cutoff, config.h:10 and forward.cu:134). hit_cache: blended hits per
+  // pixel the forward stores so that trace_bwd reads them instead of re-traversing
+  // (0: re-traverse, as the reference does; same gradients either way).
+  void set_options(int k, double cutoff, int chunk_mode, int hit_cache);
+  std::tuple<int, double, int, int> get_options() const;
   std::string variant_name() const;
 
   ~OptixTracer();
@@ -85,5 +92,6 @@ class OptixTracer {
   int k_ = 16;
   float cutoff_ = 1e-4f;
   int chunk_mode_ = TT_CHUNK_COMPAT;
+  int hit_cache_ = 32;
   int device_ = -1;
 };

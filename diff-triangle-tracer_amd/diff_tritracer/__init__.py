@@ -71,6 +71,7 @@ class _Tracer(torch.autograd.Function):
         res = _call_with_snapshot(tracer.trace_forward, args, tracer_settings.debug, "snapshot_fw.dump", "forward")
         color, others, gpu_time, scaling, density_factor, max_blending = res.data
 
+        ctx.hits = (res.hits, res.hit_counts)  # stored blended hits (None: the backward re-traverses)
         ctx.tracer = tracer
         ctx.tracer_settings = tracer_settings
         ctx.number_of_points = number_of_points
@@ -85,7 +86,7 @@ class _Tracer(torch.autograd.Function):
         st = ctx.tracer_settings
         args = (ray_o, ray_d, st.bg, triangles_points, sigma, num_points_per_triangle, cumsum_of_points_per_triangle,
                 color, others, ctx.number_of_points, shs, st.sh_degree, opacities, grad_color, grad_others, scaling,
-                st.debug)
+                st.debug, *ctx.hits)
         res = _call_with_snapshot(ctx.tracer.trace_backward, args, st.debug, "snapshot_bw.dump", "backward")
         grad_triangles, grad_sigma, grad_opacities, grad_sh = res.data
         # gradient layouts as returned by the reference (__init__.py:205-227)
@@ -98,18 +99,19 @@ class Tracer(nn.Module):
 
     ``Tracer("forward.ptx", "backward.ptx")`` for plain rendering; pass
     "reflect.ptx", "refract.ptx" or "envlight.ptx" as the first argument for the
-    sphere-bounce programs. ``set_options(k, cutoff, chunk_mode)`` changes the
-    hits gathered per chunk (8/16/32), the transmittance cutoff (reference: 1e-4)
-    and the chunk resume rule ("compat" or "exact").
+    sphere-bounce programs. ``set_options(k, cutoff, chunk_mode, hit_cache)`` changes
+    the hits gathered per chunk (8/16/32), the transmittance cutoff (reference: 1e-4),
+    the chunk resume rule ("compat" or "exact") and the blended hits per pixel the
+    forward stores for the backward (0: the backward re-traverses, as the reference).
     """
 
     def __init__(self, forwad_ptx="forward.ptx", backward_ptx="backward.ptx") -> None:
         super().__init__()
         self.tracer = _C.OptixTracer(PACKAGE_DIR, forwad_ptx, backward_ptx)
 
-    def set_options(self, k: int = 16, cutoff: float = 1e-4, chunk_mode="compat"):
+    def set_options(self, k: int = 16, cutoff: float = 1e-4, chunk_mode="compat", hit_cache: int = 32):
         mode = {"compat": 0, "exact": 1}[chunk_mode] if isinstance(chunk_mode, str) else int(chunk_mode)
-        self.tracer.set_options(int(k), float(cutoff), mode)
+        self.tracer.set_options(int(k), float(cutoff), mode, int(hit_cache))
         return self
 
     def forward(self, ray_o, ray_d, triangles_points, sigma, num_points_per_triangle, cumsum_of_points_per_triangle,

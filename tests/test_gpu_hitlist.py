@@ -8,8 +8,8 @@ Bars:
                                            the pixel, in order, for i < min(count, capacity)
   counts ................................. the oracle's blended count; -1 where it exceeds the capacity
   backward from the list ................. elementwise within 1e-3 max(|ref|, 1e-3 max|ref|) of the FP64
-                                           oracle, also at a capacity of 3 (most pixels then overflow and
-                                           are re-traversed: mixed list / re-traversal launch)
+                                           oracle, also at a capacity of 3 (pixels with more blended hits
+                                           are re-traversed: a mixed list / re-traversal launch)
 """
 import numpy as np
 import pytest
