@@ -110,9 +110,12 @@ def multiview(args, dev):
     sp = stream.cuda_stream
     h = capi.TracerHandle()
     h.build(verts, cumsum, P, capi.BUILD, sp)
+    cap = args.hitlist_cap
+    hl = capi.hitlist_buffers(Rt, cap, dev) if cap > 0 else None
     a = capi.multi_args(ws, hs, ro, rd, verts, npts, cumsum, shs, 3, opac, sigma, bg, color, others, maxb, dens, scal,
                         dL_dpix=dpix, dL_dothers=doth, dverts=dverts, dsigma=dsig, dopac=dop, dsh=dsh, k=16,
-                        cutoff=args.cutoff, chunk_mode="exact", block=128, tiles_per_fetch=1, zero_grads=True)
+                        cutoff=args.cutoff, chunk_mode="exact", block=128, tiles_per_fetch=1, zero_grads=True,
+                        hitlist=hl)
     t0 = time.perf_counter()
     sweep = h.multi_autotune(a, ks=(8, 16, 32), blocks=(64, 128, 256), tpf=(1, 2, 4), reps=3, stream=sp)
     print(f"[bench] autotuner: {len(sweep)} configurations in {time.perf_counter() - t0:.1f} s; best k={a.k} "
@@ -131,25 +134,26 @@ def multiview(args, dev):
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / args.steps
     # the same batch as one forward + one backward launch per camera (same k, exact mode)
-    launches, off = [], 0
+    launches, off, hc_keep = [], 0, []
     for wc, hc in zip(ws, hs):
         R = wc * hc
         sl = slice(off, off + R)
         col, oth = color[3 * off:3 * (off + R)], others[7 * off:7 * (off + R)]
+        hc_ = capi.hitlist_buffers(R, cap, dev) if cap > 0 else None
         fa = capi.fwd_args(wc, hc, ro[sl], rd[sl], verts, npts, cumsum, shs, 3, opac, sigma, bg, col, oth, maxb, dens,
-                           scal, k=a.k, cutoff=args.cutoff, chunk_mode="exact")
+                           scal, k=a.k, cutoff=args.cutoff, chunk_mode="exact", hitlist=hc_)
         ba = capi.bwd_args(wc, hc, ro[sl], rd[sl], verts, npts, cumsum, shs, 3, opac, sigma, col, oth,
                            dpix[3 * off:3 * (off + R)], doth[7 * off:7 * (off + R)], dverts, dsig, dop, dsh, k=a.k,
-                           cutoff=args.cutoff, chunk_mode="exact", zero_grads=False)
+                           cutoff=args.cutoff, chunk_mode="exact", zero_grads=False, hitlist=hc_)
+        hc_keep.append(hc_)
         launches.append((fa, ba))
         off += R
 
-    def per_camera():
+    def per_camera():  # each camera's backward right after its forward (its schedule and record regions)
         for t in (dverts, dsig, dop, dsh):
             t.zero_()
-        for fa, _ in launches:
+        for fa, ba in launches:
             h.fwd(fa, sp)
-        for _, ba in launches:
             h.bwd(ba, sp)
 
     for _ in range(args.warmup):
@@ -171,7 +175,7 @@ def multiview(args, dev):
         "data": "synthetic (seeded clustered triangle splats, 8 pinhole cameras on random poses at radius 3-5)",
         "config": {"workload": f"multi-view fwd+bwd, {P} clustered splats (64 clusters, seed 2), cameras "
                                + ", ".join(f"{w}x{hh}" for w, hh in zip(ws, hs)),
-                   "rays": Rt, "k": a.k, "block": a.block, "tiles_per_fetch": a.tiles_per_fetch,
+                   "rays": Rt, "k": a.k, "block": a.block, "tiles_per_fetch": a.tiles_per_fetch, "hitlist_cap": cap,
                    "chunk_mode": "exact", "cutoff": args.cutoff,
                    "per_camera_launches_ms": round(ms_pc, 4),
                    "per_camera_launches_mrays": round(Rt / (ms_pc * 1e-3) / 1e6, 4),

@@ -35,6 +35,10 @@ struct tt_tracer {
   int *rec_off = nullptr;    // their exclusive prefix sums: the backward's record regions
   int tile_cap = 0;
   int sched_W = -1, sched_H = -1;
+  // the last multi-camera forward's tile count, block and rays (its schedule is valid for a
+  // backward of the same batch)
+  int multi_tiles = -1, multi_block = -1;
+  long long multi_rays = -1;
   // backward gradient records: 2048 rows of 80 B per block of the one-shot backward
   float4 *rec_rows = nullptr;
   size_t rec_blocks = 0;  // rows allocated
@@ -119,8 +123,8 @@ void free_records(tt_tracer *tr) {
 
 // Gradient-record rows for a backward: 16 rows of 80 B per launched lane (2.6 GB for a
 // 1920x1080 one-shot backward).
-int prepare_records(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
-  const size_t rows = lanes * 16;
+int prepare_records(tt_tracer *tr, size_t lanes, tt::TraceParams &p, int lane_rows = 16) {
+  const size_t rows = lanes * (size_t)lane_rows;
   if (rows > (size_t)0x7fffffff) return fail(TT_ERR_INVALID_ARG, "too many rays for one backward launch");
   if (rows > tr->rec_blocks) {
     free_records(tr);
@@ -132,6 +136,33 @@ int prepare_records(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
   }
   p.rec_rows = tr->rec_rows;
   p.rec_cap = (int)rows;
+  p.rec_lane_rows = lane_rows;
+  return TT_OK;
+}
+
+// Tile schedule buffers (cost, orders, blended hits, record offsets) for nt tiles.
+int ensure_tiles(tt_tracer *tr, int nt) {
+  if (nt <= tr->tile_cap) return TT_OK;
+  if (tr->tile_cost) (void)hipFree(tr->tile_cost);
+  if (tr->tile_order) (void)hipFree(tr->tile_order);
+  if (tr->probe_order) (void)hipFree(tr->probe_order);
+  if (tr->tile_hits) (void)hipFree(tr->tile_hits);
+  if (tr->rec_off) (void)hipFree(tr->rec_off);
+  tr->tile_cost = nullptr;
+  tr->tile_order = nullptr;
+  tr->probe_order = nullptr;
+  tr->tile_hits = nullptr;
+  tr->rec_off = nullptr;
+  tr->tile_cap = 0;
+  tr->sched_W = tr->sched_H = -1;
+  tr->multi_tiles = -1;
+  if (hipMalloc(&tr->tile_cost, (size_t)nt * sizeof(float)) != hipSuccess ||
+      hipMalloc(&tr->tile_order, (size_t)nt * sizeof(int)) != hipSuccess ||
+      hipMalloc(&tr->probe_order, (size_t)nt * sizeof(int)) != hipSuccess ||
+      hipMalloc(&tr->tile_hits, (size_t)nt * sizeof(int)) != hipSuccess ||
+      hipMalloc(&tr->rec_off, (size_t)nt * sizeof(int)) != hipSuccess)
+    return fail(TT_ERR_HIP, "hipMalloc of the tile schedule failed");
+  tr->tile_cap = nt;
   return TT_OK;
 }
 
@@ -313,6 +344,7 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
     p.hl = reinterpret_cast<uint2 *>(a->hitlist_dev);
     p.hl_count = a->hitlist_count_dev;
     p.hl_cap = a->hitlist_cap;
+    p.hl_stride = (size_t)a->width * a->height;
   }
   if (a->variant == TT_VARIANT_BOUNCE) {
     p.refl = a->reflectivity_dev;
@@ -324,27 +356,7 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   const int nt = tt::trace_tiles(a->width, a->height);
-  if (nt > tr->tile_cap) {
-    if (tr->tile_cost) (void)hipFree(tr->tile_cost);
-    if (tr->tile_order) (void)hipFree(tr->tile_order);
-  if (tr->probe_order) (void)hipFree(tr->probe_order);
-  if (tr->tile_hits) (void)hipFree(tr->tile_hits);
-  if (tr->rec_off) (void)hipFree(tr->rec_off);
-    tr->tile_cost = nullptr;
-    tr->tile_order = nullptr;
-    tr->probe_order = nullptr;
-    tr->tile_hits = nullptr;
-    tr->rec_off = nullptr;
-    tr->tile_cap = 0;
-    if (hipMalloc(&tr->tile_cost, (size_t)nt * sizeof(float)) != hipSuccess ||
-        hipMalloc(&tr->tile_order, (size_t)nt * sizeof(int)) != hipSuccess ||
-        hipMalloc(&tr->probe_order, (size_t)nt * sizeof(int)) != hipSuccess ||
-        hipMalloc(&tr->tile_hits, (size_t)nt * sizeof(int)) != hipSuccess ||
-        hipMalloc(&tr->rec_off, (size_t)nt * sizeof(int)) != hipSuccess)
-      return fail(TT_ERR_HIP, "hipMalloc of the tile schedule failed");
-    tr->tile_cap = nt;
-    tr->sched_W = tr->sched_H = -1;
-  }
+  if ((rc = ensure_tiles(tr, nt)) != TT_OK) return rc;
   p.tile_cost = tr->tile_cost;
   p.tile_order = tr->tile_order;
   p.probe_order = tr->probe_order;
@@ -354,6 +366,7 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   tr->sched_W = a->width;
   tr->sched_H = a->height;
+  tr->multi_tiles = -1;
   return ok();
 }
 
@@ -420,6 +433,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   if (a->hitlist_dev) {
     p.hl_in = reinterpret_cast<const uint2 *>(a->hitlist_dev);
     p.hl_count_in = a->hitlist_count_dev;
+    p.hl_stride = (size_t)a->width * a->height;
   }
   if (a->variant == TT_VARIANT_BOUNCE) {
     p.refl = a->reflectivity_dev;
@@ -515,9 +529,34 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
     hipError_t e = hipMalloc(&tr->heads, 8 * sizeof(int));
     if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("queue allocation failed: ") + hipGetErrorString(e));
   }
-  if (backward) {
-    const int rc = prepare_records(tr, (size_t)grid * a->block, p);
-    if (rc != TT_OK) return rc;
+  if (a->hitlist_dev || a->hitlist_count_dev) {
+    TT_ARG(a->hitlist_dev && a->hitlist_count_dev && a->hitlist_cap > 0,
+           "the hit list needs hitlist, hitlist_count and hitlist_cap > 0");
+    p.hl_stride = (size_t)rays;
+    if (!backward) {
+      p.hl = reinterpret_cast<uint2 *>(a->hitlist_dev);
+      p.hl_count = a->hitlist_count_dev;
+      p.hl_cap = a->hitlist_cap;
+    } else {
+      p.hl_in = reinterpret_cast<const uint2 *>(a->hitlist_dev);
+      p.hl_count_in = a->hitlist_count_dev;
+    }
+  }
+  if ((rc = ensure_tiles(tr, tiles)) != TT_OK) return rc;
+  if (!backward) {  // schedule: probe order for the forward, measured tile times for the backward
+    p.tile_cost = tr->tile_cost;
+    p.tile_order = tr->tile_order;
+    p.probe_order = tr->probe_order;
+    tr->sched_W = tr->sched_H = -1;
+    tr->multi_tiles = tiles;
+    tr->multi_block = a->block;
+    tr->multi_rays = rays;
+  } else if (tr->multi_tiles == tiles && tr->multi_block == a->block && tr->multi_rays == rays) {
+    p.tile_sched = tr->tile_order;
+  }
+  if (backward) {  // a fixed region per resident block, reused tile after tile: 64 rows per lane
+    const int rc2 = prepare_records(tr, (size_t)grid * a->block, p, 64);
+    if (rc2 != TT_OK) return rc2;
   }
   return prepare_spill_lanes(tr, (size_t)grid * a->block, p);
 }

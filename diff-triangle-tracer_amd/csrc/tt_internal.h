@@ -122,6 +122,8 @@ struct TraceParams {
   int hl_cap;
   const uint2 *hl_in;
   const int *hl_count_in;
+  size_t hl_stride;  // entries between hit i and hit i + 1 of a pixel (the launch's total rays)
+  int rec_lane_rows;  // rows per launched lane of a fixed record region (no per-tile regions)
 };
 // Per-pixel record of the bounce variant: L_r[3] L_t[3] R Q o_r[3] d_r[3] o_t[3] d_t[3]
 // (d_r = 0: no secondary rays).
@@ -150,6 +152,8 @@ void set_last_error(const std::string &msg);
 
 // Tiles (blocks) of a one-shot trace launch for an image of W x H.
 int trace_tiles(int W, int H);
+// Tiles ordered by cost, most expensive first (one block), into order[n].
+std::string launch_tile_order(const float *cost, int n, int *order, hipStream_t stream);
 // Lanes a trace launch uses for an image of W x H (the stack spill needs one slot each).
 size_t trace_lanes(int W, int H);
 constexpr int kLdsStack = 16;  // traversal stack entries per lane kept in LDS; deeper ones spill

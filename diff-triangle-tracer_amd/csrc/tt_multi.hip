@@ -56,8 +56,23 @@ TT_D TraceParams camera_view(const TraceParams &P, const MultiCams &mc, int tile
   if (P.fothers) Q.fothers = P.fothers + 7 * off;
   if (P.dpix) Q.dpix = P.dpix + 3 * off;
   if (P.dothers) Q.dothers = P.dothers + 7 * off;
+  if (P.hl) Q.hl = P.hl + off;  // hit lists keep the launch-wide stride P.hl_stride
+  if (P.hl_count) Q.hl_count = P.hl_count + off;
+  if (P.hl_in) Q.hl_in = P.hl_in + off;
+  if (P.hl_count_in) Q.hl_count_in = P.hl_count_in + off;
   local_tile = tile - mc.tile_begin[c];
   return Q;
+}
+
+// Cost estimate of every global tile (probe_tile on its camera), one lane per tile.
+template <int BLOCK>
+__global__ __launch_bounds__(64) void k_probe_multi(TraceParams P, MultiCams mc, float *cost) {
+  __shared__ int s_st[kProbeStack * 64];
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= mc.total_tiles) return;
+  int lt;
+  const TraceParams Q = camera_view(P, mc, t, lt);
+  cost[t] = probe_tile<BLOCK>(Q, lt, s_st + threadIdx.x, 64);
 }
 
 template <int K, int BLOCK>
@@ -74,10 +89,16 @@ __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, Multi
     const int b = s_range[0], e = s_range[1];
     __syncthreads();
     if (b < 0) break;
-    for (int t = b; t < e; t++) {
+    for (int pos = b; pos < e; pos++) {
+      const int t = P.tile_sched ? P.tile_sched[pos] : pos;
+      const long long c0 = P.tile_cost ? wall_clock64() : 0;
       int lt, x, y;
       const TraceParams Q = camera_view(P, mc, t, lt);
       if (tile_pixel<BLOCK>(Q, lt, x, y)) forward_pixel<K, BLOCK, 0, false>(Q, y * Q.W + x, s_node);
+      if (P.tile_cost) {  // the tile's wall time: the backward's schedule key
+        __syncthreads();
+        if (threadIdx.x == 0) P.tile_cost[t] = (float)(wall_clock64() - c0);
+      }
     }
   }
 }
@@ -98,7 +119,8 @@ __global__ __launch_bounds__(BLOCK, 3) void k_backward_multi(TraceParams P, Mult
     const int b = s_range[0], e = s_range[1];
     __syncthreads();
     if (b < 0) break;
-    for (int t = b; t < e; t++) {
+    for (int pos = b; pos < e; pos++) {
+      const int t = P.tile_sched ? P.tile_sched[pos] : pos;
       int lt, x, y;
       const TraceParams Q = camera_view(P, mc, t, lt);
       const bool valid = tile_pixel<BLOCK>(Q, lt, x, y);
@@ -150,13 +172,28 @@ std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, 
   const void *fn = multi_kernel(k, block, backward);
   if (!fn) return "unsupported (k, block): k in {8, 16, 32}, block in {64, 128, 256}";
   if (mc.total_tiles == 0) return "";
-  std::string e = launch_fill_int(heads, 8, 0, stream);
-  if (!e.empty()) return e;
   TraceParams pp = p;
   MultiCams m = mc;
+  const int n = mc.total_tiles;
+  if (!backward && p.tile_cost && p.probe_order) {  // heaviest tiles first, by the probe's estimate
+    const dim3 g((n + 63) / 64), b(64);
+    if (block == 64) hipLaunchKernelGGL(k_probe_multi<64>, g, b, 0, stream, p, mc, p.tile_cost);
+    if (block == 128) hipLaunchKernelGGL(k_probe_multi<128>, g, b, 0, stream, p, mc, p.tile_cost);
+    if (block == 256) hipLaunchKernelGGL(k_probe_multi<256>, g, b, 0, stream, p, mc, p.tile_cost);
+    std::string e = launch_tile_order(p.tile_cost, n, p.probe_order, stream);
+    if (!e.empty()) return e;
+    pp.tile_sched = p.probe_order;
+  }
+  if (pp.tile_sched) {  // one queue in schedule order
+    m.q_begin[0] = 0;
+    for (int q = 1; q <= 8; q++) m.q_begin[q] = n;
+  }
+  std::string e = launch_fill_int(heads, 8, 0, stream);
+  if (!e.empty()) return e;
   void *args[] = {&pp, &m, &heads, &G};
   hipError_t err = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
   if (err != hipSuccess) return std::string("persistent launch failed: ") + hipGetErrorString(err);
+  if (!backward && p.tile_cost && p.tile_order) return launch_tile_order(p.tile_cost, n, p.tile_order, stream);
   return "";
 }
 

@@ -59,45 +59,13 @@ __global__ __launch_bounds__(1024) void k_tile_offsets(const int *hits, int n, i
   }
 }
 
-// Cost estimate of every tile before the forward: the tile's centre ray walks the
-// 4-wide BVH down to subtrees of at most kProbeCut leaves and sums the leaf counts of
-// the subtrees and leaves whose boxes its line crosses -- how much geometry the tile's
-// rays pass, at a few dozen node visits per tile (a few percent of a full trace).
-// One lane per tile; the LDS stack drops entries past kProbeStack (the estimate then
-// undercounts; it only orders tiles).
-constexpr int kProbeCut = 32, kProbeStack = 48, kProbeBlock = 64;
+// Cost estimate of every tile before the forward (probe_tile), one lane per tile.
+constexpr int kProbeBlock = 64;
 __global__ __launch_bounds__(kProbeBlock) void k_tile_probe(TraceParams P, float *cost) {
   __shared__ int s_st[kProbeStack * kProbeBlock];
   const int tile = blockIdx.x * kProbeBlock + threadIdx.x;
   if (tile >= P.n_tiles) return;
-  const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
-  const int x = min(tx * kTileW + kTileW / 2, P.W - 1), y = min(ty * kTileH + kTileH / 2, P.H - 1);
-  const size_t pix = (size_t)y * P.W + x;
-  const RayPre r = prep_ray(ld3(P.ray_o + 3 * pix), ld3(P.ray_d + 3 * pix));
-  const int lane = threadIdx.x;
-  float est = 0.0f;
-  int sp = 0, node = P.n_leaves > 0 ? 0 : -1;
-  const float inf = __builtin_huge_valf();
-  while (node >= 0) {
-    const float4 *nd = P.nodes4 + 8 * (size_t)node;
-    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-    const int4 ref = *reinterpret_cast<const int4 *>(nd + 6), cnt = *reinterpret_cast<const int4 *>(nd + 7);
-    float tn[4], tf[4];
-    slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn[0], tf[0]);
-    slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn[1], tf[1]);
-    slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn[2], tf[2]);
-    slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn[3], tf[3]);
-    const int rf[4] = {ref.x, ref.y, ref.z, ref.w}, ct[4] = {cnt.x, cnt.y, cnt.z, cnt.w};
-    for (int c = 0; c < 4; c++) {
-      if (rf[c] == kEmptyChild || !box_ok(tn[c], tf[c], 0.0f, inf)) continue;
-      if (rf[c] < 0 || ct[c] <= kProbeCut)
-        est += (float)ct[c];
-      else if (sp < kProbeStack)
-        s_st[sp++ * kProbeBlock + lane] = rf[c];
-    }
-    node = sp > 0 ? s_st[--sp * kProbeBlock + lane] : -1;
-  }
-  cost[tile] = est;
+  cost[tile] = probe_tile<kBlock>(P, tile, s_st + threadIdx.x, kProbeBlock);
 }
 
 // Tiles ordered by cost, most expensive first (counting sort over 1024 buckets of
@@ -227,6 +195,14 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
     e = hipGetLastError();
     if (e != hipSuccess) return std::string("tile order launch failed: ") + hipGetErrorString(e);
   }
+  return "";
+}
+
+std::string launch_tile_order(const float *cost, int n, int *order, hipStream_t stream) {
+  if (n == 0) return "";
+  hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, cost, n, order);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("tile order launch failed: ") + hipGetErrorString(e);
   return "";
 }
 

@@ -180,6 +180,45 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
   }
 }
 
+// Cost estimate of tile `tile` for scheduling: the tile's centre ray walks the 4-wide
+// BVH down to subtrees of at most kProbeCut leaves and sums the leaf counts of the
+// subtrees and leaves whose boxes its line crosses -- how much geometry the tile's rays
+// pass, at a few dozen node visits (a few percent of a full trace). s_st: this lane's
+// LDS stack of kProbeStack entries `stride` apart; entries past it are dropped (the
+// estimate then undercounts; it only orders tiles).
+constexpr int kProbeCut = 32, kProbeStack = 48;
+template <int BLOCK>
+TT_D float probe_tile(const TraceParams &P, int tile, int *s_st, int stride) {
+  const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
+  constexpr int TW = TileShape<BLOCK>::W, TH = TileShape<BLOCK>::H;
+  const int x = min(tx * TW + TW / 2, P.W - 1), y = min(ty * TH + TH / 2, P.H - 1);
+  const size_t pix = (size_t)y * P.W + x;
+  const RayPre r = prep_ray(ld3(P.ray_o + 3 * pix), ld3(P.ray_d + 3 * pix));
+  float est = 0.0f;
+  int sp = 0, node = P.n_leaves > 0 ? 0 : -1;
+  const float inf = __builtin_huge_valf();
+  while (node >= 0) {
+    const float4 *nd = P.nodes4 + 8 * (size_t)node;
+    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+    const int4 ref = *reinterpret_cast<const int4 *>(nd + 6), cnt = *reinterpret_cast<const int4 *>(nd + 7);
+    float tn[4], tf[4];
+    slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn[0], tf[0]);
+    slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn[1], tf[1]);
+    slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn[2], tf[2]);
+    slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn[3], tf[3]);
+    const int rf[4] = {ref.x, ref.y, ref.z, ref.w}, ct[4] = {cnt.x, cnt.y, cnt.z, cnt.w};
+    for (int c = 0; c < 4; c++) {
+      if (rf[c] == kEmptyChild || !box_ok(tn[c], tf[c], 0.0f, inf)) continue;
+      if (rf[c] < 0 || ct[c] <= kProbeCut)
+        est += (float)ct[c];
+      else if (sp < kProbeStack)
+        s_st[sp++ * stride] = rf[c];
+    }
+    node = sp > 0 ? s_st[--sp * stride] : -1;
+  }
+  return est;
+}
+
 // Tile of a block in a one-shot launch: workgroups go to the 8 XCDs round-robin,
 // so block i is mapped to tile (i % 8) * (n/8) + i / 8 (with the remainder spread
 // over the first XCDs): each XCD's tiles form one contiguous band of the image.
@@ -348,7 +387,7 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
           }
           st.T = testT;
           if (FULL && P.hl && st.blended < P.hl_cap)
-            P.hl[(size_t)st.blended * P.R + ray] = make_uint2(prim, __float_as_uint(t));
+            P.hl[(size_t)st.blended * P.hl_stride + ray] = make_uint2(prim, __float_as_uint(t));
           st.blended++;
           code = 2;
         }
@@ -728,13 +767,9 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
 // region of P.rec_rows, rows [s_rec[1], s_rec[1] + s_rec[2]); the row index comes from one
 // LDS atomic per wave (counter s_rec[0]). A one-shot backward that follows a forward of the
 // same image sizes each tile's region to that tile's blended-hit count in the forward
-// (P.tile_hits, prefix sums in P.rec_off); otherwise a block has rec_per_block rows. Past
-// its region a hit's gradient goes to the global buffers with atomics instead.
+// (P.tile_hits, prefix sums in P.rec_off); otherwise a block has P.rec_lane_rows rows per
+// lane. Past its region a hit's gradient goes to the global buffers with atomics instead.
 constexpr int kRecLds = 4;  // s_rec: counter, region start, region rows, work items of the aggregation
-template <int BLOCK>
-constexpr int rec_per_block() {  // rows of a fixed region: 16 per lane
-  return 16 * BLOCK;
-}
 template <int BLOCK>
 constexpr int rec_batch() {  // records grouped per pass (two shorts each in the traversal LDS)
   return 16 * BLOCK;
@@ -994,7 +1029,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
       far_t = iv.y;
     }
     backward_seg<K, BLOCK, 0, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal,
-                                   nullptr, nullptr, tlo, plo, far_t, s_node, s_keys, s_acc, hl, n_hl, R);
+                                   nullptr, nullptr, tlo, plo, far_t, s_node, s_keys, s_acc, hl, n_hl, P.hl_stride);
   } else {
     const float *a = P.aux + (size_t)kBounceAux * pix;
     float Lr[3], Lt[3];
@@ -1073,7 +1108,8 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   int *t_item = s_tab + 4 * kAggSlots;  // per slot: its first work item (segment of <= kSegRecords records)
   float *stage = reinterpret_cast<float *>(s_tab + 5 * kAggSlots);
   if (threadIdx.x == 0) {
-    int start = blockIdx.x * rec_per_block<BLOCK>(), rows = rec_per_block<BLOCK>();
+    const int lane_rows = P.rec_lane_rows > 0 ? P.rec_lane_rows : 16;
+    int start = blockIdx.x * BLOCK * lane_rows, rows = BLOCK * lane_rows;
     if (P.rec_off) {
       start = P.rec_off[tile];
       rows = max(0, min(P.tile_hits[tile], P.rec_cap - start));

@@ -87,6 +87,7 @@ class MultiArgs(C.Structure):
         ("scaling_dev", _fp), ("stats_dev", _fp),
         ("dL_dpix_dev", _fp), ("dL_dothers_dev", _fp), ("zero_grads", C.c_int), ("dL_dvertices_rows", C.c_int),
         ("dL_dvertices_dev", _fp), ("dL_dsigma_dev", _fp), ("dL_dopacity_dev", _fp), ("dL_dsh_dev", _fp),
+        ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int),
     ]
 
 
@@ -376,9 +377,10 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
 def multi_args(widths, heights, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others,
                maxb, density, scaling, dL_dpix=None, dL_dothers=None, dverts=None, dsigma=None, dopac=None, dsh=None,
                k=16, cutoff=1e-3, chunk_mode="exact", block=128, tiles_per_fetch=1, zero_grads=True, stats=None,
-               near=0.2, far=1000.0) -> MultiArgs:
+               near=0.2, far=1000.0, hitlist=None) -> MultiArgs:
     """Arguments of the batched multi-camera path; per-camera buffers are concatenated
-    in camera order (rays [sum R_c][3], colour [3][R_c] per camera, others [7][R_c])."""
+    in camera order (rays [sum R_c][3], colour [3][R_c] per camera, others [7][R_c]).
+    hitlist: (hits [cap, sum R_c, 2], counts [sum R_c]) written by the forward, read by the backward."""
     a = MultiArgs()
     n = len(widths)
     a.n_cameras = n
@@ -403,4 +405,7 @@ def multi_args(widths, heights, ray_o, ray_d, verts, npts, cumsum, shs, sh_degre
     a.zero_grads = 1 if zero_grads else 0
     a.dL_dvertices_rows = int(dverts.numel() // 3) if dverts is not None else 0
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
+    if hitlist is not None:
+        hits, counts = hitlist
+        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[0])
     return a

@@ -230,6 +230,11 @@ typedef struct {
   int zero_grads;               /* tt_multi_backward zeroes the gradient buffers first */
   int dL_dvertices_rows;
   float *dL_dvertices_dev, *dL_dsigma_dev, *dL_dopacity_dev, *dL_dsh_dev;
+  /* optional stored blended hits of all cameras, as tt_fwd_args.hitlist_* with R = sum R_c:
+     written by tt_multi_forward, read by tt_multi_backward instead of re-traversing */
+  unsigned int *hitlist_dev;    /* [hitlist_cap][sum R_c][2] */
+  int *hitlist_count_dev;       /* [sum R_c] */
+  int hitlist_cap;
 } tt_multi_args;
 
 /* Forward of every camera in ONE persistent launch (blocks pull tiles from 8 work queues). */

@@ -1,5 +1,7 @@
-"""Persistent multi-camera path (one launch for all cameras), its hipGraph replay and
-the autotuner, against per-camera one-shot launches through the C ABI.
+"""Persistent multi-camera path (one launch for all cameras, tiles in the probe's order
+forward and the forward's measured order backward, optionally with the forward's hit
+list), its hipGraph replay and the autotuner, against per-camera one-shot launches
+through the C ABI.
 
 Bars: colour/others bit-exact and statistics identical to per-camera launches
 (same per-pixel code, exact chunk mode); gradients = sum of the per-camera
@@ -72,13 +74,18 @@ def _per_camera(mv, cuda, k=16):
     return {k_: v.cpu().numpy() for k_, v in B.items()}
 
 
-def _multi_args(mv, B, k=16, block=128, tpf=1):
+def _multi_args(mv, B, k=16, block=128, tpf=1, hitlist_cap=0):
     sc, w, h, S, tr, Rt = mv
+    hl = None
+    if hitlist_cap:
+        import torch
+        hl = capi.hitlist_buffers(Rt, hitlist_cap, S["ro"].device)
+        B["_hits"], B["_hit_counts"] = hl
     return capi.multi_args(w, h, S["ro"], S["rd"], S["verts"], S["npts"], S["cumsum"], S["shs"], 3, S["opac"],
                            S["sigma"], S["bg"], B["color"], B["others"], B["maxb"], B["dens"], B["scal"],
                            dL_dpix=S["dpix"], dL_dothers=S["doth"], dverts=B["dverts"], dsigma=B["dsig"],
                            dopac=B["dop"], dsh=B["dsh"], k=k, cutoff=1e-3, chunk_mode="exact", block=block,
-                           tiles_per_fetch=tpf, zero_grads=True, stats=B["stats"])
+                           tiles_per_fetch=tpf, zero_grads=True, stats=B["stats"], hitlist=hl)
 
 
 @pytest.fixture(scope="module")
@@ -99,28 +106,29 @@ def _check(out, ref, grads=True):
             assert ok, (key, worst)
 
 
-@pytest.mark.parametrize("block,tpf,k", [(128, 1, 16), (64, 2, 8), (256, 4, 32)])
-def test_multi_equals_per_camera(mv, ref, cuda, block, tpf, k):
+@pytest.mark.parametrize("block,tpf,k,cap", [(128, 1, 16, 0), (64, 2, 8, 0), (256, 4, 32, 0), (128, 1, 16, 32),
+                                             (64, 2, 16, 3), (256, 1, 8, 32)])
+def test_multi_equals_per_camera(mv, ref, cuda, block, tpf, k, cap):
     import torch
 
     sc, w, h, S, tr, Rt = mv
-    _say(f"persistent multi-camera block={block} tiles/fetch={tpf} k={k}")
+    _say(f"persistent multi-camera block={block} tiles/fetch={tpf} k={k} hit list capacity={cap}")
     B = _bufs(torch, cuda, sc.P, Rt, 16)
-    a = _multi_args(mv, B, k=k, block=block, tpf=tpf)
+    a = _multi_args(mv, B, k=k, block=block, tpf=tpf, hitlist_cap=cap)
     s = torch.cuda.current_stream().cuda_stream
     tr.multi_fwd(a, s)
     tr.multi_bwd(a, s)
     torch.cuda.synchronize()
-    _check({k_: v.cpu().numpy() for k_, v in B.items()}, ref)
+    _check({k_: v.cpu().numpy() for k_, v in B.items() if not k_.startswith("_")}, ref)
 
 
 def test_graph_replay_equals_eager(mv, ref, cuda):
     import torch
 
     sc, w, h, S, tr, Rt = mv
-    _say("hipGraph capture + 3 replays")
+    _say("hipGraph capture + 3 replays (hit list on)")
     B = _bufs(torch, cuda, sc.P, Rt, 16)
-    a = _multi_args(mv, B)
+    a = _multi_args(mv, B, hitlist_cap=32)
     s = torch.cuda.current_stream().cuda_stream
     tr.multi_capture(a, 3, s)
     for rep in range(3):
@@ -130,7 +138,7 @@ def test_graph_replay_equals_eager(mv, ref, cuda):
         B["dens"].zero_()
         tr.multi_replay(s)
         torch.cuda.synchronize()
-        _check({k_: v.cpu().numpy() for k_, v in B.items()}, ref)
+        _check({k_: v.cpu().numpy() for k_, v in B.items() if not k_.startswith("_")}, ref)
 
 
 def test_autotuner_picks_a_measured_configuration(mv, ref, cuda):
