@@ -209,12 +209,14 @@ def bounce(args, dev):
              dreflectivity=torch.zeros(fr.P, device=dev), denvmap=torch.zeros(512, 1024, 3, device=dev))
     stats = torch.zeros(fr.R, 2, dtype=torch.int32, device=dev)
     c = capi
+    cap = args.hitlist_cap
+    hl = c.hitlist_buffers(fr.R, cap, dev, segments=3) if cap > 0 else None
     fa = c.fwd_args(W, H, fr.ro, fr.rd, fr.verts, fr.npts, fr.cumsum, fr.shs, 3, fr.opac, fr.sigma, fr.bg, fr.color,
                     fr.others, fr.maxb, fr.dens, fr.scal, k=args.k, cutoff=args.cutoff, chunk_mode="exact",
-                    variant="bounce", bounce=B)
+                    variant="bounce", bounce=B, hitlist=hl)
     ba = c.bwd_args(W, H, fr.ro, fr.rd, fr.verts, fr.npts, fr.cumsum, fr.shs, 3, fr.opac, fr.sigma, fr.color,
                     fr.others, fr.dpix, fr.doth, fr.dverts, fr.dsig, fr.dop, fr.dsh, k=args.k, cutoff=args.cutoff,
-                    chunk_mode="exact", zero_grads=True, bounce=B)
+                    chunk_mode="exact", zero_grads=True, bounce=B, hitlist=hl)
     fs = c.fwd_args(W, H, fr.ro, fr.rd, fr.verts, fr.npts, fr.cumsum, fr.shs, 3, fr.opac, fr.sigma, fr.bg, fr.color,
                     fr.others, fr.maxb, fr.dens, fr.scal, k=args.k, cutoff=args.cutoff, chunk_mode="exact",
                     variant="bounce", bounce=B, stats=stats)
@@ -235,7 +237,7 @@ def bounce(args, dev):
         "data": "synthetic (clustered splats, Beta(2,5) reflectivity, U[1,1.6] IOR, lognormal 1024x512 environment)",
         "config": {"workload": f"bounce variant fwd+bwd, {scene.P} clustered splats (64 clusters, seed 3), one "
                                f"{W}x{H} camera, k={args.k}, cutoff {args.cutoff:g}, exact chunk mode",
-                   "fwd_ms": round(per["fwd"], 4), "bwd_ms": round(per["bwd"], 4),
+                   "fwd_ms": round(per["fwd"], 4), "bwd_ms": round(per["bwd"], 4), "hitlist_cap": cap,
                    "hits_consumed_per_pixel": round(float(st[0]) / R, 3),
                    "hits_blended_per_pixel": round(float(st[1]) / R, 3)},
     }

@@ -310,7 +310,8 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
     if (a->interval_in_dev)
       TT_ARG(a->chunk_mode == TT_CHUNK_EXACT, "interval_in requires TT_CHUNK_EXACT");
     if (a->hitlist_dev || a->hitlist_count_dev) {
-      TT_ARG(a->variant == TT_VARIANT_FORWARD, "the hit list is supported by the forward variant only");
+      TT_ARG(a->variant == TT_VARIANT_FORWARD || a->variant == TT_VARIANT_BOUNCE,
+             "the hit list is supported by TT_VARIANT_FORWARD and TT_VARIANT_BOUNCE");
       TT_ARG(a->hitlist_dev && a->hitlist_count_dev && a->hitlist_cap > 0,
              "the hit list needs hitlist, hitlist_count and hitlist_cap > 0");
     }
@@ -413,7 +414,6 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
          "forward outputs / upstream gradients are NULL");
   if (a->interval_dev) TT_ARG(a->chunk_mode == TT_CHUNK_EXACT, "interval requires TT_CHUNK_EXACT");
   if (a->hitlist_dev || a->hitlist_count_dev) {
-    TT_ARG(a->variant == TT_VARIANT_FORWARD, "the hit list is supported by the forward variant only");
     TT_ARG(a->hitlist_dev && a->hitlist_count_dev && a->hitlist_cap > 0,
            "the hit list needs hitlist, hitlist_count and hitlist_cap > 0");
   }
@@ -433,6 +433,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   if (a->hitlist_dev) {
     p.hl_in = reinterpret_cast<const uint2 *>(a->hitlist_dev);
     p.hl_count_in = a->hitlist_count_dev;
+    p.hl_cap = a->hitlist_cap;
     p.hl_stride = (size_t)a->width * a->height;
   }
   if (a->variant == TT_VARIANT_BOUNCE) {
@@ -540,6 +541,7 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
     } else {
       p.hl_in = reinterpret_cast<const uint2 *>(a->hitlist_dev);
       p.hl_count_in = a->hitlist_count_dev;
+      p.hl_cap = a->hitlist_cap;
     }
   }
   if ((rc = ensure_tiles(tr, tiles)) != TT_OK) return rc;

@@ -297,7 +297,7 @@ TT_D void seg_init(SegState &st) {
 // IVAL: the interval features (start / cap, emitted interval) are compiled in.
 template <int K, int BLOCK, bool FULL, bool COUNT, bool BOUNCE = false, bool IVAL = false>
 TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node,
-                        float tstart = 0.0f, float cap = kInfDist) {
+                        float tstart = 0.0f, float cap = kInfDist, int hseg = -1) {
   RayPre r = prep_ray(o, d);
   KBuf<K> b;
   float last = 0.0f, tlo = IVAL ? tstart : 0.0f;
@@ -386,8 +386,8 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
             st.D += bw * t;
           }
           st.T = testT;
-          if (FULL && P.hl && st.blended < P.hl_cap)
-            P.hl[(size_t)st.blended * P.hl_stride + ray] = make_uint2(prim, __float_as_uint(t));
+          if (hseg >= 0 && P.hl && st.blended < P.hl_cap)  // hit list of segment hseg
+            P.hl[((size_t)hseg * P.hl_cap + st.blended) * P.hl_stride + ray] = make_uint2(prim, __float_as_uint(t));
           st.blended++;
           code = 2;
         }
@@ -431,29 +431,34 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
       tstart = iv.x;
       cap = iv.y;
     }
-    trace_segment<K, BLOCK, true, COUNT, false, IVAL>(P, o, d, st, pix, o, s_node, tstart, cap);
+    trace_segment<K, BLOCK, true, COUNT, false, IVAL>(P, o, d, st, pix, o, s_node, tstart, cap, 0);
     if (IVAL && P.ival_out)
       P.ival_out[pix] = make_float4(st.t_first, st.t_last, st.t_term, __uint_as_float(st.p_first));
     if (P.hl_count) P.hl_count[pix] = st.blended <= P.hl_cap ? st.blended : -1;
   } else if (VARIANT == 4) {
-    trace_segment<K, BLOCK, true, COUNT, true>(P, o, d, st, pix, o, s_node);
+    trace_segment<K, BLOCK, true, COUNT, true>(P, o, d, st, pix, o, s_node, 0.0f, kInfDist, 0);
+    int nb[3] = {st.blended, 0, 0};  // blended hits of the primary, reflected and refracted segment
     const Bounce g = bounce_rays(o, d, st.T, st.N, st.D, st.eta);
     float Lr[3] = {0.0f, 0.0f, 0.0f}, Lt[3] = {0.0f, 0.0f, 0.0f};
     if (g.valid) {
       float E[3];
       SegState sr;
       seg_init(sr);
-      trace_segment<K, BLOCK, false, false>(P, g.orr, g.dr, sr, pix, g.orr, s_node);
+      trace_segment<K, BLOCK, false, false>(P, g.orr, g.dr, sr, pix, g.orr, s_node, 0.0f, kInfDist, 1);
       env_value(P.env, P.env_w, P.env_h, g.dr, E);
       for (int c = 0; c < 3; c++) Lr[c] = sr.C[c] + sr.T * E[c];
       SegState stt;
       seg_init(stt);
-      trace_segment<K, BLOCK, false, false>(P, g.ot, g.dt, stt, pix, g.ot, s_node);
+      trace_segment<K, BLOCK, false, false>(P, g.ot, g.dt, stt, pix, g.ot, s_node, 0.0f, kInfDist, 2);
       env_value(P.env, P.env_w, P.env_h, g.dt, E);
       for (int c = 0; c < 3; c++) Lt[c] = stt.C[c] + stt.T * E[c];
       st.consumed += sr.consumed + stt.consumed;
       st.blended += sr.blended + stt.blended;
+      nb[1] = sr.blended;
+      nb[2] = stt.blended;
     }
+    if (P.hl_count)
+      for (int sg = 0; sg < 3; sg++) P.hl_count[sg * P.hl_stride + pix] = nb[sg] <= P.hl_cap ? nb[sg] : -1;
     float E0[3];
     env_value(P.env, P.env_w, P.env_h, d, E0);
     const size_t R = (size_t)P.R;
@@ -1035,8 +1040,16 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
     float Lr[3], Lt[3];
     for (int c = 0; c < 3; c++) Lr[c] = a[c], Lt[c] = a[3 + c];
     const float Rw = a[6], Qw = a[7];
+    const uint2 *hls[3] = {nullptr, nullptr, nullptr};  // the forward's hit lists of the three segments
+    int nhl[3] = {0, 0, 0};
+    if (P.hl_in)
+      for (int sg = 0; sg < 3; sg++) {
+        nhl[sg] = P.hl_count_in[sg * P.hl_stride + pix];
+        if (nhl[sg] >= 0) hls[sg] = P.hl_in + (size_t)sg * P.hl_cap * P.hl_stride + pix;
+      }
     const float Tp = backward_seg<K, BLOCK, 1, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
-                                                    final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_keys, s_acc);
+                                                    final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_keys, s_acc,
+                                                    hls[0], nhl[0], P.hl_stride);
     const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
     env_scatter(P, d, ge);
     const v3f dr = mk(a[11], a[12], a[13]);
@@ -1048,7 +1061,8 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
         const v3f sd = s == 0 ? dr : mk(a[17], a[18], a[19]);
         const float gs[3] = {wgt * dL_dpixel[0], wgt * dL_dpixel[1], wgt * dL_dpixel[2]};
         const float Ts = backward_seg<K, BLOCK, 2, REC>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero,
-                                                        nullptr, nullptr, 0.0f, -1, P.far_t, s_node, s_keys, s_acc);
+                                                        nullptr, nullptr, 0.0f, -1, P.far_t, s_node, s_keys, s_acc,
+                                                        hls[1 + s], nhl[1 + s], P.hl_stride);
         const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
         env_scatter(P, sd, gse);
       }

@@ -290,11 +290,12 @@ class TracerHandle:
         return out
 
 
-def hitlist_buffers(R, cap, device):
-    """Hit list of a forward for its backward: (hits int32 [cap, R, 2], counts int32 [R])."""
+def hitlist_buffers(R, cap, device, segments=1):
+    """Hit list of a forward for its backward: (hits int32 [segments, cap, R, 2], counts int32
+    [segments, R]); segments = 3 for the bounce variant (primary, reflected, refracted)."""
     import torch
-    return (torch.empty(int(cap), int(R), 2, dtype=torch.int32, device=device),
-            torch.empty(int(R), dtype=torch.int32, device=device))
+    return (torch.empty(int(segments), int(cap), int(R), 2, dtype=torch.int32, device=device),
+            torch.empty(int(segments), int(R), dtype=torch.int32, device=device))
 
 
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
@@ -331,7 +332,7 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
         a.env_height, a.env_width = int(bounce["envmap"].shape[0]), int(bounce["envmap"].shape[1])
     if hitlist is not None:
         hits, counts = hitlist
-        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[0])
+        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[-3])
     if log is not None:
         prim, t, code, count = log
         a.hit_log_cap = int(prim.shape[-1])
@@ -364,7 +365,7 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.interval_dev = ptr(interval)
     if hitlist is not None:
         hits, counts = hitlist
-        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[0])
+        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[-3])
     if bounce is not None:
         a.variant = VARIANTS["bounce"]
         a.reflectivity_dev, a.envmap_dev = ptr(bounce["reflectivity"]), ptr(bounce["envmap"])
@@ -407,5 +408,5 @@ def multi_args(widths, heights, ray_o, ray_d, verts, npts, cumsum, shs, sh_degre
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
     if hitlist is not None:
         hits, counts = hitlist
-        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[0])
+        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[-3])
     return a

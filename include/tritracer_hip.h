@@ -141,13 +141,15 @@ typedef struct {
   int env_width, env_height;
   float *bounce_aux_dev;         /* [R][20] written: the record tt_trace_bwd reads */
   /* Stored blended hits (no counterpart in the reference, whose backward always re-traverses;
-     TT_VARIANT_FORWARD). Optional: the forward writes blended hit i of pixel p as
-     (triangle id, t bits) to hitlist[i * R + p] for i < hitlist_cap and the number of blended
-     hits to hitlist_count[p], or -1 when the pixel blended more than hitlist_cap. Handing
-     both to tt_trace_bwd replaces its re-traversal by a read of the same hits (same
-     gradients); pixels with count -1 are re-traversed. */
-  unsigned int *hitlist_dev;     /* [hitlist_cap][R][2] written */
-  int *hitlist_count_dev;        /* [R] written */
+     TT_VARIANT_FORWARD and TT_VARIANT_BOUNCE). Optional: the forward writes blended hit i of
+     segment s of pixel p as (triangle id, t bits) to hitlist[(s * hitlist_cap + i) * R + p] for
+     i < hitlist_cap and the segment's number of blended hits to hitlist_count[s * R + p], or -1
+     when it blended more than hitlist_cap. Segments: 0 the primary ray; for TT_VARIANT_BOUNCE
+     also 1 the reflected and 2 the refracted ray. Handing both to tt_trace_bwd replaces its
+     re-traversal by a read of the same hits (same gradients); segments with count -1 are
+     re-traversed. */
+  unsigned int *hitlist_dev;     /* [segments][hitlist_cap][R][2] written */
+  int *hitlist_count_dev;        /* [segments][R] written */
   int hitlist_cap;
 } tt_fwd_args;
 
@@ -189,8 +191,8 @@ typedef struct {
   const float *bounce_aux_dev;
   float *dL_dreflectivity_dev;   /* [P] */
   float *dL_denvmap_dev;         /* [env_height][env_width][3] */
-  /* optional: the hit list and counts a tt_trace_fwd of the same rays and scene wrote
-     (TT_VARIANT_FORWARD; takes precedence over interval_dev) */
+  /* optional: the hit list and counts a tt_trace_fwd of the same rays, scene and variant wrote
+     (takes precedence over interval_dev) */
   const unsigned int *hitlist_dev;
   const int *hitlist_count_dev;
   int hitlist_cap;

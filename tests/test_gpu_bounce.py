@@ -4,6 +4,8 @@ against the oracle (40k splats, 144x112 camera, exact chunk mode):
 
   colour / others ........ <= 2e-6 abs vs the oracle's FP32 path, <= 1e-5 vs its FP64 replay
   statistics ............. consumed / blended hits over all three rays identical
+  both with the backward re-traversing and with it reading the forward's hit lists of the
+  three segments (capacity 32)
   gradients (FP64) ....... opacity, sigma, SH, reflectivity and environment texels elementwise
                            within 1e-3 max(|ref|, 1e-3 max|ref|); vertices the same for all but
                            1e-4 of the entries, with relative L2 error <= 1e-4 (a vertex gradient
@@ -25,12 +27,12 @@ def _say(*a):
     print("[gpu-bounce]", *a, flush=True)
 
 
-@pytest.fixture(scope="module")
-def case(cuda):
-    return run_case(cuda)
+@pytest.fixture(scope="module", params=[0, 32], ids=["retrace", "hitlist32"])
+def case(cuda, request):
+    return run_case(cuda, request.param)
 
 
-def run_case(cuda):
+def run_case(cuda, hitlist_cap=0):
     import torch
 
     sc = scenes.uniform_scene(40_000, 6, offset_std=0.08)
@@ -49,10 +51,11 @@ def run_case(cuda):
     stats = torch.zeros(R, 2, dtype=torch.int32, device=cuda)
     s = torch.cuda.current_stream().cuda_stream
     RO, RD = d(ro), d(rd)
-    _say("bounce forward")
+    hl = capi.hitlist_buffers(R, hitlist_cap, cuda, segments=3) if hitlist_cap else None
+    _say(f"bounce forward (hit list capacity {hitlist_cap})")
     a = capi.fwd_args(ro.shape[1], ro.shape[0], RO, RD, g.verts, g.npts, g.cumsum, g.shs, 3, g.opac, g.sigma,
                       torch.zeros(3, device=cuda), color, others, maxb, dens, scal, k=16, cutoff=1e-3,
-                      chunk_mode="exact", variant="bounce", stats=stats, bounce=B)
+                      chunk_mode="exact", variant="bounce", stats=stats, bounce=B, hitlist=hl)
     g.h.fwd(a, s)
     dpix = rng.standard_normal((3, R)).astype(np.float32)
     doth = np.zeros((7, R), np.float32)
@@ -63,7 +66,7 @@ def run_case(cuda):
     _say("bounce backward")
     b = capi.bwd_args(ro.shape[1], ro.shape[0], RO, RD, g.verts, g.npts, g.cumsum, g.shs, 3, g.opac, g.sigma, color,
                       others, d(dpix), d(doth), dverts, dsig, dop, dsh, k=16, cutoff=1e-3, chunk_mode="exact",
-                      zero_grads=True, bounce=B)
+                      zero_grads=True, bounce=B, hitlist=hl)
     g.h.bwd(b, s)
     torch.cuda.synchronize()
     out = dict(color=color.cpu().numpy(), others=others.cpu().numpy(), stats=stats.cpu().numpy(),

@@ -44,7 +44,7 @@ def test_hitlist_forward_and_backward(small, mode, cap):
     opts = O.make_opts(k=16, cutoff=1e-3, chunk_mode=mode)
     log = O.Log(S, ro, rd, opts)
     lists = log_lists(log.export(), R)
-    hits, counts = out["hitlist"]
+    hits, counts = out["hitlist"][0][0], out["hitlist"][1][0]  # segment 0: the primary rays
     bad_count = bad_hits = overflow = 0
     for r in range(R):
         if lists[r] is None:
