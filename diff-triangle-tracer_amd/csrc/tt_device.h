@@ -230,11 +230,12 @@ TT_D bool intersect(const RayPre &r, v3f v0, v3f v1, v3f v2, float &t) {
 }
 
 // Chunk key: a candidate must satisfy t <= far and (t > tlo or (t == tlo and prim > plo)).
+// (bitwise | and &: no short-circuit branches in the compiled code)
 TT_D bool after_key(float t, uint32_t prim, float tlo, long long plo) {
-  return t > tlo || (t == tlo && (long long)prim > plo);
+  return (t > tlo) | ((t == tlo) & ((long long)prim > plo));
 }
 TT_D bool hit_less(float t, uint32_t p, float bt, uint32_t bp) {
-  return t < bt || (t == bt && p < bp);
+  return (t < bt) | ((t == bt) & (p < bp));
 }
 
 // Conservative slab test: boxes are padded at build time and the interval is

@@ -59,19 +59,22 @@ TT_D void kinit(KBuf<K> &b) {
 
 // Sorted insertion (the anyhit compare-and-swap chain of closestk.h:111-147) with a
 // deterministic (t, prim) order instead of traversal order for equal t.
+// Slot i takes slot i-1 if the key is less than slot i-1, the key if it is less than slot
+// i only, and keeps its value otherwise; slots are rewritten from the last down, each from
+// the old contents, so the steps do not depend on one another.
 template <int K>
 TT_D void kinsert(KBuf<K> &b, float t, uint32_t p) {
-  if (!hit_less(t, p, b.t[K - 1], b.p[K - 1])) return;
+  bool lt = hit_less(t, p, b.t[K - 1], b.p[K - 1]);  // key < slot i
+  if (!lt) return;
 #pragma unroll
-  for (int i = 0; i < K; i++) {
-    bool lt = hit_less(t, p, b.t[i], b.p[i]);
-    float ot = b.t[i];
-    uint32_t op = b.p[i];
-    b.t[i] = lt ? t : ot;
-    b.p[i] = lt ? p : op;
-    t = lt ? ot : t;
-    p = lt ? op : p;
+  for (int i = K - 1; i > 0; i--) {
+    const bool lt_prev = hit_less(t, p, b.t[i - 1], b.p[i - 1]);
+    b.t[i] = lt_prev ? b.t[i - 1] : (lt ? t : b.t[i]);
+    b.p[i] = lt_prev ? b.p[i - 1] : (lt ? p : b.p[i]);
+    lt = lt_prev;
   }
+  b.t[0] = lt ? t : b.t[0];
+  b.p[0] = lt ? p : b.p[0];
 }
 template <int K>
 TT_D void kpop(KBuf<K> &b) {
