@@ -141,14 +141,25 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn1, tf1);
       slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn2, tf2);
       slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn3, tf3);
-      bool h0 = ref.x != kEmptyChild && box_ok(tn0, tf0, tlo_c, thi_c);
-      bool h1 = ref.y != kEmptyChild && box_ok(tn1, tf1, tlo_c, thi_c);
-      bool h2 = ref.z != kEmptyChild && box_ok(tn2, tf2, tlo_c, thi_c);
-      bool h3 = ref.w != kEmptyChild && box_ok(tn3, tf3, tlo_c, thi_c);
-      if (h0 && ref.x < 0) s_leaf[nq++ * BLOCK + lane] = ~ref.x, h0 = false;
-      if (h1 && ref.y < 0) s_leaf[nq++ * BLOCK + lane] = ~ref.y, h1 = false;
-      if (h2 && ref.z < 0) s_leaf[nq++ * BLOCK + lane] = ~ref.z, h2 = false;
-      if (h3 && ref.w < 0) s_leaf[nq++ * BLOCK + lane] = ~ref.w, h3 = false;
+      bool h0 = (ref.x != kEmptyChild) & box_ok(tn0, tf0, tlo_c, thi_c);
+      bool h1 = (ref.y != kEmptyChild) & box_ok(tn1, tf1, tlo_c, thi_c);
+      bool h2 = (ref.z != kEmptyChild) & box_ok(tn2, tf2, tlo_c, thi_c);
+      bool h3 = (ref.w != kEmptyChild) & box_ok(tn3, tf3, tlo_c, thi_c);
+      // hit leaves to the queue without branches: slots nq .. nq + 3 are free (nq <= kLeafQ - 4);
+      // a write that is not counted is overwritten by the next one or lies past the queue's end
+      const bool l0 = h0 & (ref.x < 0), l1 = h1 & (ref.y < 0), l2 = h2 & (ref.z < 0), l3 = h3 & (ref.w < 0);
+      s_leaf[nq * BLOCK + lane] = ~ref.x;
+      nq += l0;
+      s_leaf[nq * BLOCK + lane] = ~ref.y;
+      nq += l1;
+      s_leaf[nq * BLOCK + lane] = ~ref.z;
+      nq += l2;
+      s_leaf[nq * BLOCK + lane] = ~ref.w;
+      nq += l3;
+      h0 &= !l0;
+      h1 &= !l1;
+      h2 &= !l2;
+      h3 &= !l3;
       const float inf = __builtin_huge_valf();
       float d0 = h0 ? tn0 : inf, d1 = h1 ? tn1 : inf, d2 = h2 ? tn2 : inf, d3 = h3 ? tn3 : inf;
       int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
@@ -158,13 +169,23 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       cswap(d0, c0, d2, c2);
       cswap(d1, c1, d3, c3);
       cswap(d1, c1, d2, c2);
-      if (cnt >= 4) push(c3);
-      if (cnt >= 3) push(c2);
-      if (cnt >= 2) push(c1);
+      if (sp + 3 <= kStack) {  // the usual case: LDS slots sp .. sp + 2, written without branches
+        s_node[sp * BLOCK + lane] = c3;
+        sp += cnt >= 4;
+        s_node[sp * BLOCK + lane] = c2;
+        sp += cnt >= 3;
+        s_node[sp * BLOCK + lane] = c1;
+        sp += cnt >= 2;
+      } else {
+        if (cnt >= 4) push(c3);
+        if (cnt >= 3) push(c2);
+        if (cnt >= 2) push(c1);
+      }
       node = cnt >= 1 ? c0 : -1;
       if (node < 0 && sp > 0) {
         sp--;
-        node = sp < kStack ? s_node[sp * BLOCK + lane] : ov[sp - kStack];
+        node = s_node[min(sp, kStack - 1) * BLOCK + lane];
+        if (sp >= kStack) node = ov[sp - kStack];
       }
     }
     const bool walked = node < 0;
