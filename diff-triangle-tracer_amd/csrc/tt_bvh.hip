@@ -294,13 +294,16 @@ __global__ __launch_bounds__(256) void k_level_boxes(const int *__restrict__ lev
   nd[2] = make_float4(lo[0][2], hi[0][2], lo[1][2], hi[1][2]);
 }
 
-// 4-wide nodes: node4[n] holds n's children, each internal child replaced by its
-// two children. The traversal visits binary depths 0, 2, 4, ... only.
+// 4-wide nodes: node4[n] holds up to four disjoint subtrees that cover n's leaves:
+// starting from n's two children, the internal entry with the largest surface area is
+// replaced by its two children, twice (the greedy rule for collapsing a binary tree into
+// a wide one: it opens the boxes a ray is most likely to hit). A 4-wide step descends at
+// least one binary level, so a walk's path holds at most `height` 4-wide nodes.
 __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ nodes, int ni, float4 *nodes4) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ni) return;
   float lo[3][4], hi[3][4];
-  int ref[4];
+  int ref[4], cnt[4] = {0, 0, 0, 0};  // child, leaves under it
   int e = 0;
   auto slot_box = [&](int node, int slot, float l[3], float h[3]) {
     const float4 a = nodes[4 * (size_t)node + slot], z = nodes[4 * (size_t)node + 2];
@@ -308,31 +311,38 @@ __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ no
     l[2] = slot ? z.z : z.x;
     h[2] = slot ? z.w : z.y;
   };
-  int cnt[4] = {0, 0, 0, 0};  // leaves under each child
-  auto put = [&](int r, const float l[3], const float h[3]) {
-    for (int a = 0; a < 3; a++) lo[a][e] = l[a], hi[a][e] = h[a];
-    ref[e] = r;
-    cnt[e] = r < 0 ? 1 : reinterpret_cast<const int4 *>(nodes + 4 * (size_t)r + 3)->w;
-    e++;
+  auto set = [&](int i, int r, const float l[3], const float h[3]) {
+    for (int a = 0; a < 3; a++) lo[a][i] = l[a], hi[a][i] = h[a];
+    ref[i] = r;
+    cnt[i] = r < 0 ? 1 : reinterpret_cast<const int4 *>(nodes + 4 * (size_t)r + 3)->w;
   };
   const int4 li = *reinterpret_cast<const int4 *>(nodes + 4 * (size_t)n + 3);
-  for (int slot = 0; slot < 2; slot++) {
-    const int c = slot ? li.y : li.x;
-    float l[3], h[3];
-    if (c < 0) {
-      slot_box(n, slot, l, h);
-      put(c, l, h);
-    } else {
-      const int4 lc = *reinterpret_cast<const int4 *>(nodes + 4 * (size_t)c + 3);
-      slot_box(c, 0, l, h);
-      put(lc.x, l, h);
-      slot_box(c, 1, l, h);
-      put(lc.y, l, h);
+  float l[3], h[3];
+  slot_box(n, 0, l, h);
+  set(e++, li.x, l, h);
+  slot_box(n, 1, l, h);
+  set(e++, li.y, l, h);
+  for (int open = 0; open < 2; open++) {
+    int best = -1;
+    float best_area = -1.0f;
+    for (int i = 0; i < e; i++) {
+      if (ref[i] < 0) continue;
+      const float dx = hi[0][i] - lo[0][i], dy = hi[1][i] - lo[1][i], dz = hi[2][i] - lo[2][i];
+      const float area = dx * dy + dy * dz + dz * dx;
+      if (area > best_area) best_area = area, best = i;
     }
+    if (best < 0) break;
+    const int c = ref[best];
+    const int4 lc = *reinterpret_cast<const int4 *>(nodes + 4 * (size_t)c + 3);
+    slot_box(c, 0, l, h);
+    set(best, lc.x, l, h);
+    slot_box(c, 1, l, h);
+    set(e++, lc.y, l, h);
   }
   for (; e < 4; e++) {
     for (int a = 0; a < 3; a++) lo[a][e] = 1.0f, hi[a][e] = 0.0f;
     ref[e] = kEmptyChild;
+    cnt[e] = 0;
   }
   float4 *o = nodes4 + 8 * (size_t)n;
   o[0] = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);

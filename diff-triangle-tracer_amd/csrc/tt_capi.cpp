@@ -91,10 +91,11 @@ int check_common(const tt_tracer *tr, int W, int H, const float *ro, const float
   return TT_OK;
 }
 
-// Stack spill for a launch of `lanes` lanes: a 4-wide walk defers at most 3 children
-// at each of at most ceil(height/2) nodes on its path; entries beyond the LDS part spill.
+// Stack spill for a launch of `lanes` lanes: a 4-wide walk defers at most 3 children at
+// each of at most `height` 4-wide nodes on its path (each 4-wide step descends at least one
+// binary level); entries beyond the LDS part spill.
 int prepare_spill_lanes(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
-  const int depth = 3 * ((tr->bvh.height_max + 1) / 2) + 1 - tt::kLdsStack;
+  const int depth = 3 * tr->bvh.height_max + 1 - tt::kLdsStack;
   p.ov_depth = depth > 0 ? depth : 0;
   p.ov_stack = tr->ov_stack;
   if (p.ov_depth == 0 || tr->bvh.n_leaves == 0) return TT_OK;
