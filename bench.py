@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernels on the default workload (1M splats, 1920x1080,
 # k=16), 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE from rocprofv3 counter runs of this script
 # (profiles/trace_fwd_bwd_1M_1080p.md; the factors come from a 256 MiB copy).
-MEASURED_TRAFFIC = {"k_forward<16,0,false,true>": 6.710e9}
+MEASURED_TRAFFIC = {"k_forward<16,0,false,true>": 6.426e9}
 TRAIN_POSES, HELD_OUT_EXTRA = 16, 3
 
 
