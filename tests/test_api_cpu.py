@@ -161,3 +161,29 @@ def test_scene_generators_deterministic():
     np.testing.assert_allclose(np.linalg.norm(rd.reshape(-1, 3), axis=1), 1.0, atol=1e-6)
     c = scenes.clustered_scene(2000, 8, seed=1)
     assert c.P == 2000 and np.all(np.abs(c.verts) < 2.5)
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Field offsets and sizes of the ctypes mirrors equal the C header's (gcc on the header)."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    pairs = [(capi.FwdArgs, "tt_fwd_args"), (capi.BwdArgs, "tt_bwd_args"), (capi.MultiArgs, "tt_multi_args"),
+             (capi.ProposalHparams, "tt_proposal_hparams")]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tritracer_hip.h"', "int main(void) {"]
+    want = []
+    for cls, cname in pairs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(C.sizeof(cls))
+        for name, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {name}));')
+            want.append(getattr(cls, name).offset)
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want
