@@ -29,7 +29,8 @@ Prints one JSON line. Extra objects:
                   timed on the host cores
 Other workloads: --workload multiview (2M splats, 8 uneven cameras, one persistent
 hipGraph launch, autotuned), --workload proposal (3M splats, 128 clusters: the
-learned-proposal configuration).
+learned-proposal configuration), --workload bounce (secondary rays and environment
+light), --workload train (fwd + bwd + fused Adam + LBVH refit).
 """
 from __future__ import annotations
 
@@ -240,6 +241,55 @@ def bounce(args, dev):
                    "fwd_ms": round(per["fwd"], 4), "bwd_ms": round(per["bwd"], 4), "hitlist_cap": cap,
                    "hits_consumed_per_pixel": round(float(st[0]) / R, 3),
                    "hits_blended_per_pixel": round(float(st[1]) / R, 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def train(args, dev):
+    """Training step on the default frame: forward (storing the hit list) + backward reading it
+    + the fused Adam step on vertices, opacity, sigma and SH (zeroing the gradients it
+    consumed) + LBVH refit to the moved vertices. value = rays per second of the whole step."""
+    import numpy as np
+    import torch
+    from diff_tritracer import capi, scenes
+
+    W, H = args.width, args.height
+    scene = scenes.clustered_scene(args.triangles, 64, seed=1)
+    fr = Frame(dev, scene, args.k, args.cutoff, args.hitlist_cap)
+    fr.h.build(fr.verts, fr.cumsum, fr.P, capi.BUILD, fr.sp)
+    fr.buffers(W, H)
+    eye = 4.0 * np.array([0.35, 0.3, 1.0]) / np.linalg.norm([0.35, 0.3, 1.0])
+    fr.camera(scenes.camera_basis(W, H, eye))
+    fr.dpix.mul_(1e-3)  # a loss gradient small enough that 2000 steps keep the scene sane
+    params = [(fr.verts, fr.dverts, 1e-4), (fr.opac, fr.dop, 1e-2), (fr.sigma, fr.dsig, 1e-2), (fr.shs, fr.dsh, 2.5e-3)]
+    moments = [(torch.zeros_like(p), torch.zeros_like(p)) for p, _, _ in params]
+    groups = [(p, g, m, v, lr) for (p, g, lr), (m, v) in zip(params, moments)]
+    fa, ba = fr.fwd_args("exact"), fr.bwd_args("exact")
+    sp = fr.sp
+    state = {"step": 0}
+    fr.zero_grads()
+
+    def adam():
+        state["step"] += 1
+        capi.adam_step(groups, state["step"], zero_grads=True, stream=sp)
+
+    launches = [("fwd", lambda: fr.h.fwd(fa, sp)), ("bwd", lambda: fr.h.bwd(ba, sp)), ("adam", adam),
+                ("refit", lambda: fr.h.build(fr.verts, fr.cumsum, fr.P, capi.REFIT, sp))]
+    ms, per = timed(fr, launches, args.steps, args.warmup)
+    R = fr.R
+    n_params = sum(int(p.numel()) for p, _, _ in params)
+    print(f"[bench] train {ms:.3f} ms/step " + " ".join(f"{k} {v:.3f}" for k, v in per.items()) +
+          f"  ({R / ms / 1e3:.2f} Mrays/s)", file=sys.stderr, flush=True)
+    line = {
+        "metric": "training step Mrays/s (fwd + bwd + Adam + LBVH refit)",
+        "value": round(R / (ms * 1e-3) / 1e6, 4), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded clustered triangle splats, pinhole camera)",
+        "config": {"workload": f"training step, {scene.P} clustered splats (64 clusters, seed 1), one {W}x{H} "
+                               f"camera, k={args.k}, cutoff {args.cutoff:g}, exact chunk mode, hit list "
+                               f"{args.hitlist_cap}, Adam on {n_params} parameters (4 groups), refit",
+                   "kernel_ms": {k: round(v, 4) for k, v in per.items()},
+                   "adam_gbs": round(7 * 4 * n_params / (per["adam"] * 1e-3) / 1e9, 1)},
     }
     print(json.dumps(line), flush=True)
 
@@ -545,7 +595,7 @@ def main():
     ap.add_argument("--hitlist-cap", type=int, default=64,
                     help="blended hits per pixel the forward stores for the backward (0: the backward re-traverses)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce"], default="frame")
+    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce", "train"], default="frame")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates rocprof traffic counters)")
     args = ap.parse_args()
@@ -575,6 +625,9 @@ def main():
     if args.workload == "bounce":
         args.triangles = args.triangles or 1_000_000
         return bounce(args, dev)
+    if args.workload == "train":
+        args.triangles = args.triangles or 1_000_000
+        return train(args, dev)
     t0 = time.perf_counter()
     if args.workload == "proposal":
         P = args.triangles or 3_000_000

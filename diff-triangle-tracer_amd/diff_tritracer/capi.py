@@ -24,7 +24,7 @@ EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build"
             "tt_num_primitives", "tt_last_build_ms", "tt_multi_forward", "tt_multi_backward", "tt_multi_capture",
             "tt_multi_replay", "tt_multi_autotune", "tt_proposal_default_hparams", "tt_proposal_num_params",
             "tt_proposal_create", "tt_proposal_destroy", "tt_proposal_set_params", "tt_proposal_get_params",
-            "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays"]
+            "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays", "tt_adam_step"]
 MAX_CAMERAS = 32
 
 _vp = C.c_void_p
@@ -91,6 +91,11 @@ class MultiArgs(C.Structure):
     ]
 
 
+class AdamGroup(C.Structure):
+    _fields_ = [("param_dev", _fp), ("grad_dev", _fp), ("exp_avg_dev", _fp), ("exp_avg_sq_dev", _fp),
+                ("n", C.c_longlong), ("lr", C.c_double)]
+
+
 class ProposalHparams(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("enc_scale", "t_scale", "t_skip", "tau_start", "tau_end", "lr", "beta1",
                                         "beta2", "eps")]
@@ -126,6 +131,7 @@ def lib():
                                         C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_float), _vp]
         _d3 = C.POINTER(C.c_double)
         L.tt_camera_rays.argtypes = [C.c_int, C.c_int, _d3, _d3, _d3, _d3, C.c_double, C.c_double, _fp, _fp, _vp]
+        L.tt_adam_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _vp]
         L.tt_proposal_default_hparams.argtypes = [C.POINTER(ProposalHparams)]
         L.tt_proposal_default_hparams.restype = None
         L.tt_proposal_create.argtypes = [C.POINTER(_vp), C.POINTER(ProposalHparams), C.c_ulonglong]
@@ -288,6 +294,16 @@ class TracerHandle:
                     if v >= 0:
                         out[(k, b, t)] = float(v)
         return out
+
+
+def adam_step(groups, step, beta1=0.9, beta2=0.999, eps=1e-8, zero_grads=False, stream=None):
+    """One Adam step (torch.optim.Adam semantics) on up to 8 groups of (param, grad, exp_avg,
+    exp_avg_sq, lr), float32 device tensors of equal sizes per group."""
+    arr = (AdamGroup * max(1, len(groups)))()
+    for i, (p, g, m, v, lr) in enumerate(groups):
+        arr[i] = AdamGroup(ptr(p), ptr(g), ptr(m), ptr(v), int(p.numel()), float(lr))
+    check(lib().tt_adam_step(C.cast(arr, C.c_void_p), len(groups), int(step), float(beta1), float(beta2), float(eps),
+                             1 if zero_grads else 0, stream), "tt_adam_step")
 
 
 def hitlist_buffers(R, cap, device, segments=1):

@@ -305,6 +305,28 @@ int tt_proposal_train(tt_proposal *p, const float *ray_o_dev, const float *ray_d
 int tt_camera_rays(int width, int height, const double *eye, const double *forward, const double *right,
                    const double *up, double tan_x, double tan_y, float *ray_o_dev, float *ray_d_dev, void *stream);
 
+/* Optimizer step (no counterpart in the reference, whose caller runs torch.optim.Adam on
+   the gradients its autograd Function returns): Adam on up to TT_ADAM_MAX_GROUPS parameter
+   groups (vertices, opacity, sigma, SH, ...) in one launch, following torch.optim.Adam's
+   single-tensor update without weight decay or amsgrad:
+     m = lerp(m, g, 1 - beta1); v = v beta2 + ((1 - beta2) g) g;
+     p -= lr / (1 - beta1^step) * m / (sqrt(v) / sqrt(1 - beta2^step) + eps),
+   with the scalars formed in double and rounded once, as torch does. zero_grads: the
+   gradients are set to 0 afterwards (the next backward can accumulate without a separate
+   zeroing launch). step counts from 1; m and v start at 0. After updating vertices, refit
+   the acceleration structure with tt_build(..., TT_REFIT, ...). */
+#define TT_ADAM_MAX_GROUPS 8
+typedef struct {
+  float *param_dev;       /* [n] updated in place */
+  float *grad_dev;        /* [n] read (zeroed when zero_grads) */
+  float *exp_avg_dev;     /* [n] first moment, in/out */
+  float *exp_avg_sq_dev;  /* [n] second moment, in/out */
+  long long n;
+  double lr;
+} tt_adam_group;
+int tt_adam_step(const tt_adam_group *groups, int n_groups, int step, double beta1, double beta2, double eps,
+                 int zero_grads, void *stream);
+
 /* Number of triangles of the current acceleration structure (-1 if none). */
 int tt_num_primitives(const tt_tracer *tr);
 

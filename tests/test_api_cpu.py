@@ -171,7 +171,7 @@ def test_ctypes_structs_match_the_header(tmp_path):
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     pairs = [(capi.FwdArgs, "tt_fwd_args"), (capi.BwdArgs, "tt_bwd_args"), (capi.MultiArgs, "tt_multi_args"),
-             (capi.ProposalHparams, "tt_proposal_hparams")]
+             (capi.ProposalHparams, "tt_proposal_hparams"), (capi.AdamGroup, "tt_adam_group")]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tritracer_hip.h"', "int main(void) {"]
     want = []
     for cls, cname in pairs:
