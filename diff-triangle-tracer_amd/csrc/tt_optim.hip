@@ -122,7 +122,7 @@ extern "C" int tt_adam_step(const tt_adam_group *groups, int n_groups, int step,
   s.zero_grads = zero_grads ? 1 : 0;
   const long long work = vec4 ? nmax / 4 : nmax;
   const long long blocks = (work + 255) / 256;
-  const unsigned gx = (unsigned)(blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048);
+  const unsigned gx = (unsigned)(blocks < 65535 ? (blocks > 0 ? blocks : 1) : 65535);
   hipLaunchKernelGGL(k_adam, dim3(gx, (unsigned)n_groups), dim3(256), 0, static_cast<hipStream_t>(stream), G, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
