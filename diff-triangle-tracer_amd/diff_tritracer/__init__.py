@@ -129,4 +129,6 @@ class Tracer(nn.Module):
                              density_factor, tracer_settings)
 
 
-__all__ = ["TracerSettings", "Tracer", "_Tracer", "PACKAGE_DIR", "_C"]
+from .optim import FusedAdam  # noqa: E402  (the optimizer step on the library's Adam kernel)
+
+__all__ = ["TracerSettings", "Tracer", "_Tracer", "PACKAGE_DIR", "_C", "FusedAdam"]

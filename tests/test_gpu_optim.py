@@ -72,3 +72,28 @@ def test_vertex_step_then_refit_equals_rebuild(cuda):
     _say("refit after an Adam step on the vertices vs rebuild")
     assert np.array_equal(a["color"].view(np.uint32), b["color"].view(np.uint32))
     assert np.array_equal(a["stats"], b["stats"])
+
+
+def test_fused_adam_optimizer_matches_torch_adam(cuda):
+    import torch
+    from diff_tritracer import FusedAdam
+
+    rng = np.random.Generator(np.random.PCG64(23))
+    shapes = [(300, 3), (100, 1), (100, 16, 3)]
+    init = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+    a = [torch.tensor(x, device=cuda, requires_grad=True) for x in init]
+    b = [torch.tensor(x, requires_grad=True) for x in init]
+    fa = FusedAdam([{"params": [a[0]], "lr": 1e-3}, {"params": a[1:], "lr": 1e-2}], betas=(0.9, 0.99), eps=1e-6)
+    ta = torch.optim.Adam([{"params": [b[0]], "lr": 1e-3}, {"params": b[1:], "lr": 1e-2}], betas=(0.9, 0.99), eps=1e-6)
+    _say("FusedAdam vs torch.optim.Adam, 4 steps")
+    for _ in range(4):
+        grads = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+        for p, q, g in zip(a, b, grads):
+            p.grad = torch.tensor(g, device=cuda)
+            q.grad = torch.tensor(g)
+        fa.step()
+        ta.step()
+    torch.cuda.synchronize()
+    for p, q in zip(a, b):
+        x, y = p.detach().cpu().numpy().astype(np.float64), q.detach().numpy().astype(np.float64)
+        assert np.all(np.abs(x - y) <= 2e-6 * np.abs(y) + 1e-7)
