@@ -30,7 +30,8 @@ Prints one JSON line. Extra objects:
 Other workloads: --workload multiview (2M splats, 8 uneven cameras, one persistent
 hipGraph launch, autotuned), --workload proposal (3M splats, 128 clusters: the
 learned-proposal configuration), --workload bounce (secondary rays and environment
-light), --workload train (fwd + bwd + fused Adam + LBVH refit).
+light), --workload train (fwd + bwd + fused Adam + LBVH refit), --workload densify
+(pruning + densification from the forward's statistics).
 """
 from __future__ import annotations
 
@@ -291,6 +292,91 @@ def train(args, dev):
                    "kernel_ms": {k: round(v, 4) for k, v in per.items()},
                    # Adam moves 32 B per parameter: p, g, m, v read; p, m, v and the zeroed g written
                    "adam_gbs": round(8 * 4 * n_params / (per["adam"] * 1e-3) / 1e9, 1)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def densify_bench(args, dev):
+    """Pruning + densification of a trained-size scene from real forward statistics: the
+    1080p exact forward fills max_blending / density / scaling, then each step runs
+    tt_densify_plan (3 launches) and tt_densify_apply on the whole parameter set with Adam
+    moments (vertices + 2 moments as vertex rows, opacity, sigma, SH + 2 moments each: 12
+    arrays, 708 B per triangle). value = input triangles per second of plan + apply."""
+    import numpy as np
+    import torch
+    from diff_tritracer import capi, scenes
+
+    W, H = args.width, args.height
+    scene = scenes.clustered_scene(args.triangles, 64, seed=1)
+    fr = Frame(dev, scene, args.k, args.cutoff)
+    fr.h.build(fr.verts, fr.cumsum, fr.P, capi.BUILD, fr.sp)
+    fr.buffers(W, H)
+    eye = 4.0 * np.array([0.35, 0.3, 1.0]) / np.linalg.norm([0.35, 0.3, 1.0])
+    fr.camera(scenes.camera_basis(W, H, eye))
+    fr.h.fwd(fr.fwd_args("exact", hitlist=False), fr.sp)
+    torch.cuda.synchronize()
+    P, sp = fr.P, fr.sp
+    # thresholds: drop triangles below 1/255 peak blending weight, split the blended
+    # triangles in the top 5% of angular size, cap the growth at 10%
+    blended = fr.scal[fr.dens > 0]
+    split_scaling = float(torch.quantile(blended[:1 << 24].float(), 0.95)) if blended.numel() else float("inf")
+    kw = dict(min_blending=1.0 / 255.0, min_opacity=0.005, split_scaling=split_scaling, max_out=int(1.1 * P))
+    work = torch.empty(capi.densify_work_ints(P), dtype=torch.int32, device=dev)
+    action = torch.empty(P, dtype=torch.uint8, device=dev)
+    offset = torch.empty(P + 1, dtype=torch.int32, device=dev)
+    pa = capi.DensifyPlanArgs()
+    pa.P = P
+    pa.max_blending_dev, pa.opacity_dev, pa.density_dev, pa.scaling_dev = (capi.ptr(t) for t in
+                                                                           (fr.maxb, fr.opac, fr.dens, fr.scal))
+    pa.min_blending, pa.min_opacity, pa.split_scaling = kw["min_blending"], kw["min_opacity"], kw["split_scaling"]
+    pa.max_out = kw["max_out"]
+    pa.action_dev, pa.offset_dev, pa.work_dev = capi.ptr(action), capi.ptr(offset), capi.ptr(work)
+    L = capi.lib()
+    capi.check(L.tt_densify_plan(capi.C.byref(pa), sp), "tt_densify_plan")
+    P_out = int(offset[-1].item())
+    acts = torch.bincount(action.long(), minlength=3).cpu().tolist()
+    mom = lambda t: (torch.zeros_like(t).normal_(), torch.zeros_like(t).normal_())
+    (vm, vv), (om, ov), (sm, sv), (hm, hv) = mom(fr.verts), mom(fr.opac), mom(fr.sigma), mom(fr.shs)
+    srcs = [(fr.verts, 9, capi.ROWS_VERTEX, capi.SPLIT_SUBDIVIDE), (vm, 9, capi.ROWS_VERTEX, capi.SPLIT_ZERO),
+            (vv, 9, capi.ROWS_VERTEX, capi.SPLIT_ZERO)]
+    for t, m, v, w in ((fr.opac, om, ov, 1), (fr.sigma, sm, sv, 1), (fr.shs, hm, hv, 48)):
+        srcs += [(t, w, capi.ROWS_TRIANGLE, capi.SPLIT_COPY), (m, w, capi.ROWS_TRIANGLE, capi.SPLIT_ZERO),
+                 (v, w, capi.ROWS_TRIANGLE, capi.SPLIT_ZERO)]
+    outs = [torch.empty(P_out * w, device=dev) for _, w, _, _ in srcs]
+    npts_o = torch.empty(P_out, dtype=torch.int32, device=dev)
+    cs_o = torch.empty(P_out, dtype=torch.int32, device=dev)
+    aa = capi.densify_apply_args(P, P_out, offset, fr.cumsum, [(s, o, w, r, p) for (s, w, r, p), o in
+                                                              zip(srcs, outs)], npts_o, cs_o)
+
+    def plan():
+        capi.check(L.tt_densify_plan(capi.C.byref(pa), sp), "tt_densify_plan")
+
+    def apply():
+        capi.check(L.tt_densify_apply(capi.C.byref(aa), sp), "tt_densify_apply")
+
+    ms, per = timed(fr, [("plan", plan), ("apply", apply)], args.steps, args.warmup)
+    n_keep, n_split = acts[1] + acts[2], acts[2]
+    row = sum(4 * w for _, w, _, _ in srcs)                      # 708 B of all arrays per triangle
+    copy_row = sum(4 * w for _, w, _, p in srcs if p != capi.SPLIT_ZERO)  # read for a split parent
+    apply_bytes = (acts[1] * row + n_split * copy_row + P_out * (row + 8) + 4 * (2 * P + 1))
+    plan_bytes = P * (2 * 16 + 4 + 1)
+    achieved = apply_bytes / (per["apply"] * 1e-3) / 1e9
+    print(f"[bench] densify {ms:.3f} ms/step plan {per['plan']:.3f} apply {per['apply']:.3f}  P {P} -> {P_out} "
+          f"(drop {acts[0]}, keep {acts[1]}, split {acts[2]})  apply {achieved:.0f} GB/s", file=sys.stderr, flush=True)
+    line = {
+        "metric": "prune + densify Mtriangles/s (plan + apply, parameters and Adam moments)",
+        "value": round(P / (ms * 1e-3) / 1e6, 4), "unit": "Mtriangles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded clustered triangle splats, statistics of a "
+                                                   "1080p exact-mode forward)",
+        "config": {"workload": f"prune + split {P} clustered splats (64 clusters, seed 1) -> {P_out}: drop "
+                               f"{acts[0]}, keep {acts[1]}, split {acts[2]}; 12 arrays (vertices, opacity, sigma, "
+                               f"SH + Adam moments)",
+                   "kernel_ms": {k: round(v, 4) for k, v in per.items()},
+                   "plan_gbs": round(plan_bytes / (per["plan"] * 1e-3) / 1e9, 1)},
+        "roofline": dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel="k_apply",
+                         bytes_per_launch=int(apply_bytes), avg_launch_ms=round(per["apply"], 4)),
     }
     print(json.dumps(line), flush=True)
 
@@ -596,7 +682,7 @@ def main():
     ap.add_argument("--hitlist-cap", type=int, default=64,
                     help="blended hits per pixel the forward stores for the backward (0: the backward re-traverses)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce", "train"], default="frame")
+    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce", "train", "densify"], default="frame")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates rocprof traffic counters)")
     args = ap.parse_args()
@@ -629,6 +715,9 @@ def main():
     if args.workload == "train":
         args.triangles = args.triangles or 1_000_000
         return train(args, dev)
+    if args.workload == "densify":
+        args.triangles = args.triangles or 1_000_000
+        return densify_bench(args, dev)
     t0 = time.perf_counter()
     if args.workload == "proposal":
         P = args.triangles or 3_000_000

@@ -24,7 +24,8 @@ EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build"
             "tt_num_primitives", "tt_last_build_ms", "tt_multi_forward", "tt_multi_backward", "tt_multi_capture",
             "tt_multi_replay", "tt_multi_autotune", "tt_proposal_default_hparams", "tt_proposal_num_params",
             "tt_proposal_create", "tt_proposal_destroy", "tt_proposal_set_params", "tt_proposal_get_params",
-            "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays", "tt_adam_step"]
+            "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays", "tt_adam_step",
+            "tt_densify_plan", "tt_densify_apply"]
 MAX_CAMERAS = 32
 
 _vp = C.c_void_p
@@ -96,6 +97,34 @@ class AdamGroup(C.Structure):
                 ("n", C.c_longlong), ("lr", C.c_double)]
 
 
+DENSIFY_DROP, DENSIFY_KEEP, DENSIFY_SPLIT = 0, 1, 2
+ROWS_TRIANGLE, ROWS_VERTEX = 0, 1
+SPLIT_COPY, SPLIT_ZERO, SPLIT_SUBDIVIDE = 0, 1, 2
+DENSIFY_MAX_ARRAYS = 16
+
+
+def densify_work_ints(P: int) -> int:
+    """TT_DENSIFY_WORK_INTS(P): scratch ints of tt_densify_plan."""
+    return 2 * ((int(P) + 2047) // 2048) + 2
+
+
+class DensifyPlanArgs(C.Structure):
+    _fields_ = [("P", C.c_int), ("max_blending_dev", _fp), ("opacity_dev", _fp), ("density_dev", _fp),
+                ("scaling_dev", _fp), ("score_dev", _fp), ("action_in_dev", _fp), ("min_blending", C.c_float),
+                ("min_opacity", C.c_float), ("split_scaling", C.c_float), ("split_score", C.c_float),
+                ("max_out", C.c_longlong), ("action_dev", _fp), ("offset_dev", _fp), ("work_dev", _fp)]
+
+
+class DensifyArray(C.Structure):
+    _fields_ = [("src_dev", _fp), ("dst_dev", _fp), ("width", C.c_int), ("rows", C.c_int), ("on_split", C.c_int)]
+
+
+class DensifyApplyArgs(C.Structure):
+    _fields_ = [("P", C.c_int), ("P_out", C.c_int), ("offset_dev", _fp), ("cumsum_dev", _fp),
+                ("npts_out_dev", _fp), ("cumsum_out_dev", _fp), ("n_arrays", C.c_int),
+                ("arrays", DensifyArray * DENSIFY_MAX_ARRAYS)]
+
+
 class ProposalHparams(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("enc_scale", "t_scale", "t_skip", "tau_start", "tau_end", "lr", "beta1",
                                         "beta2", "eps")]
@@ -132,6 +161,8 @@ def lib():
         _d3 = C.POINTER(C.c_double)
         L.tt_camera_rays.argtypes = [C.c_int, C.c_int, _d3, _d3, _d3, _d3, C.c_double, C.c_double, _fp, _fp, _vp]
         L.tt_adam_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _vp]
+        L.tt_densify_plan.argtypes = [C.POINTER(DensifyPlanArgs), _vp]
+        L.tt_densify_apply.argtypes = [C.POINTER(DensifyApplyArgs), _vp]
         L.tt_proposal_default_hparams.argtypes = [C.POINTER(ProposalHparams)]
         L.tt_proposal_default_hparams.restype = None
         L.tt_proposal_create.argtypes = [C.POINTER(_vp), C.POINTER(ProposalHparams), C.c_ulonglong]
@@ -304,6 +335,46 @@ def adam_step(groups, step, beta1=0.9, beta2=0.999, eps=1e-8, zero_grads=False, 
         arr[i] = AdamGroup(ptr(p), ptr(g), ptr(m), ptr(v), int(p.numel()), float(lr))
     check(lib().tt_adam_step(C.cast(arr, C.c_void_p), len(groups), int(step), float(beta1), float(beta2), float(eps),
                              1 if zero_grads else 0, stream), "tt_adam_step")
+
+
+def densify_plan(max_blending, opacity, density=None, scaling=None, score=None, action_in=None, min_blending=0.0,
+                 min_opacity=0.0, split_scaling=float("inf"), split_score=0.0, max_out=0, stream=None):
+    """tt_densify_plan on [P] float32 device tensors: returns (action uint8 [P], offset int32
+    [P + 1]) on the device; offset[P] is the new triangle count."""
+    import torch
+    P = int(max_blending.numel() if max_blending is not None else action_in.numel())
+    dev = (max_blending if max_blending is not None else action_in).device
+    action = torch.empty(P, dtype=torch.uint8, device=dev)
+    offset = torch.empty(P + 1, dtype=torch.int32, device=dev)
+    work = torch.empty(densify_work_ints(P), dtype=torch.int32, device=dev)
+    a = DensifyPlanArgs()
+    a.P = P
+    a.max_blending_dev, a.opacity_dev, a.density_dev = ptr(max_blending), ptr(opacity), ptr(density)
+    a.scaling_dev, a.score_dev, a.action_in_dev = ptr(scaling), ptr(score), ptr(action_in)
+    a.min_blending, a.min_opacity = float(min_blending), float(min_opacity)
+    a.split_scaling, a.split_score, a.max_out = float(split_scaling), float(split_score), int(max_out)
+    a.action_dev, a.offset_dev, a.work_dev = ptr(action), offset.data_ptr(), work.data_ptr()
+    check(lib().tt_densify_plan(C.byref(a), stream), "tt_densify_plan")
+    return action, offset
+
+
+def densify_apply_args(P, P_out, offset, cumsum, arrays, npts_out=None, cumsum_out=None) -> DensifyApplyArgs:
+    """arrays: list of (src, dst, width, rows, on_split) with float32 device tensors."""
+    if len(arrays) > DENSIFY_MAX_ARRAYS:
+        raise ValueError(f"at most {DENSIFY_MAX_ARRAYS} arrays per call")
+    a = DensifyApplyArgs()
+    a.P, a.P_out = int(P), int(P_out)
+    a.offset_dev, a.cumsum_dev = ptr(offset), ptr(cumsum)
+    a.npts_out_dev, a.cumsum_out_dev = ptr(npts_out), ptr(cumsum_out)
+    a.n_arrays = len(arrays)
+    for k, (src, dst, width, rows, on_split) in enumerate(arrays):
+        a.arrays[k] = DensifyArray(ptr(src), ptr(dst), int(width), int(rows), int(on_split))
+    return a
+
+
+def densify_apply(P, P_out, offset, cumsum, arrays, npts_out=None, cumsum_out=None, stream=None):
+    a = densify_apply_args(P, P_out, offset, cumsum, arrays, npts_out, cumsum_out)
+    check(lib().tt_densify_apply(C.byref(a), stream), "tt_densify_apply")
 
 
 def hitlist_buffers(R, cap, device, segments=1):

@@ -327,6 +327,81 @@ typedef struct {
 int tt_adam_step(const tt_adam_group *groups, int n_groups, int step, double beta1, double beta2, double eps,
                  int zero_grads, void *stream);
 
+/* Training-statistics consumer (no counterpart in the reference, whose callers prune and
+   densify in Python). The forward's per-triangle statistics -- max_blending (the largest
+   alpha*T the triangle received, forward.cu:138-143), density_factor (1 once it was
+   blended, forward.cu:134) and scaling (its largest angular radius seen from the ray
+   origin, forward.cu:34-74) -- become a new triangle set: a triangle whose max_blending is
+   below min_blending or whose opacity is below min_opacity is dropped; a kept triangle that
+   was blended, has scaling > split_scaling and (when score_dev is given) score >
+   split_score is a split candidate; candidates are split into four by midpoint
+   subdivision in index order while the new count stays within max_out, the rest are kept
+   whole. Survivors keep their order; the children of triangle (a, b, c) with midpoints
+   ab, bc, ca are (a, ab, ca), (ab, b, bc), (ca, bc, c), (ab, bc, ca), in that order, all
+   with the parent's orientation.
+   tt_densify_plan writes action[i] (TT_DENSIFY_*) and offset[i], the exclusive prefix sum
+   of the output triangles per input triangle, with offset[P] = the new count; read
+   offset[P] back to size the outputs (the only value the host needs), then
+   tt_densify_apply writes the new arrays. */
+#define TT_DENSIFY_DROP 0
+#define TT_DENSIFY_KEEP 1
+#define TT_DENSIFY_SPLIT 2
+/* scratch ints tt_densify_plan needs for P triangles */
+#define TT_DENSIFY_WORK_INTS(P) (2 * (((long long)(P) + 2047) / 2048) + 2)
+typedef struct {
+  int P;
+  const float *max_blending_dev;     /* [P] */
+  const float *opacity_dev;          /* [P] */
+  const float *density_dev;          /* [P] nonzero = blended, or NULL (every kept triangle counts as blended) */
+  const float *scaling_dev;          /* [P], or NULL: no split candidates */
+  const float *score_dev;            /* [P] caller's extra split criterion (e.g. accumulated gradient norm), or NULL */
+  const unsigned char *action_in_dev; /* [P] caller's own decision (TT_DENSIFY_*), or NULL; replaces the
+                                         statistics rules, max_out still applies to its splits */
+  float min_blending;
+  float min_opacity;
+  float split_scaling;
+  float split_score;
+  long long max_out;                 /* cap on the new count (<= 0: 2^31 - 1); kept triangles are never dropped to
+                                        meet it, only splits are withheld */
+  unsigned char *action_dev;         /* out [P] (TT_DENSIFY_SPLIT only where the split was granted); NULL if P = 0 */
+  int *offset_dev;                   /* out [P + 1] */
+  int *work_dev;                     /* scratch, TT_DENSIFY_WORK_INTS(P) ints */
+} tt_densify_plan_args;
+int tt_densify_plan(const tt_densify_plan_args *args, void *stream);
+
+/* One array remapped by tt_densify_apply. Rows of the input are per triangle, either
+   TT_ROWS_TRIANGLE (row i = width floats at i * width: opacity, sigma, SH [P][M][3] with
+   width 3M, optimizer moments of these) or TT_ROWS_VERTEX (row i = the 9 floats of the
+   triangle's three points at 3 * cumsum[i]: vertices and their moments; width must be 9).
+   The output is always dense: row j at j * width (for vertex rows: npts = 3, cumsum = 3j).
+   A kept triangle's row is copied; a split triangle's four children get, by on_split:
+   TT_SPLIT_COPY the parent's row, TT_SPLIT_ZERO zeros (optimizer moments of new
+   triangles), TT_SPLIT_SUBDIVIDE (vertex rows only) the midpoint-subdivided points. */
+#define TT_DENSIFY_MAX_ARRAYS 16
+#define TT_ROWS_TRIANGLE 0
+#define TT_ROWS_VERTEX 1
+#define TT_SPLIT_COPY 0
+#define TT_SPLIT_ZERO 1
+#define TT_SPLIT_SUBDIVIDE 2
+typedef struct {
+  const float *src_dev;
+  float *dst_dev;  /* [P_out * width]; must not overlap src */
+  int width;
+  int rows;        /* TT_ROWS_* */
+  int on_split;    /* TT_SPLIT_* */
+} tt_densify_array;
+typedef struct {
+  int P;
+  int P_out;                 /* offset[P] of the plan */
+  const int *offset_dev;     /* [P + 1] from tt_densify_plan (output counts 0, 1 or 4 per triangle) */
+  const int *cumsum_dev;     /* [P], needed by TT_ROWS_VERTEX arrays */
+  int *npts_out_dev;         /* [P_out] set to 3, or NULL */
+  int *cumsum_out_dev;       /* [P_out] set to 3j, or NULL */
+  int n_arrays;
+  tt_densify_array arrays[TT_DENSIFY_MAX_ARRAYS];
+} tt_densify_apply_args;
+int tt_densify_apply(const tt_densify_apply_args *args, void *stream);
+
 /* Number of triangles of the current acceleration structure (-1 if none). */
 int tt_num_primitives(const tt_tracer *tr);
 

@@ -46,6 +46,40 @@ def test_c_abi_argument_errors_without_device():
     assert L.tt_num_primitives(None) == -1
 
 
+def test_densify_argument_errors_without_device():
+    L = capi.lib()
+    assert L.tt_densify_plan(None, None) == 1
+    p = capi.DensifyPlanArgs()
+    p.P = 10
+    assert L.tt_densify_plan(C.byref(p), None) == 1 and "required" in capi.last_error()
+    p.P = -1
+    assert L.tt_densify_plan(C.byref(p), None) == 1 and "non-negative" in capi.last_error()
+    fake = 256  # never dereferenced: every check below fails before a launch
+    checks = [
+        (dict(width=0), "width"),
+        (dict(width=9, rows=2), "layout"),
+        (dict(width=9, rows=0, on_split=3), "policy"),
+        (dict(width=8, rows=1, on_split=0), "vertex rows"),
+        (dict(width=9, rows=0, on_split=2), "subdivision"),
+    ]
+    for fields, msg in checks:
+        a = capi.DensifyApplyArgs()
+        a.P, a.P_out, a.offset_dev, a.cumsum_dev, a.n_arrays = 4, 4, fake, fake, 1
+        arr = capi.DensifyArray(fake, fake, 9, 0, 0)
+        for k, v in fields.items():
+            setattr(arr, k, v)
+        a.arrays[0] = arr
+        assert L.tt_densify_apply(C.byref(a), None) == 1, fields
+        assert msg in capi.last_error(), (fields, capi.last_error())
+    a = capi.DensifyApplyArgs()
+    a.P, a.P_out, a.n_arrays = 4, 17, 0
+    assert L.tt_densify_apply(C.byref(a), None) == 1 and "4 P" in capi.last_error()
+    a.P_out, a.n_arrays = 4, 17
+    assert L.tt_densify_apply(C.byref(a), None) == 1 and "n_arrays" in capi.last_error()
+    with pytest.raises(ValueError):
+        capi.densify_apply_args(4, 4, None, None, [(None, None, 1, 0, 0)] * 17)
+
+
 def test_extension_binds_reference_names():
     for name in ("OptixTracer", "FwdTraceResult", "BwdTraceResult"):
         assert hasattr(D._C, name)
@@ -171,7 +205,9 @@ def test_ctypes_structs_match_the_header(tmp_path):
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     pairs = [(capi.FwdArgs, "tt_fwd_args"), (capi.BwdArgs, "tt_bwd_args"), (capi.MultiArgs, "tt_multi_args"),
-             (capi.ProposalHparams, "tt_proposal_hparams"), (capi.AdamGroup, "tt_adam_group")]
+             (capi.ProposalHparams, "tt_proposal_hparams"), (capi.AdamGroup, "tt_adam_group"),
+             (capi.DensifyPlanArgs, "tt_densify_plan_args"), (capi.DensifyArray, "tt_densify_array"),
+             (capi.DensifyApplyArgs, "tt_densify_apply_args")]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tritracer_hip.h"', "int main(void) {"]
     want = []
     for cls, cname in pairs:
