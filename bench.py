@@ -31,7 +31,8 @@ Other workloads: --workload multiview (2M splats, 8 uneven cameras, one persiste
 hipGraph launch, autotuned), --workload proposal (3M splats, 128 clusters: the
 learned-proposal configuration), --workload bounce (secondary rays and environment
 light), --workload train (fwd + bwd + fused Adam + LBVH refit), --workload densify
-(pruning + densification from the forward's statistics).
+(pruning + densification from the forward's statistics), --workload sceneio (scene-file rows
+<-> tracer layout on the GPU).
 """
 from __future__ import annotations
 
@@ -381,6 +382,92 @@ def densify_bench(args, dev):
     print(json.dumps(line), flush=True)
 
 
+def sceneio_bench(args, dev):
+    """Scene files: the GPU halves of save (tt_scene_pack: tracer layout -> PLY vertex and
+    face rows) and load (tt_scene_unpack: rows -> tracer layout, vertices gathered through
+    the indices) on a 1M-splat scene with inputs resident in HBM. value = triangles per
+    second of pack + unpack. The host legs (file write / read, PCIe) are timed once and
+    reported in config."""
+    import ctypes
+    import tempfile
+
+    import numpy as np
+    import torch
+    from diff_tritracer import capi, scenes, sceneio
+
+    scene = scenes.clustered_scene(args.triangles, 64, seed=1)
+    P = scene.P
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    verts, cumsum, opac, sigma, shs = T(scene.verts), T(scene.cumsum), T(scene.opac), T(scene.sigma), T(scene.shs)
+    M = int(shs.shape[1])
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    vert_rows, face_rows, _ = sceneio.pack(verts, cumsum, opac, sigma, shs)
+    pa = capi.ScenePackArgs(P, M, capi.ptr(verts), capi.ptr(cumsum), capi.ptr(opac), capi.ptr(sigma), capi.ptr(shs),
+                            capi.ptr(face_rows), capi.ptr(vert_rows))
+    S = capi.scene_face_bytes(M)
+    ua = capi.SceneUnpackArgs()
+    ua.P, ua.V, ua.face_stride, ua.M = P, 3 * P, S, M
+    ua.count_offset, ua.index_offset, ua.opacity_offset, ua.sigma_offset = 0, 1, 13, 17
+    for k in range(capi.SCENE_MAX_COEFFS * 3):
+        ua.sh_offset[k] = -1
+    for c in range(3):
+        ua.sh_offset[c] = 21 + 4 * c
+        for j in range(1, M):
+            ua.sh_offset[3 * j + c] = 21 + 12 + 4 * (c * (M - 1) + j - 1)
+    ua.vert_stride = 12
+    ua.xyz_offset[0], ua.xyz_offset[1], ua.xyz_offset[2] = 0, 4, 8
+    out_v, out_o, out_s = torch.empty_like(verts), torch.empty_like(opac), torch.empty_like(sigma)
+    out_h = torch.empty_like(shs)
+    out_n, out_c = torch.empty_like(cumsum), torch.empty_like(cumsum)
+    bad = torch.zeros(1, dtype=torch.int32, device=dev)
+    ua.face_dev, ua.vert_dev = capi.ptr(face_rows), capi.ptr(vert_rows)
+    ua.vertices_out_dev, ua.npts_out_dev, ua.cumsum_out_dev = capi.ptr(out_v), capi.ptr(out_n), capi.ptr(out_c)
+    ua.opacity_out_dev, ua.sigma_out_dev, ua.shs_out_dev, ua.bad_dev = (capi.ptr(out_o), capi.ptr(out_s),
+                                                                          capi.ptr(out_h), capi.ptr(bad))
+    L = capi.lib()
+
+    class _Fr:  # what timed() needs
+        pass
+    fr = _Fr()
+    fr.torch, fr.stream = torch, stream
+    launches = [("pack", lambda: capi.check(L.tt_scene_pack(ctypes.byref(pa), sp), "tt_scene_pack")),
+                ("unpack", lambda: capi.check(L.tt_scene_unpack(ctypes.byref(ua), sp), "tt_scene_unpack"))]
+    ms, per = timed(fr, launches, args.steps, args.warmup)
+    assert int(bad.item()) == 0 and torch.equal(out_h, shs) and torch.equal(out_o, opac)
+    # host legs, once: save (pack + copy + write) and load (read + copy + unpack) of a real file
+    with tempfile.TemporaryDirectory() as d:
+        path = d + "/scene.ply"
+        t0 = time.perf_counter()
+        sceneio.save_ply(path, verts, cumsum, opac, sigma, shs)
+        t1 = time.perf_counter()
+        sceneio.load_ply(path, dev)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        size = os.path.getsize(path)
+    unpack_bytes = P * (S + 36 + 244)   # face rows + gathered vertex rows read, tracer layout written
+    pack_bytes = P * (240 + S + 36)     # tracer layout + cumsum read, face and vertex rows written
+    ach = unpack_bytes / (per["unpack"] * 1e-3) / 1e9
+    print(f"[bench] sceneio pack {per['pack']:.3f} ms ({pack_bytes / (per['pack'] * 1e-3) / 1e9:.0f} GB/s) unpack "
+          f"{per['unpack']:.3f} ms ({ach:.0f} GB/s); file {size / 1e6:.0f} MB save {t1 - t0:.2f} s load {t2 - t1:.2f} s",
+          file=sys.stderr, flush=True)
+    line = {
+        "metric": "scene file pack + unpack Mtriangles/s (device side of save / load)",
+        "value": round(P / (ms * 1e-3) / 1e6, 4), "unit": "Mtriangles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8/fp32 (byte moves)", "data": "synthetic (seeded clustered triangle splats)",
+        "config": {"workload": f"PLY scene rows <-> tracer layout, {P} clustered splats (64 clusters, seed 1), "
+                               f"M={M}, {S}-byte face rows",
+                   "kernel_ms": {k: round(v, 4) for k, v in per.items()},
+                   "pack_gbs": round(pack_bytes / (per["pack"] * 1e-3) / 1e9, 1),
+                   "file_mb": round(size / 1e6, 1), "save_s": round(t1 - t0, 3), "load_s": round(t2 - t1, 3)},
+        "roofline": dict(bound="hbm", achieved=round(ach, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(ach / HBM_PEAK_GBS, 5), traffic=None, kernel="k_scene_unpack",
+                         bytes_per_launch=int(unpack_bytes), avg_launch_ms=round(per["unpack"], 4)),
+    }
+    print(json.dumps(line), flush=True)
+
+
 class Frame:
     """Scene tensors, the tracer handle and per-frame buffers of one workload."""
 
@@ -682,7 +769,7 @@ def main():
     ap.add_argument("--hitlist-cap", type=int, default=64,
                     help="blended hits per pixel the forward stores for the backward (0: the backward re-traverses)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce", "train", "densify"], default="frame")
+    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce", "train", "densify", "sceneio"], default="frame")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates rocprof traffic counters)")
     args = ap.parse_args()
@@ -718,6 +805,9 @@ def main():
     if args.workload == "densify":
         args.triangles = args.triangles or 1_000_000
         return densify_bench(args, dev)
+    if args.workload == "sceneio":
+        args.triangles = args.triangles or 1_000_000
+        return sceneio_bench(args, dev)
     t0 = time.perf_counter()
     if args.workload == "proposal":
         P = args.triangles or 3_000_000

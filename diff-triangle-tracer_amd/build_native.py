@@ -23,7 +23,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["tt_bvh.hip", "tt_trace.hip", "tt_multi.hip", "tt_proposal.hip", "tt_camera.hip", "tt_optim.hip",
-               "tt_densify.hip", "tt_capi.cpp"]
+               "tt_densify.hip", "tt_sceneio.hip", "tt_capi.cpp"]
 EXT_SOURCES = ["ext.cpp", "tracer.cpp"]
 
 # -ffp-contract=off: the kernels round where the CPU oracle rounds (bit-exact hit lists).

@@ -25,7 +25,7 @@ EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build"
             "tt_multi_replay", "tt_multi_autotune", "tt_proposal_default_hparams", "tt_proposal_num_params",
             "tt_proposal_create", "tt_proposal_destroy", "tt_proposal_set_params", "tt_proposal_get_params",
             "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays", "tt_adam_step",
-            "tt_densify_plan", "tt_densify_apply"]
+            "tt_densify_plan", "tt_densify_apply", "tt_scene_unpack", "tt_scene_pack"]
 MAX_CAMERAS = 32
 
 _vp = C.c_void_p
@@ -125,6 +125,28 @@ class DensifyApplyArgs(C.Structure):
                 ("arrays", DensifyArray * DENSIFY_MAX_ARRAYS)]
 
 
+SCENE_MAX_COEFFS = 16
+
+
+def scene_face_bytes(M: int) -> int:
+    """TT_SCENE_FACE_BYTES(M): bytes of a face row as tt_scene_pack writes it."""
+    return 1 + 12 + 4 * (2 + 3 * int(M))
+
+
+class SceneUnpackArgs(C.Structure):
+    _fields_ = [("P", C.c_int), ("V", C.c_longlong), ("face_dev", _fp), ("face_stride", C.c_int),
+                ("count_offset", C.c_int), ("index_offset", C.c_int), ("opacity_offset", C.c_int),
+                ("sigma_offset", C.c_int), ("sh_offset", C.c_int * (SCENE_MAX_COEFFS * 3)), ("M", C.c_int),
+                ("vert_dev", _fp), ("vert_stride", C.c_int), ("xyz_offset", C.c_int * 3),
+                ("vertices_out_dev", _fp), ("npts_out_dev", _fp), ("cumsum_out_dev", _fp),
+                ("opacity_out_dev", _fp), ("sigma_out_dev", _fp), ("shs_out_dev", _fp), ("bad_dev", _fp)]
+
+
+class ScenePackArgs(C.Structure):
+    _fields_ = [("P", C.c_int), ("M", C.c_int), ("vertices_dev", _fp), ("cumsum_dev", _fp), ("opacity_dev", _fp),
+                ("sigma_dev", _fp), ("shs_dev", _fp), ("face_out_dev", _fp), ("vert_out_dev", _fp)]
+
+
 class ProposalHparams(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("enc_scale", "t_scale", "t_skip", "tau_start", "tau_end", "lr", "beta1",
                                         "beta2", "eps")]
@@ -163,6 +185,8 @@ def lib():
         L.tt_adam_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _vp]
         L.tt_densify_plan.argtypes = [C.POINTER(DensifyPlanArgs), _vp]
         L.tt_densify_apply.argtypes = [C.POINTER(DensifyApplyArgs), _vp]
+        L.tt_scene_unpack.argtypes = [C.POINTER(SceneUnpackArgs), _vp]
+        L.tt_scene_pack.argtypes = [C.POINTER(ScenePackArgs), _vp]
         L.tt_proposal_default_hparams.argtypes = [C.POINTER(ProposalHparams)]
         L.tt_proposal_default_hparams.restype = None
         L.tt_proposal_create.argtypes = [C.POINTER(_vp), C.POINTER(ProposalHparams), C.c_ulonglong]

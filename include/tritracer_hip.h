@@ -402,6 +402,53 @@ typedef struct {
 } tt_densify_apply_args;
 int tt_densify_apply(const tt_densify_apply_args *args, void *stream);
 
+/* Scene I/O (no counterpart in the reference, which neither loads nor saves scenes): the
+   byte layout of a binary little-endian PLY scene file on the device. The file holds
+     element vertex V    (x, y, z float, other properties allowed)
+     element face P      (property list uchar int vertex_indices, float opacity, float sigma,
+                          float f_dc_0..2, float f_rest_0..3(M-1)-1, other properties allowed)
+   with SH coefficient j >= 1 of colour channel c at f_rest_{c (M-1) + j - 1} (channel-major,
+   the usual splat-PLY order) and f_dc_c the DC term of channel c. The host parses the
+   header and hands over the two element bodies as separate device buffers.
+   tt_scene_unpack turns them into the tracer's layout: vertices [3P][3] gathered through
+   the indices (npts = 3, cumsum = 3i), opacity [P], sigma [P], shs [P][M][3]; absent
+   properties (offset -1) read as 0. bad_dev (one int, accumulated) counts faces whose
+   list count is not 3 or whose index lies outside [0, V); their vertices are written as 0.
+   tt_scene_pack writes the canonical rows this build saves: face rows of
+   TT_SCENE_FACE_BYTES(M) bytes (count 3, indices 3i..3i+2, opacity, sigma, f_dc, f_rest
+   in that order) and vertex rows x, y, z [3P][3] read through cumsum. */
+#define TT_SCENE_MAX_COEFFS 16
+#define TT_SCENE_FACE_BYTES(M) (1 + 12 + 4 * (2 + 3 * (M)))
+typedef struct {
+  int P;
+  long long V;
+  const unsigned char *face_dev;  /* P rows of face_stride bytes, 16-byte aligned */
+  int face_stride;
+  int count_offset;               /* uchar list count */
+  int index_offset;               /* three int32 indices (little endian) */
+  int opacity_offset, sigma_offset;
+  int sh_offset[TT_SCENE_MAX_COEFFS * 3]; /* byte offset of coefficient j, channel c at [3 j + c], or -1 */
+  int M;                          /* coefficients per channel written to shs (1..16) */
+  const unsigned char *vert_dev;  /* V rows of vert_stride bytes */
+  int vert_stride;
+  int xyz_offset[3];
+  float *vertices_out_dev;        /* [3P][3] */
+  int *npts_out_dev, *cumsum_out_dev; /* [P] or NULL */
+  float *opacity_out_dev, *sigma_out_dev; /* [P] */
+  float *shs_out_dev;             /* [P][M][3] */
+  int *bad_dev;                   /* [1] incremented, or NULL */
+} tt_scene_unpack_args;
+int tt_scene_unpack(const tt_scene_unpack_args *args, void *stream);
+typedef struct {
+  int P, M;
+  const float *vertices_dev;      /* triangle i at rows cumsum[i] .. cumsum[i] + 2 */
+  const int *cumsum_dev;
+  const float *opacity_dev, *sigma_dev, *shs_dev; /* [P], [P], [P][M][3] */
+  unsigned char *face_out_dev;    /* P * TT_SCENE_FACE_BYTES(M) bytes, 16-byte aligned */
+  float *vert_out_dev;            /* [3P][3] */
+} tt_scene_pack_args;
+int tt_scene_pack(const tt_scene_pack_args *args, void *stream);
+
 /* Number of triangles of the current acceleration structure (-1 if none). */
 int tt_num_primitives(const tt_tracer *tr);
 

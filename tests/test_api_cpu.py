@@ -80,6 +80,31 @@ def test_densify_argument_errors_without_device():
         capi.densify_apply_args(4, 4, None, None, [(None, None, 1, 0, 0)] * 17)
 
 
+def test_scene_io_argument_errors_without_device():
+    L = capi.lib()
+    assert L.tt_scene_unpack(None, None) == 1 and L.tt_scene_pack(None, None) == 1
+    fake = 256  # never dereferenced: every check below fails before a launch
+    base = dict(P=4, V=12, face_dev=fake, face_stride=213, count_offset=0, index_offset=1, opacity_offset=13,
+                sigma_offset=17, M=16, vert_dev=fake, vert_stride=12, vertices_out_dev=fake, opacity_out_dev=fake,
+                sigma_out_dev=fake, shs_out_dev=fake)
+    cases = [(dict(M=17), "M must"), (dict(face_stride=12), "face_stride"), (dict(face_stride=40000), "face_stride"), (dict(index_offset=205), "outside"),
+             (dict(opacity_offset=210), "opacity"), (dict(face_dev=fake + 4), "aligned"),
+             (dict(vert_stride=8), "x, y, z"), (dict(shs_out_dev=None), "NULL")]
+    for fields, msg in cases:
+        a = capi.SceneUnpackArgs()
+        for k, v in {**base, **fields}.items():
+            setattr(a, k, v)
+        for k in range(48):
+            a.sh_offset[k] = -1
+        a.xyz_offset[0], a.xyz_offset[1], a.xyz_offset[2] = 0, 4, 8
+        assert L.tt_scene_unpack(C.byref(a), None) == 1, fields
+        assert msg in capi.last_error(), (fields, capi.last_error())
+    a = capi.ScenePackArgs(4, 0, fake, fake, fake, fake, fake, fake, fake)
+    assert L.tt_scene_pack(C.byref(a), None) == 1 and "M must" in capi.last_error()
+    a = capi.ScenePackArgs(4, 16, fake, fake, fake, fake, fake, fake + 4, fake)
+    assert L.tt_scene_pack(C.byref(a), None) == 1 and "aligned" in capi.last_error()
+
+
 def test_extension_binds_reference_names():
     for name in ("OptixTracer", "FwdTraceResult", "BwdTraceResult"):
         assert hasattr(D._C, name)
@@ -207,7 +232,8 @@ def test_ctypes_structs_match_the_header(tmp_path):
     pairs = [(capi.FwdArgs, "tt_fwd_args"), (capi.BwdArgs, "tt_bwd_args"), (capi.MultiArgs, "tt_multi_args"),
              (capi.ProposalHparams, "tt_proposal_hparams"), (capi.AdamGroup, "tt_adam_group"),
              (capi.DensifyPlanArgs, "tt_densify_plan_args"), (capi.DensifyArray, "tt_densify_array"),
-             (capi.DensifyApplyArgs, "tt_densify_apply_args")]
+             (capi.DensifyApplyArgs, "tt_densify_apply_args"), (capi.SceneUnpackArgs, "tt_scene_unpack_args"),
+             (capi.ScenePackArgs, "tt_scene_pack_args")]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tritracer_hip.h"', "int main(void) {"]
     want = []
     for cls, cname in pairs:
