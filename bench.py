@@ -49,8 +49,9 @@ sys.path.insert(0, os.path.join(ROOT, "diff-triangle-tracer_amd"))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernels on the default workload (1M splats, 1920x1080,
 # k=16), 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE from rocprofv3 counter runs of this script
-# (profiles/trace_fwd_bwd_1M_1080p.md; the factors come from a 256 MiB copy).
-MEASURED_TRAFFIC = {"k_forward<16,0,false,true>": 6.426e9}
+# (profiles/trace_fwd_bwd_1M_1080p.md; the factors come from a 256 MiB copy), and of the
+# densify and scene-file kernels at 1M splats (profiles/densify_sceneio_1M.md).
+MEASURED_TRAFFIC = {"k_forward<16,0,false,true>": 6.426e9, "k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
 TRAIN_POSES, HELD_OUT_EXTRA = 16, 3
 
 
@@ -376,7 +377,8 @@ def densify_bench(args, dev):
                    "kernel_ms": {k: round(v, 4) for k, v in per.items()},
                    "plan_gbs": round(plan_bytes / (per["plan"] * 1e-3) / 1e9, 1)},
         "roofline": dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                         frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel="k_apply",
+                         frac=round(achieved / HBM_PEAK_GBS, 5),
+                         traffic=MEASURED_TRAFFIC["k_apply"] if P == 1_000_000 else None, kernel="k_apply",
                          bytes_per_launch=int(apply_bytes), avg_launch_ms=round(per["apply"], 4)),
     }
     print(json.dumps(line), flush=True)
@@ -462,7 +464,9 @@ def sceneio_bench(args, dev):
                    "pack_gbs": round(pack_bytes / (per["pack"] * 1e-3) / 1e9, 1),
                    "file_mb": round(size / 1e6, 1), "save_s": round(t1 - t0, 3), "load_s": round(t2 - t1, 3)},
         "roofline": dict(bound="hbm", achieved=round(ach, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                         frac=round(ach / HBM_PEAK_GBS, 5), traffic=None, kernel="k_scene_unpack",
+                         frac=round(ach / HBM_PEAK_GBS, 5),
+                         traffic=MEASURED_TRAFFIC["k_scene_unpack"] if P == 1_000_000 else None,
+                         kernel="k_scene_unpack",
                          bytes_per_launch=int(unpack_bytes), avg_launch_ms=round(per["unpack"], 4)),
     }
     print(json.dumps(line), flush=True)
