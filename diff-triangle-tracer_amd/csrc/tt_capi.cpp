@@ -42,6 +42,11 @@ struct tt_tracer {
   // backward gradient records: 2048 rows of 80 B per block of the one-shot backward
   float4 *rec_rows = nullptr;
   size_t rec_blocks = 0;  // rows allocated
+  // global grouping of the bounce variant's secondary-segment records: per record row a key
+  // and the radix-sort scratch (keys, values x 2, digit counts)
+  uint32_t *rec_keys = nullptr, *sort_keys2 = nullptr, *sort_vals = nullptr, *sort_vals2 = nullptr;
+  uint32_t *sort_hist = nullptr;
+  size_t sort_cap = 0;
 };
 
 namespace {
@@ -114,6 +119,36 @@ int prepare_spill_lanes(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
 }
 int prepare_spill(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
   return prepare_spill_lanes(tr, tt::trace_lanes(W, H), p);
+}
+
+void free_rec_sort(tt_tracer *tr) {
+  for (uint32_t **q : {&tr->rec_keys, &tr->sort_keys2, &tr->sort_vals, &tr->sort_vals2, &tr->sort_hist}) {
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+  }
+  tr->sort_cap = 0;
+}
+
+// Key and sort buffers for p.rec_cap record rows (the bounce backward's global grouping).
+int prepare_rec_sort(tt_tracer *tr, tt::TraceParams &p) {
+  const size_t n = (size_t)p.rec_cap;
+  if (n > tr->sort_cap) {
+    free_rec_sort(tr);
+    const size_t counts = 256 * ((n + 4095) / 4096), hist = counts + (counts + 4095) / 4096 + 1;
+    if (hipMalloc(&tr->rec_keys, n * 4) != hipSuccess || hipMalloc(&tr->sort_keys2, n * 4) != hipSuccess ||
+        hipMalloc(&tr->sort_vals, n * 4) != hipSuccess || hipMalloc(&tr->sort_vals2, n * 4) != hipSuccess ||
+        hipMalloc(&tr->sort_hist, hist * 4) != hipSuccess) {
+      free_rec_sort(tr);
+      return fail(TT_ERR_HIP, "hipMalloc of the record sort buffers failed");
+    }
+    tr->sort_cap = n;
+  }
+  p.rec_keys = tr->rec_keys;
+  p.sort_keys2 = tr->sort_keys2;
+  p.sort_vals = tr->sort_vals;
+  p.sort_vals2 = tr->sort_vals2;
+  p.sort_hist = tr->sort_hist;
+  return TT_OK;
 }
 
 void free_records(tt_tracer *tr) {
@@ -236,6 +271,7 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->tile_hits) (void)hipFree(tr->tile_hits);
   if (tr->rec_off) (void)hipFree(tr->rec_off);
   free_records(tr);
+  free_rec_sort(tr);
   (void)hipSetDevice(dev);
   delete tr;
   return ok();
@@ -448,6 +484,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   if ((rc = prepare_records(tr, tt::trace_lanes(a->width, a->height), p)) != TT_OK) return rc;
+  if (a->variant == TT_VARIANT_BOUNCE && (rc = prepare_rec_sort(tr, p)) != TT_OK) return rc;
   if (tr->sched_W == a->width && tr->sched_H == a->height) {  // the last forward's schedule and record regions
     p.tile_sched = tr->tile_order;
     p.tile_hits = tr->tile_hits;

@@ -124,6 +124,12 @@ struct TraceParams {
   const int *hl_count_in;
   size_t hl_stride;  // entries between hit i and hit i + 1 of a pixel (the launch's total rays)
   int rec_lane_rows;  // rows per launched lane of a fixed record region (no per-tile regions)
+  // global grouping of records (the bounce variant's secondary segments, whose rays are too
+  // incoherent for per-block grouping): the triangle of every record row (rows never
+  // written hold 0xffffffff) and the radix-sort scratch (rec_cap entries each, hist:
+  // 256 per 4096 rows + one per 4096 of those + 1); null: per-block grouping only
+  uint32_t *rec_keys;
+  uint32_t *sort_keys2, *sort_vals, *sort_vals2, *sort_hist;
 };
 // Per-pixel record of the bounce variant: L_r[3] L_t[3] R Q o_r[3] d_r[3] o_t[3] d_t[3]
 // (d_r = 0: no secondary rays).

@@ -56,6 +56,40 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t *data, int n) {
   }
 }
 
+// Exclusive scan of n counts in place over many blocks (the digit counts of a sort of tens
+// of millions of keys): kScanChunk counts per block scanned locally with the chunk total to
+// totals[block], the totals scanned by k_scan, then added back.
+constexpr int kScanChunk = 256 * 16;
+__global__ __launch_bounds__(256) void k_scan_local(uint32_t *data, int n, uint32_t *totals) {
+  __shared__ uint32_t s[256];
+  const int base = blockIdx.x * kScanChunk + threadIdx.x * 16;
+  uint32_t v[16], sum = 0;
+  for (int k = 0; k < 16; k++) {
+    v[k] = base + k < n ? data[base + k] : 0u;
+    sum += v[k];
+  }
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t x = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - sum;
+  for (int k = 0; k < 16; k++) {
+    if (base + k < n) data[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) totals[blockIdx.x] = s[255];
+}
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t *data, int n, const uint32_t *offs) {
+  const uint32_t o = offs[blockIdx.x];
+  const int base = blockIdx.x * kScanChunk;
+  for (int k = threadIdx.x; k < kScanChunk; k += 256)
+    if (base + k < n) data[base + k] += o;
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_scatter(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ vin, uint32_t *kout,
                                                            uint32_t *vout, int n, const int *n_dev, int shift,

@@ -4,6 +4,7 @@
 
 #include <string>
 
+#include "tt_sort.h"
 #include "tt_trace_dev.h"
 
 namespace tt {
@@ -218,9 +219,56 @@ std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream
   if (k != 8 && k != 16 && k != 32) return "k must be 8, 16 or 32";
   if (p.rec_rows) {
     if (variant == 4) {
-      if (k == 8) hipLaunchKernelGGL((k_backward_rec<8, 4>), g, bl, 0, stream, p);
-      if (k == 16) hipLaunchKernelGGL((k_backward_rec<16, 4>), g, bl, 0, stream, p);
-      if (k == 32) hipLaunchKernelGGL((k_backward_rec<32, 4>), g, bl, 0, stream, p);
+      // the primary segment (coherent rays) with per-block grouping of its records
+      TraceParams p1 = p;
+      p1.rec_keys = nullptr;
+      if (k == 8) hipLaunchKernelGGL((k_backward_rec<8, 5>), g, bl, 0, stream, p1);
+      if (k == 16) hipLaunchKernelGGL((k_backward_rec<16, 5>), g, bl, 0, stream, p1);
+      if (k == 32) hipLaunchKernelGGL((k_backward_rec<32, 5>), g, bl, 0, stream, p1);
+      if (p.rec_keys && p.sort_keys2 && p.sort_vals && p.sort_vals2 && p.sort_hist) {
+        // the secondary segments: their records (keys = triangle ids) are sorted by triangle
+        // over the whole launch and summed run by run
+        const int n = p.rec_cap;
+        hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, stream,
+                           reinterpret_cast<int *>(p.rec_keys), n, -1);
+        if (k == 8) hipLaunchKernelGGL((k_backward_rec<8, 8>), g, bl, 0, stream, p);
+        if (k == 16) hipLaunchKernelGGL((k_backward_rec<16, 8>), g, bl, 0, stream, p);
+        if (k == 32) hipLaunchKernelGGL((k_backward_rec<32, 8>), g, bl, 0, stream, p);
+        hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, stream, p.sort_vals, n);
+        const int sblk = (n + kSortTile - 1) / kSortTile;
+        int bits = 1;  // digits covering P, so that 0xffffffff (masked) sorts after every triangle id
+        while (bits < 32 && (1ll << bits) <= (long long)p.P) bits++;
+        uint32_t *kb[2] = {p.rec_keys, p.sort_keys2}, *vb[2] = {p.sort_vals, p.sort_vals2};
+        int cur = 0;
+        for (int shift = 0; shift < bits; shift += 8) {
+          hipLaunchKernelGGL(k_hist, dim3(sblk), dim3(kSortThreads), 0, stream, kb[cur], n, nullptr, shift,
+                             p.sort_hist, sblk);
+          const int nh = 256 * sblk, nc = (nh + kScanChunk - 1) / kScanChunk;
+          uint32_t *totals = p.sort_hist + nh;
+          hipLaunchKernelGGL(k_scan_local, dim3(nc), dim3(256), 0, stream, p.sort_hist, nh, totals);
+          hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, totals, nc);
+          hipLaunchKernelGGL(k_scan_add, dim3(nc), dim3(256), 0, stream, p.sort_hist, nh, totals);
+          hipLaunchKernelGGL(k_scatter, dim3(sblk), dim3(kSortThreads), 0, stream, kb[cur], vb[cur], kb[cur ^ 1],
+                             vb[cur ^ 1], n, nullptr, shift, p.sort_hist, sblk);
+          cur ^= 1;
+        }
+        const int chunks = (n + kRedChunk - 1) / kRedChunk;
+        hipLaunchKernelGGL(k_rec_reduce_sorted, dim3((chunks + 255) / 256), dim3(256), 0, stream, p, kb[cur],
+                           vb[cur], n);
+      } else {  // one launch per secondary segment with per-block grouping
+        if (k == 8) {
+          hipLaunchKernelGGL((k_backward_rec<8, 6>), g, bl, 0, stream, p1);
+          hipLaunchKernelGGL((k_backward_rec<8, 7>), g, bl, 0, stream, p1);
+        }
+        if (k == 16) {
+          hipLaunchKernelGGL((k_backward_rec<16, 6>), g, bl, 0, stream, p1);
+          hipLaunchKernelGGL((k_backward_rec<16, 7>), g, bl, 0, stream, p1);
+        }
+        if (k == 32) {
+          hipLaunchKernelGGL((k_backward_rec<32, 6>), g, bl, 0, stream, p1);
+          hipLaunchKernelGGL((k_backward_rec<32, 7>), g, bl, 0, stream, p1);
+        }
+      }
     } else {
       if (k == 8) hipLaunchKernelGGL((k_backward_rec<8, 0>), g, bl, 0, stream, p);
       if (k == 16) hipLaunchKernelGGL((k_backward_rec<16, 0>), g, bl, 0, stream, p);

@@ -26,6 +26,7 @@
 // hits, and the global FP32 atomics (which execute past the per-XCD L2) were
 // the backward's bound.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include "tt_device.h"
@@ -824,6 +825,7 @@ TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const
   base = __shfl(base, leader);
   const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
   if (idx < s_rec[2]) {
+    if (P.rec_keys) P.rec_keys[(size_t)s_rec[1] + idx] = prim;
     float4 *row = P.rec_rows + 5 * ((size_t)s_rec[1] + idx);
     const int cl = (c0 ? 1 : 0) | (c1 ? 2 : 0) | (c2 ? 4 : 0);
     row[0] = make_float4(__uint_as_float(prim), __int_as_float(cl), d.x, d.y);
@@ -1027,7 +1029,10 @@ TT_D void env_scatter(const TraceParams &P, v3f d, const float g[3]) {
 }
 
 // One pixel of the backward (backward.cu:191-391). VARIANT 4: the bounce variant's
-// primary, its environment term and its two secondary rays (upstream R g and Q g).
+// primary, its environment term and its two secondary rays (upstream R g and Q g);
+// VARIANT 5, 6, 7: only its primary (with the environment term), only the reflected, only
+// the refracted segment -- three launches whose sum is VARIANT 4, each with the register
+// footprint of one segment; VARIANT 8: both secondary segments (records grouped globally).
 template <int K, int BLOCK, int VARIANT, bool REC = false>
 TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, float *s_acc) {
   const size_t R = (size_t)P.R;
@@ -1060,6 +1065,9 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
     backward_seg<K, BLOCK, 0, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal,
                                    nullptr, nullptr, tlo, plo, far_t, s_node, s_keys, s_acc, hl, n_hl, P.hl_stride);
   } else {
+    constexpr bool kPrim = VARIANT == 4 || VARIANT == 5;
+    constexpr bool kRefl = VARIANT == 4 || VARIANT == 6 || VARIANT == 8;
+    constexpr bool kRefr = VARIANT == 4 || VARIANT == 7 || VARIANT == 8;
     const float *a = P.aux + (size_t)kBounceAux * pix;
     float Lr[3], Lt[3];
     for (int c = 0; c < 3; c++) Lr[c] = a[c], Lt[c] = a[3 + c];
@@ -1068,18 +1076,22 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
     int nhl[3] = {0, 0, 0};
     if (P.hl_in)
       for (int sg = 0; sg < 3; sg++) {
+        if (!(sg == 0 ? kPrim : (sg == 1 ? kRefl : kRefr))) continue;
         nhl[sg] = P.hl_count_in[sg * P.hl_stride + pix];
         if (nhl[sg] >= 0) hls[sg] = P.hl_in + (size_t)sg * P.hl_cap * P.hl_stride + pix;
       }
-    const float Tp = backward_seg<K, BLOCK, 1, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
-                                                    final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_keys, s_acc,
-                                                    hls[0], nhl[0], P.hl_stride);
-    const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
-    env_scatter(P, d, ge);
+    if (kPrim) {
+      const float Tp = backward_seg<K, BLOCK, 1, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2,
+                                                      final_depth, final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node,
+                                                      s_keys, s_acc, hls[0], nhl[0], P.hl_stride);
+      const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
+      env_scatter(P, d, ge);
+    }
     const v3f dr = mk(a[11], a[12], a[13]);
-    if (dr.x != 0.0f || dr.y != 0.0f || dr.z != 0.0f) {
+    if ((kRefl || kRefr) && (dr.x != 0.0f || dr.y != 0.0f || dr.z != 0.0f)) {
       const v3f zero = mk(0.0f, 0.0f, 0.0f);
-      for (int s = 0; s < 2; s++) {
+      auto secondary = [&](auto which) {  // which: 0 reflected, 1 refracted (a compile-time constant)
+        constexpr int s = decltype(which)::value;
         const float wgt = s == 0 ? Rw : Qw;
         const v3f so = s == 0 ? mk(a[8], a[9], a[10]) : mk(a[14], a[15], a[16]);
         const v3f sd = s == 0 ? dr : mk(a[17], a[18], a[19]);
@@ -1089,7 +1101,9 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
                                                         hls[1 + s], nhl[1 + s], P.hl_stride);
         const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
         env_scatter(P, sd, gse);
-      }
+      };
+      if (kRefl) secondary(std::integral_constant<int, 0>());
+      if (kRefr) secondary(std::integral_constant<int, 1>());
     }
   }
 }
@@ -1160,6 +1174,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   __syncthreads();
   if (valid) backward_ray<K, BLOCK, VARIANT, true>(P, pix, s_node, s_rec, nullptr);
   __syncthreads();
+  if (VARIANT == 8) return;  // records grouped globally (sorted by triangle) by the caller
   const int n_all = min(s_rec[0], s_rec[2]);
   const size_t start = (size_t)s_rec[1];
   short *slot_of = reinterpret_cast<short *>(s_node), *list = slot_of + kBatch;
@@ -1242,6 +1257,42 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
     }
   }
   __syncthreads();
+}
+
+// Sums of records sorted by triangle (keys) -- rows[j] is the record row of sorted entry j
+// -- with one thread per kRedChunk consecutive entries: each run of one triangle inside a
+// chunk is summed in registers and flushed once (FP32 atomics; a run crossing chunks is
+// flushed once per chunk). Entries with key 0xffffffff (rows never written) end the list.
+constexpr int kRedChunk = 64;
+__global__ __launch_bounds__(256) void k_rec_reduce_sorted(TraceParams P, const uint32_t *keys, const uint32_t *rows,
+                                                           int n) {
+  const long long b = ((long long)blockIdx.x * 256 + threadIdx.x) * kRedChunk;
+  if (b >= n) return;
+  const long long e = min((long long)n, b + kRedChunk);
+  uint32_t cur = 0xffffffffu;
+  float acc[11];
+  ShSink sh;
+  auto reset = [&]() {
+    for (int f = 0; f < 11; f++) acc[f] = 0.0f;
+    for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
+  };
+  reset();
+  for (long long j = b; j < e; j++) {
+    const uint32_t key = keys[j];
+    if (key == 0xffffffffu) break;
+    if (key != cur) {
+      if (cur != 0xffffffffu) flush_sums(P, cur, acc, sh);
+      reset();
+      cur = key;
+    }
+    add_record(P, rows[j], acc, sh);
+  }
+  if (cur != 0xffffffffu) flush_sums(P, cur, acc, sh);
+}
+
+__global__ void k_iota(uint32_t *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
 }
 
 __global__ void k_fill(float *p, size_t n, float v) {
