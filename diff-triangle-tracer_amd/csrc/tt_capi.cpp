@@ -129,9 +129,10 @@ void free_rec_sort(tt_tracer *tr) {
   tr->sort_cap = 0;
 }
 
-// Key and sort buffers for p.rec_cap record rows (the bounce backward's global grouping).
-int prepare_rec_sort(tt_tracer *tr, tt::TraceParams &p) {
-  const size_t n = (size_t)p.rec_cap;
+// Key and sort buffers for n entries: the bounce backward's global grouping of its p.rec_cap
+// record rows, the bounce forward's binning of its R secondary-ray pixels (shared: the two
+// never run at once on a tracer's stream).
+int prepare_rec_sort(tt_tracer *tr, tt::TraceParams &p, size_t n) {
   if (n > tr->sort_cap) {
     free_rec_sort(tr);
     const size_t counts = 256 * ((n + 4095) / 4096), hist = counts + (counts + 4095) / 4096 + 1;
@@ -393,6 +394,11 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
     p.aux = a->bounce_aux_dev;
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
+  if (a->variant == TT_VARIANT_BOUNCE && (long long)a->width * a->height > 0) {  // secondary-ray binning
+    if ((rc = prepare_rec_sort(tr, p, (size_t)a->width * a->height)) != TT_OK) return rc;
+    p.ray_keys = p.rec_keys;
+    p.rec_keys = nullptr;
+  }
   const int nt = tt::trace_tiles(a->width, a->height);
   if ((rc = ensure_tiles(tr, nt)) != TT_OK) return rc;
   p.tile_cost = tr->tile_cost;
@@ -484,7 +490,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
   if ((rc = prepare_records(tr, tt::trace_lanes(a->width, a->height), p)) != TT_OK) return rc;
-  if (a->variant == TT_VARIANT_BOUNCE && (rc = prepare_rec_sort(tr, p)) != TT_OK) return rc;
+  if (a->variant == TT_VARIANT_BOUNCE && (rc = prepare_rec_sort(tr, p, (size_t)p.rec_cap)) != TT_OK) return rc;
   if (tr->sched_W == a->width && tr->sched_H == a->height) {  // the last forward's schedule and record regions
     p.tile_sched = tr->tile_order;
     p.tile_hits = tr->tile_hits;

@@ -130,6 +130,10 @@ struct TraceParams {
   // 256 per 4096 rows + one per 4096 of those + 1); null: per-block grouping only
   uint32_t *rec_keys;
   uint32_t *sort_keys2, *sort_vals, *sort_vals2, *sort_hist;
+  // the bounce forward's secondary rays binned for coherence: one key per pixel (pixel
+  // region, reflected-direction bin, pixel; 0xffffffff: no secondary rays), sorted with the
+  // sort_* scratch (R entries each); null: all three segments in one launch
+  uint32_t *ray_keys;
 };
 // Per-pixel record of the bounce variant: L_r[3] L_t[3] R Q o_r[3] d_r[3] o_t[3] d_t[3]
 // (d_r = 0: no secondary rays).

@@ -37,6 +37,16 @@ __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   }
 }
 
+// The bounce variant's secondary segments, one lane per pixel in the order of the sorted
+// ray keys (pixels without secondary rays sort last and are skipped).
+template <int K>
+__global__ __launch_bounds__(kBlock, 3) void k_bounce_secondary(TraceParams P, const uint32_t *keys,
+                                                                const uint32_t *order) {
+  __shared__ int s_node[kLdsInts * kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < P.R && keys[i] != 0xffffffffu) bounce_secondary<K, kBlock>(P, (int)order[i], s_node);
+}
+
 // Exclusive prefix sums of the tiles' blended-hit counts (one block): the backward's
 // per-tile record regions.
 __global__ __launch_bounds__(1024) void k_tile_offsets(const int *hits, int n, int *off) {
@@ -147,7 +157,30 @@ hipError_t fwd_k(const TraceParams &p, int variant, hipStream_t s) {
       hipLaunchKernelGGL((k_forward<K, 3>), g, bl, 0, s, p);
       break;
     default:
-      hipLaunchKernelGGL((k_forward<K, 4>), g, bl, 0, s, p);
+      if (p.ray_keys && p.sort_keys2 && p.sort_vals && p.sort_vals2 && p.sort_hist && p.log_cap == 0) {
+        // primary segments, then the secondary rays binned by direction and pixel locality
+        hipLaunchKernelGGL((k_forward<K, 5>), g, bl, 0, s, p);
+        const int n = p.R;
+        const int sblk = (n + kSortTile - 1) / kSortTile;
+        hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, p.sort_vals, n);
+        uint32_t *kb[2] = {p.ray_keys, p.sort_keys2}, *vb[2] = {p.sort_vals, p.sort_vals2};
+        int cur = 0;
+        for (int shift = 0; shift < 32; shift += 8) {
+          hipLaunchKernelGGL(k_hist, dim3(sblk), dim3(kSortThreads), 0, s, kb[cur], n, nullptr, shift, p.sort_hist,
+                             sblk);
+          const int nh = 256 * sblk, nc = (nh + kScanChunk - 1) / kScanChunk;
+          uint32_t *totals = p.sort_hist + nh;
+          hipLaunchKernelGGL(k_scan_local, dim3(nc), dim3(256), 0, s, p.sort_hist, nh, totals);
+          hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, totals, nc);
+          hipLaunchKernelGGL(k_scan_add, dim3(nc), dim3(256), 0, s, p.sort_hist, nh, totals);
+          hipLaunchKernelGGL(k_scatter, dim3(sblk), dim3(kSortThreads), 0, s, kb[cur], vb[cur], kb[cur ^ 1],
+                             vb[cur ^ 1], n, nullptr, shift, p.sort_hist, sblk);
+          cur ^= 1;
+        }
+        hipLaunchKernelGGL((k_bounce_secondary<K>), dim3((n + kBlock - 1) / kBlock), bl, 0, s, p, kb[cur], vb[cur]);
+      } else {
+        hipLaunchKernelGGL((k_forward<K, 4>), g, bl, 0, s, p);
+      }
       break;
   }
   return hipGetLastError();

@@ -442,8 +442,34 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
   }
 }
 
+// Sort key of a bounce pixel's secondary rays: the pixel's 32 x 32 region (Morton order,
+// 12 bits), then the reflected direction's octahedral bin (5 + 5 bits), then the pixel's
+// Morton order inside the region (10 bits): a wave traces rays that start close together
+// and point in similar directions.
+TT_D uint32_t morton2(uint32_t x, uint32_t y, int bits) {
+  uint32_t m = 0;
+  for (int b = 0; b < bits; b++) m |= ((x >> b) & 1u) << (2 * b) | ((y >> b) & 1u) << (2 * b + 1);
+  return m;
+}
+TT_D uint32_t ray_bin_key(v3f dir, int x, int y) {
+  const float l1 = fabsf(dir.x) + fabsf(dir.y) + fabsf(dir.z);
+  float u = dir.x / l1, v = dir.y / l1;
+  if (dir.z < 0.0f) {
+    const float uu = (1.0f - fabsf(v)) * (u >= 0.0f ? 1.0f : -1.0f);
+    v = (1.0f - fabsf(u)) * (v >= 0.0f ? 1.0f : -1.0f);
+    u = uu;
+  }
+  const uint32_t qu = (uint32_t)min(31, max(0, (int)((u * 0.5f + 0.5f) * 32.0f)));
+  const uint32_t qv = (uint32_t)min(31, max(0, (int)((v * 0.5f + 0.5f) * 32.0f)));
+  const uint32_t region = morton2((uint32_t)min(x >> 5, 63), (uint32_t)min(y >> 5, 63), 6);
+  return region << 20 | morton2(qu, qv, 5) << 10 | morton2((uint32_t)(x & 31), (uint32_t)(y & 31), 5);
+}
+
 // One pixel of the forward programs (forward.cu:161-215 and the variants' raygens).
-// Returns the pixel's blended hits over all its traced segments.
+// Returns the pixel's blended hits over all its traced segments. VARIANT 5: the bounce
+// variant's primary segment only -- the colour buffer then holds the primary's composite
+// C and aux its transmittance (slot 5), R, Q and the two secondary rays, and ray_keys the
+// pixel's sort key (of its reflected ray), for bounce_secondary.
 template <int K, int BLOCK, int VARIANT, bool COUNT, bool IVAL = false>
 TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
   v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
@@ -460,6 +486,33 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
     if (IVAL && P.ival_out)
       P.ival_out[pix] = make_float4(st.t_first, st.t_last, st.t_term, __uint_as_float(st.p_first));
     if (P.hl_count) P.hl_count[pix] = st.blended <= P.hl_cap ? st.blended : -1;
+  } else if (VARIANT == 5) {
+    trace_segment<K, BLOCK, true, COUNT, true>(P, o, d, st, pix, o, s_node, 0.0f, kInfDist, 0);
+    const Bounce g = bounce_rays(o, d, st.T, st.N, st.D, st.eta);
+    if (P.hl_count) {
+      P.hl_count[pix] = st.blended <= P.hl_cap ? st.blended : -1;
+      P.hl_count[P.hl_stride + pix] = 0;
+      P.hl_count[2 * P.hl_stride + pix] = 0;
+    }
+    const size_t R = (size_t)P.R;
+    float *a = P.aux + (size_t)kBounceAux * pix;
+    const v3f zero = mk(0.0f, 0.0f, 0.0f);
+    const v3f orr = g.valid ? g.orr : zero, dr = g.valid ? g.dr : zero, ot = g.valid ? g.ot : zero,
+              dt = g.valid ? g.dt : zero;
+    const float rec[kBounceAux] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, st.T, st.R, st.Q, orr.x, orr.y,
+                                   orr.z, dr.x, dr.y, dr.z, ot.x, ot.y, ot.z, dt.x, dt.y, dt.z};
+    for (int i = 0; i < kBounceAux; i++) a[i] = rec[i];
+    if (g.valid) {
+      for (int c = 0; c < 3; c++) P.color[c * R + pix] = st.C[c];
+      P.ray_keys[pix] = ray_bin_key(g.dr, pix % P.W, pix / P.W);
+    } else {  // no secondary rays: the final colour now (Lr = Lt = 0), as VARIANT 4 forms it
+      float E0[3];
+      env_value(P.env, P.env_w, P.env_h, d, E0);
+      for (int c = 0; c < 3; c++)
+        P.color[c * R + pix] = ((st.C[c] + st.R * 0.0f) + st.Q * 0.0f) + st.T * E0[c];
+      a[5] = 0.0f;
+      P.ray_keys[pix] = 0xffffffffu;
+    }
   } else if (VARIANT == 4) {
     trace_segment<K, BLOCK, true, COUNT, true>(P, o, d, st, pix, o, s_node, 0.0f, kInfDist, 0);
     int nb[3] = {st.blended, 0, 0};  // blended hits of the primary, reflected and refracted segment
@@ -553,9 +606,9 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
     }
   }
   const size_t R = (size_t)P.R;
-  if (VARIANT != 4)
+  if (VARIANT != 4 && VARIANT != 5)
     for (int c = 0; c < 3; c++) P.color[c * R + pix] = st.C[c] + st.T * P.bg[c];
-  if (VARIANT == 0 || VARIANT == 4) {
+  if (VARIANT == 0 || VARIANT == 4 || VARIANT == 5) {
     P.others[0 * R + pix] = 0.0f;
     P.others[1 * R + pix] = 1.0f - st.T;
     P.others[2 * R + pix] = st.N[0];
@@ -570,6 +623,50 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
     P.stats[2 * (size_t)pix + 1] = st.blended;
   }
   return st.blended;
+}
+
+// The secondary segments of bounce pixel pix after its primary (VARIANT 5): the reflected
+// and refracted rays from aux, their radiance L = C' + T' E(d') into aux 0..5, the final
+// colour ((C + R L_r) + Q L_t) + T E(d) (the operation order of VARIANT 4), the hit counts
+// of both segments, and the statistics and the tile's blended hits added to.
+template <int K, int BLOCK>
+TT_D void bounce_secondary(const TraceParams &P, int pix, int *s_node) {
+  float *a = P.aux + (size_t)kBounceAux * pix;
+  const v3f orr = mk(a[8], a[9], a[10]), dr = mk(a[11], a[12], a[13]);
+  const v3f ot = mk(a[14], a[15], a[16]), dt = mk(a[17], a[18], a[19]);
+  float E[3], Lr[3], Lt[3];
+  SegState sr;
+  seg_init(sr);
+  trace_segment<K, BLOCK, false, false>(P, orr, dr, sr, pix, orr, s_node, 0.0f, kInfDist, 1);
+  env_value(P.env, P.env_w, P.env_h, dr, E);
+  for (int c = 0; c < 3; c++) Lr[c] = sr.C[c] + sr.T * E[c];
+  SegState stt;
+  seg_init(stt);
+  trace_segment<K, BLOCK, false, false>(P, ot, dt, stt, pix, ot, s_node, 0.0f, kInfDist, 2);
+  env_value(P.env, P.env_w, P.env_h, dt, E);
+  for (int c = 0; c < 3; c++) Lt[c] = stt.C[c] + stt.T * E[c];
+  if (P.hl_count) {
+    P.hl_count[P.hl_stride + pix] = sr.blended <= P.hl_cap ? sr.blended : -1;
+    P.hl_count[2 * P.hl_stride + pix] = stt.blended <= P.hl_cap ? stt.blended : -1;
+  }
+  const size_t R = (size_t)P.R;
+  const float T = a[5], Rw = a[6], Qw = a[7];
+  float E0[3];
+  env_value(P.env, P.env_w, P.env_h, ld3(P.ray_d + 3 * (size_t)pix), E0);
+  for (int c = 0; c < 3; c++) {
+    const float C = P.color[c * R + pix];
+    P.color[c * R + pix] = ((C + Rw * Lr[c]) + Qw * Lt[c]) + T * E0[c];
+  }
+  for (int c = 0; c < 3; c++) a[c] = Lr[c], a[3 + c] = Lt[c];
+  const int nb = sr.blended + stt.blended;
+  if (P.stats) {
+    P.stats[2 * (size_t)pix] += sr.consumed + stt.consumed;
+    P.stats[2 * (size_t)pix + 1] += nb;
+  }
+  if (P.tile_cost && P.tile_hits && nb > 0) {
+    const int x = pix % P.W, y = pix / P.W;
+    atomicAdd(P.tile_hits + (y / TileShape<BLOCK>::H) * P.tiles_x + x / TileShape<BLOCK>::W, nb);
+  }
 }
 
 // ------------------------------------------------------------------ backward
