@@ -106,3 +106,53 @@ def test_bounce_gradients_vs_oracle(case):
     rel = np.linalg.norm(g - r) / np.linalg.norm(r)
     _say(f"  dverts: {bad} of {r.size} entries outside the elementwise bound, relative L2 error {rel:.3g}")
     assert bad <= 1e-4 * r.size and rel <= 1e-4
+
+
+def test_two_pass_forward_equals_single_launch(cuda):
+    """The default bounce forward (primary pass, then the secondary rays binned by region and
+    direction) against the single launch the library uses when a debug hit log is asked
+    for: colour, others, the backward record (aux), statistics, hit lists and hit counts
+    bit-identical."""
+    import torch
+
+    sc = scenes.uniform_scene(20_000, 9, offset_std=0.08)
+    ro, rd = scenes.look_at_rays(96, 80, scenes.orbit_eye(3.0, 4), fov_deg=50.0)
+    rng = np.random.Generator(np.random.PCG64(5))
+    refl = rng.beta(2.0, 5.0, sc.P).astype(np.float32)
+    ior = rng.uniform(1.0, 1.6, sc.P).astype(np.float32)
+    env = np.exp(rng.normal(0.0, 0.5, (32, 64, 3))).astype(np.float32)
+    g = GpuScene(sc, cuda)
+    R = ro.shape[0] * ro.shape[1]
+    d = lambda a: to_dev(a, cuda)
+    s = torch.cuda.current_stream().cuda_stream
+    outs = []
+    for logged in (False, True):
+        B = dict(reflectivity=d(refl), ior=d(ior), envmap=d(env), aux=torch.zeros(R, 20, device=cuda))
+        color, others = torch.zeros(3, R, device=cuda), torch.zeros(7, R, device=cuda)
+        maxb, dens, scal = (torch.zeros(sc.P, device=cuda) for _ in range(3))
+        stats = torch.zeros(R, 2, dtype=torch.int32, device=cuda)
+        hl = capi.hitlist_buffers(R, 32, cuda, segments=3)
+        hl[0].fill_(-7)
+        log = None
+        if logged:
+            log = (torch.full((R, 8), -1, dtype=torch.int32, device=cuda), torch.zeros(R, 8, device=cuda),
+                   torch.full((R, 8), -1, dtype=torch.int32, device=cuda), torch.zeros(R, dtype=torch.int32, device=cuda))
+        _say(f"bounce forward, {'single launch (logged)' if logged else 'two passes'}")
+        a = capi.fwd_args(ro.shape[1], ro.shape[0], d(ro), d(rd), g.verts, g.npts, g.cumsum, g.shs, 3, g.opac,
+                          g.sigma, torch.zeros(3, device=cuda), color, others, maxb, dens, scal, k=16, cutoff=1e-3,
+                          chunk_mode="exact", variant="bounce", stats=stats, bounce=B, hitlist=hl, log=log)
+        g.h.fwd(a, s)
+        torch.cuda.synchronize()
+        counts = hl[1].cpu().numpy()
+        hits = hl[0].cpu().numpy()
+        # only the stored entries are defined: mask each pixel's list beyond its count
+        idx = np.arange(32)[None, :, None]
+        valid = idx < np.maximum(counts, 0)[:, None, :]
+        outs.append(dict(color=color.cpu().numpy(), others=others.cpu().numpy(), aux=B["aux"].cpu().numpy(),
+                         stats=stats.cpu().numpy(), counts=counts, hits=np.where(valid[..., None], hits, 0),
+                         maxb=maxb.cpu().numpy()))
+    a, b = outs
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint32) if a[k].dtype == np.float32 else a[k],
+                              b[k].view(np.uint32) if b[k].dtype == np.float32 else b[k]), k
+    assert (a["counts"][1:] > 0).any()
