@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
 // Backward through gradient records aggregated per block (no LDS atomics on the
 // gradient fields; 16 KB of LDS per block instead of 42 KB).
 template <int K, int VARIANT>
-__global__ __launch_bounds__(kBlock, 3) void k_backward_rec(TraceParams P) {
+__global__ __launch_bounds__(kBlock, 2) void k_backward_rec(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
   __shared__ int s_rec[kRecLds];
   __shared__ int s_tab[rec_tab_ints<kBlock>()];
