@@ -112,16 +112,6 @@ __global__ __launch_bounds__(1024) void k_tile_order(const float *cost, int n, i
   for (int i = t; i < n; i += 1024) order[atomicAdd(&hist[bucket(i)], 1u)] = i;
 }
 
-template <int K, int VARIANT = 0>
-__global__ __launch_bounds__(kBlock) void k_backward(TraceParams P) {
-  __shared__ int s_node[kLdsInts * kBlock];
-  __shared__ int s_keys[kBlock];
-  __shared__ float s_acc[kFields * kBlock];
-  int x, y;
-  const bool valid = tile_pixel<kBlock>(P, P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles), x, y);
-  backward_block<K, kBlock, VARIANT>(P, valid, y * P.W + x, s_node, s_keys, s_acc);
-}
-
 // Backward through gradient records aggregated per block (no LDS atomics on the
 // gradient fields; 16 KB of LDS per block instead of 42 KB).
 template <int K, int VARIANT>
@@ -307,14 +297,8 @@ std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream
       if (k == 16) hipLaunchKernelGGL((k_backward_rec<16, 0>), g, bl, 0, stream, p);
       if (k == 32) hipLaunchKernelGGL((k_backward_rec<32, 0>), g, bl, 0, stream, p);
     }
-  } else if (variant == 4) {
-    if (k == 8) hipLaunchKernelGGL((k_backward<8, 4>), g, bl, 0, stream, p);
-    if (k == 16) hipLaunchKernelGGL((k_backward<16, 4>), g, bl, 0, stream, p);
-    if (k == 32) hipLaunchKernelGGL((k_backward<32, 4>), g, bl, 0, stream, p);
   } else {
-    if (k == 8) hipLaunchKernelGGL(k_backward<8>, g, bl, 0, stream, p);
-    if (k == 16) hipLaunchKernelGGL(k_backward<16>, g, bl, 0, stream, p);
-    if (k == 32) hipLaunchKernelGGL(k_backward<32>, g, bl, 0, stream, p);
+    return "the backward needs its gradient-record rows";
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("backward launch failed: ") + hipGetErrorString(e);

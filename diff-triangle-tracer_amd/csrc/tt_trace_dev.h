@@ -728,8 +728,7 @@ TT_D void dne_dp3(v3f p1, v3f p2, v3f p3, float out[9]) {
   for (int i = 0; i < 9; i++) out[i] = m[i] - d[i];
 }
 
-// ---- per-block gradient accumulation
-constexpr int kFields = 59;  // 9 vertex coordinates, opacity, sigma, 16 x 3 SH coefficients
+// ---- gradient fields: 9 vertex coordinates, opacity, sigma, 16 x 3 SH coefficients
 constexpr int F_OPAC = 9, F_SIGMA = 10, F_SH = 11;
 
 // Slot of `prim` in the block's table of SLOTS entries (Fibonacci hash, linear
@@ -746,26 +745,6 @@ TT_D int claim_slot(int *keys, uint32_t prim) {
   }
   return -1;
 }
-
-// Where one blended hit's gradient goes: the LDS slot of its triangle, or (table
-// full) straight to the global buffers.
-struct GradSink {
-  float *acc;
-  int slot, stride;  // stride = slots per field
-  float *g_verts, *g_opac, *g_sigma, *g_sh;
-  TT_D void add(int f, float v) {
-    if (slot >= 0)
-      atomicAdd(&acc[f * stride + slot], v);
-    else if (f < 9)
-      unsafeAtomicAdd(g_verts + f, v);
-    else if (f == F_OPAC)
-      unsafeAtomicAdd(g_opac, v);
-    else if (f == F_SIGMA)
-      unsafeAtomicAdd(g_sigma, v);
-    else
-      unsafeAtomicAdd(g_sh + (f - F_SH), v);
-  }
-};
 
 // Register sinks: the 11 per-hit scalars (9 vertex coordinates, opacity, sigma) of a
 // gradient record, and the 48 SH sums of the record reduce.
@@ -1002,10 +981,10 @@ TT_D void flush_sums(const TraceParams &P, uint32_t prim, const float acc[11], c
 // bounce variant's primary (per-hit colours extended by rho L_r + (1 - rho) L_t, with the
 // reflectivity gradient); 2: a bounce secondary ray (colour only). Returns the segment's
 // final transmittance.
-template <int K, int BLOCK, int MODE, bool REC>
+template <int K, int BLOCK, int MODE>
 TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], const float g[3], float dL_ddepth,
                         v3f dL_dn2, float final_depth, v3f final_normal, const float *Lr, const float *Lt, float tlo,
-                        long long plo, float far_t, int *s_node, int *s_keys, float *s_acc,
+                        long long plo, float far_t, int *s_node, int *s_rec,
                         const uint2 *hl = nullptr, int n_hl = 0, size_t stride = 0) {
   float C[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f}, D = 0.0f, T = 1.0f;
   // one hit at or past the near plane; false: the ray terminates before blending it
@@ -1053,27 +1032,12 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
     const float dL_dC = opacity * dL_dalpha;
     const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
     const v3f dL_dn3 = MODE != 2 ? dL_dn2 * bw : mk(0.0f, 0.0f, 0.0f);
-    if (REC) {
-      HitSink hs;
-      for (int f = 0; f < 11; f++) hs.v[f] = 0.0f;
-      hs.v[F_OPAC] = dL_dalpha * w.Cx;
-      if (w.phi_final > 0.0f) hs.v[F_SIGMA] = (dL_dC * w.Cx) * log_tt(w.phi_final);
-      window_bwd(V, w, dL_dphi, dL_dn3, hs);
-      emit_record<BLOCK>(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_keys);
-    } else {
-      GradSink sink;
-      sink.acc = s_acc;
-      sink.stride = BLOCK;
-      sink.slot = claim_slot<BLOCK>(s_keys, prim);
-      sink.g_verts = P.dverts + 3 * (size_t)off;
-      sink.g_opac = P.dopac + prim;
-      sink.g_sigma = P.dsigma + prim;
-      sink.g_sh = P.dsh + (size_t)prim * P.M * 3;
-      sink.add(F_OPAC, dL_dalpha * w.Cx);
-      if (w.phi_final > 0.0f) sink.add(F_SIGMA, (dL_dC * w.Cx) * log_tt(w.phi_final));
-      window_bwd(V, w, dL_dphi, dL_dn3, sink);
-      sh_bwd(P.deg, d, c0, c1, c2, dL_dcolor, sink);
-    }
+    HitSink hs;
+    for (int f = 0; f < 11; f++) hs.v[f] = 0.0f;
+    hs.v[F_OPAC] = dL_dalpha * w.Cx;
+    if (w.phi_final > 0.0f) hs.v[F_SIGMA] = (dL_dC * w.Cx) * log_tt(w.phi_final);
+    window_bwd(V, w, dL_dphi, dL_dn3, hs);
+    emit_record<BLOCK>(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_rec);
     T = testT;
     return true;
   };
@@ -1130,8 +1094,8 @@ TT_D void env_scatter(const TraceParams &P, v3f d, const float g[3]) {
 // VARIANT 5, 6, 7: only its primary (with the environment term), only the reflected, only
 // the refracted segment -- three launches whose sum is VARIANT 4, each with the register
 // footprint of one segment; VARIANT 8: both secondary segments (records grouped globally).
-template <int K, int BLOCK, int VARIANT, bool REC = false>
-TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, float *s_acc) {
+template <int K, int BLOCK, int VARIANT>
+TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec) {
   const size_t R = (size_t)P.R;
   const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
   const float final_depth = P.fothers[pix];
@@ -1159,8 +1123,8 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
       plo = (long long)__float_as_uint(iv.w) - 1;
       far_t = iv.y;
     }
-    backward_seg<K, BLOCK, 0, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal,
-                                   nullptr, nullptr, tlo, plo, far_t, s_node, s_keys, s_acc, hl, n_hl, P.hl_stride);
+    backward_seg<K, BLOCK, 0>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal, nullptr,
+                              nullptr, tlo, plo, far_t, s_node, s_rec, hl, n_hl, P.hl_stride);
   } else {
     constexpr bool kPrim = VARIANT == 4 || VARIANT == 5;
     constexpr bool kRefl = VARIANT == 4 || VARIANT == 6 || VARIANT == 8;
@@ -1178,9 +1142,9 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
         if (nhl[sg] >= 0) hls[sg] = P.hl_in + (size_t)sg * P.hl_cap * P.hl_stride + pix;
       }
     if (kPrim) {
-      const float Tp = backward_seg<K, BLOCK, 1, REC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2,
-                                                      final_depth, final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node,
-                                                      s_keys, s_acc, hls[0], nhl[0], P.hl_stride);
+      const float Tp = backward_seg<K, BLOCK, 1>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
+                                                 final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_rec, hls[0],
+                                                 nhl[0], P.hl_stride);
       const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
       env_scatter(P, d, ge);
     }
@@ -1193,9 +1157,9 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
         const v3f so = s == 0 ? mk(a[8], a[9], a[10]) : mk(a[14], a[15], a[16]);
         const v3f sd = s == 0 ? dr : mk(a[17], a[18], a[19]);
         const float gs[3] = {wgt * dL_dpixel[0], wgt * dL_dpixel[1], wgt * dL_dpixel[2]};
-        const float Ts = backward_seg<K, BLOCK, 2, REC>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero,
-                                                        nullptr, nullptr, 0.0f, -1, P.far_t, s_node, s_keys, s_acc,
-                                                        hls[1 + s], nhl[1 + s], P.hl_stride);
+        const float Ts = backward_seg<K, BLOCK, 2>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
+                                                   nullptr, 0.0f, -1, P.far_t, s_node, s_rec, hls[1 + s], nhl[1 + s],
+                                                   P.hl_stride);
         const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
         env_scatter(P, sd, gse);
       };
@@ -1203,38 +1167,6 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_keys, 
       if (kRefr) secondary(std::integral_constant<int, 1>());
     }
   }
-}
-
-// One block's share of the backward: clear the LDS table, run the lane's ray (if
-// any), then scatter the table to the gradient buffers. Every lane of the block
-// must call this (it contains block barriers).
-template <int K, int BLOCK, int VARIANT = 0>
-TT_D void backward_block(const TraceParams &P, bool valid, int pix, int *s_node, int *s_keys, float *s_acc) {
-  for (int i = threadIdx.x; i < BLOCK; i += BLOCK) s_keys[i] = -1;
-  for (int i = threadIdx.x; i < kFields * BLOCK; i += BLOCK) s_acc[i] = 0.0f;
-  __syncthreads();
-  if (valid) backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_keys, s_acc);
-  __syncthreads();
-  const int nsh3 = 3 * (P.deg + 1) * (P.deg + 1);
-  const int s = threadIdx.x;
-  const int prim = s_keys[s];
-  if (prim >= 0) {
-    float *gv = P.dverts + 3 * (size_t)P.cumsum[prim];
-    for (int f = 0; f < 9; f++) {
-      const float v = s_acc[f * BLOCK + s];
-      if (v != 0.0f) unsafeAtomicAdd(gv + f, v);
-    }
-    float v = s_acc[F_OPAC * BLOCK + s];
-    if (v != 0.0f) unsafeAtomicAdd(P.dopac + prim, v);
-    v = s_acc[F_SIGMA * BLOCK + s];
-    if (v != 0.0f) unsafeAtomicAdd(P.dsigma + prim, v);
-    float *gs = P.dsh + (size_t)prim * P.M * 3;
-    for (int j = 0; j < nsh3; j++) {
-      v = s_acc[(F_SH + j) * BLOCK + s];
-      if (v != 0.0f) unsafeAtomicAdd(gs + j, v);
-    }
-  }
-  __syncthreads();
 }
 
 // The record path of one block's backward (tile `tile`): the lanes trace their rays
@@ -1269,7 +1201,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   }
   for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cnt[i] = 0, t_fill[i] = 0;
   __syncthreads();
-  if (valid) backward_ray<K, BLOCK, VARIANT, true>(P, pix, s_node, s_rec, nullptr);
+  if (valid) backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_rec);
   __syncthreads();
   if (VARIANT == 8) return;  // records grouped globally (sorted by triangle) by the caller
   const int n_all = min(s_rec[0], s_rec[2]);
