@@ -50,7 +50,7 @@ def _moments(optimizer, p):
 def prune_and_split(vertices, cumsum, triangle_params, max_blending, opacity, density=None, scaling=None, score=None,
                     optimizer=None, action_in=None, min_blending=0.0, min_opacity=0.0, split_scaling=float("inf"),
                     split_score=0.0, max_out=0):
-    """vertices: [V, 3] tensor (triangle i's points at rows cumsum[i] .. cumsum[i] + 2);
+    """vertices: [V, 3] (or flat [3 V]) tensor (triangle i's points at rows cumsum[i] .. cumsum[i] + 2);
     triangle_params: [P, ...] tensors (opacity, sigma, SH, ...). Every tensor's .data is
     replaced by the new set (vertices dense: [3 P_out, 3]); children copy the parent's
     per-triangle values; Adam moments (exp_avg, exp_avg_sq) in `optimizer`'s state are
@@ -62,9 +62,10 @@ def prune_and_split(vertices, cumsum, triangle_params, max_blending, opacity, de
                                  min_opacity, split_scaling, split_score, max_out)
     jobs = []  # (tensor owner, setter, src, width, rows, on_split, new shape)
     _check(vertices, "vertices")
-    jobs.append((vertices, None, vertices, 9, capi.ROWS_VERTEX, capi.SPLIT_SUBDIVIDE, (3 * P_out, 3)))
+    vshape = (3 * P_out, 3) if vertices.dim() == 2 else (9 * P_out,)  # [V, 3] or flat, as given
+    jobs.append((vertices, None, vertices, 9, capi.ROWS_VERTEX, capi.SPLIT_SUBDIVIDE, vshape))
     for k in _moments(optimizer, vertices):
-        jobs.append((vertices, k, optimizer.state[vertices][k], 9, capi.ROWS_VERTEX, capi.SPLIT_ZERO, (3 * P_out, 3)))
+        jobs.append((vertices, k, optimizer.state[vertices][k], 9, capi.ROWS_VERTEX, capi.SPLIT_ZERO, vshape))
     for i, t in enumerate(triangle_params):
         _check(t, f"triangle_params[{i}]")
         if t.shape[0] != P:
