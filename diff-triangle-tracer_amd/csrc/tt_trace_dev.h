@@ -118,7 +118,7 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
   const int lane = threadIdx.x;
   int *ov = P.ov_stack + ((size_t)blockIdx.x * BLOCK + lane) * P.ov_depth;
   int *s_leaf = s_node + kStack * BLOCK;
-  const float tlo_c = tlo - fabsf(tlo) * 1e-6f - 1e-7f;
+  const float lo_w = tlo - fabsf(tlo) * kWideRel - kWideAbs;
   int node = 0;
   int sp = 0;
   int nq = 0;  // queued leaves of this lane
@@ -136,16 +136,16 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
       const int4 ref = *reinterpret_cast<const int4 *>(nd + 6);
       const float thi = fminf(far_t, b.t[K - 1]);
-      const float thi_c = thi + fabsf(thi) * 1e-6f + 1e-7f;
+      const float hi_w = thi + fabsf(thi) * kWideRel + kWideAbs;
       float tn0, tf0, tn1, tf1, tn2, tf2, tn3, tf3;
-      slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn0, tf0);
-      slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn1, tf1);
-      slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn2, tf2);
-      slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn3, tf3);
-      bool h0 = (ref.x != kEmptyChild) & box_ok(tn0, tf0, tlo_c, thi_c);
-      bool h1 = (ref.y != kEmptyChild) & box_ok(tn1, tf1, tlo_c, thi_c);
-      bool h2 = (ref.z != kEmptyChild) & box_ok(tn2, tf2, tlo_c, thi_c);
-      bool h3 = (ref.w != kEmptyChild) & box_ok(tn3, tf3, tlo_c, thi_c);
+      slab_raw(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn0, tf0);
+      slab_raw(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn1, tf1);
+      slab_raw(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn2, tf2);
+      slab_raw(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn3, tf3);
+      bool h0 = (ref.x != kEmptyChild) & box_ok_wide(tn0, tf0, lo_w, hi_w);
+      bool h1 = (ref.y != kEmptyChild) & box_ok_wide(tn1, tf1, lo_w, hi_w);
+      bool h2 = (ref.z != kEmptyChild) & box_ok_wide(tn2, tf2, lo_w, hi_w);
+      bool h3 = (ref.w != kEmptyChild) & box_ok_wide(tn3, tf3, lo_w, hi_w);
       // hit leaves to the queue without branches: slots nq .. nq + 3 are free (nq <= kLeafQ - 4);
       // a write that is not counted is overwritten by the next one or lies past the queue's end
       const bool l0 = h0 & (ref.x < 0), l1 = h1 & (ref.y < 0), l2 = h2 & (ref.z < 0), l3 = h3 & (ref.w < 0);
