@@ -156,3 +156,50 @@ def test_two_pass_forward_equals_single_launch(cuda):
         assert np.array_equal(a[k].view(np.uint32) if a[k].dtype == np.float32 else a[k],
                               b[k].view(np.uint32) if b[k].dtype == np.float32 else b[k]), k
     assert (a["counts"][1:] > 0).any()
+
+
+def test_backward_without_a_forward_of_the_same_size(cuda):
+    """A bounce backward whose tracer last ran a forward of another image size (no per-tile
+    record regions: fixed rows per lane, also for the globally sorted secondary records)
+    gives the gradients of the regular one-shot pair up to FP32 summation order (the parity
+    bar, 1e-3 of max(|ref|, 1e-3 max|ref|): SH sums of many cancelling terms differ by a few
+    1e-4 between the two record layouts)."""
+    import torch
+
+    out, _, _, _, _, dpix, doth = run_case(cuda, 32)
+    sc = scenes.uniform_scene(40_000, 6, offset_std=0.08)
+    ro, rd = scenes.look_at_rays(144, 112, scenes.orbit_eye(3.0, 6), fov_deg=50.0)
+    rng = np.random.Generator(np.random.PCG64(3))
+    refl = rng.beta(2.0, 5.0, sc.P).astype(np.float32)
+    ior = rng.uniform(1.0, 1.6, sc.P).astype(np.float32)
+    env = np.exp(rng.normal(0.0, 0.5, (64, 128, 3))).astype(np.float32)
+    g = GpuScene(sc, cuda)
+    R = ro.shape[0] * ro.shape[1]
+    d = lambda a: to_dev(a, cuda)
+    s = torch.cuda.current_stream().cuda_stream
+    B = dict(reflectivity=d(refl), ior=d(ior), envmap=d(env), aux=torch.zeros(R, 20, device=cuda),
+             dreflectivity=torch.zeros(sc.P, device=cuda), denvmap=torch.zeros(64, 128, 3, device=cuda))
+    color, others = torch.zeros(3, R, device=cuda), torch.zeros(7, R, device=cuda)
+    maxb, dens, scal = (torch.zeros(sc.P, device=cuda) for _ in range(3))
+    hl = capi.hitlist_buffers(R, 32, cuda, segments=3)
+    a = capi.fwd_args(ro.shape[1], ro.shape[0], d(ro), d(rd), g.verts, g.npts, g.cumsum, g.shs, 3, g.opac, g.sigma,
+                      torch.zeros(3, device=cuda), color, others, maxb, dens, scal, k=16, cutoff=1e-3,
+                      chunk_mode="exact", variant="bounce", bounce=B, hitlist=hl)
+    g.h.fwd(a, s)
+    # a forward of another size on the same tracer drops the first one's record regions
+    o2, d2 = scenes.look_at_rays(40, 24, scenes.orbit_eye(3.0, 6), fov_deg=50.0)
+    g.forward(o2, d2, k=16, cutoff=1e-3, chunk_mode="exact")
+    dverts = torch.zeros(g.verts.shape[0], 3, device=cuda)
+    dsig, dop = torch.zeros(sc.P, device=cuda), torch.zeros(sc.P, device=cuda)
+    dsh = torch.zeros(sc.P, 16, 3, device=cuda)
+    _say("bounce backward with fixed record rows")
+    b = capi.bwd_args(ro.shape[1], ro.shape[0], d(ro), d(rd), g.verts, g.npts, g.cumsum, g.shs, 3, g.opac, g.sigma,
+                      color, others, d(dpix), d(doth), dverts, dsig, dop, dsh, k=16, cutoff=1e-3, chunk_mode="exact",
+                      zero_grads=True, bounce=B, hitlist=hl)
+    g.h.bwd(b, s)
+    torch.cuda.synchronize()
+    got = dict(dverts=dverts.cpu().numpy(), dsigma=dsig.cpu().numpy(), dopac=dop.cpu().numpy(), dsh=dsh.cpu().numpy(),
+               drefl=B["dreflectivity"].cpu().numpy(), denv=B["denvmap"].cpu().numpy())
+    for k in got:
+        ok, worst = grad_close(got[k], out[k], rel=1e-3)
+        assert ok, (k, worst)

@@ -228,3 +228,23 @@ def test_apply_unaligned_rows_take_the_scalar_path(cuda):
     torch.cuda.synchronize()
     assert np.array_equal(dst[1:].cpu().numpy().view(np.uint32), ref.view(np.uint32))
     assert dst[0].item() == -3.0
+
+
+def test_prune_and_split_keeps_a_flat_vertex_tensor_flat(cuda):
+    """The reference passes triangles_points flat ([9 P]); prune_and_split keeps that layout
+    and writes the same values as for a [3 P, 3] tensor."""
+    import torch
+    from diff_tritracer import densify
+
+    P = 2000
+    verts0, cumsum, _, act = _apply_case(P, 31)
+    rows = to_dev(verts0, cuda)
+    flat = to_dev(verts0.reshape(-1), cuda)
+    opac = [to_dev(np.ones((P, 1), np.float32), cuda) for _ in range(2)]
+    a = torch.from_numpy(act).to(cuda)
+    _say("prune_and_split, flat vs [V, 3] vertices")
+    densify.prune_and_split(rows, to_dev(cumsum, cuda), [opac[0]], None, None, action_in=a)
+    densify.prune_and_split(flat, to_dev(cumsum, cuda), [opac[1]], None, None, action_in=a)
+    torch.cuda.synchronize()
+    assert flat.dim() == 1 and rows.dim() == 2 and flat.numel() == rows.numel()
+    assert np.array_equal(flat.cpu().numpy(), rows.cpu().numpy().reshape(-1))
