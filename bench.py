@@ -52,6 +52,8 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/trace_fwd_bwd_1M_1080p.md; the factors come from a 256 MiB copy), and of the
 # densify and scene-file kernels at 1M splats (profiles/densify_sceneio_1M.md).
 MEASURED_TRAFFIC = {"k_forward<16,0,false,true>": 6.424e9, "k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
+# VALU busy (SQ_ACTIVE_INST_VALU x 4 / (SIMDs x cycles)) of the dominant kernel, same profile
+MEASURED_VALU_BUSY = {"k_forward<16,0,false,true>": 0.69}
 TRAIN_POSES, HELD_OUT_EXTRA = 16, 3
 
 
@@ -729,6 +731,10 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
                     frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic, kernel=dom,
                     bytes_per_launch=int(dom_bytes), avg_launch_ms=round(dom_ms, 4),
                     step_frac=round((bf + bb) / ((per["fwd"] + per["bwd"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5))
+    if default:  # what does bound the kernel (rocprofv3 counters of this workload)
+        roofline["valu_busy"] = MEASURED_VALU_BUSY.get(dom)
+        roofline["limiter"] = ("VALU issue and dependent BVH-node fetch latency at 3 waves/SIMD, not HBM "
+                               "(profiles/trace_fwd_bwd_1M_1080p.md)")
     mlp_flops = 2 * 14848 * R
     cfg = {
         "workload": f"{label}: fwd+bwd, {P} clustered triangle splats, one {W}x{H} camera (held out from the "
