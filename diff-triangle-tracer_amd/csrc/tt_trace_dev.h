@@ -1089,13 +1089,48 @@ TT_D void env_scatter(const TraceParams &P, v3f d, const float g[3]) {
         if (g[c] != 0.0f) unsafeAtomicAdd(P.denv + 3 * e.idx[j] + c, e.w[j] * g[c]);
 }
 
+// The environment gradient of the bounce primaries of one wave (all 64 lanes must call it,
+// `on` false for lanes without one): the four bilinear taps of each lane's direction d
+// get w g, summed over the lanes sharing a texel before one atomic per texel and channel
+// -- a wave of neighbouring primary rays looks up a handful of texels, so the per-lane
+// form's atomics collided on the same addresses.
+TT_D void env_scatter_wave(const TraceParams &P, v3f d, const float g[3], bool on) {
+  EnvTaps e{};
+  const bool any = on && (g[0] != 0.0f || g[1] != 0.0f || g[2] != 0.0f);
+  if (any) e = env_taps(d, P.env_w, P.env_h);
+  const int lane = threadIdx.x & 63;
+  for (int j = 0; j < 4; j++) {
+    const int idx = any && e.w[j] != 0.0f ? e.idx[j] : -1;
+    const float v0 = any ? e.w[j] * g[0] : 0.0f, v1 = any ? e.w[j] * g[1] : 0.0f, v2 = any ? e.w[j] * g[2] : 0.0f;
+    unsigned long long pending = __ballot(idx >= 0);
+    while (pending) {  // one texel per round: its lanes' values summed across the wave
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const int lidx = __shfl(idx, leader);
+      const bool mine = idx == lidx;
+      const unsigned long long peers = __ballot(mine);
+      float s0 = mine ? v0 : 0.0f, s1 = mine ? v1 : 0.0f, s2 = mine ? v2 : 0.0f;
+      for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+      }
+      if (lane == leader) {
+        if (s0 != 0.0f) unsafeAtomicAdd(P.denv + 3 * (size_t)lidx, s0);
+        if (s1 != 0.0f) unsafeAtomicAdd(P.denv + 3 * (size_t)lidx + 1, s1);
+        if (s2 != 0.0f) unsafeAtomicAdd(P.denv + 3 * (size_t)lidx + 2, s2);
+      }
+      pending &= ~peers;
+    }
+  }
+}
+
 // One pixel of the backward (backward.cu:191-391). VARIANT 4: the bounce variant's
 // primary, its environment term and its two secondary rays (upstream R g and Q g);
 // VARIANT 5, 6, 7: only its primary (with the environment term), only the reflected, only
 // the refracted segment -- three launches whose sum is VARIANT 4, each with the register
 // footprint of one segment; VARIANT 8: both secondary segments (records grouped globally).
 template <int K, int BLOCK, int VARIANT>
-TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec) {
+TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, float *env_g = nullptr) {
   const size_t R = (size_t)P.R;
   const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
   const float final_depth = P.fothers[pix];
@@ -1146,7 +1181,11 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec) {
                                                  final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_rec, hls[0],
                                                  nhl[0], P.hl_stride);
       const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
-      env_scatter(P, d, ge);
+      if (env_g) {  // the caller aggregates it across the wave (env_scatter_wave)
+        for (int c = 0; c < 3; c++) env_g[c] = ge[c];
+      } else {
+        env_scatter(P, d, ge);
+      }
     }
     const v3f dr = mk(a[11], a[12], a[13]);
     if ((kRefl || kRefr) && (dr.x != 0.0f || dr.y != 0.0f || dr.z != 0.0f)) {
@@ -1201,7 +1240,13 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   }
   for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cnt[i] = 0, t_fill[i] = 0;
   __syncthreads();
-  if (valid) backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_rec);
+  if (VARIANT == 5) {  // the bounce primaries: environment gradient aggregated per wave
+    float eg[3] = {0.0f, 0.0f, 0.0f};
+    if (valid) backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_rec, eg);
+    env_scatter_wave(P, valid ? ld3(P.ray_d + 3 * (size_t)pix) : mk(0.0f, 0.0f, 1.0f), eg, valid);
+  } else if (valid) {
+    backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_rec);
+  }
   __syncthreads();
   if (VARIANT == 8) return;  // records grouped globally (sorted by triangle) by the caller
   const int n_all = min(s_rec[0], s_rec[2]);
