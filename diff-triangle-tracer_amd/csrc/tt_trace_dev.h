@@ -325,7 +325,7 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
                         float tstart = 0.0f, float cap = kInfDist, int hseg = -1) {
   RayPre r = prep_ray(o, d);
   KBuf<K> b;
-  float last = 0.0f, tlo = IVAL ? tstart : 0.0f;
+  float last = 0.0f, tlo = tstart;
   long long plo = -1;
   bool capped = IVAL && cap < P.far_t && cap >= tstart;  // a cap before the start is no cap
   bool hard = IVAL && P.ival_hard && capped;
@@ -442,6 +442,12 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
   }
 }
 
+// Start of the bounce variant's secondary rays (this project's model): in exact chunk mode
+// they begin at the key (near_t, -1) -- hits before near_t would only be consumed and
+// skipped, so the blended result is the same and the gather does not collect them; in
+// compat mode at 0 (its chunk boundaries depend on every consumed hit).
+TT_D float bounce_start(const TraceParams &P) { return P.chunk_mode == 1 ? P.near_t : 0.0f; }
+
 // Sort key of a bounce pixel's secondary rays: the pixel's 32 x 32 region (Morton order,
 // 12 bits), then the reflected direction's octahedral bin (5 + 5 bits), then the pixel's
 // Morton order inside the region (10 bits): a wave traces rays that start close together
@@ -522,12 +528,12 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
       float E[3];
       SegState sr;
       seg_init(sr);
-      trace_segment<K, BLOCK, false, false>(P, g.orr, g.dr, sr, pix, g.orr, s_node, 0.0f, kInfDist, 1);
+      trace_segment<K, BLOCK, false, false>(P, g.orr, g.dr, sr, pix, g.orr, s_node, bounce_start(P), kInfDist, 1);
       env_value(P.env, P.env_w, P.env_h, g.dr, E);
       for (int c = 0; c < 3; c++) Lr[c] = sr.C[c] + sr.T * E[c];
       SegState stt;
       seg_init(stt);
-      trace_segment<K, BLOCK, false, false>(P, g.ot, g.dt, stt, pix, g.ot, s_node, 0.0f, kInfDist, 2);
+      trace_segment<K, BLOCK, false, false>(P, g.ot, g.dt, stt, pix, g.ot, s_node, bounce_start(P), kInfDist, 2);
       env_value(P.env, P.env_w, P.env_h, g.dt, E);
       for (int c = 0; c < 3; c++) Lt[c] = stt.C[c] + stt.T * E[c];
       st.consumed += sr.consumed + stt.consumed;
@@ -637,12 +643,12 @@ TT_D void bounce_secondary(const TraceParams &P, int pix, int *s_node) {
   float E[3], Lr[3], Lt[3];
   SegState sr;
   seg_init(sr);
-  trace_segment<K, BLOCK, false, false>(P, orr, dr, sr, pix, orr, s_node, 0.0f, kInfDist, 1);
+  trace_segment<K, BLOCK, false, false>(P, orr, dr, sr, pix, orr, s_node, bounce_start(P), kInfDist, 1);
   env_value(P.env, P.env_w, P.env_h, dr, E);
   for (int c = 0; c < 3; c++) Lr[c] = sr.C[c] + sr.T * E[c];
   SegState stt;
   seg_init(stt);
-  trace_segment<K, BLOCK, false, false>(P, ot, dt, stt, pix, ot, s_node, 0.0f, kInfDist, 2);
+  trace_segment<K, BLOCK, false, false>(P, ot, dt, stt, pix, ot, s_node, bounce_start(P), kInfDist, 2);
   env_value(P.env, P.env_w, P.env_h, dt, E);
   for (int c = 0; c < 3; c++) Lt[c] = stt.C[c] + stt.T * E[c];
   if (P.hl_count) {
@@ -1197,8 +1203,8 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
         const v3f sd = s == 0 ? dr : mk(a[17], a[18], a[19]);
         const float gs[3] = {wgt * dL_dpixel[0], wgt * dL_dpixel[1], wgt * dL_dpixel[2]};
         const float Ts = backward_seg<K, BLOCK, 2>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
-                                                   nullptr, 0.0f, -1, P.far_t, s_node, s_rec, hls[1 + s], nhl[1 + s],
-                                                   P.hl_stride);
+                                                   nullptr, bounce_start(P), -1, P.far_t, s_node, s_rec, hls[1 + s],
+                                                   nhl[1 + s], P.hl_stride);
         const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
         env_scatter(P, sd, gse);
       };

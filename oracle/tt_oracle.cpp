@@ -801,6 +801,13 @@ struct FwdSinks {
   V3f scaling_org;
 };
 
+// Start key of the bounce variant's secondary rays (this project's model, no reference
+// counterpart): in exact chunk mode they begin at (near_t, -1) -- the hits before near_t
+// would only be consumed and skipped, so the blended result is the same and the gather
+// does not collect them; in compat mode they begin at 0 (chunk boundaries there depend on
+// every consumed hit).
+inline float bounce_secondary_start(const Opts &op) { return op.mode == 1 ? op.near_t : 0.0f; }
+
 // The chunk loop of forward.cu:76-155 (FULL) and of the variants (colour only).
 // Returns with C, T, N updated; counts consumed/blended hits; optionally logs hits.
 void trace_segment_f32(const Scene &s, const Opts &op, V3f o, V3f d, float C[3], float &T, float N[3],
@@ -1208,11 +1215,12 @@ int or_forward_f32(void *scene, const float *o, const float *d, int R, const or_
       const BounceGeom g = bounce_geometry(ro, rd, T, N, acc);
       float Lr[3] = {0, 0, 0}, Lt[3] = {0, 0, 0};
       if (g.valid) {
+        const float t0 = bounce_secondary_start(op);
         float Cr[3] = {0, 0, 0}, Tr = 1.0f, Nr[3] = {0, 0, 0}, Er[3];
-        trace_segment_f32(s, op, g.orr, g.dr, Cr, Tr, Nr, consumed, blended, nullptr, nullptr);
+        trace_segment_f32(s, op, g.orr, g.dr, Cr, Tr, Nr, consumed, blended, nullptr, nullptr, t0);
         env_f32(op, g.dr, Er);
         float Ct[3] = {0, 0, 0}, Tt = 1.0f, Nt[3] = {0, 0, 0}, Et[3];
-        trace_segment_f32(s, op, g.ot, g.dt, Ct, Tt, Nt, consumed, blended, nullptr, nullptr);
+        trace_segment_f32(s, op, g.ot, g.dt, Ct, Tt, Nt, consumed, blended, nullptr, nullptr, t0);
         env_f32(op, g.dt, Et);
         for (int c = 0; c < 3; c++) Lr[c] = Cr[c] + Tr * Er[c], Lt[c] = Ct[c] + Tt * Et[c];
       }
@@ -1389,7 +1397,8 @@ void *or_record(void *scene, const float *o, const float *d, int R, const or_opt
           sb.pix = pix, sb.o = role == 1 ? g.orr : g.ot, sb.d = role == 1 ? g.dr : g.dt, sb.weight = 1.0;
           sb.role = role;
           float Cl[3] = {0, 0, 0}, Tl = 1.0f, Nb[3] = {0, 0, 0};
-          trace_segment_f32(s, op, sb.o, sb.d, Cl, Tl, Nb, consumed, blended, nullptr, &sb.hits);
+          trace_segment_f32(s, op, sb.o, sb.d, Cl, Tl, Nb, consumed, blended, nullptr, &sb.hits,
+                            bounce_secondary_start(op));
           per[pix].push_back(std::move(sb));
         }
       }
