@@ -32,6 +32,8 @@ struct tt_tracer {
   int *tile_order = nullptr;
   int *probe_order = nullptr;
   int *tile_hits = nullptr;  // blended hits per tile in the last forward
+  int *tile_hits_b = nullptr;    // the same folded into the backward's (16x16) tiles
+  float *tile_cost_b = nullptr;  // wall times folded likewise (the backward's order)
   int *rec_off = nullptr;    // their exclusive prefix sums: the backward's record regions
   int tile_cap = 0;
   int sched_W = -1, sched_H = -1;
@@ -185,11 +187,15 @@ int ensure_tiles(tt_tracer *tr, int nt) {
   if (tr->probe_order) (void)hipFree(tr->probe_order);
   if (tr->tile_hits) (void)hipFree(tr->tile_hits);
   if (tr->rec_off) (void)hipFree(tr->rec_off);
+  if (tr->tile_hits_b) (void)hipFree(tr->tile_hits_b);
+  if (tr->tile_cost_b) (void)hipFree(tr->tile_cost_b);
   tr->tile_cost = nullptr;
   tr->tile_order = nullptr;
   tr->probe_order = nullptr;
   tr->tile_hits = nullptr;
   tr->rec_off = nullptr;
+  tr->tile_hits_b = nullptr;
+  tr->tile_cost_b = nullptr;
   tr->tile_cap = 0;
   tr->sched_W = tr->sched_H = -1;
   tr->multi_tiles = -1;
@@ -197,7 +203,9 @@ int ensure_tiles(tt_tracer *tr, int nt) {
       hipMalloc(&tr->tile_order, (size_t)nt * sizeof(int)) != hipSuccess ||
       hipMalloc(&tr->probe_order, (size_t)nt * sizeof(int)) != hipSuccess ||
       hipMalloc(&tr->tile_hits, (size_t)nt * sizeof(int)) != hipSuccess ||
-      hipMalloc(&tr->rec_off, (size_t)nt * sizeof(int)) != hipSuccess)
+      hipMalloc(&tr->rec_off, (size_t)nt * sizeof(int)) != hipSuccess ||
+      hipMalloc(&tr->tile_hits_b, (size_t)nt * sizeof(int)) != hipSuccess ||
+      hipMalloc(&tr->tile_cost_b, (size_t)nt * sizeof(float)) != hipSuccess)
     return fail(TT_ERR_HIP, "hipMalloc of the tile schedule failed");
   tr->tile_cap = nt;
   return TT_OK;
@@ -271,6 +279,8 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->probe_order) (void)hipFree(tr->probe_order);
   if (tr->tile_hits) (void)hipFree(tr->tile_hits);
   if (tr->rec_off) (void)hipFree(tr->rec_off);
+  if (tr->tile_hits_b) (void)hipFree(tr->tile_hits_b);
+  if (tr->tile_cost_b) (void)hipFree(tr->tile_cost_b);
   free_records(tr);
   free_rec_sort(tr);
   (void)hipSetDevice(dev);
@@ -406,6 +416,8 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.probe_order = tr->probe_order;
   p.tile_hits = tr->tile_hits;
   p.rec_off_out = tr->rec_off;
+  p.tile_hits_b = tr->tile_hits_b;
+  p.tile_cost_b = tr->tile_cost_b;
   std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   tr->sched_W = a->width;
@@ -493,7 +505,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   if (a->variant == TT_VARIANT_BOUNCE && (rc = prepare_rec_sort(tr, p, (size_t)p.rec_cap)) != TT_OK) return rc;
   if (tr->sched_W == a->width && tr->sched_H == a->height) {  // the last forward's schedule and record regions
     p.tile_sched = tr->tile_order;
-    p.tile_hits = tr->tile_hits;
+    p.tile_hits = tr->tile_hits_b;  // the backward's tiles
     p.rec_off = tr->rec_off;
   }
   std::string err = tt::launch_backward(p, a->k, a->variant, s);

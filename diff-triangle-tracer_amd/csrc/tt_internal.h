@@ -113,6 +113,10 @@ struct TraceParams {
   int *tile_hits;
   const int *rec_off;
   int *rec_off_out;
+  // the backward tiles 16x16 (two forward tiles): the forward folds its tiles' blended hits
+  // and wall times into these (its rec_off_out and tile_order are then over backward tiles)
+  int *tile_hits_b;
+  float *tile_cost_b;
   // stored blended hits of the primary rays (variant 0, one-shot launches): the forward
   // writes hit i of pixel p as (prim, t bits) to hl[i * R + p] for i < hl_cap and the
   // count to hl_count[p] (-1: more than hl_cap); a backward given hl_in reads them
