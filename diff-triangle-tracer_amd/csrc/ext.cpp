@@ -30,7 +30,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
            pybind11::arg("dL_dothers"), pybind11::arg("scaling"), pybind11::arg("debug"),
            pybind11::arg("hits") = pybind11::none(), pybind11::arg("hit_counts") = pybind11::none())
       .def("set_options", &OptixTracer::set_options, pybind11::arg("k") = 16, pybind11::arg("cutoff") = 1e-4,
-           pybind11::arg("chunk_mode") = 0, pybind11::arg("hit_cache") = 32)
+           pybind11::arg("chunk_mode") = 0, pybind11::arg("hit_cache") = 64)
       .def("get_options", &OptixTracer::get_options)
       .def_property_readonly("variant", &OptixTracer::variant_name);
 

@@ -92,6 +92,6 @@ class OptixTracer {
   int k_ = 16;
   float cutoff_ = 1e-4f;
   int chunk_mode_ = TT_CHUNK_COMPAT;
-  int hit_cache_ = 32;
+  int hit_cache_ = 64;
   int device_ = -1;
 };

@@ -109,7 +109,7 @@ class Tracer(nn.Module):
         super().__init__()
         self.tracer = _C.OptixTracer(PACKAGE_DIR, forwad_ptx, backward_ptx)
 
-    def set_options(self, k: int = 16, cutoff: float = 1e-4, chunk_mode="compat", hit_cache: int = 32):
+    def set_options(self, k: int = 16, cutoff: float = 1e-4, chunk_mode="compat", hit_cache: int = 64):
         mode = {"compat": 0, "exact": 1}[chunk_mode] if isinstance(chunk_mode, str) else int(chunk_mode)
         self.tracer.set_options(int(k), float(cutoff), mode, int(hit_cache))
         return self

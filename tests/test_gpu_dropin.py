@@ -57,7 +57,7 @@ def _run(cuda, sc, ro, rd, dpix, doth, hit_cache):
                 dsigma=sigma.grad.cpu().numpy(), dopac=opac.grad.cpu().numpy(), dsh=shs.grad.cpu().numpy())
 
 
-@pytest.mark.parametrize("hit_cache", [0, 32])
+@pytest.mark.parametrize("hit_cache", [0, 32, 64])
 def test_dropin_autograd_vs_oracle(cuda, case, hit_cache):
     sc, ro, rd, log = case
     R = ro.shape[0] * ro.shape[1]
