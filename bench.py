@@ -51,9 +51,9 @@ HBM_PEAK_GBS = 8000.0
 # k=16), 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE from rocprofv3 counter runs of this script
 # (profiles/trace_fwd_bwd_1M_1080p.md; the factors come from a 256 MiB copy), and of the
 # densify and scene-file kernels at 1M splats (profiles/densify_sceneio_1M.md).
-MEASURED_TRAFFIC = {"k_forward<16,0,false,true>": 6.424e9, "k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
+MEASURED_TRAFFIC = {"k_forward<16,0,false,true>": 6.560e9, "k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
 # VALU busy (SQ_ACTIVE_INST_VALU x 4 / (SIMDs x cycles)) of the dominant kernel, same profile
-MEASURED_VALU_BUSY = {"k_forward<16,0,false,true>": 0.69}
+MEASURED_VALU_BUSY = {"k_forward<16,0,false,true>": 0.67}
 TRAIN_POSES, HELD_OUT_EXTRA = 16, 3
 
 
