@@ -448,10 +448,10 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
 // compat mode at 0 (its chunk boundaries depend on every consumed hit).
 TT_D float bounce_start(const TraceParams &P) { return P.chunk_mode == 1 ? P.near_t : 0.0f; }
 
-// Sort key of a bounce pixel's secondary rays: the pixel's 32 x 32 region (Morton order,
-// 12 bits), then the reflected direction's octahedral bin (5 + 5 bits), then the pixel's
-// Morton order inside the region (10 bits): a wave traces rays that start close together
-// and point in similar directions.
+// Sort key of a bounce pixel's secondary rays: the pixel's 128 x 128 region (Morton order,
+// 8 bits), then the reflected direction's octahedral bin (6 + 6 bits), then the pixel's
+// Morton order inside the region at 2-pixel granularity (12 bits): a wave traces rays that
+// start close together and point in similar directions.
 TT_D uint32_t morton2(uint32_t x, uint32_t y, int bits) {
   uint32_t m = 0;
   for (int b = 0; b < bits; b++) m |= ((x >> b) & 1u) << (2 * b) | ((y >> b) & 1u) << (2 * b + 1);
@@ -465,10 +465,10 @@ TT_D uint32_t ray_bin_key(v3f dir, int x, int y) {
     v = (1.0f - fabsf(u)) * (v >= 0.0f ? 1.0f : -1.0f);
     u = uu;
   }
-  const uint32_t qu = (uint32_t)min(31, max(0, (int)((u * 0.5f + 0.5f) * 32.0f)));
-  const uint32_t qv = (uint32_t)min(31, max(0, (int)((v * 0.5f + 0.5f) * 32.0f)));
-  const uint32_t region = morton2((uint32_t)min(x >> 5, 63), (uint32_t)min(y >> 5, 63), 6);
-  return region << 20 | morton2(qu, qv, 5) << 10 | morton2((uint32_t)(x & 31), (uint32_t)(y & 31), 5);
+  const uint32_t qu = (uint32_t)min(63, max(0, (int)((u * 0.5f + 0.5f) * 64.0f)));
+  const uint32_t qv = (uint32_t)min(63, max(0, (int)((v * 0.5f + 0.5f) * 64.0f)));
+  const uint32_t region = morton2((uint32_t)min(x >> 7, 15), (uint32_t)min(y >> 7, 15), 4);
+  return region << 24 | morton2(qu, qv, 6) << 12 | morton2((uint32_t)((x & 127) >> 1), (uint32_t)((y & 127) >> 1), 6);
 }
 
 // One pixel of the forward programs (forward.cu:161-215 and the variants' raygens).
