@@ -468,7 +468,9 @@ TT_D uint32_t ray_bin_key(v3f dir, int x, int y) {
   const uint32_t qu = (uint32_t)min(63, max(0, (int)((u * 0.5f + 0.5f) * 64.0f)));
   const uint32_t qv = (uint32_t)min(63, max(0, (int)((v * 0.5f + 0.5f) * 64.0f)));
   const uint32_t region = morton2((uint32_t)min(x >> 7, 15), (uint32_t)min(y >> 7, 15), 4);
-  return region << 24 | morton2(qu, qv, 6) << 12 | morton2((uint32_t)((x & 127) >> 1), (uint32_t)((y & 127) >> 1), 6);
+  const uint32_t key =
+      region << 24 | morton2(qu, qv, 6) << 12 | morton2((uint32_t)((x & 127) >> 1), (uint32_t)((y & 127) >> 1), 6);
+  return min(key, 0xfffffffeu);  // 0xffffffff marks pixels without secondary rays
 }
 
 // One pixel of the forward programs (forward.cu:161-215 and the variants' raygens).
