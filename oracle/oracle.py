@@ -78,6 +78,7 @@ def lib():
         L.or_sincosf.argtypes = [C.c_float, fp, fp]
         L.or_intersect.restype = C.c_int
         L.or_intersect.argtypes = [fp, fp, fp, fp, fp, fp]
+        L.or_cos_hemi.argtypes = [C.c_uint, fp, C.c_int, fp]
         L.or_window_f32.argtypes = [fp, fp, C.c_float, fp]
         L.or_window_f64.argtypes = [dp, dp, C.c_double, dp]
         L.or_sh_f32.argtypes = [fp, C.c_int, fp, fp, ip]
@@ -159,6 +160,16 @@ def sincosf(x):
     s, c = C.c_float(), C.c_float()
     lib().or_sincosf(float(x), C.byref(s), C.byref(c))
     return np.float32(s.value), np.float32(c.value)
+
+
+def cos_hemi(seed, normal, count):
+    """`count` successive envlight hemisphere directions (envlight.cu:25-51) around the unit
+    `normal`, drawn from one LCG state `seed`: float32 [count, 3]."""
+    n = _f32(normal)
+    out = np.zeros((int(count), 3), np.float32)
+    lib().or_cos_hemi(int(seed) & 0xFFFFFFFF, n.ctypes.data_as(C.POINTER(C.c_float)), int(count),
+                      out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
 
 
 def intersect(o, d, v0, v1, v2):

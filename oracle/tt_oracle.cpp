@@ -1120,6 +1120,15 @@ int or_intersect(const float *o, const float *d, const float *v0, const float *v
   return intersect(r, ld3<float>(v0), ld3<float>(v1), ld3<float>(v2), *t) ? 1 : 0;
 }
 
+void or_cos_hemi(unsigned int seed, const float *nrm, int count, float *out) {
+  uint32_t u = seed;
+  const V3f n = ld3<float>(nrm);
+  for (int i = 0; i < count; i++) {
+    const V3f w = cos_hemi(u, n);
+    out[3 * i] = w.x, out[3 * i + 1] = w.y, out[3 * i + 2] = w.z;
+  }
+}
+
 void or_window_f32(const float *verts, const float *hit, float sigma, float *out) {
   V3f V[3] = {ld3<float>(verts), ld3<float>(verts + 3), ld3<float>(verts + 6)};
   Win<float> w;

@@ -28,6 +28,12 @@
  *   loss : the f64 replay as a scalar loss with hit distance t and incenter
  *          distance frozen -- the surrogate whose finite differences pin the
  *          reference's partial vertex gradient.
+ *
+ * Parity unpinned: the refract direction (refract.cu:34-35, formula kept with
+ * its quirks) at oblique incidence; only its straight-through case has an
+ * independent closed form. Everything else here is pinned by known answers
+ * (closed forms, libm in double, brute force, finite differences, the LCG's
+ * published sequence).
  */
 #pragma once
 #include <stdint.h>
@@ -71,6 +77,9 @@ float or_powf(float x, float y);
 void or_sincosf(float x, float *s, float *c);
 /* watertight intersection; returns 1 on hit and writes t */
 int or_intersect(const float *o, const float *d, const float *v0, const float *v1, const float *v2, float *t);
+/* envlight hemisphere samples: `count` successive cosine-weighted directions around the unit
+   normal nrm[3] from one LCG state `seed` -> out[3 * count] */
+void or_cos_hemi(unsigned int seed, const float *nrm, int count, float *out);
 /* window function: verts[9], hit[3] -> out[0]=Cx out[1]=phi_final out[2]=phi_max out[3..5]=edge dists
    out[6..8]=face normal out[9]=incenter dist */
 void or_window_f32(const float *verts, const float *hit, float sigma, float *out);
