@@ -127,14 +127,37 @@ __global__ __launch_bounds__(64) void k_prop_infer(const float *ro, const float 
   for (int sub = 0; sub < kInferTiles; sub++) {
     const long long base = ((long long)blockIdx.x * kInferTiles + sub) * 16;
     if (base >= R) break;  // uniform across the block
-    {  // encoding: lane l fills features 32*(l/16) .. +31 of ray l%16
+    {  // encoding: lane l fills features 32 g .. 32 g + 31 of ray l % 16, g = l / 16 -- feature()
+       // unrolled: feature 32 g + i is raw input i (g = 0, i < 6) or a sinusoid of input
+       // 2g - 1 (i < 6), 2g (6 <= i < 22) or 2g + 1 (i >= 22), octave ((i + 10) & 15) / 2,
+       // cos for odd i; the three inputs are picked once per lane, the row stored 16 B at a time
       const long long ray = base + (l & 15);
+      const int g = l >> 4;
       float v[6] = {0, 0, 0, 0, 0, 0};
       if (ray < R) {
         const float inv = 1.0f / hp.enc_scale;
         for (int c = 0; c < 3; c++) v[c] = ro[3 * ray + c] * inv, v[3 + c] = rd[3 * ray + c];
       }
-      for (int e = 32 * (l >> 4); e < 32 * (l >> 4) + 32; e++) X[(l & 15) * SX + e] = (bf16)(ray < R ? feature(v, e) : 0.0f);
+      float vd[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int dim = 2 * g - 1 + k;
+        float x = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 6; c++) x = dim == c ? v[c] : x;
+        vd[k] = x;
+      }
+      bf16x8 pk[4];
+#pragma unroll
+      for (int i = 0; i < 32; i++) {
+        const float u = ldexpf(vd[i < 6 ? 0 : (i < 22 ? 1 : 2)], (((i + 10) & 15) >> 1) - 1);
+        float val = (i & 1) ? __builtin_amdgcn_cosf(u) : __builtin_amdgcn_sinf(u);
+        if (i < 6 && g == 0) val = v[i];
+        if (32 * g + i >= kIn || ray >= R) val = 0.0f;
+        pk[i >> 3][i & 7] = (bf16)val;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) *reinterpret_cast<bf16x8 *>(X + (l & 15) * SX + 32 * g + 8 * j) = pk[j];
     }
     __syncthreads();
     f32x4 acc[4];
