@@ -49,6 +49,14 @@ struct tt_tracer {
   uint32_t *rec_keys = nullptr, *sort_keys2 = nullptr, *sort_vals = nullptr, *sort_vals2 = nullptr;
   uint32_t *sort_hist = nullptr;
   size_t sort_cap = 0;
+  // State generations. A captured graph holds the pointers, stack depth and tile schedule of
+  // the moment it was recorded: mem_gen changes with every TT_BUILD (new nodes, possibly a
+  // deeper tree) and every (re)allocation of the tracer's scratch; sched_gen with every eager
+  // launch that rewrites the tile schedule for a different tile layout. A replay checks both
+  // (the schedule only for a backward-only graph, whose order comes from outside the graph).
+  unsigned long long mem_gen = 1, sched_gen = 1;
+  unsigned long long graph_mem_gen = 0, graph_sched_gen = 0;
+  int graph_what = 0;
 };
 
 namespace {
@@ -115,6 +123,7 @@ int prepare_spill_lanes(tt_tracer *tr, size_t lanes, tt::TraceParams &p) {
     if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("traversal stack spill allocation failed: ") +
                                                      hipGetErrorString(e));
     tr->ov_cap = need;
+    tr->mem_gen++;
   }
   p.ov_stack = tr->ov_stack;
   return TT_OK;
@@ -145,6 +154,7 @@ int prepare_rec_sort(tt_tracer *tr, tt::TraceParams &p, size_t n) {
       return fail(TT_ERR_HIP, "hipMalloc of the record sort buffers failed");
     }
     tr->sort_cap = n;
+    tr->mem_gen++;
   }
   p.rec_keys = tr->rec_keys;
   p.sort_keys2 = tr->sort_keys2;
@@ -172,6 +182,7 @@ int prepare_records(tt_tracer *tr, size_t lanes, tt::TraceParams &p, int lane_ro
       return fail(TT_ERR_HIP, "hipMalloc of the gradient records failed");
     }
     tr->rec_blocks = rows;
+    tr->mem_gen++;
   }
   p.rec_rows = tr->rec_rows;
   p.rec_cap = (int)rows;
@@ -182,6 +193,7 @@ int prepare_records(tt_tracer *tr, size_t lanes, tt::TraceParams &p, int lane_ro
 // Tile schedule buffers (cost, orders, blended hits, record offsets) for nt tiles.
 int ensure_tiles(tt_tracer *tr, int nt) {
   if (nt <= tr->tile_cap) return TT_OK;
+  tr->mem_gen++;
   if (tr->tile_cost) (void)hipFree(tr->tile_cost);
   if (tr->tile_order) (void)hipFree(tr->tile_order);
   if (tr->probe_order) (void)hipFree(tr->probe_order);
@@ -304,6 +316,7 @@ int tt_build(tt_tracer *tr, const float *vertices_dev, int num_vertices, const i
       return fail(TT_ERR_HIP, "hipEventCreate failed");
   }
   (void)hipEventRecord(tr->ev0, s);
+  if (mode == TT_BUILD) tr->mem_gen++;  // new nodes and tree height: captured graphs are stale
   std::string err = tt::bvh_build(tr->bvh, vertices_dev, num_vertices, cumsum_dev, P, mode, s);
   if (!err.empty()) {
     tr->build_timed = false;
@@ -423,6 +436,7 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   tr->sched_W = a->width;
   tr->sched_H = a->height;
   tr->multi_tiles = -1;
+  tr->sched_gen++;  // tile_order now holds the one-shot tiles of this image
   return ok();
 }
 
@@ -585,6 +599,7 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
   if (!tr->heads) {
     hipError_t e = hipMalloc(&tr->heads, 8 * sizeof(int));
     if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("queue allocation failed: ") + hipGetErrorString(e));
+    tr->mem_gen++;
   }
   if (a->hitlist_dev || a->hitlist_count_dev) {
     TT_ARG(a->hitlist_dev && a->hitlist_count_dev && a->hitlist_cap > 0,
@@ -606,6 +621,7 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
     p.tile_order = tr->tile_order;
     p.probe_order = tr->probe_order;
     tr->sched_W = tr->sched_H = -1;
+    if (tr->multi_tiles != tiles || tr->multi_block != a->block || tr->multi_rays != rays) tr->sched_gen++;
     tr->multi_tiles = tiles;
     tr->multi_block = a->block;
     tr->multi_rays = rays;
@@ -691,13 +707,25 @@ int tt_multi_capture(tt_tracer *tr, const tt_multi_args *a, int what, void *stre
   tr->graph = nullptr;
   e = hipGraphInstantiate(&tr->graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    tr->graph = nullptr;
+    return fail(TT_ERR_HIP, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(e));
+  }
+  tr->graph_mem_gen = tr->mem_gen;
+  tr->graph_sched_gen = tr->sched_gen;
+  tr->graph_what = what;
   return ok();
 }
 
 int tt_multi_replay(tt_tracer *tr, void *stream) {
   TT_ARG(tr, "tracer is NULL");
   if (!tr->graph) return fail(TT_ERR_STATE, "no graph has been captured");
+  if (tr->graph_mem_gen != tr->mem_gen)
+    return fail(TT_ERR_STATE, "the captured graph is stale: the acceleration structure was rebuilt or the tracer's "
+                              "scratch buffers were reallocated since tt_multi_capture; capture again");
+  if (!(tr->graph_what & 1) && tr->graph_sched_gen != tr->sched_gen)
+    return fail(TT_ERR_STATE, "the captured backward-only graph is stale: a forward of another tile layout rewrote "
+                              "the tile schedule since tt_multi_capture; capture again");
   hipError_t e = hipGraphLaunch(tr->graph, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("hipGraphLaunch failed: ") + hipGetErrorString(e));
   return ok();

@@ -244,9 +244,15 @@ int tt_multi_forward(tt_tracer *tr, const tt_multi_args *a, void *stream);
 /* Backward of every camera in ONE persistent launch; gradients of all cameras accumulate. */
 int tt_multi_backward(tt_tracer *tr, const tt_multi_args *a, void *stream);
 /* Record one training step into a hipGraph owned by the tracer: what = 1 forward, 2 backward
-   (with zeroing when a->zero_grads), 3 both. Buffers must stay valid while the graph is used. */
+   (with zeroing when a->zero_grads), 3 both. Buffers must stay valid while the graph is used.
+   The graph holds the tracer's state of the moment it was recorded: after tt_build(TT_BUILD)
+   (a rebuild, even of the same triangles) or a call that needed larger tracer scratch (a
+   bigger one-shot image, another block size), capture again. tt_build(TT_REFIT) keeps the
+   graph valid (same nodes, updated boxes). */
 int tt_multi_capture(tt_tracer *tr, const tt_multi_args *a, int what, void *stream);
-/* Launch the captured graph on `stream`. */
+/* Launch the captured graph on `stream`. TT_ERR_STATE (nothing launched) when the graph is
+   stale as described above, or, for a backward-only graph, when a forward of another tile
+   layout (any tt_trace_fwd, a tt_multi_forward of another batch) rewrote the tile schedule. */
 int tt_multi_replay(tt_tracer *tr, void *stream);
 /* Autotuner: times forward+backward for every (k, block, tiles_per_fetch) in the given lists
    (k is swept only in TT_CHUNK_EXACT mode, where results do not depend on it), writes the

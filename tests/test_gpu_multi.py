@@ -159,3 +159,41 @@ def test_autotuner_picks_a_measured_configuration(mv, ref, cuda):
     tr.multi_bwd(a2, s)
     torch.cuda.synchronize()
     _check({k_: v.cpu().numpy() for k_, v in B2.items()}, ref)
+
+
+def test_graph_goes_stale_after_rebuild(mv, ref, cuda):
+    """A graph records the tracer's nodes, stack depth and schedule: a refit keeps it valid,
+    a rebuild or a forward of another tile layout (backward-only graph) makes the replay fail
+    with TT_ERR_STATE before anything is launched."""
+    import torch
+
+    sc, w, h, S, tr, Rt = mv
+    s = torch.cuda.current_stream().cuda_stream
+    B = _bufs(torch, cuda, sc.P, Rt, 16)
+    a = _multi_args(mv, B)
+    _say("graph staleness: capture, refit, replay")
+    tr.multi_capture(a, 3, s)
+    tr.build(S["verts"], S["cumsum"], sc.P, capi.REFIT, s)
+    B["maxb"].zero_()
+    tr.multi_replay(s)
+    torch.cuda.synchronize()
+    _check({k_: v.cpu().numpy() for k_, v in B.items() if not k_.startswith("_")}, ref)
+    _say("graph staleness: rebuild, replay must fail")
+    tr.build(S["verts"], S["cumsum"], sc.P, capi.BUILD, s)
+    with pytest.raises(RuntimeError, match="stale"):
+        tr.multi_replay(s)
+    _say("graph staleness: backward-only graph, one-shot forward in between")
+    tr.multi_fwd(a, s)
+    tr.multi_capture(a, 2, s)
+    tr.multi_replay(s)
+    torch.cuda.synchronize()
+    W0, H0 = w[0], h[0]
+    col0 = torch.zeros(3 * W0 * H0, device=cuda)
+    oth0 = torch.zeros(7 * W0 * H0, device=cuda)
+    fa = capi.fwd_args(W0, H0, S["ro"][:W0 * H0], S["rd"][:W0 * H0], S["verts"], S["npts"], S["cumsum"], S["shs"], 3,
+                       S["opac"], S["sigma"], S["bg"], col0, oth0, B["maxb"], B["dens"], B["scal"], k=16,
+                       cutoff=1e-3, chunk_mode="exact")
+    tr.fwd(fa, s)
+    with pytest.raises(RuntimeError, match="stale"):
+        tr.multi_replay(s)
+    torch.cuda.synchronize()
