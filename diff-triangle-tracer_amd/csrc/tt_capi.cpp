@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,7 +58,38 @@ struct tt_tracer {
   unsigned long long mem_gen = 1, sched_gen = 1;
   unsigned long long graph_mem_gen = 0, graph_sched_gen = 0;
   int graph_what = 0;
+  int noalloc_grid[3][3] = {};  // persistent forward grid per (k, block) of tt_multi_forward_noalloc (0: unset)
 };
+
+// Scratch of tt_multi_forward_noalloc: one static device array that every GPU the library
+// runs on holds (its own copy, at its own address) until the process ends. In host code the
+// name `scratch` is a host-side stand-in, not a device address: kernels and fills get the
+// address hipGetSymbolAddress returned for the current device, kept in g_scratch_dev.
+// Layout (ints): [0, 64) work-queue heads, then TT_NOALLOC_MAX_TILES tile costs (float),
+// TT_NOALLOC_MAX_TILES tile order entries, then the traversal stack spill.
+static __device__ __align__(256) int scratch[TT_NOALLOC_SCRATCH_BYTES / sizeof(int)];
+
+namespace {
+constexpr int kScratchDevices = 64;
+constexpr size_t kScratchInts = TT_NOALLOC_SCRATCH_BYTES / sizeof(int);
+constexpr size_t kScratchCostOff = 64, kScratchOrderOff = kScratchCostOff + TT_NOALLOC_MAX_TILES;
+constexpr size_t kScratchSpillOff = kScratchOrderOff + TT_NOALLOC_MAX_TILES;
+static_assert(kScratchSpillOff % 64 == 0, "spill area must stay 256-byte aligned");
+std::mutex g_scratch_mu;
+int *g_scratch_dev[kScratchDevices];  // device address of `scratch` on device d (null: not set up)
+
+// Once per device, with that device current (called from tt_create).
+hipError_t setup_scratch(int dev) {
+  if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev]) return hipSuccess;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_dev[dev] = static_cast<int *>(ptr);
+  return hipSuccess;
+}
+}  // namespace
 
 namespace {
 thread_local std::string g_err;
@@ -269,6 +301,11 @@ int tt_create(tt_tracer **out) {
   if (e != hipSuccess) {
     delete tr;
     return fail(TT_ERR_HIP, std::string("hipGetDevice failed: ") + hipGetErrorString(e));
+  }
+  e = setup_scratch(tr->device);
+  if (e != hipSuccess && e != hipErrorInvalidDevice) {  // beyond kScratchDevices: the no-alloc path reports it
+    delete tr;
+    return fail(TT_ERR_HIP, std::string("hipGetSymbolAddress of the static scratch failed: ") + hipGetErrorString(e));
   }
   *out = tr;
   return ok();
@@ -532,8 +569,9 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
 // ------------------------------------------------------------- multi-camera
 namespace {
 
-int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiCams &mc, tt::TraceParams &p,
-                int &grid) {
+// Validation, camera table and launch parameters of a multi-camera launch (no allocation).
+int multi_params(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiCams &mc, tt::TraceParams &p,
+                 long long &rays_out) {
   TT_ARG(a, "args is NULL");
   TT_ARG(a->n_cameras >= 1 && a->n_cameras <= TT_MAX_CAMERAS, "n_cameras must be in [1, 32]");
   TT_ARG(a->width && a->height, "camera size arrays are NULL");
@@ -588,19 +626,6 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
     p.dopac = a->dL_dopacity_dev;
     p.dsh = a->dL_dsh_dev;
   }
-  const long long key = ((long long)a->k << 20) | ((long long)a->block << 1) | (backward ? 1 : 0);
-  auto it = tr->grids.find(key);
-  if (it == tr->grids.end()) {
-    const int g = tt::multi_grid(a->k, a->block, backward);
-    if (g <= 0) return fail(TT_ERR_HIP, "could not size the persistent grid");
-    it = tr->grids.emplace(key, g).first;
-  }
-  grid = it->second;
-  if (!tr->heads) {
-    hipError_t e = hipMalloc(&tr->heads, 8 * sizeof(int));
-    if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("queue allocation failed: ") + hipGetErrorString(e));
-    tr->mem_gen++;
-  }
   if (a->hitlist_dev || a->hitlist_count_dev) {
     TT_ARG(a->hitlist_dev && a->hitlist_count_dev && a->hitlist_cap > 0,
            "the hit list needs hitlist, hitlist_count and hitlist_cap > 0");
@@ -614,6 +639,31 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
       p.hl_count_in = a->hitlist_count_dev;
       p.hl_cap = a->hitlist_cap;
     }
+  }
+  rays_out = rays;
+  return TT_OK;
+}
+
+// multi_params plus the tracer's persistent grid, work-queue heads, tile schedule, record
+// rows and stack spill (allocated or grown here).
+int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiCams &mc, tt::TraceParams &p,
+                int &grid) {
+  long long rays = 0;
+  int rc = multi_params(tr, a, backward, mc, p, rays);
+  if (rc != TT_OK) return rc;
+  const int tiles = mc.total_tiles;
+  const long long key = ((long long)a->k << 20) | ((long long)a->block << 1) | (backward ? 1 : 0);
+  auto it = tr->grids.find(key);
+  if (it == tr->grids.end()) {
+    const int g = tt::multi_grid(a->k, a->block, backward);
+    if (g <= 0) return fail(TT_ERR_HIP, "could not size the persistent grid");
+    it = tr->grids.emplace(key, g).first;
+  }
+  grid = it->second;
+  if (!tr->heads) {
+    hipError_t e = hipMalloc(&tr->heads, 8 * sizeof(int));
+    if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("queue allocation failed: ") + hipGetErrorString(e));
+    tr->mem_gen++;
   }
   if ((rc = ensure_tiles(tr, tiles)) != TT_OK) return rc;
   if (!backward) {  // schedule: probe order for the forward, measured tile times for the backward
@@ -677,6 +727,48 @@ int tt_multi_forward(tt_tracer *tr, const tt_multi_args *a, void *stream) {
 int tt_multi_backward(tt_tracer *tr, const tt_multi_args *a, void *stream) {
   int rc = multi_issue(tr, a, 2, static_cast<hipStream_t>(stream));
   return rc == TT_OK ? ok() : rc;
+}
+
+int tt_multi_forward_noalloc(tt_tracer *tr, const tt_multi_args *a, void *stream) {
+  TT_ARG(tr, "tracer is NULL");
+  tt::MultiCams mc;
+  tt::TraceParams p;
+  long long rays = 0;
+  int rc = multi_params(tr, a, false, mc, p, rays);
+  if (rc != TT_OK) return rc;
+  if (mc.total_tiles > TT_NOALLOC_MAX_TILES)
+    return fail(TT_ERR_INVALID_ARG, "the batch has " + std::to_string(mc.total_tiles) + " tiles; the no-alloc forward "
+                                    "takes at most TT_NOALLOC_MAX_TILES (" + std::to_string(TT_NOALLOC_MAX_TILES) +
+                                    "); use tt_multi_forward or a larger block");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != tr->device)
+    return fail(TT_ERR_STATE, "the current device is not the tracer's device");
+  int *base = dev >= 0 && dev < kScratchDevices ? g_scratch_dev[dev] : nullptr;
+  if (!base) return fail(TT_ERR_STATE, "the static scratch is not set up on this device (device index >= 64)");
+  const int ki = a->k == 8 ? 0 : (a->k == 16 ? 1 : 2), bi = a->block == 64 ? 0 : (a->block == 128 ? 1 : 2);
+  int &g = tr->noalloc_grid[ki][bi];
+  if (g == 0) g = tt::multi_grid(a->k, a->block, false);
+  if (g <= 0) return fail(TT_ERR_HIP, "could not size the persistent grid");
+  int grid = g;
+  const int depth = 3 * tr->bvh.height_max + 1 - tt::kLdsStack;
+  p.ov_depth = depth > 0 ? depth : 0;
+  p.ov_stack = base + kScratchSpillOff;
+  if (p.ov_depth > 0 && tr->bvh.n_leaves > 0) {  // the grid whose stack spill fits the array
+    const size_t fit = (kScratchInts - kScratchSpillOff) / ((size_t)a->block * p.ov_depth);
+    if (fit < 1)
+      return fail(TT_ERR_UNSUPPORTED, "the acceleration structure is too deep for the static scratch's stack spill");
+    if ((size_t)grid > fit) grid = (int)fit;
+  }
+  p.tile_cost = reinterpret_cast<float *>(base + kScratchCostOff);
+  p.probe_order = base + kScratchOrderOff;
+  p.tile_order = nullptr;  // no schedule for a backward: the tracer's own is left alone
+  tr->sched_W = tr->sched_H = -1;
+  tr->multi_tiles = -1;    // a following tt_multi_backward takes its tiles in queue order
+  tr->sched_gen++;
+  std::string e = tt::launch_multi(p, mc, base, a->k, a->block, a->tiles_per_fetch, grid, false,
+                                   static_cast<hipStream_t>(stream));
+  if (!e.empty()) return fail(TT_ERR_HIP, e);
+  return ok();
 }
 
 int tt_multi_capture(tt_tracer *tr, const tt_multi_args *a, int what, void *stream) {

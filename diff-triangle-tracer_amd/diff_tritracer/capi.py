@@ -21,7 +21,8 @@ BUILD, REFIT, REUSE = 0, 1, 2
 CHUNK = {"compat": 0, "exact": 1}
 
 EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build", "tt_trace_fwd", "tt_trace_bwd",
-            "tt_num_primitives", "tt_last_build_ms", "tt_multi_forward", "tt_multi_backward", "tt_multi_capture",
+            "tt_num_primitives", "tt_last_build_ms", "tt_multi_forward", "tt_multi_backward", "tt_multi_forward_noalloc",
+            "tt_multi_capture",
             "tt_multi_replay", "tt_multi_autotune", "tt_proposal_default_hparams", "tt_proposal_num_params",
             "tt_proposal_create", "tt_proposal_destroy", "tt_proposal_set_params", "tt_proposal_get_params",
             "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays", "tt_adam_step",
@@ -176,6 +177,7 @@ def lib():
         L.tt_last_build_ms.argtypes = [_vp]
         L.tt_multi_forward.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
         L.tt_multi_backward.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
+        L.tt_multi_forward_noalloc.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
         L.tt_multi_capture.argtypes = [_vp, C.POINTER(MultiArgs), C.c_int, _vp]
         L.tt_multi_replay.argtypes = [_vp, _vp]
         L.tt_multi_autotune.argtypes = [_vp, C.POINTER(MultiArgs), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
@@ -323,6 +325,10 @@ class TracerHandle:
 
     def multi_fwd(self, args: "MultiArgs", stream=None):
         check(lib().tt_multi_forward(self.h, C.byref(args), stream), "tt_multi_forward")
+
+    def multi_fwd_noalloc(self, args: "MultiArgs", stream=None):
+        """tt_multi_forward with its scratch in the library's static device array (no allocation)."""
+        check(lib().tt_multi_forward_noalloc(self.h, C.byref(args), stream), "tt_multi_forward_noalloc")
 
     def multi_bwd(self, args: "MultiArgs", stream=None):
         check(lib().tt_multi_backward(self.h, C.byref(args), stream), "tt_multi_backward")

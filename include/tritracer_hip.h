@@ -243,6 +243,22 @@ typedef struct {
 int tt_multi_forward(tt_tracer *tr, const tt_multi_args *a, void *stream);
 /* Backward of every camera in ONE persistent launch; gradients of all cameras accumulate. */
 int tt_multi_backward(tt_tracer *tr, const tt_multi_args *a, void *stream);
+/* Allocation-free variant of tt_multi_forward. No counterpart in the reference; for callers
+   that cannot allocate device memory (code under hipGraph stream capture, tight serving
+   loops, harnesses that forbid hipMalloc). Same launches and the same outputs as
+   tt_multi_forward, bit for bit; every scratch buffer (work-queue heads, tile cost estimates
+   and order, traversal stack spill) lives in one static device array of
+   TT_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process ends;
+   the call makes no hipMalloc / hipFree or other allocating call. Limits: at most
+   TT_NOALLOC_MAX_TILES tiles per batch (TT_ERR_INVALID_ARG beyond, nothing launched); the
+   persistent grid shrinks until its stack spill fits the array (TT_ERR_UNSUPPORTED if not one
+   block fits). Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+   (other streams, threads or tracers) race. The tracer's own tile schedule is not written,
+   so a following tt_multi_backward takes its tiles in queue order. The acceleration
+   structure must have been built with tt_build on the calling (current) device. */
+#define TT_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define TT_NOALLOC_MAX_TILES (1 << 18)
+int tt_multi_forward_noalloc(tt_tracer *tr, const tt_multi_args *a, void *stream);
 /* Record one training step into a hipGraph owned by the tracer: what = 1 forward, 2 backward
    (with zeroing when a->zero_grads), 3 both. Buffers must stay valid while the graph is used.
    The graph holds the tracer's state of the moment it was recorded: after tt_build(TT_BUILD)

@@ -1,0 +1,100 @@
+"""Allocation-free multi-camera forward (tt_multi_forward_noalloc) against tt_multi_forward,
+through the C ABI: bit-identical colour, others, statistics and hit lists on two calls in a
+row (the second on a smaller batch), no device memory allocated by the call, and the tile
+limit rejected before anything is launched.
+
+`scaling` is compared on the one-camera call only: the reference fills it in place from the
+origin of whichever ray first blends the triangle (forward.cu:34-74), so with several camera
+origins its value depends on the tile order, which differs between the two launches."""
+import numpy as np
+import pytest
+
+from diff_tritracer import capi, scenes
+from helpers import to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-noalloc]", *a, flush=True)
+
+
+def _batch(torch, cuda, sc, S, sizes, seed, cap):
+    w, h, ro, rd = scenes.multiview_rays(sizes, seed=seed)
+    Rt = ro.shape[0]
+    B = dict(ro=to_dev(ro, cuda), rd=to_dev(rd, cuda), color=torch.full((3 * Rt,), 7.0, device=cuda),
+             others=torch.full((7 * Rt,), 7.0, device=cuda), maxb=torch.zeros(sc.P, device=cuda),
+             dens=torch.zeros(sc.P, device=cuda), scal=torch.zeros(sc.P, device=cuda),
+             stats=torch.full((Rt, 2), -1, dtype=torch.int32, device=cuda))
+    hl = capi.hitlist_buffers(Rt, cap, cuda)
+    hl[0].fill_(-7)
+    hl[1].fill_(-7)
+    B["hits"], B["counts"] = hl
+    a = capi.multi_args(w, h, B["ro"], B["rd"], S["verts"], S["npts"], S["cumsum"], S["shs"], 3, S["opac"], S["sigma"],
+                        S["bg"], B["color"], B["others"], B["maxb"], B["dens"], B["scal"], k=16, cutoff=1e-3,
+                        chunk_mode="exact", block=128, tiles_per_fetch=1, stats=B["stats"], hitlist=hl)
+    return a, B
+
+
+def _host(B):
+    return {k: v.cpu().numpy() for k, v in B.items() if k not in ("ro", "rd")}
+
+
+def _same(x, y, skip=()):
+    for k in x:
+        if k in skip:
+            continue
+        assert np.array_equal(x[k].view(np.uint32) if x[k].dtype == np.float32 else x[k],
+                              y[k].view(np.uint32) if y[k].dtype == np.float32 else y[k]), k
+
+
+def test_noalloc_forward_equals_multi_forward(cuda):
+    import torch
+
+    sc = scenes.clustered_scene(200_000, 16, seed=5)
+    d = lambda a: to_dev(a, cuda)
+    S = dict(verts=d(sc.verts), npts=d(sc.npts), cumsum=d(sc.cumsum), shs=d(sc.shs), opac=d(sc.opac.reshape(-1)),
+             sigma=d(sc.sigma.reshape(-1)), bg=torch.tensor([0.1, 0.2, 0.3], device=cuda))
+    s = torch.cuda.current_stream().cuda_stream
+    fresh = capi.TracerHandle()         # never ran a multi-camera launch: owns no scratch yet
+    fresh.build(S["verts"], S["cumsum"], sc.P, capi.BUILD, s)
+    ref = capi.TracerHandle()
+    ref.build(S["verts"], S["cumsum"], sc.P, capi.BUILD, s)
+    # warm-up on the reference tracer: the runtime's own first-launch set-up (code object,
+    # private-segment memory of the kernels) is not the entry point's allocation
+    aw, _ = _batch(torch, cuda, sc, S, [(64, 64)], 9, 4)
+    ref.multi_fwd(aw, s)
+    torch.cuda.synchronize()
+    for call, (sizes, seed) in enumerate((([(160, 120), (96, 64), (200, 100)], 6), ([(72, 40)], 7))):
+        _say(f"call {call + 1}: cameras {sizes}")
+        a1, B1 = _batch(torch, cuda, sc, S, sizes, seed, 24)
+        a2, B2 = _batch(torch, cuda, sc, S, sizes, seed, 24)
+        torch.cuda.synchronize()
+        free0 = torch.cuda.mem_get_info()[0]
+        fresh.multi_fwd_noalloc(a1, s)
+        torch.cuda.synchronize()
+        free1 = torch.cuda.mem_get_info()[0]
+        _say(f"  free device memory {free0} -> {free1}")
+        assert free0 - free1 < (2 << 20)  # the allocating path takes ~80 MB of stack spill here
+        ref.multi_fwd(a2, s)
+        torch.cuda.synchronize()
+        x, y = _host(B1), _host(B2)
+        _say(f"  blended hits {int(x['stats'][:, 1].sum())}, mean colour {float(x['color'].mean()):.4f}")
+        _same(x, y, skip=("scal",) if len(sizes) > 1 else ())
+
+
+def test_noalloc_rejects_too_many_tiles(cuda):
+    import torch
+
+    sc = scenes.uniform_scene(1000, 0)
+    d = lambda a: to_dev(a, cuda)
+    S = dict(verts=d(sc.verts), npts=d(sc.npts), cumsum=d(sc.cumsum), shs=d(sc.shs), opac=d(sc.opac.reshape(-1)),
+             sigma=d(sc.sigma.reshape(-1)), bg=torch.zeros(3, device=cuda))
+    h = capi.TracerHandle()
+    h.build(S["verts"], S["cumsum"], sc.P)
+    one = torch.zeros(3, device=cuda)
+    # 4 cameras of 4096 x 4160 on 8x8 tiles: 266,240 tiles > TT_NOALLOC_MAX_TILES (nothing is launched)
+    a = capi.multi_args([4096] * 4, [4160] * 4, one, one, S["verts"], S["npts"], S["cumsum"], S["shs"], 3, S["opac"],
+                        S["sigma"], S["bg"], one, one, one, one, one, block=64)
+    with pytest.raises(RuntimeError, match="TT_NOALLOC_MAX_TILES"):
+        h.multi_fwd_noalloc(a)
