@@ -17,14 +17,20 @@ cutoff 1e-3, random N(0,1) image-space loss gradient; exact chunk mode.
                  the interval the forward emitted (same gradients)
   proposal_hard: lossy variant in config: proposed start and a hard cap, with
                  fallbacks to the full trace; its PSNR against plain is reported
+  proposal_ceiling: each ray's own emitted interval of this frame as (start, cap), trusted
+                 (hard cap, no fallback; rays that blend nothing are not traced) -- what a
+                 perfect proposal could save (same image as plain)
   value        : Mrays/s of the proposal step; plain and the ablations are in config
 
-One step = one pass over the frame with inputs resident in HBM. The LBVH build,
-refit and the proposal's training are timed separately and reported beside it.
+One step = one pass over the frame with inputs resident in HBM; times are medians
+over the timed steps. The LBVH build, refit and the proposal's training are timed
+separately and reported beside it.
 
 Prints one JSON line. Extra objects:
-  roofline      : the dominant kernel's algorithmic bytes per launch / its average
-                  launch time (HIP events on the launch stream) against 8 TB/s
+  roofline      : the dominant traversal kernel's algorithmic bytes per launch (per ray
+                  64 + 48 Hc + 204 Hb forward, 104 + 48 Hc + 428 Hb backward) / its median
+                  launch time (the library's HIP events right around that kernel on the
+                  launch stream) against 8 TB/s
   cpu_baseline  : the scalar C++/OpenMP oracle (same algorithm, FP32) on the frame,
                   timed on the host cores
 Other workloads: --workload multiview (2M splats, 8 uneven cameras, one persistent
@@ -47,14 +53,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "diff-triangle-tracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the dominant kernels on the default workload (1M splats, 1920x1080,
-# k=16), 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE from rocprofv3 counter runs of this script
-# (profiles/trace_fwd_bwd_1M_1080p.md; the factors come from a 256 MiB copy), and of the
-# densify and scene-file kernels at 1M splats (profiles/densify_sceneio_1M.md).
-MEASURED_TRAFFIC = {"k_forward<16,0,false,true>": 6.560e9, "k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
-# VALU busy (SQ_ACTIVE_INST_VALU x 4 / (SIMDs x cycles)) of the dominant kernel, same profile
-MEASURED_VALU_BUSY = {"k_forward<16,0,false,true>": 0.67}
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+# Counters of the default workload's traversal kernels (1M splats, 1920x1080, k=16), copied from
+# the rocprofv3 runs summarised in PROFILE_SOURCE (they cannot be read inside this process):
+# traffic = HBM bytes per launch, 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE (factors from a 256 MiB
+# copy); avg_ms = the profiler's average duration; l2_hit = TCC_HIT / (TCC_HIT + TCC_MISS);
+# valu_busy = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x GRBM_GUI_ACTIVE / XCDs).
+PROFILE_SOURCE = "profiles/trace_fwd_bwd_1M_1080p.md"
+MEASURED = {
+    "k_forward<16,0,false,true>": dict(traffic=6.210e9, avg_ms=5.073, l2_hit=0.642, valu_busy=0.67),
+    "k_backward_rec<16,0>": dict(traffic=7.437e9, avg_ms=3.198, l2_hit=0.588, valu_busy=0.40),
+}
+# densify and scene-file kernels at 1M splats (profiles/densify_sceneio_1M.md)
+MEASURED_TRAFFIC = {"k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
 TRAIN_POSES, HELD_OUT_EXTRA = 16, 3
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(scene, ray_o, ray_d, stride, cutoff, k):
@@ -76,7 +100,7 @@ def cpu_baseline(scene, ray_o, ray_d, stride, cutoff, k):
     f = O.forward_f32(S, ro, rd, opts)
     O.backward_f32(S, ro, rd, opts, f["color"], f["others"], dpix, doth)
     t = time.perf_counter() - t0
-    return dict(value=round(R / t / 1e6, 5), unit="Mrays/s", cores=O.get_threads(), kind="port",
+    return dict(value=round(R / t / 1e6, 5), unit="Mrays/s", cores=O.get_threads(), kind="port", cpu_model=cpu_model(),
                 sample=f"every {stride}th pixel in x and y of the same 1920x1080 frame ({R} rays), full 1M-splat "
                        f"scene, FP32 fwd+bwd of the scalar C++/OpenMP oracle; its own BVH build "
                        f"{t_build:.2f} s excluded", seconds=round(t, 3))
@@ -536,9 +560,17 @@ class Frame:
             t.zero_()
 
 
-def timed(fr, launches, steps, warmup):
+def _median(xs):
+    xs = sorted(xs)
+    n = len(xs)
+    return 0.5 * (xs[(n - 1) // 2] + xs[n // 2]) if n else float("nan")
+
+
+def timed(fr, launches, steps, warmup, stats=None):
     """Run `launches` (list of (name, fn)) as one step: warmup, then `steps` timed steps
-    with HIP events around every launch on the launch stream. Returns (ms/step, {name: mean ms})."""
+    with HIP events around every launch on the launch stream (torch's current stream, which
+    every launch here uses). Returns (median ms/step, {name: median ms}); `stats`, if given,
+    receives the mean ms/step over the whole timed region."""
     torch = fr.torch
     for _ in range(warmup):
         for _, fn in launches:
@@ -554,8 +586,10 @@ def timed(fr, launches, steps, warmup):
             evs[i][j + 1].record(fr.stream)
     stop.record(fr.stream)
     torch.cuda.synchronize()
-    per = {name: sum(e[j].elapsed_time(e[j + 1]) for e in evs) / steps for j, (name, _) in enumerate(launches)}
-    return start.elapsed_time(stop) / steps, per
+    per = {name: _median([e[j].elapsed_time(e[j + 1]) for e in evs]) for j, (name, _) in enumerate(launches)}
+    if stats is not None:
+        stats["mean_ms"] = start.elapsed_time(stop) / steps
+    return _median([e[0].elapsed_time(e[-1]) for e in evs]), per
 
 
 def psnr(a, b):
@@ -612,12 +646,17 @@ def train_proposal(fr, poses, W, H, steps, batch, seed):
     return net, ms, me, info
 
 
+PROPOSAL_LEGS = ("proposal", "proposal_hard")
+
+
 def frame_benchmark(args, dev, scene, cam_eye, label):
     import numpy as np
     import torch
     from diff_tritracer import capi, scenes
 
     W, H = args.width, args.height
+    legs = [x for x in args.legs.split(",") if x] if args.legs else None
+    want = lambda name: legs is None or name in legs
     fr = Frame(dev, scene, args.k, args.cutoff, args.hitlist_cap)
     P = fr.P
     # LBVH build and refit, timed separately (median of 5 after one warm-up)
@@ -631,10 +670,12 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     print(f"[bench] LBVH build {build_ms:.3f} ms, refit {refit_ms:.3f} ms", file=sys.stderr, flush=True)
 
     # the proposal: 16 training poses on the sphere of radius 3.5-4.5 around the scene
-    rng = np.random.Generator(np.random.PCG64(101))
-    poses = [scenes.orbit_eye(float(rng.uniform(3.5, 4.5)), 101, i) for i in range(TRAIN_POSES)]
-    net, ms, me, pinfo = train_proposal(fr, poses, W, H, args.proposal_steps, args.proposal_batch, seed=3)
-    print(f"[bench] proposal: {pinfo}", file=sys.stderr, flush=True)
+    net, pinfo, ms, me = None, None, 0.0, 0.0
+    if any(want(n) for n in PROPOSAL_LEGS):
+        rng = np.random.Generator(np.random.PCG64(101))
+        poses = [scenes.orbit_eye(float(rng.uniform(3.5, 4.5)), 101, i) for i in range(TRAIN_POSES)]
+        net, ms, me, pinfo = train_proposal(fr, poses, W, H, args.proposal_steps, args.proposal_batch, seed=3)
+        print(f"[bench] proposal: {pinfo}", file=sys.stderr, flush=True)
 
     fr.buffers(W, H)
     fr.camera(scenes.camera_basis(W, H, cam_eye))
@@ -659,15 +700,21 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     ba_compat = fr.bwd_args("compat")
     fa_emit = fr.fwd_args("exact", interval_out=fr.ival)
     fa_soft = fr.fwd_args("exact", interval_in=fr.prop, interval_out=fr.ival)
-    ts = float(net.hp.t_scale)
-    fa_hard = fr.fwd_args("exact", interval_in=fr.prop2, interval_out=fr.ival, hard=True, fb_start=ms * ts,
-                          fb_end=me * ts)
     ba_bound = fr.bwd_args("exact", interval=fr.ival)
-    # exact: proposed cap only (start 0, continuation past the cap): the image equals plain
-    propose = lambda: net.forward(fr.ro, fr.rd, fr.prop, capi.PROPOSAL_CAP, ms, me, stream=sp)
-    # lossy: proposed start and hard cap, with the fallbacks
-    propose2 = lambda: net.forward(fr.ro, fr.rd, fr.prop2, capi.PROPOSAL_INTERVAL, ms, me, stream=sp)
-
+    # the proposal's ceiling: each ray's own emitted interval (plain forward of this frame) as
+    # (start, cap) -- start at the first blended (or terminating) hit, cap at the terminating or
+    # last blended hit; exact mode, so the image is plain's whatever the interval
+    perfect = torch.zeros(R, 2, device=dev)
+    if want("proposal_ceiling"):
+        fr.h.fwd(fa_emit, sp)
+        iv = fr.ival.clone()
+        first = torch.where(torch.isfinite(iv[:, 0]), iv[:, 0], iv[:, 2])
+        stop_t = torch.where(torch.isfinite(iv[:, 2]), iv[:, 2], iv[:, 1])
+        perfect = torch.stack([torch.where(torch.isfinite(first), first, torch.full_like(first, 3.0e38)),
+                               torch.where(torch.isfinite(stop_t), stop_t, torch.zeros_like(stop_t))], 1).contiguous()
+    # hard mode without fallbacks: the trace ends at the cap and rays that blend nothing (start
+    # past the far plane) are not traced; with these intervals the image is plain's
+    fa_ceiling = fr.fwd_args("exact", interval_in=perfect, interval_out=fr.ival, hard=True)
     runs = {
         "plain": [("fwd", lambda: fr.h.fwd(fa_plain, sp)), ("zero", fr.zero_grads), ("bwd", lambda: fr.h.bwd(ba_plain, sp))],
         "plain_retrace": [("fwd", lambda: fr.h.fwd(fa_retrace, sp)), ("zero", fr.zero_grads),
@@ -676,90 +723,139 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
                          ("bwd", lambda: fr.h.bwd(ba_compat, sp))],
         "handoff_only": [("fwd", lambda: fr.h.fwd(fa_emit, sp)), ("zero", fr.zero_grads),
                          ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
-        "proposal": [("mlp", propose), ("fwd", lambda: fr.h.fwd(fa_soft, sp)), ("zero", fr.zero_grads),
-                     ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
-        "proposal_hard": [("mlp", propose2), ("fwd", lambda: fr.h.fwd(fa_hard, sp)), ("zero", fr.zero_grads),
-                          ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
+        "proposal_ceiling": [("fwd", lambda: fr.h.fwd(fa_ceiling, sp)), ("zero", fr.zero_grads),
+                             ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
     }
+    if net is not None:
+        ts = float(net.hp.t_scale)
+        fa_hard = fr.fwd_args("exact", interval_in=fr.prop2, interval_out=fr.ival, hard=True, fb_start=ms * ts,
+                              fb_end=me * ts)
+        # exact: proposed cap only (start 0, continuation past the cap): the image equals plain
+        propose = lambda: net.forward(fr.ro, fr.rd, fr.prop, capi.PROPOSAL_CAP, ms, me, stream=sp)
+        # lossy: proposed start and hard cap, with the fallbacks
+        propose2 = lambda: net.forward(fr.ro, fr.rd, fr.prop2, capi.PROPOSAL_INTERVAL, ms, me, stream=sp)
+        runs["proposal"] = [("mlp", propose), ("fwd", lambda: fr.h.fwd(fa_soft, sp)), ("zero", fr.zero_grads),
+                            ("bwd", lambda: fr.h.bwd(ba_bound, sp))]
+        runs["proposal_hard"] = [("mlp", propose2), ("fwd", lambda: fr.h.fwd(fa_hard, sp)), ("zero", fr.zero_grads),
+                                 ("bwd", lambda: fr.h.bwd(ba_bound, sp))]
+    runs = {k: v for k, v in runs.items() if want(k)}
     res, images = {}, {}
     for name, launches in runs.items():
-        ms_step, per = timed(fr, launches, args.steps, args.warmup)
-        res[name] = dict(ms=ms_step, per=per)
+        fr.h.set_kernel_timing(True)
+        st_ = {}
+        ms_step, per = timed(fr, launches, args.steps, args.warmup, st_)
+        # the traversal kernels alone (library events around them, over the timed steps)
+        kf, kb = fr.h.kernel_times("fwd", args.steps), fr.h.kernel_times("bwd", args.steps)
+        fr.h.set_kernel_timing(False)
+        res[name] = dict(ms=ms_step, mean_ms=st_["mean_ms"], per=per, k_fwd=_median(kf), k_bwd=_median(kb),
+                         k_fwd_mean=sum(kf) / max(1, len(kf)), k_bwd_mean=sum(kb) / max(1, len(kb)))
         images[name] = fr.color.clone()
-        print(f"[bench] {name:14s} {ms_step:8.3f} ms/step " + " ".join(f"{k} {v:.3f}" for k, v in per.items()) +
+        print(f"[bench] {name:16s} {ms_step:8.3f} ms/step " + " ".join(f"{k} {v:.3f}" for k, v in per.items()) +
+              f"  kernels fwd {res[name]['k_fwd']:.3f} bwd {res[name]['k_bwd']:.3f}"
               f"  ({R / ms_step / 1e3:.2f} Mrays/s)", file=sys.stderr, flush=True)
-    ref_img = images["plain"].cpu().numpy()
-    exact_equal = bool(np.array_equal(images["proposal"].cpu().numpy().view(np.uint32), ref_img.view(np.uint32)))
-    hard_psnr = psnr(images["proposal_hard"].cpu().numpy(), ref_img)
-    # node visits of the proposal-bounded forward (the network's gain in traversal work)
-    propose()
-    fr.h.fwd(fr.fwd_args("exact", interval_in=fr.prop, stats=stats, trav=trav), sp)
-    torch.cuda.synchronize()
-    tv_prop = trav.to(torch.int64).sum(0).cpu().numpy()
-    # extra held-out poses: equality and hard-mode PSNR
-    rng2 = np.random.Generator(np.random.PCG64(202))
-    held = []
-    for i in range(HELD_OUT_EXTRA):
-        eye = scenes.orbit_eye(float(rng2.uniform(3.5, 4.5)), 202, i)
-        fr.camera(scenes.camera_basis(W, H, eye))
-        fr.h.fwd(fa_plain, sp)
-        a = fr.color.cpu().numpy()
+    mr = lambda name: R / (res[name]["ms"] * 1e-3) / 1e6 if name in res else None
+    same = lambda a, b: bool(np.array_equal(images[a].cpu().numpy().view(np.uint32),
+                                            images[b].cpu().numpy().view(np.uint32)))
+    cfg = {}
+    if "plain" in res and "proposal_ceiling" in res:
+        cfg["proposal_ceiling_image_equals_plain"] = same("proposal_ceiling", "plain")
+    tv_prop = None
+    if net is not None and "proposal" in res and "plain" in res:
+        ref_img = images["plain"].cpu().numpy()
+        cfg["proposal_image_equals_plain"] = same("proposal", "plain")
+        cfg["psnr_loss_db"] = 0.0 if cfg["proposal_image_equals_plain"] else None
+        if "proposal_hard" in res:
+            cfg["hard_psnr_vs_plain_db"] = round(psnr(images["proposal_hard"].cpu().numpy(), ref_img), 2)
+        # node visits of the proposal-bounded forward (the network's gain in traversal work)
         propose()
-        fr.h.fwd(fa_soft, sp)
-        b = fr.color.cpu().numpy()
-        propose2()
-        fr.h.fwd(fa_hard, sp)
-        c = fr.color.cpu().numpy()
-        held.append(dict(equal=bool(np.array_equal(a.view(np.uint32), b.view(np.uint32))), hard_psnr_db=round(psnr(c, a), 2)))
-    fr.camera(scenes.camera_basis(W, H, cam_eye))
+        fr.h.fwd(fr.fwd_args("exact", interval_in=fr.prop, stats=stats, trav=trav), sp)
+        torch.cuda.synchronize()
+        tv_prop = trav.to(torch.int64).sum(0).cpu().numpy()
+        # extra held-out poses: equality and hard-mode PSNR
+        rng2 = np.random.Generator(np.random.PCG64(202))
+        held = []
+        for i in range(HELD_OUT_EXTRA):
+            eye = scenes.orbit_eye(float(rng2.uniform(3.5, 4.5)), 202, i)
+            fr.camera(scenes.camera_basis(W, H, eye))
+            fr.h.fwd(fa_plain, sp)
+            a = fr.color.cpu().numpy()
+            propose()
+            fr.h.fwd(fa_soft, sp)
+            b = fr.color.cpu().numpy()
+            propose2()
+            fr.h.fwd(fa_hard, sp)
+            c = fr.color.cpu().numpy()
+            held.append(dict(equal=bool(np.array_equal(a.view(np.uint32), b.view(np.uint32))),
+                             hard_psnr_db=round(psnr(c, a), 2)))
+        cfg["held_out_extra"] = held
+        fr.camera(scenes.camera_basis(W, H, cam_eye))
+    tv_ceil = None
+    if "proposal_ceiling" in res:
+        fr.h.fwd(fr.fwd_args("exact", interval_in=perfect, hard=True, stats=stats, trav=trav), sp)
+        torch.cuda.synchronize()
+        tv_ceil = trav.to(torch.int64).sum(0).cpu().numpy()
 
-    mr = lambda name: R / (res[name]["ms"] * 1e-3) / 1e6
-    bf = 64 * R + 48 * Hc + 204 * Hb + 8 * R + 16 * R  # + proposed interval read, emitted interval written
-    bb = 104 * R + 48 * Hc + 428 * Hb + 16 * R         # + emitted interval read
-    if fr.hl is not None:  # hit list: the forward writes 8 B per blended hit + a count, the backward reads them
-        bf += 8 * Hb + 4 * R
-        bb += 8 * Hb + 4 * R - 48 * Hc - 16 * R
-    per = res["proposal"]["per"]
-    if per["bwd"] >= per["fwd"]:
-        dom, dom_bytes, dom_ms = f"k_backward_rec<{args.k},0>", bb, per["bwd"]
+    # roofline of the dominant traversal kernel of the measured step: algorithmic bytes per
+    # launch (per ray 64 + 48 Hc + 204 Hb forward, 104 + 48 Hc + 428 Hb backward, with the
+    # plain trace's consumed / blended hits; node traffic, stack spill, hit lists, intervals
+    # and gradient records excluded) / its median launch time (library events around it)
+    bf = 64 * R + 48 * Hc + 204 * Hb
+    bb = 104 * R + 48 * Hc + 428 * Hb
+    step = "proposal" if "proposal" in res else next(iter(res))
+    rs = res[step]
+    kfwd = f"k_forward<{args.k},0,false,{'true' if step != 'plain' and step != 'plain_retrace' and step != 'plain_compat' else 'false'}>"
+    if rs["k_bwd"] >= rs["k_fwd"]:
+        dom, dom_bytes, dom_ms = f"k_backward_rec<{args.k},0>", bb, rs["k_bwd"]
     else:
-        dom, dom_bytes, dom_ms = f"k_forward<{args.k},0,false,true>", bf, per["fwd"]
+        dom, dom_bytes, dom_ms = kfwd, bf, rs["k_fwd"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     default = P == 1_000_000 and (W, H) == (1920, 1080) and label.startswith("1080p")
-    traffic = MEASURED_TRAFFIC.get(dom) if default else None
+    prof = MEASURED.get(dom) if default else None
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic, kernel=dom,
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=prof["traffic"] if prof else None, kernel=dom,
                     bytes_per_launch=int(dom_bytes), avg_launch_ms=round(dom_ms, 4),
-                    step_frac=round((bf + bb) / ((per["fwd"] + per["bwd"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5))
-    if default:  # what does bound the kernel (rocprofv3 counters of this workload)
-        roofline["valu_busy"] = MEASURED_VALU_BUSY.get(dom)
-        roofline["limiter"] = ("VALU issue and dependent BVH-node fetch latency at 3 waves/SIMD, not HBM "
-                               "(profiles/trace_fwd_bwd_1M_1080p.md)")
-    mlp_flops = 2 * 14848 * R
-    cfg = {
+                    avg_launch_ms_mean=round(rs["k_fwd_mean"] if dom == kfwd else rs["k_bwd_mean"], 4),
+                    step_frac=round((bf + bb) / ((rs["k_fwd"] + rs["k_bwd"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5))
+    if prof:  # what bounds the kernel: rocprofv3 counters of this workload
+        roofline.update(profiled_avg_ms=prof["avg_ms"], l2_hit=prof["l2_hit"], valu_busy=prof["valu_busy"],
+                        counters_from=PROFILE_SOURCE)
+    cfg.update({
         "workload": f"{label}: fwd+bwd, {P} clustered triangle splats, one {W}x{H} camera (held out from the "
                     f"proposal's {TRAIN_POSES} training poses), k={args.k}, cutoff {args.cutoff:g}, exact chunk mode, "
                     f"N(0,1) loss gradient",
         "triangles": P, "width": W, "height": H, "k": args.k, "cutoff": args.cutoff, "chunk_mode": "exact",
-        "hitlist_cap": args.hitlist_cap,
-        "plain_mrays": round(mr("plain"), 3), "plain_retrace_mrays": round(mr("plain_retrace"), 3),
-        "plain_compat_mrays": round(mr("plain_compat"), 3),
-        "handoff_only_mrays": round(mr("handoff_only"), 3), "proposal_mrays": round(mr("proposal"), 3),
-        "proposal_hard_mrays": round(mr("proposal_hard"), 3),
-        "speedup_vs_plain": round(res["plain"]["ms"] / res["proposal"]["ms"], 4),
-        "speedup_hard_vs_plain": round(res["plain"]["ms"] / res["proposal_hard"]["ms"], 4),
-        "proposal_image_equals_plain": exact_equal, "psnr_loss_db": 0.0 if exact_equal else None,
-        "hard_psnr_vs_plain_db": round(hard_psnr, 2), "held_out_extra": held,
-        "ms": {k: round(v["ms"], 4) for k, v in res.items()},
-        "kernel_ms": {k: {n: round(x, 4) for n, x in v["per"].items()} for k, v in res.items()},
-        "mlp_tflops": round(mlp_flops / (per["mlp"] * 1e-3) / 1e12, 2),
-        "nodes_visited_per_ray_plain": round(float(tv_plain[0]) / R, 2),
-        "nodes_visited_per_ray_proposal": round(float(tv_prop[0]) / R, 2),
-        "hits_consumed_per_ray": round(Hc / R, 3), "hits_blended_per_ray": round(Hb / R, 3),
-        "lbvh_build_ms": round(build_ms, 3), "lbvh_refit_ms": round(refit_ms, 3), "proposal_training": pinfo,
-    }
-    ms_step = res["proposal"]["ms"]
-    out = dict(value=round(mr("proposal"), 4), ms_per_step=round(ms_step, 4), config=cfg, roofline=roofline)
+        "hitlist_cap": args.hitlist_cap, "timing": f"median of {args.steps} steps after {args.warmup} warm-ups",
+    })
+    for name in res:
+        cfg[f"{name}_mrays"] = round(mr(name), 3)
+    if "proposal" in res and "plain" in res:
+        cfg["speedup_vs_plain"] = round(res["plain"]["ms"] / res["proposal"]["ms"], 4)
+    if "proposal_hard" in res and "plain" in res:
+        cfg["speedup_hard_vs_plain"] = round(res["plain"]["ms"] / res["proposal_hard"]["ms"], 4)
+    if "proposal_ceiling" in res and "plain" in res:
+        cfg["speedup_ceiling_vs_plain"] = round(res["plain"]["ms"] / res["proposal_ceiling"]["ms"], 4)
+    cfg["ms"] = {k: round(v["ms"], 4) for k, v in res.items()}
+    cfg["ms_mean"] = {k: round(v["mean_ms"], 4) for k, v in res.items()}
+    cfg["kernel_ms"] = {k: {n: round(x, 4) for n, x in v["per"].items()} for k, v in res.items()}
+    cfg["traversal_kernel_ms"] = {k: {"fwd": round(v["k_fwd"], 4), "bwd": round(v["k_bwd"], 4)} for k, v in res.items()}
+    if net is not None and "proposal" in res:
+        mlp_flops = 2 * 14848 * R
+        mlp_ms = res["proposal"]["per"]["mlp"]
+        tfl = mlp_flops / (mlp_ms * 1e-3) / 1e12
+        # training: forward + backward ~ 3x the forward's MACs per ray (dW and dH per layer)
+        train_flops = 3 * 2 * 14848 * args.proposal_batch * args.proposal_steps
+        ttf = train_flops / (pinfo["train_ms"] * 1e-3) / 1e12
+        cfg.update(mlp_tflops=round(tfl, 2), mlp_mfma_frac=round(tfl / BF16_PEAK_TFLOPS, 5),
+                   mlp_train_tflops=round(ttf, 3), mlp_train_mfma_frac=round(ttf / BF16_PEAK_TFLOPS, 6),
+                   proposal_training=pinfo)
+    cfg["nodes_visited_per_ray_plain"] = round(float(tv_plain[0]) / R, 2)
+    if tv_prop is not None:
+        cfg["nodes_visited_per_ray_proposal"] = round(float(tv_prop[0]) / R, 2)
+    if tv_ceil is not None:
+        cfg["nodes_visited_per_ray_ceiling"] = round(float(tv_ceil[0]) / R, 2)
+    cfg.update({"hits_consumed_per_ray": round(Hc / R, 3), "hits_blended_per_ray": round(Hb / R, 3),
+                "lbvh_build_ms": round(build_ms, 3), "lbvh_refit_ms": round(refit_ms, 3)})
+    out = dict(value=round(mr(step), 4), ms_per_step=round(rs["ms"], 4), config=cfg, roofline=roofline, step=step)
     return out, fr
 
 
@@ -779,6 +875,9 @@ def main():
     ap.add_argument("--hitlist-cap", type=int, default=64,
                     help="blended hits per pixel the forward stores for the backward (0: the backward re-traverses)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", default="",
+                    help="comma-separated subset of the frame workload's steps (plain, plain_retrace, plain_compat, "
+                         "handoff_only, proposal_ceiling, proposal, proposal_hard); default all")
     ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce", "train", "densify", "sceneio"], default="frame")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates rocprof traffic counters)")
@@ -852,7 +951,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded clustered triangle splats, pinhole camera rays)",
-        "config": out["config"],
+        "config": dict(out["config"], value_step=out["step"]),
         "roofline": out["roofline"],
         "cpu_baseline": cpu,
     }

@@ -59,6 +59,11 @@ struct tt_tracer {
   unsigned long long graph_mem_gen = 0, graph_sched_gen = 0;
   int graph_what = 0;
   int noalloc_grid[3][3] = {};  // persistent forward grid per (k, block) of tt_multi_forward_noalloc (0: unset)
+  // per-kernel timing (tt_set_kernel_timing): a ring of event pairs per direction (0 forward,
+  // 1 backward) around the traversal kernels, and the number of timed launches
+  bool timing = false;
+  hipEvent_t tev[2][TT_TIMING_RING][2] = {};
+  unsigned long long tcount[2] = {0, 0};
 };
 
 // Scratch of tt_multi_forward_noalloc: one static device array that every GPU the library
@@ -323,6 +328,10 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->capture_stream) (void)hipStreamDestroy(tr->capture_stream);
   if (tr->ev0) (void)hipEventDestroy(tr->ev0);
   if (tr->ev1) (void)hipEventDestroy(tr->ev1);
+  for (auto &dir : tr->tev)
+    for (auto &pair : dir)
+      for (hipEvent_t &ev : pair)
+        if (ev) (void)hipEventDestroy(ev);
   if (tr->tile_cost) (void)hipFree(tr->tile_cost);
   if (tr->tile_order) (void)hipFree(tr->tile_order);
   if (tr->probe_order) (void)hipFree(tr->probe_order);
@@ -361,6 +370,40 @@ int tt_build(tt_tracer *tr, const float *vertices_dev, int num_vertices, const i
   }
   (void)hipEventRecord(tr->ev1, s);
   tr->build_timed = true;
+  return ok();
+}
+
+int tt_set_kernel_timing(tt_tracer *tr, int enable) {
+  TT_ARG(tr, "tracer is NULL");
+  if (enable && !tr->tev[0][0][0]) {
+    for (auto &dir : tr->tev)
+      for (auto &pair : dir)
+        for (hipEvent_t &ev : pair) {
+          hipError_t e = hipEventCreate(&ev);
+          if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("hipEventCreate failed: ") + hipGetErrorString(e));
+        }
+  }
+  tr->timing = enable != 0;
+  tr->tcount[0] = tr->tcount[1] = 0;
+  return ok();
+}
+
+int tt_kernel_times(tt_tracer *tr, int which, float *ms_out, int max_n, int *count_out) {
+  TT_ARG(tr && ms_out && count_out, "NULL argument");
+  TT_ARG(which == 0 || which == 1, "which must be 0 (forward) or 1 (backward)");
+  TT_ARG(max_n >= 0, "max_n must be non-negative");
+  const unsigned long long have = tr->tcount[which];
+  unsigned long long n = std::min<unsigned long long>({have, (unsigned long long)max_n, (unsigned long long)TT_TIMING_RING});
+  for (unsigned long long i = 0; i < n; i++) {
+    const unsigned slot = (unsigned)((have - n + i) % TT_TIMING_RING);
+    hipEvent_t b = tr->tev[which][slot][0], e = tr->tev[which][slot][1];
+    hipError_t r = hipEventSynchronize(e);
+    float ms = 0.0f;
+    if (r == hipSuccess) r = hipEventElapsedTime(&ms, b, e);
+    if (r != hipSuccess) return fail(TT_ERR_HIP, std::string("kernel timing events: ") + hipGetErrorString(r));
+    ms_out[i] = ms;
+  }
+  *count_out = (int)n;
   return ok();
 }
 
@@ -468,8 +511,13 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.rec_off_out = tr->rec_off;
   p.tile_hits_b = tr->tile_hits_b;
   p.tile_cost_b = tr->tile_cost_b;
-  std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream));
+  tt::KernelEvents kev;
+  const unsigned slot_f = (unsigned)(tr->tcount[0] % TT_TIMING_RING);
+  if (tr->timing) kev.before = tr->tev[0][slot_f][0], kev.after = tr->tev[0][slot_f][1];
+  std::string err = tt::launch_forward(p, a->k, a->variant, static_cast<hipStream_t>(stream),
+                                       tr->timing ? &kev : nullptr);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
+  if (tr->timing) tr->tcount[0]++;
   tr->sched_W = a->width;
   tr->sched_H = a->height;
   tr->multi_tiles = -1;
@@ -559,8 +607,12 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
     p.tile_hits = tr->tile_hits_b;  // the backward's tiles
     p.rec_off = tr->rec_off;
   }
-  std::string err = tt::launch_backward(p, a->k, a->variant, s);
+  tt::KernelEvents kev;
+  const unsigned slot_b = (unsigned)(tr->tcount[1] % TT_TIMING_RING);
+  if (tr->timing) kev.before = tr->tev[1][slot_b][0], kev.after = tr->tev[1][slot_b][1];
+  std::string err = tt::launch_backward(p, a->k, a->variant, s, tr->timing ? &kev : nullptr);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
+  if (tr->timing) tr->tcount[1]++;
   return ok();
 }
 

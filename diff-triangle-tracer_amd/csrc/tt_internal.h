@@ -172,9 +172,17 @@ std::string launch_tile_order(const float *cost, int n, int *order, hipStream_t 
 size_t trace_lanes(int W, int H);
 constexpr int kLdsStack = 16;  // traversal stack entries per lane kept in LDS; deeper ones spill
 
+// Optional HIP events recorded on the launch stream right before and after the traversal
+// kernel(s) of a launch (not the schedule kernels around them).
+struct KernelEvents {
+  hipEvent_t before = nullptr, after = nullptr;
+};
+
 // Launch the forward kernel for (k, variant). Returns an error message or "".
-std::string launch_forward(const TraceParams &p, int k, int variant, hipStream_t stream);
-std::string launch_backward(const TraceParams &p, int k, int variant, hipStream_t stream);
+std::string launch_forward(const TraceParams &p, int k, int variant, hipStream_t stream,
+                           const KernelEvents *ev = nullptr);
+std::string launch_backward(const TraceParams &p, int k, int variant, hipStream_t stream,
+                            const KernelEvents *ev = nullptr);
 std::string launch_fill(float *ptr, size_t n, float v, hipStream_t stream);
 
 }  // namespace tt

@@ -210,7 +210,7 @@ static void tiles(TraceParams &p, int th = kTileH) {
   p.n_tiles = p.tiles_x * ((p.H + th - 1) / th);
 }
 
-std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_t stream) {
+std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_t stream, const KernelEvents *ev) {
   TraceParams p = p0;
   tiles(p);
   if (p.n_tiles == 0) return "";
@@ -225,6 +225,7 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
     if (e != hipSuccess) return std::string("tile probe launch failed: ") + hipGetErrorString(e);
     p.tile_sched = p.probe_order;
   }
+  if (ev) (void)hipEventRecord(ev->before, stream);
   if (k == 8)
     e = fwd_k<8>(p, variant, stream);
   else if (k == 16)
@@ -233,6 +234,7 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
     e = fwd_k<32>(p, variant, stream);
   else
     return "k must be 8, 16 or 32";
+  if (ev) (void)hipEventRecord(ev->after, stream);
   if (e != hipSuccess) return std::string("forward launch failed: ") + hipGetErrorString(e);
   if (p.tile_cost && p.tile_order && p.tile_cost_b) {  // the backward's schedule and record regions
     TraceParams pb = p;
@@ -260,12 +262,13 @@ int trace_tiles(int W, int H) {  // the forward's tiles (at least as many as the
   return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH);
 }
 
-std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream_t stream) {
+std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream_t stream, const KernelEvents *ev) {
   TraceParams p = p0;
   tiles(p, 2 * kTileH);
   if (p.n_tiles == 0) return "";
   dim3 g(p.n_tiles), bl(kBlockB);
   if (k != 8 && k != 16 && k != 32) return "k must be 8, 16 or 32";
+  if (ev) (void)hipEventRecord(ev->before, stream);
   if (p.rec_rows) {
     if (variant == 4) {
       // the primary segment (coherent rays) with per-block grouping of its records
@@ -326,6 +329,7 @@ std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream
   } else {
     return "the backward needs its gradient-record rows";
   }
+  if (ev) (void)hipEventRecord(ev->after, stream);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return std::string("backward launch failed: ") + hipGetErrorString(e);
   return "";

@@ -332,6 +332,10 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
   bool probe = IVAL && P.ival_hard && tstart > 0.0f;  // first gather of a hard interval with a start
   const bool logging = FULL && P.log_cap > 0;
   TravCount tc;
+  if (IVAL && tstart > P.far_t) {  // an interval past the far plane: nothing to gather, no fallback
+    if (COUNT) P.trav_stats[2 * (size_t)ray] = P.trav_stats[2 * (size_t)ray + 1] = 0;
+    return;
+  }
   while (true) {
     gather<K, BLOCK, COUNT>(P, r, tlo, plo, capped ? cap : P.far_t, b, s_node, tc);
     if (probe) {

@@ -26,8 +26,10 @@ EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build"
             "tt_multi_replay", "tt_multi_autotune", "tt_proposal_default_hparams", "tt_proposal_num_params",
             "tt_proposal_create", "tt_proposal_destroy", "tt_proposal_set_params", "tt_proposal_get_params",
             "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays", "tt_adam_step",
-            "tt_densify_plan", "tt_densify_apply", "tt_scene_unpack", "tt_scene_pack"]
+            "tt_densify_plan", "tt_densify_apply", "tt_scene_unpack", "tt_scene_pack", "tt_set_kernel_timing",
+            "tt_kernel_times"]
 MAX_CAMERAS = 32
+TIMING_RING = 64
 
 _vp = C.c_void_p
 _fp = C.c_void_p  # device pointers travel as void*
@@ -178,6 +180,8 @@ def lib():
         L.tt_multi_forward.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
         L.tt_multi_backward.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
         L.tt_multi_forward_noalloc.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
+        L.tt_set_kernel_timing.argtypes = [_vp, C.c_int]
+        L.tt_kernel_times.argtypes = [_vp, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]
         L.tt_multi_capture.argtypes = [_vp, C.POINTER(MultiArgs), C.c_int, _vp]
         L.tt_multi_replay.argtypes = [_vp, _vp]
         L.tt_multi_autotune.argtypes = [_vp, C.POINTER(MultiArgs), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
@@ -325,6 +329,17 @@ class TracerHandle:
 
     def multi_fwd(self, args: "MultiArgs", stream=None):
         check(lib().tt_multi_forward(self.h, C.byref(args), stream), "tt_multi_forward")
+
+    def set_kernel_timing(self, enable=True):
+        """Record HIP events around every traversal kernel (ring of TIMING_RING per direction)."""
+        check(lib().tt_set_kernel_timing(self.h, 1 if enable else 0), "tt_set_kernel_timing")
+
+    def kernel_times(self, which: str = "fwd", n: int = TIMING_RING):
+        """Durations (ms) of the last n timed forward ('fwd') or backward ('bwd') traversal kernels."""
+        buf = (C.c_float * max(1, n))()
+        cnt = C.c_int(0)
+        check(lib().tt_kernel_times(self.h, 0 if which == "fwd" else 1, buf, int(n), C.byref(cnt)), "tt_kernel_times")
+        return [buf[i] for i in range(cnt.value)]
 
     def multi_fwd_noalloc(self, args: "MultiArgs", stream=None):
         """tt_multi_forward with its scratch in the library's static device array (no allocation)."""

@@ -117,7 +117,7 @@ typedef struct {
      interval_in: optional [R][2] (start, cap) per ray. The gather starts at t >= start; the
      first gathers stop at cap and go on past it only while the ray is still open, so the
      result equals a trace that starts at `start` (start <= near_t: the full trace; a cap
-     below start is ignored).
+     below start is ignored; a start beyond far_t: an empty trace, in both modes below).
      interval_out: optional [R][4] per ray: t of the first and of the last blended hit, t of
      the hit that ended the ray (inf if none), first blended triangle id (int bits); the
      first field is inf for a ray with no blended hit. */
@@ -470,6 +470,18 @@ typedef struct {
   float *vert_out_dev;            /* [3P][3] */
 } tt_scene_pack_args;
 int tt_scene_pack(const tt_scene_pack_args *args, void *stream);
+
+/* Per-kernel timing (the counterpart of the reference's debug timing of its launch,
+   tracer.cpp:110-137, at kernel granularity): while enabled, every tt_trace_fwd / tt_trace_bwd
+   records HIP events on its stream right before and after its traversal kernel(s) -- not the
+   tile-schedule kernels around them -- into a ring of TT_TIMING_RING launches per direction.
+   Enabling (or disabling) clears the ring. */
+#define TT_TIMING_RING 64
+int tt_set_kernel_timing(tt_tracer *tr, int enable);
+/* Durations in ms of the last min(max_n, TT_TIMING_RING, launches timed) launches of the forward
+   (which = 0) or backward (which = 1) traversal kernels, oldest first; *count_out = how many were
+   written. Waits for those launches to finish. */
+int tt_kernel_times(tt_tracer *tr, int which, float *ms_out, int max_n, int *count_out);
 
 /* Number of triangles of the current acceleration structure (-1 if none). */
 int tt_num_primitives(const tt_tracer *tr);

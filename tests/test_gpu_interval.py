@@ -138,6 +138,26 @@ def test_hard_cap_with_the_emitted_interval_reproduces_plain(small):
     assert np.array_equal(out["color"][:, ~has], plain["color"][:, ~has])
 
 
+def test_hard_cap_with_perfect_intervals_reproduces_plain(small):
+    # the proposal's ceiling: each ray's own blended range as (start, cap), rays that only
+    # terminate start at their terminating hit, rays that blend nothing start past the far
+    # plane (empty trace, no fallback); the image, normals and alpha equal plain bit for bit
+    sc, ro, rd, g, S, plain, log = small
+    iv = plain["interval"]
+    first = np.where(np.isfinite(iv[:, 0]), iv[:, 0], iv[:, 2])
+    stop = np.where(np.isfinite(iv[:, 2]), iv[:, 2], iv[:, 1])
+    ivi = np.stack([np.where(np.isfinite(first), first, 3.0e38), np.where(np.isfinite(stop), stop, 0.0)],
+                   1).astype(np.float32)
+    empty = ~np.isfinite(first)
+    _say(f"hard cap on perfect intervals; {int(empty.sum())} rays skipped as empty")
+    assert empty.sum() > 50
+    out = g.forward(ro, rd, k=K, cutoff=CUT, chunk_mode="exact", interval_in=ivi, hard_cap=True)
+    assert np.array_equal(out["color"], plain["color"]) and np.array_equal(out["others"], plain["others"])
+    assert np.all(out["stats"][empty] == 0)
+    soft = g.forward(ro, rd, k=K, cutoff=CUT, chunk_mode="exact", interval_in=ivi)
+    assert np.array_equal(soft["color"], plain["color"])
+
+
 def test_backward_bounded_by_forward_interval(small):
     sc, ro, rd, g, S, plain, log = small
     R = ro.shape[0] * ro.shape[1]
