@@ -39,7 +39,7 @@ class GpuScene:
 
     def forward(self, ray_o, ray_d, k=16, cutoff=1e-3, chunk_mode="compat", variant="forward", extra=None,
                 bg=(0.0, 0.0, 0.0), log_cap=0, stats=True, interval_in=None, interval_out=False, hard_cap=False,
-                fallback=(0.0, 0.0), hitlist_cap=0):
+                fallback=(0.0, 0.0), hitlist_cap=0, trav=False):
         from diff_tritracer import capi
 
         torch = self.torch
@@ -53,6 +53,7 @@ class GpuScene:
         dens = torch.zeros(self.P, device=dev)
         scal = torch.zeros(self.P, device=dev)
         st = torch.zeros(R, 2, dtype=torch.int32, device=dev) if stats else None
+        tv = torch.zeros(R, 2, dtype=torch.int32, device=dev) if trav else None
         bgt = torch.tensor(bg, dtype=torch.float32, device=dev)
         ext = to_dev(np.asarray(extra, np.float32), dev) if extra is not None else None
         log = None
@@ -70,13 +71,16 @@ class GpuScene:
         a = capi.fwd_args(W, H, ro, rd, self.verts, self.npts, self.cumsum, self.shs, 3, self.opac, self.sigma, bgt,
                           color, others, maxb, dens, scal, k=k, cutoff=cutoff, chunk_mode=chunk_mode,
                           variant=variant, extra=ext, stats=st, log=log, interval_in=ivi, interval_out=ivo,
-                          hard_cap=hard_cap, fallback_start=fallback[0], fallback_end=fallback[1], hitlist=hl)
+                          hard_cap=hard_cap, fallback_start=fallback[0], fallback_end=fallback[1], hitlist=hl,
+                          trav_stats=tv)
         self.h.fwd(a, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         out = dict(color=color.cpu().numpy(), others=others.cpu().numpy(), max_blending=maxb.cpu().numpy(),
                    density=dens.cpu().numpy(), scaling=scal.cpu().numpy(), _color=color, _others=others)
         if stats:
             out["stats"] = st.cpu().numpy()
+        if trav:
+            out["trav"] = tv.cpu().numpy()
         if log is not None:
             out["log"] = tuple(x.cpu().numpy() for x in log)
         if ivo is not None:
@@ -161,3 +165,33 @@ def grad_close(g, ref, rel=1e-3):
     ratio = np.abs(g - ref) / den
     worst = float(ratio.max())
     return worst <= rel, worst
+
+
+def padded_log(export, R, cap):
+    """Oracle decision log (single-segment renders) as padded arrays like the GPU debug log:
+    prim [R, cap] (-1 past the end), t [R, cap] (0), code [R, cap] (-1), count [R]."""
+    off, spix, prims, ts, codes = export
+    cnt = np.diff(off).astype(np.int64)
+    count = np.zeros(R, np.int32)
+    count[spix] = cnt
+    n = int(off[-1])
+    pix = np.repeat(spix.astype(np.int64), cnt)
+    idx = np.arange(n, dtype=np.int64) - np.repeat(off[:-1].astype(np.int64), cnt)
+    m = idx < cap
+    P = np.full((R, cap), -1, np.int32)
+    T = np.zeros((R, cap), np.float32)
+    Cd = np.full((R, cap), -1, np.int32)
+    P[pix[m], idx[m]] = prims[m]
+    T[pix[m], idx[m]] = ts[m]
+    Cd[pix[m], idx[m]] = codes[m]
+    return P, T, Cd, count
+
+
+def compare_hit_logs_fast(gpu_log, export, R):
+    """Vectorised compare_hit_logs: the number of rays whose count or first `cap` logged
+    hits (prim, t bits, decision) differ (the GPU buffers must start as -1 / 0 / -1)."""
+    gprim, gt, gcode, gcount = gpu_log
+    P, T, Cd, count = padded_log(export, R, gprim.shape[1])
+    bad = (gcount != count) | np.any(gprim != P, axis=1) | np.any(gcode != Cd, axis=1) | \
+        np.any(gt.view(np.uint32) != T.view(np.uint32), axis=1)
+    return int(bad.sum())
