@@ -32,6 +32,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
       .def("set_options", &OptixTracer::set_options, pybind11::arg("k") = 16, pybind11::arg("cutoff") = 1e-4,
            pybind11::arg("chunk_mode") = 0, pybind11::arg("hit_cache") = 64)
       .def("get_options", &OptixTracer::get_options)
+      .def("set_store_hits", &OptixTracer::set_store_hits, pybind11::arg("store"))
       .def_property_readonly("variant", &OptixTracer::variant_name);
 
   m.def("version", []() { return std::string(tt_version()); });

@@ -70,7 +70,8 @@ OptixTracer::~OptixTracer() {
 
 OptixTracer::OptixTracer(OptixTracer &&o) noexcept
     : fwdState(std::move(o.fwdState)), bwdState(std::move(o.bwdState)), tr_(o.tr_), variant_(o.variant_), k_(o.k_),
-      cutoff_(o.cutoff_), chunk_mode_(o.chunk_mode_), hit_cache_(o.hit_cache_), device_(o.device_) {
+      cutoff_(o.cutoff_), chunk_mode_(o.chunk_mode_), hit_cache_(o.hit_cache_), store_hits_(o.store_hits_),
+      device_(o.device_) {
   o.tr_ = nullptr;
 }
 
@@ -85,6 +86,7 @@ OptixTracer &OptixTracer::operator=(OptixTracer &&o) noexcept {
     cutoff_ = o.cutoff_;
     chunk_mode_ = o.chunk_mode_;
     hit_cache_ = o.hit_cache_;
+    store_hits_ = o.store_hits_;
     device_ = o.device_;
     o.tr_ = nullptr;
   }
@@ -210,7 +212,7 @@ FwdTraceResult OptixTracer::trace_fwd(const Tensor &ray_o, const Tensor &ray_d, 
   a.density_dev = dens.data_ptr<float>();
   a.scaling_dev = scal.data_ptr<float>();
   Tensor hits, hit_counts;
-  if (hit_cache_ > 0 && variant_ == TT_VARIANT_FORWARD && (long long)H * W > 0) {
+  if (hit_cache_ > 0 && store_hits_ && variant_ == TT_VARIANT_FORWARD && (long long)H * W > 0) {
     auto iopts = torch::TensorOptions().dtype(torch::kInt32).device(torch::kCUDA, dev);
     hits = torch::empty({hit_cache_, (long long)H * W, 2}, iopts);
     hit_counts = torch::empty({(long long)H * W}, iopts);

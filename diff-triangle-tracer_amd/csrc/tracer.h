@@ -76,6 +76,9 @@ class OptixTracer {
   // pixel the forward stores so that trace_bwd reads them instead of re-traversing
   // (0: re-traverse, as the reference does; same gradients either way).
   void set_options(int k, double cutoff, int chunk_mode, int hit_cache);
+  // Whether the next trace_fwd stores its blended hits (default true). The Python autograd
+  // Function clears it when no input needs a gradient (no backward will read them).
+  void set_store_hits(bool store) { store_hits_ = store; }
   std::tuple<int, double, int, int> get_options() const;
   std::string variant_name() const;
 
@@ -93,5 +96,6 @@ class OptixTracer {
   float cutoff_ = 1e-4f;
   int chunk_mode_ = TT_CHUNK_COMPAT;
   int hit_cache_ = 64;
+  bool store_hits_ = true;
   int device_ = -1;
 };

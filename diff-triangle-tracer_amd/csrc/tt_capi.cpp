@@ -45,6 +45,13 @@ struct tt_tracer {
   // backward gradient records: 2048 rows of 80 B per block of the one-shot backward
   float4 *rec_rows = nullptr;
   size_t rec_blocks = 0;  // rows allocated
+  // blended hits of the last one-shot forwards (their record-region total), copied to pinned
+  // host memory after each forward; read only once its event has completed, so the pool
+  // grows to the largest total seen without making a backward wait for its forward
+  int *hits_total_host = nullptr;
+  hipEvent_t hits_total_ev = nullptr;
+  bool hits_total_pending = false;
+  size_t hits_high_water = 0;
   // global grouping of the bounce variant's secondary-segment records: per record row a key
   // and the radix-sort scratch (keys, values x 2, digit counts)
   uint32_t *rec_keys = nullptr, *sort_keys2 = nullptr, *sort_vals = nullptr, *sort_vals2 = nullptr;
@@ -252,12 +259,19 @@ int ensure_tiles(tt_tracer *tr, int nt) {
       hipMalloc(&tr->tile_order, (size_t)nt * sizeof(int)) != hipSuccess ||
       hipMalloc(&tr->probe_order, (size_t)nt * sizeof(int)) != hipSuccess ||
       hipMalloc(&tr->tile_hits, (size_t)nt * sizeof(int)) != hipSuccess ||
-      hipMalloc(&tr->rec_off, (size_t)nt * sizeof(int)) != hipSuccess ||
+      hipMalloc(&tr->rec_off, ((size_t)nt + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc(&tr->tile_hits_b, (size_t)nt * sizeof(int)) != hipSuccess ||
       hipMalloc(&tr->tile_cost_b, (size_t)nt * sizeof(float)) != hipSuccess)
     return fail(TT_ERR_HIP, "hipMalloc of the tile schedule failed");
   tr->tile_cap = nt;
   return TT_OK;
+}
+
+// The record-region total of an earlier forward, if its copy has landed (never waits).
+void poll_hits_total(tt_tracer *tr) {
+  if (!tr->hits_total_pending || hipEventQuery(tr->hits_total_ev) != hipSuccess) return;
+  tr->hits_total_pending = false;
+  if (*tr->hits_total_host > 0) tr->hits_high_water = std::max(tr->hits_high_water, (size_t)*tr->hits_total_host);
 }
 
 void fill_common(tt::TraceParams &p, const tt_tracer *tr, int W, int H, const float *ro, const float *rd,
@@ -341,6 +355,8 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->tile_cost_b) (void)hipFree(tr->tile_cost_b);
   free_records(tr);
   free_rec_sort(tr);
+  if (tr->hits_total_host) (void)hipHostFree(tr->hits_total_host);
+  if (tr->hits_total_ev) (void)hipEventDestroy(tr->hits_total_ev);
   (void)hipSetDevice(dev);
   delete tr;
   return ok();
@@ -518,6 +534,20 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
                                        tr->timing ? &kev : nullptr);
   if (!err.empty()) return fail(TT_ERR_HIP, err);
   if (tr->timing) tr->tcount[0]++;
+  poll_hits_total(tr);
+  if (!tr->hits_total_pending && a->variant == TT_VARIANT_FORWARD) {  // publish this frame's record total
+    if (!tr->hits_total_host && hipHostMalloc((void **)&tr->hits_total_host, sizeof(int)) != hipSuccess)
+      tr->hits_total_host = nullptr;
+    if (tr->hits_total_host && !tr->hits_total_ev && hipEventCreateWithFlags(&tr->hits_total_ev, hipEventDisableTiming) != hipSuccess)
+      tr->hits_total_ev = nullptr;
+    if (tr->hits_total_host && tr->hits_total_ev) {
+      const int nb = tt::backward_tiles(a->width, a->height);
+      hipStream_t s = static_cast<hipStream_t>(stream);
+      if (hipMemcpyAsync(tr->hits_total_host, tr->rec_off + nb, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess &&
+          hipEventRecord(tr->hits_total_ev, s) == hipSuccess)
+        tr->hits_total_pending = true;
+    }
+  }
   tr->sched_W = a->width;
   tr->sched_H = a->height;
   tr->multi_tiles = -1;
@@ -600,7 +630,13 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
     p.denv = a->dL_denvmap_dev;
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
-  if ((rc = prepare_records(tr, tt::trace_lanes(a->width, a->height), p)) != TT_OK) return rc;
+  {  // 16 rows per lane, or the largest record total a forward has needed so far
+    poll_hits_total(tr);
+    const size_t lanes = tt::trace_lanes(a->width, a->height);
+    const size_t want = std::min<size_t>(tr->hits_high_water, (size_t)1 << 28);  // at most 20 GiB of rows
+    const int lane_rows = (int)std::max<size_t>(16, (want + lanes - 1) / lanes);
+    if ((rc = prepare_records(tr, lanes, p, lane_rows)) != TT_OK) return rc;
+  }
   if (a->variant == TT_VARIANT_BOUNCE && (rc = prepare_rec_sort(tr, p, (size_t)p.rec_cap)) != TT_OK) return rc;
   if (tr->sched_W == a->width && tr->sched_H == a->height) {  // the last forward's schedule and record regions
     p.tile_sched = tr->tile_order;

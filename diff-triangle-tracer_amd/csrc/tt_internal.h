@@ -164,8 +164,10 @@ std::string launch_fill_int(int *ptr, int n, int v, hipStream_t stream);
 // Message returned by tt_last_error() on this thread (shared by every entry point).
 void set_last_error(const std::string &msg);
 
-// Tiles (blocks) of a one-shot trace launch for an image of W x H.
+// Tiles (blocks) of a one-shot trace launch for an image of W x H, and of the backward
+// (whose tiles are two forward tiles stacked).
 int trace_tiles(int W, int H);
+int backward_tiles(int W, int H);
 // Tiles ordered by cost, most expensive first (one block), into order[n].
 std::string launch_tile_order(const float *cost, int n, int *order, hipStream_t stream);
 // Lanes a trace launch uses for an image of W x H (the stack spill needs one slot each).

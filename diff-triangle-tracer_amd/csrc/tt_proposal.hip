@@ -54,13 +54,18 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// sin / cos of 2 pi u. v_sin / v_cos take revolutions and are accurate for |u| <= 256
+// only, so the whole revolutions are removed first (u - rint(u) is exact in fp32).
+__device__ __forceinline__ float sin_rev(float u) { return __builtin_amdgcn_sinf(u - rintf(u)); }
+__device__ __forceinline__ float cos_rev(float u) { return __builtin_amdgcn_cosf(u - rintf(u)); }
+
 // Feature e of the encoding of v[6] (e >= 102: padding).
 __device__ __forceinline__ float feature(const float v[6], int e) {
   if (e < 6) return v[e];
   if (e >= kIn) return 0.0f;
   const int q = e - 6, dim = q >> 4, f = (q & 15) >> 1;
-  const float u = ldexpf(v[dim], f - 1);  // 2^f pi v = 2 pi u: v_sin/v_cos take revolutions
-  return (q & 1) ? __builtin_amdgcn_cosf(u) : __builtin_amdgcn_sinf(u);
+  const float u = ldexpf(v[dim], f - 1);  // 2^f pi v = 2 pi u
+  return (q & 1) ? cos_rev(u) : sin_rev(u);
 }
 
 __device__ __forceinline__ bf16x8 frag(const bf16x8 *F, int slot) {
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(64) void k_prop_infer(const float *ro, const float 
 #pragma unroll
       for (int i = 0; i < 32; i++) {
         const float u = ldexpf(vd[i < 6 ? 0 : (i < 22 ? 1 : 2)], (((i + 10) & 15) >> 1) - 1);
-        float val = (i & 1) ? __builtin_amdgcn_cosf(u) : __builtin_amdgcn_sinf(u);
+        float val = (i & 1) ? cos_rev(u) : sin_rev(u);
         if (i < 6 && g == 0) val = v[i];
         if (32 * g + i >= kIn || ray >= R) val = 0.0f;
         pk[i >> 3][i & 7] = (bf16)val;

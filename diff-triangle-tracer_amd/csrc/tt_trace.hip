@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_bounce_secondary(TraceParams P, c
 }
 
 // Exclusive prefix sums of the tiles' blended-hit counts (one block): the backward's
-// per-tile record regions.
+// per-tile record regions; off[n] = the total (capped at 2^31 - 1).
 __global__ __launch_bounds__(1024) void k_tile_offsets(const int *hits, int n, int *off) {
   __shared__ long long s_part[1024];
   const int t = threadIdx.x;
@@ -73,6 +73,7 @@ __global__ __launch_bounds__(1024) void k_tile_offsets(const int *hits, int n, i
     off[i] = (int)min(run, (long long)0x7fffffff);
     run += hits[i];
   }
+  if (t == 1023) off[n] = (int)min(s_part[1023], (long long)0x7fffffff);
 }
 
 // Cost estimate of every tile before the forward (probe_tile), one lane per tile.
@@ -260,6 +261,10 @@ std::string launch_tile_order(const float *cost, int n, int *order, hipStream_t 
 
 int trace_tiles(int W, int H) {  // the forward's tiles (at least as many as the backward's)
   return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH);
+}
+
+int backward_tiles(int W, int H) {
+  return ((W + kTileW - 1) / kTileW) * ((H + 2 * kTileH - 1) / (2 * kTileH));
 }
 
 std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream_t stream, const KernelEvents *ev) {

@@ -68,7 +68,15 @@ class _Tracer(torch.autograd.Function):
                 number_of_points, shs, tracer_settings.sh_degree, colors_precomp, opacities, scaling, density_factor,
                 tracer_settings.bg, tracer_settings.extra_params, tracer_settings.rebuild_gas,
                 tracer_settings.debug)
-        res = _call_with_snapshot(tracer.trace_forward, args, tracer_settings.debug, "snapshot_fw.dump", "forward")
+        # the hit list is only worth writing when a backward can read it
+        store = any(ctx.needs_input_grad)
+        if not store:
+            tracer.set_store_hits(False)
+        try:
+            res = _call_with_snapshot(tracer.trace_forward, args, tracer_settings.debug, "snapshot_fw.dump", "forward")
+        finally:
+            if not store:
+                tracer.set_store_hits(True)
         color, others, gpu_time, scaling, density_factor, max_blending = res.data
 
         ctx.hits = (res.hits, res.hit_counts)  # stored blended hits (None: the backward re-traverses)
@@ -124,9 +132,16 @@ class Tracer(nn.Module):
             shs = torch.empty(0, device=device)
         if colors_precomp is None:
             colors_precomp = torch.empty(0, device=device)
-        return _Tracer.apply(self.tracer, ray_o, ray_d, triangles_points, sigma, num_points_per_triangle,
-                             cumsum_of_points_per_triangle, number_of_points, shs, colors_precomp, opacities, scaling,
-                             density_factor, tracer_settings)
+        grad = torch.is_grad_enabled()  # no backward under no_grad: the forward stores no hit list
+        if not grad:
+            self.tracer.set_store_hits(False)
+        try:
+            return _Tracer.apply(self.tracer, ray_o, ray_d, triangles_points, sigma, num_points_per_triangle,
+                                 cumsum_of_points_per_triangle, number_of_points, shs, colors_precomp, opacities,
+                                 scaling, density_factor, tracer_settings)
+        finally:
+            if not grad:
+                self.tracer.set_store_hits(True)
 
 
 from .optim import FusedAdam  # noqa: E402  (the optimizer step on the library's Adam kernel)

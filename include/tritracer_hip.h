@@ -282,7 +282,8 @@ int tt_multi_autotune(tt_tracer *tr, tt_multi_args *a, const int *ks, int nk, co
    mapping a ray to (start, end) of the depth range where its blended hits lie; trained
    on the GPU with Adam on the intervals tt_trace_fwd emits (interval_out), and used to
    fill tt_fwd_args.interval_in. Input encoding: (o / enc_scale, d) and sin, cos of
-   2^f pi v, f = 0..7, for each of those six values (|o| / enc_scale must stay below 64). */
+   2^f pi v, f = 0..7, for each of those six values (whole periods removed first, exact in
+   fp32, so any ray origin is valid). */
 typedef struct tt_proposal tt_proposal;
 
 typedef struct {

@@ -134,10 +134,16 @@ class _FakeTracer:
         self.fwd_args = None
         self.bwd_args = None
 
+    store = True
+
+    def set_store_hits(self, store):
+        self.store = store
+
     def trace_forward(self, *args):
         if self.fail:
             raise RuntimeError("forced failure")
         self.fwd_args = args
+        self.stored = self.store
         ray_o, P = args[0], args[6]
         H, W = ray_o.shape[:2]
         color = torch.full((3, H, W), 0.5)
@@ -183,6 +189,27 @@ def test_autograd_wiring_and_gradient_layouts():
     assert torch.all(x["verts"].grad == 1.0) and x["verts"].grad.shape == x["verts"].shape
     assert torch.all(x["sigma"].grad == 2.0) and x["sigma"].grad.shape == (5, 1)
     assert torch.all(x["opac"].grad == 3.0) and torch.all(x["shs"].grad == 4.0)
+
+
+def test_hit_list_stored_only_when_a_backward_can_need_it():
+    tr = D.Tracer.__new__(D.Tracer)
+    torch.nn.Module.__init__(tr)
+    fake = tr.tracer = _FakeTracer()
+
+    def call(x):
+        return tr(x["ray_o"], x["ray_d"], x["verts"], x["sigma"], x["npts"], x["cumsum"], x["P"], x["opac"],
+                  x["scaling"], x["density"], x["shs"], None, x["st"])
+
+    call(_inputs())
+    assert fake.stored is True
+    with torch.no_grad():
+        call(_inputs())
+    assert fake.stored is False and fake.store is True   # restored after the call
+    y = _inputs()
+    for k in ("verts", "sigma", "opac", "shs"):
+        y[k] = y[k].detach()
+    call(y)
+    assert fake.stored is False and fake.store is True
 
 
 def test_tracer_requires_exactly_one_colour_source():
