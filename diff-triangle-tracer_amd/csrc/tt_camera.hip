@@ -8,28 +8,24 @@
 #include <string>
 
 #include "../../include/tritracer_hip.h"
+#include "tt_device.h"
 #include "tt_internal.h"
 
 namespace {
 
 struct Cam {
-  double eye[3], fwd[3], right[3], up[3];
-  double tx, ty;
+  double c[tt::kCamDoubles];  // eye, forward, right, up, tan_x, tan_y
   int W, H;
 };
 
 __global__ void k_camera_rays(Cam c, float *ro, float *rd) {
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (long long)c.W * c.H) return;
-  const int x = (int)(p % c.W), y = (int)(p / c.W);
-  const double xs = (2.0 * ((double)x + 0.5) / (double)c.W - 1.0) * c.tx;
-  const double ys = (1.0 - 2.0 * ((double)y + 0.5) / (double)c.H) * c.ty;
-  double d[3];
-  for (int i = 0; i < 3; i++) d[i] = (c.fwd[i] + xs * c.right[i]) + ys * c.up[i];
-  const double n = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+  float o[3], d[3];
+  tt::pinhole_ray(c.c, c.W, c.H, (int)(p % c.W), (int)(p / c.W), o, d);
   for (int i = 0; i < 3; i++) {
-    rd[3 * p + i] = (float)(d[i] / n);
-    ro[3 * p + i] = (float)c.eye[i];
+    rd[3 * p + i] = d[i];
+    ro[3 * p + i] = o[i];
   }
 }
 
@@ -48,9 +44,9 @@ extern "C" int tt_camera_rays(int width, int height, const double *eye, const do
     return TT_ERR_INVALID_ARG;
   }
   Cam c;
-  for (int i = 0; i < 3; i++) c.eye[i] = eye[i], c.fwd[i] = forward[i], c.right[i] = right[i], c.up[i] = up[i];
-  c.tx = tan_x;
-  c.ty = tan_y;
+  for (int i = 0; i < 3; i++) c.c[i] = eye[i], c.c[3 + i] = forward[i], c.c[6 + i] = right[i], c.c[9 + i] = up[i];
+  c.c[12] = tan_x;
+  c.c[13] = tan_y;
   c.W = width;
   c.H = height;
   const long long n = (long long)width * height;

@@ -65,6 +65,8 @@ struct tt_tracer {
   unsigned long long mem_gen = 1, sched_gen = 1;
   unsigned long long graph_mem_gen = 0, graph_sched_gen = 0;
   int graph_what = 0;
+  float *loss_part = nullptr;  // per-tile sums of the fused loss of a multi-camera forward
+  int loss_cap = 0;
   int noalloc_grid[3][3] = {};  // persistent forward grid per (k, block) of tt_multi_forward_noalloc (0: unset)
   // per-kernel timing (tt_set_kernel_timing): a ring of event pairs per direction (0 forward,
   // 1 backward) around the traversal kernels, and the number of timed launches
@@ -78,14 +80,16 @@ struct tt_tracer {
 // name `scratch` is a host-side stand-in, not a device address: kernels and fills get the
 // address hipGetSymbolAddress returned for the current device, kept in g_scratch_dev.
 // Layout (ints): [0, 64) work-queue heads, then TT_NOALLOC_MAX_TILES tile costs (float),
-// TT_NOALLOC_MAX_TILES tile order entries, then the traversal stack spill.
+// TT_NOALLOC_MAX_TILES tile order entries, TT_NOALLOC_MAX_TILES per-tile loss sums (float, the
+// fused loss), then the traversal stack spill.
 static __device__ __align__(256) int scratch[TT_NOALLOC_SCRATCH_BYTES / sizeof(int)];
 
 namespace {
 constexpr int kScratchDevices = 64;
 constexpr size_t kScratchInts = TT_NOALLOC_SCRATCH_BYTES / sizeof(int);
 constexpr size_t kScratchCostOff = 64, kScratchOrderOff = kScratchCostOff + TT_NOALLOC_MAX_TILES;
-constexpr size_t kScratchSpillOff = kScratchOrderOff + TT_NOALLOC_MAX_TILES;
+constexpr size_t kScratchLossOff = kScratchOrderOff + TT_NOALLOC_MAX_TILES;
+constexpr size_t kScratchSpillOff = kScratchLossOff + TT_NOALLOC_MAX_TILES;
 static_assert(kScratchSpillOff % 64 == 0, "spill area must stay 256-byte aligned");
 std::mutex g_scratch_mu;
 int *g_scratch_dev[kScratchDevices];  // device address of `scratch` on device d (null: not set up)
@@ -355,6 +359,7 @@ int tt_destroy(tt_tracer *tr) {
   if (tr->tile_cost_b) (void)hipFree(tr->tile_cost_b);
   free_records(tr);
   free_rec_sort(tr);
+  if (tr->loss_part) (void)hipFree(tr->loss_part);
   if (tr->hits_total_host) (void)hipHostFree(tr->hits_total_host);
   if (tr->hits_total_ev) (void)hipEventDestroy(tr->hits_total_ev);
   (void)hipSetDevice(dev);
@@ -685,12 +690,23 @@ int multi_params(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::Multi
   mc.tile_begin[a->n_cameras] = tiles;
   mc.total_tiles = tiles;
   for (int q = 0; q <= 8; q++) mc.q_begin[q] = (int)((long long)tiles * q / 8);
-  int rc = check_common(tr, 1, 1, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev, a->cumsum_dev, a->P,
-                        a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->k, a->chunk_mode);
+  const int with_rays = a->cameras_dev ? 0 : 1;  // fused ray generation reads no ray buffers
+  int rc = check_common(tr, with_rays, with_rays, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev,
+                        a->cumsum_dev, a->P, a->shs_dev, a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->k,
+                        a->chunk_mode);
   if (rc != TT_OK) return rc;
   fill_common(p, tr, 1, 1, a->ray_o_dev, a->ray_d_dev, a->vertices_dev, a->npts_dev, a->cumsum_dev, a->P, a->shs_dev,
               a->M, a->sh_degree, a->opacities_dev, a->sigma_dev, a->chunk_mode, a->cutoff, a->near_t, a->far_t);
   p.bg = a->bg_dev;
+  p.cams = a->cameras_dev;
+  if (!backward && a->target_dev) {
+    TT_ARG(a->loss_kind == TT_LOSS_L2 || a->loss_kind == TT_LOSS_L1, "loss_kind must be TT_LOSS_L2 or TT_LOSS_L1");
+    TT_ARG(a->loss_grad_dev && a->loss_dev, "the fused loss needs loss_grad_dev and loss_dev");
+    p.target = a->target_dev;
+    p.loss_grad = a->loss_grad_dev;
+    p.loss_kind = a->loss_kind;
+    p.loss_norm = (float)(1.0 / (3.0 * (double)rays));
+  }
   if (!backward) {
     TT_ARG(a->color_dev && a->others_dev && a->bg_dev, "colour / others / background buffers are NULL");
     if (a->P > 0) TT_ARG(a->max_blending_dev && a->density_dev && a->scaling_dev, "statistics buffers are NULL");
@@ -754,6 +770,18 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
     tr->mem_gen++;
   }
   if ((rc = ensure_tiles(tr, tiles)) != TT_OK) return rc;
+  if (p.target) {
+    if (tiles > tr->loss_cap) {
+      if (tr->loss_part) (void)hipFree(tr->loss_part);
+      tr->loss_part = nullptr;
+      tr->loss_cap = 0;
+      if (hipMalloc(&tr->loss_part, (size_t)tiles * sizeof(float)) != hipSuccess)
+        return fail(TT_ERR_HIP, "hipMalloc of the loss partial sums failed");
+      tr->loss_cap = tiles;
+      tr->mem_gen++;
+    }
+    p.loss_part = tr->loss_part;
+  }
   if (!backward) {  // schedule: probe order for the forward, measured tile times for the backward
     p.tile_cost = tr->tile_cost;
     p.tile_order = tr->tile_order;
@@ -789,7 +817,8 @@ int multi_issue(tt_tracer *tr, const tt_multi_args *a, int what, hipStream_t s) 
   int grid = 0, rc;
   if (what & 1) {
     if ((rc = multi_setup(tr, a, false, mc, p, grid)) != TT_OK) return rc;
-    std::string e = tt::launch_multi(p, mc, tr->heads, a->k, a->block, a->tiles_per_fetch, grid, false, s);
+    std::string e = tt::launch_multi(p, mc, tr->heads, a->k, a->block, a->tiles_per_fetch, grid, false, s,
+                                     a->loss_dev);
     if (!e.empty()) return fail(TT_ERR_HIP, e);
   }
   if (what & 2) {
@@ -849,12 +878,13 @@ int tt_multi_forward_noalloc(tt_tracer *tr, const tt_multi_args *a, void *stream
   }
   p.tile_cost = reinterpret_cast<float *>(base + kScratchCostOff);
   p.probe_order = base + kScratchOrderOff;
+  if (p.target) p.loss_part = reinterpret_cast<float *>(base + kScratchLossOff);
   p.tile_order = nullptr;  // no schedule for a backward: the tracer's own is left alone
   tr->sched_W = tr->sched_H = -1;
   tr->multi_tiles = -1;    // a following tt_multi_backward takes its tiles in queue order
   tr->sched_gen++;
   std::string e = tt::launch_multi(p, mc, base, a->k, a->block, a->tiles_per_fetch, grid, false,
-                                   static_cast<hipStream_t>(stream));
+                                   static_cast<hipStream_t>(stream), a->loss_dev);
   if (!e.empty()) return fail(TT_ERR_HIP, e);
   return ok();
 }

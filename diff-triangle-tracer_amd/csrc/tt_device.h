@@ -34,6 +34,24 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f, SH
 struct v3f {
   float x, y, z;
 };
+
+// Pinhole camera ray of pixel (x, y) of a W x H image: cam = eye[3] forward[3] right[3] up[3]
+// tan_x tan_y (double). The direction forward + xs right + ys up with
+// xs = (2 (x + 0.5) / W - 1) tan_x, ys = (1 - 2 (y + 0.5) / H) tan_y is formed and normalised
+// in double in this fixed order and rounded once to float (tt_camera_rays and the fused ray
+// generation of the multi-camera launches both use this routine).
+constexpr int kCamDoubles = 14;
+__device__ __forceinline__ void pinhole_ray(const double *cam, int W, int H, int x, int y, float o[3], float d[3]) {
+  const double xs = (2.0 * ((double)x + 0.5) / (double)W - 1.0) * cam[12];
+  const double ys = (1.0 - 2.0 * ((double)y + 0.5) / (double)H) * cam[13];
+  double v[3];
+  for (int i = 0; i < 3; i++) v[i] = (cam[3 + i] + xs * cam[6 + i]) + ys * cam[9 + i];
+  const double n = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+  for (int i = 0; i < 3; i++) {
+    d[i] = (float)(v[i] / n);
+    o[i] = (float)cam[i];
+  }
+}
 struct v3d {
   double x, y, z;
 };

@@ -128,6 +128,16 @@ struct TraceParams {
   const int *hl_count_in;
   size_t hl_stride;  // entries between hit i and hit i + 1 of a pixel (the launch's total rays)
   int rec_lane_rows;  // rows per launched lane of a fixed record region (no per-tile regions)
+  // fused ray generation: this view's pinhole camera (14 doubles in device memory: eye,
+  // forward, right, up, tan_x, tan_y), or null (rays read from ray_o / ray_d); cams = the
+  // camera table [n][14] of a multi-camera launch
+  const double *cam, *cams;
+  // fused loss of a forward: target image [3][R] of this view; dL/dcolour is written to
+  // loss_grad [3][R] and each tile's loss sum to loss_part[tile] (global tile index)
+  const float *target;
+  float *loss_grad, *loss_part;
+  int loss_kind;    // 1: mean of (c - t)^2, 2: mean of |c - t| over the batch's channels and pixels
+  float loss_norm;  // 1 / (3 x the batch's rays)
   // global grouping of records (the bounce variant's secondary segments, whose rays are too
   // incoherent for per-block grouping): the triangle of every record row (rows never
   // written hold 0xffffffff) and the radix-sort scratch (rec_cap entries each, hist:
@@ -157,8 +167,9 @@ void multi_tile_shape(int block, int &tw, int &th);
 // Blocks that fill the device for the persistent kernel (k, block, backward?), 0 if unsupported.
 int multi_grid(int k, int block, bool backward);
 // Persistent launch over all cameras; `heads` = 8 device ints (reset inside, graph-safe).
+// A forward with a fused loss (p.loss_part) writes the batch's loss to *loss_out.
 std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, int k, int block, int tiles_per_fetch,
-                         int grid, bool backward, hipStream_t stream);
+                         int grid, bool backward, hipStream_t stream, float *loss_out = nullptr);
 std::string launch_fill_int(int *ptr, int n, int v, hipStream_t stream);
 
 // Message returned by tt_last_error() on this thread (shared by every entry point).

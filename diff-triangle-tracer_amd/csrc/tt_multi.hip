@@ -56,6 +56,9 @@ TT_D TraceParams camera_view(const TraceParams &P, const MultiCams &mc, int tile
   if (P.fothers) Q.fothers = P.fothers + 7 * off;
   if (P.dpix) Q.dpix = P.dpix + 3 * off;
   if (P.dothers) Q.dothers = P.dothers + 7 * off;
+  if (P.cams) Q.cam = P.cams + kCamDoubles * c;
+  if (P.target) Q.target = P.target + 3 * off;
+  if (P.loss_grad) Q.loss_grad = P.loss_grad + 3 * off;
   if (P.hl) Q.hl = P.hl + off;  // hit lists keep the launch-wide stride P.hl_stride
   if (P.hl_count) Q.hl_count = P.hl_count + off;
   if (P.hl_in) Q.hl_in = P.hl_in + off;
@@ -79,6 +82,7 @@ template <int K, int BLOCK>
 __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, MultiCams mc, int *heads, int G) {
   __shared__ int s_node[kLdsInts * BLOCK];
   __shared__ int s_range[2];
+  __shared__ float s_loss[BLOCK / 64];
   while (true) {
     if (threadIdx.x == 0) {
       int e = 0;
@@ -94,7 +98,19 @@ __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, Multi
       const long long c0 = P.tile_cost ? wall_clock64() : 0;
       int lt, x, y;
       const TraceParams Q = camera_view(P, mc, t, lt);
-      if (tile_pixel<BLOCK>(Q, lt, x, y)) forward_pixel<K, BLOCK, 0, false>(Q, y * Q.W + x, s_node);
+      float loss = 0.0f;
+      if (tile_pixel<BLOCK>(Q, lt, x, y)) forward_pixel<K, BLOCK, 0, false>(Q, y * Q.W + x, s_node, &loss);
+      if (P.loss_part) {  // the tile's loss: lanes summed in a fixed order (wave tree, then the waves)
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) s_loss[threadIdx.x >> 6] = loss;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          float sum = 0.0f;
+          for (int w = 0; w < BLOCK / 64; w++) sum += s_loss[w];
+          P.loss_part[t] = sum;
+        }
+      }
       if (P.tile_cost) {  // the tile's wall time: the backward's schedule key
         __syncthreads();
         if (threadIdx.x == 0) P.tile_cost[t] = (float)(wall_clock64() - c0);
@@ -127,6 +143,23 @@ __global__ __launch_bounds__(BLOCK, 2) void k_backward_multi(TraceParams P, Mult
       backward_block_rec<K, BLOCK, 0>(Q, valid, y * Q.W + x, lt, s_node, s_rec, s_tab);
     }
   }
+}
+
+// The batch's loss from the per-tile sums (one block, a fixed order: deterministic), scaled
+// to the mean over all channels and pixels.
+__global__ __launch_bounds__(1024) void k_loss_total(const float *part, int n, float norm, float *loss) {
+  __shared__ double s[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024, b = min(n, t * per), e = min(n, b + per);
+  double sum = 0.0;
+  for (int i = b; i < e; i++) sum += (double)part[i];
+  s[t] = sum;
+  __syncthreads();
+  for (int d = 512; d > 0; d >>= 1) {
+    if (t < d) s[t] += s[t + d];
+    __syncthreads();
+  }
+  if (t == 0) *loss = (float)(s[0] * (double)norm);
 }
 
 template <int K, int BLOCK>
@@ -168,7 +201,7 @@ int multi_grid(int k, int block, bool backward) {
 }
 
 std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, int k, int block, int G, int grid,
-                         bool backward, hipStream_t stream) {
+                         bool backward, hipStream_t stream, float *loss_out) {
   const void *fn = multi_kernel(k, block, backward);
   if (!fn) return "unsupported (k, block): k in {8, 16, 32}, block in {64, 128, 256}";
   if (mc.total_tiles == 0) return "";
@@ -193,6 +226,11 @@ std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, 
   void *args[] = {&pp, &m, &heads, &G};
   hipError_t err = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
   if (err != hipSuccess) return std::string("persistent launch failed: ") + hipGetErrorString(err);
+  if (!backward && p.loss_part && loss_out) {
+    hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(1024), 0, stream, p.loss_part, n, p.loss_norm, loss_out);
+    err = hipGetLastError();
+    if (err != hipSuccess) return std::string("loss reduction launch failed: ") + hipGetErrorString(err);
+  }
   if (!backward && p.tile_cost && p.tile_order) return launch_tile_order(p.tile_cost, n, p.tile_order, stream);
   return "";
 }

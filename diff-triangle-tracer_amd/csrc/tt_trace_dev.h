@@ -205,6 +205,20 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
   }
 }
 
+// The ray of pixel pix: generated from the view's pinhole camera (fused ray generation)
+// or read from the ray buffers.
+TT_D void pixel_ray(const TraceParams &P, size_t pix, v3f &o, v3f &d) {
+  if (P.cam) {
+    float fo[3], fd[3];
+    pinhole_ray(P.cam, P.W, P.H, (int)(pix % (size_t)P.W), (int)(pix / (size_t)P.W), fo, fd);
+    o = mk(fo[0], fo[1], fo[2]);
+    d = mk(fd[0], fd[1], fd[2]);
+  } else {
+    o = ld3(P.ray_o + 3 * pix);
+    d = ld3(P.ray_d + 3 * pix);
+  }
+}
+
 // Cost estimate of tile `tile` for scheduling: the tile's centre ray walks the 4-wide
 // BVH down to subtrees of at most kProbeCut leaves and sums the leaf counts of the
 // subtrees and leaves whose boxes its line crosses -- how much geometry the tile's rays
@@ -218,7 +232,9 @@ TT_D float probe_tile(const TraceParams &P, int tile, int *s_st, int stride) {
   constexpr int TW = TileShape<BLOCK>::W, TH = TileShape<BLOCK>::H;
   const int x = min(tx * TW + TW / 2, P.W - 1), y = min(ty * TH + TH / 2, P.H - 1);
   const size_t pix = (size_t)y * P.W + x;
-  const RayPre r = prep_ray(ld3(P.ray_o + 3 * pix), ld3(P.ray_d + 3 * pix));
+  v3f ro, rd;
+  pixel_ray(P, pix, ro, rd);
+  const RayPre r = prep_ray(ro, rd);
   float est = 0.0f;
   int sp = 0, node = P.n_leaves > 0 ? 0 : -1;
   const float inf = __builtin_huge_valf();
@@ -483,8 +499,9 @@ TT_D uint32_t ray_bin_key(v3f dir, int x, int y) {
 // C and aux its transmittance (slot 5), R, Q and the two secondary rays, and ray_keys the
 // pixel's sort key (of its reflected ray), for bounce_secondary.
 template <int K, int BLOCK, int VARIANT, bool COUNT, bool IVAL = false>
-TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
-  v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
+TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss = nullptr) {
+  v3f o, d;
+  pixel_ray(P, (size_t)pix, o, d);
   SegState st;
   seg_init(st);
   if (VARIANT == 0) {
@@ -620,6 +637,20 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node) {
   const size_t R = (size_t)P.R;
   if (VARIANT != 4 && VARIANT != 5)
     for (int c = 0; c < 3; c++) P.color[c * R + pix] = st.C[c] + st.T * P.bg[c];
+  if (VARIANT == 0 && P.target) {  // fused loss: dL/dcolour for the backward, this pixel's loss
+    float l = 0.0f;
+    for (int c = 0; c < 3; c++) {
+      const float diff = (st.C[c] + st.T * P.bg[c]) - P.target[c * R + pix];
+      if (P.loss_kind == 1) {
+        P.loss_grad[c * R + pix] = (2.0f * diff) * P.loss_norm;
+        l += diff * diff;
+      } else {
+        P.loss_grad[c * R + pix] = diff > 0.0f ? P.loss_norm : (diff < 0.0f ? -P.loss_norm : 0.0f);
+        l += fabsf(diff);
+      }
+    }
+    if (loss) *loss = l;
+  }
   if (VARIANT == 0 || VARIANT == 4 || VARIANT == 5) {
     P.others[0 * R + pix] = 0.0f;
     P.others[1 * R + pix] = 1.0f - st.T;
@@ -664,7 +695,9 @@ TT_D void bounce_secondary(const TraceParams &P, int pix, int *s_node) {
   const size_t R = (size_t)P.R;
   const float T = a[5], Rw = a[6], Qw = a[7];
   float E0[3];
-  env_value(P.env, P.env_w, P.env_h, ld3(P.ray_d + 3 * (size_t)pix), E0);
+  v3f po, pd;
+  pixel_ray(P, (size_t)pix, po, pd);
+  env_value(P.env, P.env_w, P.env_h, pd, E0);
   for (int c = 0; c < 3; c++) {
     const float C = P.color[c * R + pix];
     P.color[c * R + pix] = ((C + Rw * Lr[c]) + Qw * Lt[c]) + T * E0[c];
@@ -1144,7 +1177,8 @@ TT_D void env_scatter_wave(const TraceParams &P, v3f d, const float g[3], bool o
 template <int K, int BLOCK, int VARIANT>
 TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, float *env_g = nullptr) {
   const size_t R = (size_t)P.R;
-  const v3f o = ld3(P.ray_o + 3 * (size_t)pix), d = ld3(P.ray_d + 3 * (size_t)pix);
+  v3f o, d;
+  pixel_ray(P, (size_t)pix, o, d);
   const float final_depth = P.fothers[pix];
   const v3f final_normal = mk(P.fothers[2 * R + pix], P.fothers[3 * R + pix], P.fothers[4 * R + pix]);
   float final_color[3], dL_dpixel[3];
@@ -1255,7 +1289,9 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   if (VARIANT == 5) {  // the bounce primaries: environment gradient aggregated per wave
     float eg[3] = {0.0f, 0.0f, 0.0f};
     if (valid) backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_rec, eg);
-    env_scatter_wave(P, valid ? ld3(P.ray_d + 3 * (size_t)pix) : mk(0.0f, 0.0f, 1.0f), eg, valid);
+    v3f po = mk(0.0f, 0.0f, 0.0f), pd = mk(0.0f, 0.0f, 1.0f);
+    if (valid) pixel_ray(P, (size_t)pix, po, pd);
+    env_scatter_wave(P, pd, eg, valid);
   } else if (valid) {
     backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_rec);
   }

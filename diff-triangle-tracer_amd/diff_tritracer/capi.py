@@ -92,7 +92,12 @@ class MultiArgs(C.Structure):
         ("dL_dpix_dev", _fp), ("dL_dothers_dev", _fp), ("zero_grads", C.c_int), ("dL_dvertices_rows", C.c_int),
         ("dL_dvertices_dev", _fp), ("dL_dsigma_dev", _fp), ("dL_dopacity_dev", _fp), ("dL_dsh_dev", _fp),
         ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int),
+        ("cameras_dev", _fp), ("target_dev", _fp), ("loss_grad_dev", _fp), ("loss_dev", _fp), ("loss_kind", C.c_int),
     ]
+
+
+CAMERA_DOUBLES = 14
+LOSS = {"none": 0, "l2": 1, "l1": 2}
 
 
 class AdamGroup(C.Structure):
@@ -220,6 +225,17 @@ def ptr(t) -> int | None:
     if t is None or t.numel() == 0:
         return None
     return t.data_ptr()
+
+
+def camera_table(bases, device):
+    """Pinhole cameras for the fused ray generation of the multi-camera launches: a float64
+    [n, 14] device tensor (eye, forward, right, up, tan_x, tan_y per camera) from
+    scenes.camera_basis() dicts."""
+    import torch
+
+    rows = [list(b["eye"]) + list(b["forward"]) + list(b["right"]) + list(b["up"]) + [b["tan_x"], b["tan_y"]]
+            for b in bases]
+    return torch.tensor(rows, dtype=torch.float64, device=device)
 
 
 def camera_rays(basis, width, height, ray_o, ray_d, stream=None):
@@ -510,10 +526,14 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
 def multi_args(widths, heights, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others,
                maxb, density, scaling, dL_dpix=None, dL_dothers=None, dverts=None, dsigma=None, dopac=None, dsh=None,
                k=16, cutoff=1e-3, chunk_mode="exact", block=128, tiles_per_fetch=1, zero_grads=True, stats=None,
-               near=0.2, far=1000.0, hitlist=None) -> MultiArgs:
+               near=0.2, far=1000.0, hitlist=None, cameras=None, target=None, loss_grad=None, loss=None,
+               loss_kind="l2") -> MultiArgs:
     """Arguments of the batched multi-camera path; per-camera buffers are concatenated
     in camera order (rays [sum R_c][3], colour [3][R_c] per camera, others [7][R_c]).
-    hitlist: (hits [cap, sum R_c, 2], counts [sum R_c]) written by the forward, read by the backward."""
+    hitlist: (hits [cap, sum R_c, 2], counts [sum R_c]) written by the forward, read by the backward.
+    cameras: camera_table() -- rays generated in the launches (ray_o / ray_d may be None).
+    target, loss_grad, loss: the fused loss -- target images like colour, dL/dcolour written
+    to loss_grad (pass dL_dpix to feed the backward), the mean loss to loss[0]."""
     a = MultiArgs()
     n = len(widths)
     a.n_cameras = n
@@ -541,4 +561,7 @@ def multi_args(widths, heights, ray_o, ray_d, verts, npts, cumsum, shs, sh_degre
     if hitlist is not None:
         hits, counts = hitlist
         a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[-3])
+    a.cameras_dev = ptr(cameras)
+    a.target_dev, a.loss_grad_dev, a.loss_dev = ptr(target), ptr(loss_grad), ptr(loss)
+    a.loss_kind = LOSS[loss_kind] if target is not None else 0
     return a

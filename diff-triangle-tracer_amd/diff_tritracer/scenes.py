@@ -159,15 +159,24 @@ MULTIVIEW_SIZES = [(640, 480), (800, 600), (1024, 768), (1280, 720), (1280, 720)
                    (1920, 1080)]
 
 
+def multiview_eyes(n: int, seed: int = 2, rmin: float = 3.0, rmax: float = 5.0):
+    """Positions of n cameras on random poses on a sphere of radius [rmin, rmax]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return [orbit_eye(rng.uniform(rmin, rmax), seed, i) for i in range(n)]
+
+
+def multiview_bases(sizes, seed: int = 2, rmin: float = 3.0, rmax: float = 5.0, fov_deg: float = 60.0):
+    """camera_basis() of every camera of multiview_rays (same poses)."""
+    eyes = multiview_eyes(len(sizes), seed, rmin, rmax)
+    return [camera_basis(w, h, e, fov_deg=fov_deg) for (w, h), e in zip(sizes, eyes)]
+
+
 def multiview_rays(sizes, seed: int = 2, rmin: float = 3.0, rmax: float = 5.0, fov_deg: float = 60.0):
     """Cameras of uneven resolution on random poses on a sphere of radius [rmin, rmax],
     looking at the origin. Returns (widths, heights, ray_o [sum R, 3], ray_d [sum R, 3])
     with the cameras' rays concatenated in order."""
-    rng = np.random.Generator(np.random.PCG64(seed))
     os_, ds_ = [], []
-    for i, (w, h) in enumerate(sizes):
-        r = rng.uniform(rmin, rmax)
-        eye = orbit_eye(r, seed, i)
+    for (w, h), eye in zip(sizes, multiview_eyes(len(sizes), seed, rmin, rmax)):
         ro, rd = look_at_rays(w, h, eye, fov_deg=fov_deg)
         os_.append(ro.reshape(-1, 3))
         ds_.append(rd.reshape(-1, 3))

@@ -237,7 +237,26 @@ typedef struct {
   unsigned int *hitlist_dev;    /* [hitlist_cap][sum R_c][2] */
   int *hitlist_count_dev;       /* [sum R_c] */
   int hitlist_cap;
+  /* Fused ray generation (optional): pinhole cameras in device memory, TT_CAMERA_DOUBLES doubles
+     per camera -- eye[3] forward[3] right[3] up[3] tan_x tan_y, the parameters of tt_camera_rays.
+     When given, every launch forms each pixel's ray as tt_camera_rays would, bit for bit, and
+     ray_o_dev / ray_d_dev are not read (may be NULL). */
+  const double *cameras_dev;
+  /* Fused loss (optional, forward): target images laid out like color_dev. The forward writes
+     the gradient of the batch's mean loss with respect to each colour into loss_grad_dev (laid
+     out like color_dev: pass the dL_dpix_dev buffer to feed the backward) and the loss into
+     loss_dev[0]. TT_LOSS_L2: mean of (c - t)^2, dL/dc = 2 (c - t) / N; TT_LOSS_L1: mean of
+     |c - t|, dL/dc = sign(c - t) / N; N = 3 x the batch's rays. The loss is summed per tile,
+     then over the tiles in a fixed order (the same value on every run). */
+  const float *target_dev;
+  float *loss_grad_dev;
+  float *loss_dev;
+  int loss_kind;                /* TT_LOSS_* */
 } tt_multi_args;
+#define TT_CAMERA_DOUBLES 14
+#define TT_LOSS_NONE 0
+#define TT_LOSS_L2 1
+#define TT_LOSS_L1 2
 
 /* Forward of every camera in ONE persistent launch (blocks pull tiles from 8 work queues). */
 int tt_multi_forward(tt_tracer *tr, const tt_multi_args *a, void *stream);
