@@ -324,6 +324,75 @@ def train(args, dev):
     print(json.dumps(line), flush=True)
 
 
+def refit_drift(args, dev):
+    """LBVH quality over a training run: the train workload's step (forward + backward + Adam
+    on vertices, opacity, sigma and SH + tt_build(TT_REFIT_AUTO)) for --train-steps steps,
+    once with the rebuild policy off (refit only) and once at --rebuild-ratio. Every 100 steps:
+    the median forward and backward traversal-kernel time of the last 10 steps and the SAH
+    cost (1 + sum of non-root node areas / root area). value = fwd+bwd kernel Mrays/s of the
+    policy run's last window."""
+    import numpy as np
+    import torch
+    from diff_tritracer import capi, scenes
+
+    W, H = args.width, args.height
+    eye = 4.0 * np.array([0.35, 0.3, 1.0]) / np.linalg.norm([0.35, 0.3, 1.0])
+    runs = {}
+    for label, ratio in (("refit_only", 0.0), ("policy", args.rebuild_ratio), ("rebuild_every_step", 1e-6)):
+        scene = scenes.clustered_scene(args.triangles, 64, seed=1)
+        fr = Frame(dev, scene, args.k, args.cutoff, args.hitlist_cap)
+        fr.h.build(fr.verts, fr.cumsum, fr.P, capi.BUILD, fr.sp)
+        fr.buffers(W, H)
+        fr.camera(scenes.camera_basis(W, H, eye))
+        fr.dpix.mul_(1e-3)
+        params = [(fr.verts, fr.dverts, 1e-4), (fr.opac, fr.dop, 1e-2), (fr.sigma, fr.dsig, 1e-2),
+                  (fr.shs, fr.dsh, 2.5e-3)]
+        groups = [(p, g, torch.zeros_like(p), torch.zeros_like(p), lr) for p, g, lr in params]
+        fa, ba = fr.fwd_args("exact"), fr.bwd_args("exact")
+        fr.h.set_rebuild_policy(ratio)
+        fr.zero_grads()
+        fresh, fresh_cost = capi.TracerHandle(), torch.zeros(1, device=dev)  # what a rebuild would give
+        series = []
+        t0 = time.perf_counter()
+        for step in range(1, args.train_steps + 1):
+            if step % 100 == 91:
+                fr.h.set_kernel_timing(True)
+            fr.h.fwd(fa, fr.sp)
+            fr.h.bwd(ba, fr.sp)
+            capi.adam_step(groups, step, zero_grads=True, stream=fr.sp)
+            fr.h.build(fr.verts, fr.cumsum, fr.P, capi.REFIT_AUTO, fr.sp)
+            if step % 100 == 0:
+                kf, kb = fr.h.kernel_times("fwd", 10), fr.h.kernel_times("bwd", 10)
+                fr.h.set_kernel_timing(False)
+                st = fr.h.build_stats()
+                fresh.build(fr.verts, fr.cumsum, fr.P, capi.BUILD, fr.sp)
+                fresh.bvh_cost(fresh_cost, fr.sp)
+                series.append(dict(step=step, fwd_ms=round(_median(kf), 4), bwd_ms=round(_median(kb), 4),
+                                   sah=round(st[4], 4), sah_built=round(st[3], 4),
+                                   sah_fresh_build=round(float(fresh_cost.item()), 4), auto_rebuilds=st[2]))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        runs[label] = dict(series=series, wall_s=round(wall, 2), rebuilds=fr.h.build_stats()[2])
+        print(f"[bench] {label}: " + " ".join(f"{x['step']}:{x['fwd_ms']:.2f}/{x['sah']:.1f}/{x['sah_fresh_build']:.1f}"
+                                            for x in series) +
+              f"  rebuilds {runs[label]['rebuilds']} wall {wall:.1f} s", file=sys.stderr, flush=True)
+        del fr
+        torch.cuda.empty_cache()
+    last = runs["policy"]["series"][-1]
+    R = W * H
+    line = {
+        "metric": "traversal Mrays/s after Adam + refit steps (automatic rebuild policy)",
+        "value": round(R / ((last["fwd_ms"] + last["bwd_ms"]) * 1e-3) / 1e6, 4), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.train_steps, "warmup": 0, "ms_per_step": round(last["fwd_ms"] + last["bwd_ms"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded clustered triangle splats, pinhole camera, fixed N(0,1e-3) loss gradient)",
+        "config": {"workload": f"{args.train_steps} training steps (fwd + bwd + Adam + TT_REFIT_AUTO) on "
+                               f"{args.triangles} clustered splats, one {W}x{H} camera, refit only vs rebuild at "
+                               f"SAH ratio {args.rebuild_ratio}", **runs},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def densify_bench(args, dev):
     """Pruning + densification of a trained-size scene from real forward statistics: the
     1080p exact forward fills max_blending / density / scaling, then each step runs
@@ -878,7 +947,10 @@ def main():
     ap.add_argument("--legs", default="",
                     help="comma-separated subset of the frame workload's steps (plain, plain_retrace, plain_compat, "
                          "handoff_only, proposal_ceiling, proposal, proposal_hard); default all")
-    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce", "train", "densify", "sceneio"], default="frame")
+    ap.add_argument("--workload", choices=["frame", "multiview", "proposal", "bounce", "train", "densify", "sceneio",
+                                           "refit"], default="frame")
+    ap.add_argument("--train-steps", type=int, default=1000, help="--workload refit: training steps per run")
+    ap.add_argument("--rebuild-ratio", type=float, default=1.25, help="--workload refit: SAH cost ratio that rebuilds")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates rocprof traffic counters)")
     args = ap.parse_args()
@@ -917,6 +989,9 @@ def main():
     if args.workload == "sceneio":
         args.triangles = args.triangles or 1_000_000
         return sceneio_bench(args, dev)
+    if args.workload == "refit":
+        args.triangles = args.triangles or 1_000_000
+        return refit_drift(args, dev)
     t0 = time.perf_counter()
     if args.workload == "proposal":
         P = args.triangles or 3_000_000

@@ -33,6 +33,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
            pybind11::arg("chunk_mode") = 0, pybind11::arg("hit_cache") = 64)
       .def("get_options", &OptixTracer::get_options)
       .def("set_store_hits", &OptixTracer::set_store_hits, pybind11::arg("store"))
+      .def("set_rebuild_policy", &OptixTracer::set_rebuild_policy, pybind11::arg("max_ratio"))
+      .def("build_stats", &OptixTracer::build_stats)
       .def_property_readonly("variant", &OptixTracer::variant_name);
 
   m.def("version", []() { return std::string(tt_version()); });

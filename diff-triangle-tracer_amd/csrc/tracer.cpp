@@ -114,6 +114,17 @@ std::string OptixTracer::variant_name() const {
 }
 
 // pipeline.cpp:206-265. `state` supplies the vertex and cumsum buffers.
+void OptixTracer::set_rebuild_policy(double max_ratio) {
+  check_tt(tt_set_rebuild_policy(tr_, (float)max_ratio), "set_rebuild_policy");
+}
+
+std::tuple<int, int, int, double, double> OptixTracer::build_stats() {
+  int n[3];
+  float c[2];
+  check_tt(tt_build_stats(tr_, n, c), "build_stats");
+  return std::make_tuple(n[0], n[1], n[2], (double)c[0], (double)c[1]);
+}
+
 void OptixTracer::build_tri_accel(TracerState &state, uint32_t numVertices, uint32_t numTri, bool rebuild) {
   check_tt(tt_build(tr_, state.d_vertices, (int)numVertices, state.d_cumsum, (int)numTri,
                     rebuild ? TT_BUILD : TT_REFIT, cur_stream(device_)),
@@ -168,8 +179,13 @@ FwdTraceResult OptixTracer::trace_fwd(const Tensor &ray_o, const Tensor &ray_d, 
     fwdState.d_cumsum = cums.data_ptr<int>();
     if (rebuild_gas_opt == 0) fwdState.indexbuf_tensor = cums;
     build_tri_accel(fwdState, (uint32_t)num_vert, (uint32_t)P, rebuild_gas_opt == 0);
+  } else if (rebuild_gas_opt == 3) {  // this build's addition: refit, rebuild when the SAH cost has degraded
+    fwdState.d_vertices = verts.data_ptr<float>();
+    fwdState.d_cumsum = cums.data_ptr<int>();
+    check_tt(tt_build(tr_, fwdState.d_vertices, num_vert, fwdState.d_cumsum, P, TT_REFIT_AUTO, stream),
+             "acceleration structure refit (automatic rebuild)");
   } else if (rebuild_gas_opt != 2) {
-    throw Exception("rebuild_gas must be 0 (build), 1 (refit) or 2 (reuse)");
+    throw Exception("rebuild_gas must be 0 (build), 1 (refit), 2 (reuse) or 3 (refit, rebuild when degraded)");
   } else if (tt_num_primitives(tr_) < 0) {
     throw Exception("rebuild_gas = 2 (reuse) but no acceleration structure has been built");
   }

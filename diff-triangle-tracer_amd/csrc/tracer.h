@@ -79,6 +79,11 @@ class OptixTracer {
   // Whether the next trace_fwd stores its blended hits (default true). The Python autograd
   // Function clears it when no input needs a gradient (no backward will read them).
   void set_store_hits(bool store) { store_hits_ = store; }
+  // rebuild_gas = 3 (this build's addition): refit, then rebuild when the refit tree's SAH cost
+  // exceeds max_ratio x its cost right after the last build (default 1.25)
+  void set_rebuild_policy(double max_ratio);
+  // builds, refits, automatic rebuilds, SAH cost after the last build, last evaluated cost
+  std::tuple<int, int, int, double, double> build_stats();
   std::tuple<int, double, int, int> get_options() const;
   std::string variant_name() const;
 

@@ -366,11 +366,72 @@ hipError_t grow(T *&ptr, size_t &cap, size_t need) {
   return e;
 }
 
+// Surface area of a box (in double: the sums below add ~2P terms).
+__device__ __forceinline__ double box_area(float lx, float hx, float ly, float hy, float lz, float hz) {
+  const double dx = fmax(0.0, (double)hx - lx), dy = fmax(0.0, (double)hy - ly), dz = fmax(0.0, (double)hz - lz);
+  return 2.0 * ((dx * dy + dy * dz) + dz * dx);
+}
+
+// Per block: the areas of the children of its 256 internal nodes (every non-root node is
+// the child of exactly one internal node), summed in a fixed order.
+__global__ __launch_bounds__(256) void k_sah_part(const float4 *__restrict__ nodes, int ni, double *part) {
+  __shared__ double s[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (i < ni) {
+    const float4 a = nodes[4 * (size_t)i], b = nodes[4 * (size_t)i + 1], z = nodes[4 * (size_t)i + 2];
+    v = box_area(a.x, a.y, a.z, a.w, z.x, z.y) + box_area(b.x, b.y, b.z, b.w, z.z, z.w);
+  }
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int d = 128; d > 0; d >>= 1) {
+    if (threadIdx.x < d) s[threadIdx.x] += s[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// 1 + (sum of the partial sums) / SA(root), the root box being the union of node 0's children.
+__global__ __launch_bounds__(1024) void k_sah_total(const double *part, int nb, const float4 *nodes, float *out) {
+  __shared__ double s[1024];
+  const int t = threadIdx.x;
+  double v = 0.0;
+  for (int i = t; i < nb; i += 1024) v += part[i];
+  s[t] = v;
+  __syncthreads();
+  for (int d = 512; d > 0; d >>= 1) {
+    if (t < d) s[t] += s[t + d];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const float4 a = nodes[0], b = nodes[1], z = nodes[2];
+    const double root = box_area(fminf(a.x, b.x), fmaxf(a.y, b.y), fminf(a.z, b.z), fmaxf(a.w, b.w),
+                                 fminf(z.x, z.z), fmaxf(z.y, z.w));
+    out[0] = root > 0.0 ? (float)(1.0 + s[0] / root) : 1.0f;
+  }
+}
+
 }  // namespace
+
+std::string bvh_cost(Bvh &b, float *out, hipStream_t stream) {
+  if (b.P <= 0 || b.n_internal <= 0) {
+    const float one = 1.0f;
+    hipError_t e = hipMemcpyAsync(out, &one, sizeof(float), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);  // `one` must outlive the copy
+    return e == hipSuccess ? "" : std::string("SAH cost copy failed: ") + hipGetErrorString(e);
+  }
+  const int nb = (b.n_internal + 255) / 256;
+  hipError_t e = grow(b.sah_part, b.cap_sah, (size_t)nb);
+  if (e != hipSuccess) return std::string("hipMalloc of the SAH partial sums failed: ") + hipGetErrorString(e);
+  hipLaunchKernelGGL(k_sah_part, dim3(nb), dim3(256), 0, stream, b.nodes, b.n_internal, b.sah_part);
+  hipLaunchKernelGGL(k_sah_total, dim3(1), dim3(1024), 0, stream, b.sah_part, nb, b.nodes, out);
+  e = hipGetLastError();
+  return e == hipSuccess ? "" : std::string("SAH cost launch failed: ") + hipGetErrorString(e);
+}
 
 void bvh_free(Bvh &b) {
   void *ps[] = {b.nodes, b.nodes4, b.leaves, b.leaf_parent, b.keys[0], b.keys[1], b.vals[0], b.vals[1], b.hist,
-                b.bounds, b.centroids, b.hword, b.height, b.level_nodes, b.level_meta};
+                b.bounds, b.centroids, b.hword, b.height, b.level_nodes, b.level_meta, b.sah_part};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   b = Bvh();

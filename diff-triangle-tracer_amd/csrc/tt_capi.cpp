@@ -65,6 +65,11 @@ struct tt_tracer {
   unsigned long long mem_gen = 1, sched_gen = 1;
   unsigned long long graph_mem_gen = 0, graph_sched_gen = 0;
   int graph_what = 0;
+  // LBVH quality (tt_build TT_REFIT_AUTO): SAH costs on the device ([0] right after the last
+  // TT_BUILD, [1] the last one evaluated) with a pinned host copy, the policy and counters
+  float *sah_dev = nullptr, *sah_host = nullptr;
+  float rebuild_ratio = 1.25f;
+  int n_builds = 0, n_refits = 0, n_auto_rebuilds = 0;
   float *loss_part = nullptr;  // per-tile sums of the fused loss of a multi-camera forward
   int loss_cap = 0;
   int noalloc_grid[3][3] = {};  // persistent forward grid per (k, block) of tt_multi_forward_noalloc (0: unset)
@@ -360,6 +365,8 @@ int tt_destroy(tt_tracer *tr) {
   free_records(tr);
   free_rec_sort(tr);
   if (tr->loss_part) (void)hipFree(tr->loss_part);
+  if (tr->sah_dev) (void)hipFree(tr->sah_dev);
+  if (tr->sah_host) (void)hipHostFree(tr->sah_host);
   if (tr->hits_total_host) (void)hipHostFree(tr->hits_total_host);
   if (tr->hits_total_ev) (void)hipEventDestroy(tr->hits_total_ev);
   (void)hipSetDevice(dev);
@@ -370,7 +377,8 @@ int tt_destroy(tt_tracer *tr) {
 int tt_build(tt_tracer *tr, const float *vertices_dev, int num_vertices, const int *cumsum_dev, int P, int mode,
              void *stream) {
   TT_ARG(tr, "tracer is NULL");
-  TT_ARG(mode == TT_BUILD || mode == TT_REFIT || mode == TT_REUSE, "mode must be 0 (build), 1 (refit) or 2 (reuse)");
+  TT_ARG(mode == TT_BUILD || mode == TT_REFIT || mode == TT_REUSE || mode == TT_REFIT_AUTO,
+         "mode must be 0 (build), 1 (refit), 2 (reuse) or 3 (refit, rebuild when degraded)");
   TT_ARG(P >= 0 && num_vertices >= 0, "sizes must be non-negative");
   if (mode == TT_REUSE) {
     if (tr->bvh.P < 0) return fail(TT_ERR_STATE, "rebuild_gas = 2 (reuse) but no acceleration structure exists");
@@ -382,15 +390,77 @@ int tt_build(tt_tracer *tr, const float *vertices_dev, int num_vertices, const i
     if (hipEventCreate(&tr->ev0) != hipSuccess || hipEventCreate(&tr->ev1) != hipSuccess)
       return fail(TT_ERR_HIP, "hipEventCreate failed");
   }
+  if (!tr->sah_dev) {
+    if (hipMalloc(&tr->sah_dev, 2 * sizeof(float)) != hipSuccess ||
+        hipHostMalloc((void **)&tr->sah_host, 2 * sizeof(float)) != hipSuccess)
+      return fail(TT_ERR_HIP, "allocation of the SAH cost buffers failed");
+  }
   (void)hipEventRecord(tr->ev0, s);
+  const bool automatic = mode == TT_REFIT_AUTO;
   if (mode == TT_BUILD) tr->mem_gen++;  // new nodes and tree height: captured graphs are stale
-  std::string err = tt::bvh_build(tr->bvh, vertices_dev, num_vertices, cumsum_dev, P, mode, s);
+  std::string err = tt::bvh_build(tr->bvh, vertices_dev, num_vertices, cumsum_dev, P, automatic ? TT_REFIT : mode, s);
   if (!err.empty()) {
     tr->build_timed = false;
-    return fail(mode == TT_REFIT && err.find("refit") != std::string::npos ? TT_ERR_STATE : TT_ERR_HIP, err);
+    return fail(mode != TT_BUILD && err.find("refit") != std::string::npos ? TT_ERR_STATE : TT_ERR_HIP, err);
+  }
+  if (mode == TT_BUILD) {
+    tr->n_builds++;
+    err = tt::bvh_cost(tr->bvh, tr->sah_dev, s);
+    if (err.empty() && hipMemcpyAsync(tr->sah_dev + 1, tr->sah_dev, sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess)
+      err = "SAH cost copy failed";
+  } else {
+    tr->n_refits++;
+  }
+  if (automatic && err.empty()) {  // the refit tree's cost against the cost right after the last build
+    err = tt::bvh_cost(tr->bvh, tr->sah_dev + 1, s);
+    if (err.empty() && (hipMemcpyAsync(tr->sah_host, tr->sah_dev, 2 * sizeof(float), hipMemcpyDeviceToHost, s) !=
+                            hipSuccess || hipStreamSynchronize(s) != hipSuccess))
+      err = "SAH cost read-back failed";
+    if (err.empty() && tr->rebuild_ratio > 0.0f && tr->sah_host[1] > tr->rebuild_ratio * tr->sah_host[0]) {
+      tr->mem_gen++;
+      tr->n_auto_rebuilds++;
+      tr->n_builds++;
+      err = tt::bvh_build(tr->bvh, vertices_dev, num_vertices, cumsum_dev, P, TT_BUILD, s);
+      if (err.empty()) err = tt::bvh_cost(tr->bvh, tr->sah_dev, s);
+      if (err.empty() && hipMemcpyAsync(tr->sah_dev + 1, tr->sah_dev, sizeof(float), hipMemcpyDeviceToDevice, s) !=
+                             hipSuccess)
+        err = "SAH cost copy failed";
+    }
+  }
+  if (!err.empty()) {
+    tr->build_timed = false;
+    return fail(TT_ERR_HIP, err);
   }
   (void)hipEventRecord(tr->ev1, s);
   tr->build_timed = true;
+  return ok();
+}
+
+int tt_bvh_cost(tt_tracer *tr, float *cost_dev, void *stream) {
+  TT_ARG(tr && cost_dev, "NULL argument");
+  if (tr->bvh.P < 0) return fail(TT_ERR_STATE, "no acceleration structure: build it first");
+  std::string err = tt::bvh_cost(tr->bvh, cost_dev, static_cast<hipStream_t>(stream));
+  return err.empty() ? ok() : fail(TT_ERR_HIP, err);
+}
+
+int tt_set_rebuild_policy(tt_tracer *tr, float max_ratio) {
+  TT_ARG(tr, "tracer is NULL");
+  TT_ARG(max_ratio == max_ratio, "max_ratio is NaN");
+  tr->rebuild_ratio = max_ratio;
+  return ok();
+}
+
+int tt_build_stats(tt_tracer *tr, int counts[3], float costs[2]) {
+  TT_ARG(tr && counts && costs, "NULL argument");
+  counts[0] = tr->n_builds;
+  counts[1] = tr->n_refits;
+  counts[2] = tr->n_auto_rebuilds;
+  costs[0] = costs[1] = -1.0f;
+  if (tr->sah_dev && tr->n_builds > 0) {
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(costs, tr->sah_dev, 2 * sizeof(float), hipMemcpyDeviceToHost) !=
+                                                    hipSuccess)
+      return fail(TT_ERR_HIP, "SAH cost read-back failed");
+  }
   return ok();
 }
 
@@ -773,6 +843,8 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
   if (p.target) {
     if (tiles > tr->loss_cap) {
       if (tr->loss_part) (void)hipFree(tr->loss_part);
+  if (tr->sah_dev) (void)hipFree(tr->sah_dev);
+  if (tr->sah_host) (void)hipHostFree(tr->sah_host);
       tr->loss_part = nullptr;
       tr->loss_cap = 0;
       if (hipMalloc(&tr->loss_part, (size_t)tiles * sizeof(float)) != hipSuccess)

@@ -43,9 +43,14 @@ struct Bvh {
   uint32_t *bounds = nullptr;   // ordered-uint centroid bounds [6]
   float4 *centroids = nullptr;  // [P]
   const int *cumsum_seen = nullptr;  // cumsum pointer of the last build (refit checks P only)
+  double *sah_part = nullptr;        // per-block partial sums of the SAH cost
+  size_t cap_sah = 0;
 };
 
 void bvh_free(Bvh &b);
+// SAH cost of the current nodes (unit traversal and intersection costs): 1 + the sum over
+// every non-root node of SA(node) / SA(root), written to out[0] on `stream`.
+std::string bvh_cost(Bvh &b, float *out, hipStream_t stream);
 // Builds (mode 0) or refits (mode 1) on `stream`. Returns an empty string or an error message.
 std::string bvh_build(Bvh &b, const float *verts, int num_vertices, const int *cumsum, int P, int mode,
                       hipStream_t stream);

@@ -29,7 +29,10 @@ class TracerSettings(NamedTuple):
     """Per-call settings (reference __init__.py:37-43).
 
     rebuild_gas: 0 build the acceleration structure, 1 refit it to moved
-    vertices (same triangles), 2 reuse it as is. extra_params: float tensor
+    vertices (same triangles), 2 reuse it as is; 3 (added here) refit and rebuild
+    once the refit tree's SAH cost exceeds the tracer's policy ratio
+    (``Tracer.set_rebuild_policy``, default 1.25) x its cost after the last build.
+    extra_params: float tensor
     (r, cx, cy, cz[, samples]) of the sphere used by the reflect / refract /
     envlight programs. scale_modifier is accepted and unused, as in the reference.
     """
@@ -121,6 +124,15 @@ class Tracer(nn.Module):
         mode = {"compat": 0, "exact": 1}[chunk_mode] if isinstance(chunk_mode, str) else int(chunk_mode)
         self.tracer.set_options(int(k), float(cutoff), mode, int(hit_cache))
         return self
+
+    def set_rebuild_policy(self, max_ratio: float = 1.25):
+        """rebuild_gas = 3: rebuild when the refit tree's SAH cost exceeds max_ratio x the built one."""
+        self.tracer.set_rebuild_policy(float(max_ratio))
+        return self
+
+    def build_stats(self):
+        """(builds, refits, automatic rebuilds, SAH cost after the last build, last evaluated cost)."""
+        return self.tracer.build_stats()
 
     def forward(self, ray_o, ray_d, triangles_points, sigma, num_points_per_triangle, cumsum_of_points_per_triangle,
                 number_of_points, opacities, scaling, density_factor, shs, colors_precomp,

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(PACKAGE_DIR, "libtritracer_hip.so")
 
 TT_OK = 0
 VARIANTS = {"forward": 0, "reflect": 1, "refract": 2, "envlight": 3, "bounce": 4}
-BUILD, REFIT, REUSE = 0, 1, 2
+BUILD, REFIT, REUSE, REFIT_AUTO = 0, 1, 2, 3
 CHUNK = {"compat": 0, "exact": 1}
 
 EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build", "tt_trace_fwd", "tt_trace_bwd",
@@ -27,7 +27,7 @@ EXPORTED = ["tt_create", "tt_destroy", "tt_last_error", "tt_version", "tt_build"
             "tt_proposal_create", "tt_proposal_destroy", "tt_proposal_set_params", "tt_proposal_get_params",
             "tt_proposal_forward", "tt_proposal_loss_grad", "tt_proposal_train", "tt_camera_rays", "tt_adam_step",
             "tt_densify_plan", "tt_densify_apply", "tt_scene_unpack", "tt_scene_pack", "tt_set_kernel_timing",
-            "tt_kernel_times"]
+            "tt_kernel_times", "tt_set_rebuild_policy", "tt_bvh_cost", "tt_build_stats"]
 MAX_CAMERAS = 32
 TIMING_RING = 64
 
@@ -186,6 +186,9 @@ def lib():
         L.tt_multi_backward.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
         L.tt_multi_forward_noalloc.argtypes = [_vp, C.POINTER(MultiArgs), _vp]
         L.tt_set_kernel_timing.argtypes = [_vp, C.c_int]
+        L.tt_set_rebuild_policy.argtypes = [_vp, C.c_float]
+        L.tt_bvh_cost.argtypes = [_vp, _fp, _vp]
+        L.tt_build_stats.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_float)]
         L.tt_kernel_times.argtypes = [_vp, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]
         L.tt_multi_capture.argtypes = [_vp, C.POINTER(MultiArgs), C.c_int, _vp]
         L.tt_multi_replay.argtypes = [_vp, _vp]
@@ -345,6 +348,21 @@ class TracerHandle:
 
     def multi_fwd(self, args: "MultiArgs", stream=None):
         check(lib().tt_multi_forward(self.h, C.byref(args), stream), "tt_multi_forward")
+
+    def set_rebuild_policy(self, max_ratio: float):
+        """tt_build(REFIT_AUTO) rebuilds once the refit tree's SAH cost exceeds max_ratio x the built one."""
+        check(lib().tt_set_rebuild_policy(self.h, float(max_ratio)), "tt_set_rebuild_policy")
+
+    def bvh_cost(self, out, stream=None):
+        """SAH cost of the current acceleration structure into the float32 tensor out[0]."""
+        check(lib().tt_bvh_cost(self.h, ptr(out), stream), "tt_bvh_cost")
+
+    def build_stats(self):
+        """(builds, refits, automatic rebuilds, SAH cost after the last build, last evaluated cost)."""
+        n = (C.c_int * 3)()
+        c = (C.c_float * 2)()
+        check(lib().tt_build_stats(self.h, n, c), "tt_build_stats")
+        return n[0], n[1], n[2], c[0], c[1]
 
     def set_kernel_timing(self, enable=True):
         """Record HIP events around every traversal kernel (ring of TIMING_RING per direction)."""

@@ -58,6 +58,9 @@ extern "C" {
 #define TT_BUILD 0
 #define TT_REFIT 1
 #define TT_REUSE 2
+/* no counterpart in the reference: refit, then rebuild when the refit tree has degraded (see
+   tt_set_rebuild_policy) */
+#define TT_REFIT_AUTO 3
 
 /* chunk resume rule between k-closest gathers */
 #define TT_CHUNK_COMPAT 0 /* resume at t >= last_hit_t (inclusive; OptiX closed ray interval,
@@ -502,6 +505,20 @@ int tt_set_kernel_timing(tt_tracer *tr, int enable);
    (which = 0) or backward (which = 1) traversal kernels, oldest first; *count_out = how many were
    written. Waits for those launches to finish. */
 int tt_kernel_times(tt_tracer *tr, int which, float *ms_out, int max_n, int *count_out);
+
+/* Rebuild policy (no counterpart in the reference, whose caller picks build / refit / reuse
+   per call, diff_tritracer/__init__.py:42). Quality measure: the SAH cost of the current nodes
+   with unit traversal and intersection costs, 1 + the sum over every non-root node of
+   SA(node) / SA(root). tt_build(TT_REFIT_AUTO) refits, evaluates that cost, and rebuilds when
+   it exceeds max_ratio x the cost right after the last TT_BUILD (default 1.25; <= 0: never);
+   it reads the cost back, so it waits for the refit. */
+int tt_set_rebuild_policy(tt_tracer *tr, float max_ratio);
+/* The SAH cost of the current acceleration structure into cost_dev[0] on `stream`. */
+int tt_bvh_cost(tt_tracer *tr, float *cost_dev, void *stream);
+/* counts = builds (incl. automatic), refits (incl. TT_REFIT_AUTO), automatic rebuilds since
+   tt_create; costs = SAH cost right after the last build and the last evaluated one (-1 if
+   none). Waits for the device. */
+int tt_build_stats(tt_tracer *tr, int counts[3], float costs[2]);
 
 /* Number of triangles of the current acceleration structure (-1 if none). */
 int tt_num_primitives(const tt_tracer *tr);
