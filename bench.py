@@ -147,7 +147,11 @@ def multiview(args, dev):
                         cutoff=args.cutoff, chunk_mode="exact", block=128, tiles_per_fetch=1, zero_grads=True,
                         hitlist=hl)
     t0 = time.perf_counter()
-    sweep = h.multi_autotune(a, ks=(8, 16, 32), blocks=(64, 128, 256), tpf=(1, 2, 4), reps=3, stream=sp)
+    if args.no_autotune:
+        a.k, a.block, a.tiles_per_fetch = 16, 256, 1
+        sweep = {}
+    else:
+        sweep = h.multi_autotune(a, ks=(8, 16, 32), blocks=(64, 128, 256), tpf=(1, 2, 4), reps=3, stream=sp)
     print(f"[bench] autotuner: {len(sweep)} configurations in {time.perf_counter() - t0:.1f} s; best k={a.k} "
           f"block={a.block} tiles/fetch={a.tiles_per_fetch}", file=sys.stderr, flush=True)
     for key in sorted(sweep, key=sweep.get):
@@ -163,6 +167,27 @@ def multiview(args, dev):
     stop.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / args.steps
+    # the fused step: rays formed in the launches from the camera table (no ray buffers) and the
+    # loss gradient (mean L2 against a target image) written by the forward for the backward
+    cams = capi.camera_table(scenes.multiview_bases(scenes.MULTIVIEW_SIZES, seed=2), dev)
+    target = torch.rand(3 * Rt, generator=torch.Generator(device="cpu").manual_seed(6)).to(dev)
+    dpix_f, loss = torch.zeros(3 * Rt, device=dev), torch.zeros(1, device=dev)
+    af = capi.multi_args(ws, hs, None, None, verts, npts, cumsum, shs, 3, opac, sigma, bg, color, others, maxb, dens,
+                         scal, dL_dpix=dpix_f, dL_dothers=doth, dverts=dverts, dsigma=dsig, dopac=dop, dsh=dsh, k=a.k,
+                         cutoff=args.cutoff, chunk_mode="exact", block=a.block, tiles_per_fetch=a.tiles_per_fetch,
+                         zero_grads=True, hitlist=hl, cameras=cams, target=target, loss_grad=dpix_f, loss=loss)
+    h.multi_capture(af, 3, sp)
+    for _ in range(args.warmup):
+        h.multi_replay(sp)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(args.steps):
+        h.multi_replay(sp)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms_fused = start.elapsed_time(stop) / args.steps
+    print(f"[bench] fused step (rays + loss gradient in the launches) {ms_fused:.3f} ms/step, loss "
+          f"{float(loss.item()):.5f}", file=sys.stderr, flush=True)
     # the same batch as one forward + one backward launch per camera (same k, exact mode)
     launches, off, hc_keep = [], 0, []
     for wc, hc in zip(ws, hs):
@@ -207,6 +232,8 @@ def multiview(args, dev):
                                + ", ".join(f"{w}x{hh}" for w, hh in zip(ws, hs)),
                    "rays": Rt, "k": a.k, "block": a.block, "tiles_per_fetch": a.tiles_per_fetch, "hitlist_cap": cap,
                    "chunk_mode": "exact", "cutoff": args.cutoff,
+                   "fused_step_ms": round(ms_fused, 4),
+                   "fused_step_mrays": round(Rt / (ms_fused * 1e-3) / 1e6, 4),
                    "per_camera_launches_ms": round(ms_pc, 4),
                    "per_camera_launches_mrays": round(Rt / (ms_pc * 1e-3) / 1e6, 4),
                    "autotune_ms": {f"k{k_}_b{b}_t{t}": round(v, 3) for (k_, b, t), v in sorted(sweep.items())}},
@@ -944,6 +971,8 @@ def main():
     ap.add_argument("--hitlist-cap", type=int, default=64,
                     help="blended hits per pixel the forward stores for the backward (0: the backward re-traverses)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="--workload multiview: skip the sweep, use k=16, block 256, 1 tile per fetch")
     ap.add_argument("--legs", default="",
                     help="comma-separated subset of the frame workload's steps (plain, plain_retrace, plain_compat, "
                          "handoff_only, proposal_ceiling, proposal, proposal_hard); default all")

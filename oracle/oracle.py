@@ -69,6 +69,7 @@ def lib():
         L.or_scene_bvh_seconds.restype = C.c_double
         L.or_scene_bvh_seconds.argtypes = [vp]
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_abs_vertex.argtypes = [C.c_int]
         L.or_get_threads.restype = C.c_int
         for n in ("or_log2f", "or_exp2f"):
             getattr(L, n).restype = C.c_float
@@ -387,7 +388,9 @@ class Log:
         lib().or_forward_f64(self.h, _d(v), _d(o), _d(g), _d(h), _d(color), _d(others))
         return color, others
 
-    def backward_f64(self, dL_dpix, dL_dothers, **params):
+    def backward_f64(self, dL_dpix, dL_dothers, vertex_abs=False, **params):
+        """vertex_abs: dverts holds per entry the sum over the blended hits that touch it of each
+        hit's largest |vertex-gradient component| (the scale FP32 evaluation error is relative to)."""
         v, o, g, h = self.params64(**params)
         gp = np.ascontiguousarray(dL_dpix, np.float64).reshape(3, self.R)
         go = np.ascontiguousarray(dL_dothers, np.float64).reshape(7, self.R)
@@ -395,8 +398,12 @@ class Log:
         dsig = np.zeros_like(g)
         dop = np.zeros_like(o)
         dsh = np.zeros_like(h)
-        rc = lib().or_backward_f64(self.h, _d(v), _d(o), _d(g), _d(h), _d(gp), _d(go), _d(dv), _d(dsig), _d(dop),
-                                   _d(dsh))
+        lib().or_set_abs_vertex(1 if vertex_abs else 0)
+        try:
+            rc = lib().or_backward_f64(self.h, _d(v), _d(o), _d(g), _d(h), _d(gp), _d(go), _d(dv), _d(dsig), _d(dop),
+                                       _d(dsh))
+        finally:
+            lib().or_set_abs_vertex(0)
         if rc != 0:
             raise RuntimeError("f64 backward is defined for the forward variant only")
         return dict(dverts=dv, dsigma=dsig, dopac=dop, dsh=dsh)
@@ -421,15 +428,20 @@ class Log:
         go = np.ascontiguousarray(dL_dothers, np.float64).reshape(7, self.R)
         return float(lib().or_loss_f64_bounce(self.h, _d(v), _d(o), _d(g), _d(h), _d(r), _d(e), _d(gp), _d(go)))
 
-    def backward_f64_bounce(self, refl, env, dL_dpix, dL_dothers, **params):
+    def backward_f64_bounce(self, refl, env, dL_dpix, dL_dothers, vertex_abs=False, **params):
         v, o, g, h = self.params64(**params)
         r, e = self._b64(refl, env)
         gp = np.ascontiguousarray(dL_dpix, np.float64).reshape(3, self.R)
         go = np.ascontiguousarray(dL_dothers, np.float64).reshape(7, self.R)
         dv, dsig, dop, dsh = np.zeros_like(v), np.zeros_like(g), np.zeros_like(o), np.zeros_like(h)
         dr, de = np.zeros_like(r), np.zeros_like(e)
-        if lib().or_backward_f64_bounce(self.h, _d(v), _d(o), _d(g), _d(h), _d(r), _d(e), _d(gp), _d(go), _d(dv),
-                                        _d(dsig), _d(dop), _d(dsh), _d(dr), _d(de)):
+        lib().or_set_abs_vertex(1 if vertex_abs else 0)
+        try:
+            rc = lib().or_backward_f64_bounce(self.h, _d(v), _d(o), _d(g), _d(h), _d(r), _d(e), _d(gp), _d(go), _d(dv),
+                                              _d(dsig), _d(dop), _d(dsh), _d(dr), _d(de))
+        finally:
+            lib().or_set_abs_vertex(0)
+        if rc:
             raise RuntimeError("the log is not of the bounce variant")
         return dict(dverts=dv, dsigma=dsig, dopac=dop, dsh=dsh, drefl=dr, denv=de.reshape(np.shape(env)))
 

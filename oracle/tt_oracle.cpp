@@ -20,6 +20,7 @@ namespace {
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;  // RayHit::invalid_id, params.h:119
 constexpr float kInfDist = 1e6f;            // RayHit::infinite_dist, params.h:118
 constexpr double kStepEps = 1e-9;           // STEP_EPSILON, config.h:11 (a double literal)
+bool g_abs_vertex = false;  // see add_vertex_hit
 
 // SH constants, auxiliary.h:33-50 (textbook real-SH normalisations; pinned by tests).
 constexpr double SH_C0 = 0.28209479177387814;
@@ -445,9 +446,10 @@ inline void dne_dp3(V3<R> p1, V3<R> p2, V3<R> p3, R out[9]) {
 // backward.cu:104-189 computeWindowFuncBwd (hit point and incenter distance held constant).
 template <class R, class Acc>
 inline void window_bwd(const V3<R> vert[3], const V3<R> rel[3], R dL_dphi, R phi_x, const R ed[3], V3<R> dL_dn3,
-                       Acc add) {
+                       Acc add, int edge_mask = -1) {
   for (int k = 0; k < 3; k++) {
-    if (std::fabs(ed[k] - phi_x) > R(1e-6f)) continue;
+    // the max edge(s), backward.cu's tolerance test; a replay passes the FP32 pass's choice
+    if (edge_mask >= 0 ? !((edge_mask >> k) & 1) : std::fabs(ed[k] - phi_x) > R(1e-6f)) continue;
     V3<R> p1 = vert[k], p2 = vert[(k + 1) % 3], p3 = vert[(k + 2) % 3];
     V3<R> n = cross(p1 - p3, p2 - p3);
     V3<R> ne = cross(n, p1 - p2);
@@ -743,8 +745,10 @@ Opts to_opts(const or_opts *o) {
 struct LogHit {
   uint32_t prim;
   float t;
-  int code;      // 0 near skip, 1 alpha skip, 2 blended, 3 terminated (not blended)
-  double dist;   // incenter distance in double (blended hits only)
+  int code;        // 0 near skip, 1 alpha skip, 2 blended, 3 terminated (not blended)
+  double dist;     // incenter distance in double (blended hits only)
+  int edges = -1;  // blended hits: the edges the FP32 pass takes as the window's max edge
+                   // (bit k: |ed_k - phi| <= 1e-6 in FP32), a discrete decision the FP64 replay keeps
 };
 struct Segment {
   int pix;
@@ -875,7 +879,10 @@ void trace_segment_f32(const Scene &s, const Opts &op, V3f o, V3f d, float C[3],
         Win<double> wd;
         V3d hd = mk<double>(o.x, o.y, o.z) + (double)t * mk<double>(d.x, d.y, d.z);
         window<double>(Vd, hd, (double)s.sigma[prim], wd, nullptr);
-        log->push_back({h.prim, t, 2, wd.dist});
+        int edges = 0;
+        for (int k = 0; k < 3; k++)
+          if (!(std::fabs(w.ed[k] - w.phi_max) > 1e-6f)) edges |= 1 << k;
+        log->push_back({h.prim, t, 2, wd.dist, edges});
       }
     }
     if (op.mode == 1) {
@@ -1097,6 +1104,9 @@ extern "C" {
 
 void or_set_threads(int n) {
   if (n > 0) omp_set_num_threads(n);
+}
+void or_set_abs_vertex(int on) {
+  g_abs_vertex = on != 0;
 }
 int or_get_threads(void) {
   return omp_get_max_threads();
@@ -1527,6 +1537,23 @@ struct G64 {
   double *dverts, *dsigma, *dopac, *dsh, *drefl;
 };
 
+// One blended hit's vertex gradient (9 values: the triangle's three vertices), added to the
+// gradient rows of the triangle. Test aid (not part of the restated algorithm): with
+// g_abs_vertex set, every one of the 9 entries receives instead the hit's largest |value|.
+// Summed over the hits, that is the scale FP32 evaluation error is relative to: the terms
+// of one hit are differences of quantities of that size (a tiny triangle hit near one edge
+// gives two vertices gradients of ~1e3 and the third a residual of ~1e-3), and the hits'
+// terms may cancel in the sum.
+inline void add_vertex_hit(double *rows, const double hv[9]) {
+  if (g_abs_vertex) {
+    double m = 0.0;
+    for (int i = 0; i < 9; i++) m = std::fmax(m, std::fabs(hv[i]));
+    for (int i = 0; i < 9; i++) rows[i] += m;
+  } else {
+    for (int i = 0; i < 9; i++) rows[i] += hv[i];
+  }
+}
+
 // Backward of one replayed segment in double (backward.cu:191-350 per hit) for an
 // upstream gradient g on the segment's output colour `fin` (composite incl. its
 // background), optional depth / normal terms, and (bounce primary) per-hit colours
@@ -1581,10 +1608,11 @@ double backward_segment_f64(const Log &lg, const Segment &sg, const P64 &p, cons
     if (w.phi_final > 0.0) out.dsigma[prim] += dL_dC * w.Cx * std::log(w.phi_final);
     double dL_dphi = p.sigma[prim] / w.phi_max * dL_dC * w.Cx;
     V3d dL_dn3 = dL_dn2 * bw;
-    window_bwd<double>(V, w.rel, dL_dphi, w.phi_max, w.ed, dL_dn3, [&](int j, V3d gv) {
-      double *q = out.dverts + 3 * (size_t)(off + j);
-      q[0] += gv.x, q[1] += gv.y, q[2] += gv.z;
-    });
+    double hv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    window_bwd<double>(
+        V, w.rel, dL_dphi, w.phi_max, w.ed, dL_dn3,
+        [&](int j, V3d gv) { hv[3 * j] += gv.x, hv[3 * j + 1] += gv.y, hv[3 * j + 2] += gv.z; }, h.edges);
+    add_vertex_hit(out.dverts + 3 * (size_t)off, hv);
     sh_bwd<double>(s.deg, d, cl, dL_dcolor, [&](int j, V3d gv) {
       double *q = out.dsh + ((size_t)prim * s.M + j) * 3;
       q[0] += gv.x, q[1] += gv.y, q[2] += gv.z;
@@ -1712,10 +1740,11 @@ int or_backward_f64(void *log, const double *verts, const double *opac, const do
       if (w.phi_final > 0.0) dsigma[prim] += dL_dC * w.Cx * std::log(w.phi_final);
       double dL_dphi = sigma[prim] / w.phi_max * dL_dC * w.Cx;
       V3d dL_dn3 = dL_dn2 * bw;
-      window_bwd<double>(V, w.rel, dL_dphi, w.phi_max, w.ed, dL_dn3, [&](int j, V3d g) {
-        double *q = dverts + 3 * (size_t)(off + j);
-        q[0] += g.x, q[1] += g.y, q[2] += g.z;
-      });
+      double hv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      window_bwd<double>(
+          V, w.rel, dL_dphi, w.phi_max, w.ed, dL_dn3,
+          [&](int j, V3d g) { hv[3 * j] += g.x, hv[3 * j + 1] += g.y, hv[3 * j + 2] += g.z; }, h.edges);
+      add_vertex_hit(dverts + 3 * (size_t)off, hv);
       sh_bwd<double>(s.deg, d, cl, dL_dcolor, [&](int j, V3d g) {
         double *q = dsh + ((size_t)prim * s.M + j) * 3;
         q[0] += g.x, q[1] += g.y, q[2] += g.z;

@@ -69,6 +69,9 @@ void or_scene_free(void *scene);
 double or_scene_bvh_seconds(void *scene);
 void or_set_threads(int n);
 int or_get_threads(void);
+/* test aid: the FP64 backwards add to each vertex-gradient entry, per blended hit, that hit's
+   largest |vertex-gradient component| (1) instead of the gradient itself (0) */
+void or_set_abs_vertex(int on);
 
 /* ---- elementary pieces (single evaluations, for known-answer tests) ---- */
 float or_log2f(float x);

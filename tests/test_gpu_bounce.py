@@ -7,11 +7,14 @@ against the oracle (40k splats, 144x112 camera, exact chunk mode):
   both with the backward re-traversing and with it reading the forward's hit lists of the
   three segments (capacity 32)
   gradients (FP64) ....... opacity, sigma, SH, reflectivity and environment texels elementwise
-                           within 1e-3 max(|ref|, 1e-3 max|ref|); vertices the same for all but
-                           1e-4 of the entries, with relative L2 error <= 1e-4 (a vertex gradient
-                           can be a cancelling sum of large near-edge window terms from several
-                           rays; FP32 then leaves an absolute error of ~1e-5 of the tensor's max
-                           in that one entry -- 3 of 360,000 entries on this case)
+                           within 1e-3 max(|ref|, 1e-3 max|ref|); vertices elementwise within
+                           1e-3 max(|ref|, 1e-3 max|ref|) + 1e-5 S, S = the sum over the blended
+                           hits touching the entry of each hit's largest vertex-gradient component
+                           (the oracle's abs mode), and relative L2 error <= 1e-4. A hit on a tiny
+                           triangle near one edge gives two vertices gradients of ~1e3 and the third
+                           a residual of ~1e-3 formed from those large quantities, and hits' terms
+                           can cancel across rays: FP32 evaluation error scales with S, not with the
+                           entry; 1e-5 S is ~170 FP32 ulps of S (3 of 360,000 entries need it here).
 """
 import numpy as np
 import pytest
@@ -101,11 +104,15 @@ def test_bounce_gradients_vs_oracle(case):
         _say(f"  {key}: worst ratio {worst:.3g}")
         assert ok, key
     g, r = out["dverts"].reshape(-1).astype(np.float64), rb["dverts"].reshape(-1)
-    tol = 1e-3 * np.maximum(np.abs(r), 1e-3 * np.abs(r).max())
-    bad = int((np.abs(g - r) > tol).sum())
+    S = log.backward_f64_bounce(refl, env, dpix, doth, vertex_abs=True)["dverts"].reshape(-1)
+    base = 1e-3 * np.maximum(np.abs(r), 1e-3 * np.abs(r).max())
+    err = np.abs(g - r)
+    conditioned = int((err > base).sum())
+    worst = float((err / (base + 1e-5 * S)).max())
     rel = np.linalg.norm(g - r) / np.linalg.norm(r)
-    _say(f"  dverts: {bad} of {r.size} entries outside the elementwise bound, relative L2 error {rel:.3g}")
-    assert bad <= 1e-4 * r.size and rel <= 1e-4
+    _say(f"  dverts: worst ratio to the bar {worst:.3g}; {conditioned} of {r.size} entries need the 1e-5 S term; "
+         f"relative L2 error {rel:.3g}")
+    assert worst <= 1.0 and rel <= 1e-4
 
 
 def test_two_pass_forward_equals_single_launch(cuda):

@@ -378,3 +378,23 @@ def test_f32_backward_close_to_f64():
     for k in ("dopac", "dsigma", "dsh", "dverts"):
         ok, worst = grad_close(b32[k], b64[k])
         assert ok, (k, worst)
+
+
+def test_vertex_abs_mode_bounds_the_signed_sum():
+    # the test aid behind the bounce vertex-gradient bar: |sum of terms| <= sum of |terms|,
+    # equal where one ray contributes one term; the signed gradient is unchanged afterwards
+    S, ro, rd = _fd_scene()
+    opts = O.make_opts(k=4, cutoff=1e-3)
+    R = ro.shape[0]
+    rng = np.random.Generator(np.random.PCG64(10))
+    gp = rng.normal(0, 1, (3, R))
+    go = np.zeros((7, R))
+    log = O.Log(S, ro, rd, opts)
+    signed = log.backward_f64(gp, go)["dverts"]
+    absd = log.backward_f64(gp, go, vertex_abs=True)["dverts"]
+    assert np.all(absd >= np.abs(signed) - 1e-15 * absd.max())
+    assert np.any(absd > 2.0 * np.abs(signed))          # some entries do cancel
+    assert np.array_equal(log.backward_f64(gp, go)["dverts"], signed)
+    one = O.Log(S, ro[:1], rd[:1], opts)               # a single ray: every vertex entry one term or a few
+    s1, a1 = one.backward_f64(gp[:, :1], go[:, :1])["dverts"], one.backward_f64(gp[:, :1], go[:, :1], True)["dverts"]
+    assert np.all(a1 >= np.abs(s1) - 1e-15)
