@@ -843,8 +843,6 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
   if (p.target) {
     if (tiles > tr->loss_cap) {
       if (tr->loss_part) (void)hipFree(tr->loss_part);
-  if (tr->sah_dev) (void)hipFree(tr->sah_dev);
-  if (tr->sah_host) (void)hipHostFree(tr->sah_host);
       tr->loss_part = nullptr;
       tr->loss_cap = 0;
       if (hipMalloc(&tr->loss_part, (size_t)tiles * sizeof(float)) != hipSuccess)
