@@ -299,6 +299,21 @@ __global__ __launch_bounds__(256) void k_level_boxes(const int *__restrict__ lev
 // replaced by its two children, twice (the greedy rule for collapsing a binary tree into
 // a wide one: it opens the boxes a ray is most likely to hit). A 4-wide step descends at
 // least one binary level, so a walk's path holds at most `height` 4-wide nodes.
+//
+// Stored quantized, 64 B per node (4 x 16 B; 128 B as floats):
+//   [0] float origin.x, origin.y, origin.z; exponents e_x, e_y, e_z (bytes, biased by 128)
+//   [1] x planes: byte c of .x = child c's lo, of .y = its hi; y planes in .z / .w
+//   [2] z planes in .x / .y; leaves under each child (16 bits, saturated) in .z (children
+//       0, 1) and .w (2, 3)
+//   [3] int child[4] (>= 0 binary node, < 0 leaf ~s, kEmptyChild unused)
+// Child c's box decodes per axis as fmaf(q, 2^e, origin) (q * 2^e is exact: one rounding).
+// origin is the least lo of the node's children, and every q is chosen here with that same
+// decode so that the decoded box contains the child's box (lo rounded down, hi up, the
+// scale grown when 255 steps do not reach the top after rounding): a walk on the decoded
+// boxes culls no less conservatively than on the exact ones, so hit lists do not change.
+__device__ __forceinline__ float q_scale(int e) {
+  return __int_as_float((e + 127) << 23);
+}
 __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ nodes, int ni, float4 *nodes4) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ni) return;
@@ -344,15 +359,51 @@ __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ no
     ref[e] = kEmptyChild;
     cnt[e] = 0;
   }
-  float4 *o = nodes4 + 8 * (size_t)n;
-  o[0] = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
-  o[1] = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
-  o[2] = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
-  o[3] = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
-  o[4] = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
-  o[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
-  *reinterpret_cast<int4 *>(o + 6) = make_int4(ref[0], ref[1], ref[2], ref[3]);
-  *reinterpret_cast<int4 *>(o + 7) = make_int4(cnt[0], cnt[1], cnt[2], cnt[3]);
+  uint32_t qlo[3], qhi[3], eb = 0;
+  float org[3];
+  for (int a = 0; a < 3; a++) {
+    float o = __builtin_huge_valf(), top = -__builtin_huge_valf();
+    for (int c = 0; c < 4; c++)
+      if (ref[c] != kEmptyChild) o = fminf(o, lo[a][c]), top = fmaxf(top, hi[a][c]);
+    const double ext = (double)top - (double)o;
+    int sc_e = -120;
+    if (ext > 0.0) sc_e = max(-120, min(120, (int)ceil(log2(ext / 254.0))));
+    uint32_t wl = 0, wh = 0;
+    for (; sc_e <= 120; sc_e++) {  // the smallest scale whose steps reach every child's top after rounding
+      const float sc = q_scale(sc_e);
+      bool ok = true;
+      wl = wh = 0;
+      for (int c = 0; c < 4 && ok; c++) {
+        uint32_t ql = 255, qh = 0;
+        if (ref[c] != kEmptyChild) {
+          int l = (int)floor(((double)lo[a][c] - (double)o) / (double)sc);
+          int h = (int)ceil(((double)hi[a][c] - (double)o) / (double)sc);
+          l = max(0, min(255, l));
+          h = max(0, min(255, h));
+          while (l > 0 && fmaf((float)l, sc, o) > lo[a][c]) l--;
+          while (h < 255 && fmaf((float)h, sc, o) < hi[a][c]) h++;
+          if (fmaf((float)h, sc, o) < hi[a][c]) ok = false;
+          ql = (uint32_t)l;
+          qh = (uint32_t)h;
+        }
+        wl |= ql << (8 * c);
+        wh |= qh << (8 * c);
+      }
+      if (ok) break;
+    }
+    org[a] = o;
+    qlo[a] = wl;
+    qhi[a] = wh;
+    eb |= (uint32_t)(sc_e + 128) << (8 * a);
+  }
+  auto c16 = [&](int i) { return (uint32_t)min(cnt[i], 0xffff); };
+  float4 *out = nodes4 + 4 * (size_t)n;
+  out[0] = make_float4(org[0], org[1], org[2], __uint_as_float(eb));
+  out[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qhi[0]), __uint_as_float(qlo[1]),
+                       __uint_as_float(qhi[1]));
+  out[2] = make_float4(__uint_as_float(qlo[2]), __uint_as_float(qhi[2]), __uint_as_float(c16(0) | c16(1) << 16),
+                       __uint_as_float(c16(2) | c16(3) << 16));
+  *reinterpret_cast<int4 *>(out + 3) = make_int4(ref[0], ref[1], ref[2], ref[3]);
 }
 
 template <class T>
@@ -487,7 +538,7 @@ std::string bvh_build(Bvh &b, const float *verts, int num_vertices, const int *c
     b.centroids = nullptr, b.hword = nullptr, b.height = nullptr, b.level_nodes = nullptr;
     b.cap_leaves = 0;
     TT_TRY(hipMalloc(&b.nodes, sizeof(float4) * 4 * need));
-    TT_TRY(hipMalloc(&b.nodes4, sizeof(float4) * 8 * need));
+    TT_TRY(hipMalloc(&b.nodes4, sizeof(float4) * 4 * need));
     TT_TRY(hipMalloc(&b.leaves, sizeof(float4) * 3 * need));
     TT_TRY(hipMalloc(&b.leaf_parent, sizeof(int) * need));
     TT_TRY(hipMalloc(&b.keys[0], sizeof(uint32_t) * need));

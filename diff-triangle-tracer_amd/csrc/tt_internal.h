@@ -17,17 +17,19 @@ constexpr int kEmptyChild = 0x7fffffff;  // bound on LBVH height bookkeeping (a 
 //            [2] (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)   [3] int (child0, child1, parent, 0)
 //            child >= 0: internal node index; child < 0: leaf ~s (sorted position s)
 //   leaves : 3 x 16 B per leaf in Morton order: (A.xyz, prim bits) (B.xyz, 0) (C.xyz, 0)
-//   nodes4 : 8 x 16 B per binary node n, the traversal's 4-wide node: n's children, with
-//            each internal child replaced by its own two children (2 to 4 entries)
-//            [0] lo.x[4] [1] hi.x[4] [2] lo.y[4] [3] hi.y[4] [4] lo.z[4] [5] hi.z[4]
-//            [6] int child[4] (>= 0 binary node, < 0 leaf ~s, kEmptyChild unused) [7] unused
+//   nodes4 : 4 x 16 B per binary node n, the traversal's 4-wide node: n's children, with
+//            each internal child replaced by its own two children (2 to 4 entries), the
+//            child boxes as 8-bit planes over a per-axis origin and power-of-two scale
+//            [0] origin xyz + exponents [1] x, y planes [2] z planes + leaf counts
+//            [3] int child[4] (>= 0 binary node, < 0 leaf ~s, kEmptyChild unused)
 struct Bvh {
   int P = -1;           // triangles in the structure (-1: none built)
   int n_leaves = 0;     // P, or 2 when P == 1 (second leaf is an unhittable dummy)
   int n_internal = 0;   // n_leaves - 1
   size_t cap_leaves = 0;
   float4 *nodes = nullptr;
-  float4 *nodes4 = nullptr;   // 4-wide view, 8 x 16 B per binary node n: the up-to-4 grandchildren of n
+  float4 *nodes4 = nullptr;   // 4-wide view, 4 x 16 B per binary node n: the up-to-4 grandchildren of n,
+                              // their boxes quantized to 8-bit planes (layout at k_collapse4)
   float4 *leaves = nullptr;
   int *leaf_parent = nullptr;
   unsigned long long *hword = nullptr;  // per node: the two children's reported heights

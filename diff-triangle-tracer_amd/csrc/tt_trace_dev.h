@@ -110,6 +110,28 @@ TT_D void cswap(float &da, int &ca, float &db, int &cb) {
 // so the intersection code runs with most lanes busy instead of once per leaf-holding
 // lane. The K-buffer keeps the K smallest keys whatever the insertion order, so
 // postponing a leaf only weakens the culling bound for a while.
+// A 4-wide node (quantized, layout at k_collapse4) decoded to its child boxes' planes: per
+// axis fmaf(q, 2^e, origin), the builder's own decode, so the boxes contain the exact ones.
+TT_D void load_node4q(const float4 *q, float4 &lx, float4 &hx, float4 &ly, float4 &hy, float4 &lz, float4 &hz,
+                      int4 &ref) {
+  const float4 a = q[0], b = q[1], c = q[2];
+  ref = *reinterpret_cast<const int4 *>(q + 3);
+  const uint32_t eb = __float_as_uint(a.w);
+  const float sx = __uint_as_float(((eb & 0xffu) - 1u) << 23), sy = __uint_as_float((((eb >> 8) & 0xffu) - 1u) << 23),
+              sz = __uint_as_float((((eb >> 16) & 0xffu) - 1u) << 23);
+  auto dec4 = [](float w, float s, float o) {
+    const uint32_t u = __float_as_uint(w);
+    return make_float4(fmaf((float)(u & 0xffu), s, o), fmaf((float)((u >> 8) & 0xffu), s, o),
+                       fmaf((float)((u >> 16) & 0xffu), s, o), fmaf((float)(u >> 24), s, o));
+  };
+  lx = dec4(b.x, sx, a.x);
+  hx = dec4(b.y, sx, a.x);
+  ly = dec4(b.z, sy, a.y);
+  hy = dec4(b.w, sy, a.y);
+  lz = dec4(c.x, sz, a.z);
+  hz = dec4(c.y, sz, a.z);
+}
+
 template <int K, int BLOCK, bool COUNT>
 TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo, float far_t, KBuf<K> &b,
                  int *s_node, TravCount &tc) {
@@ -132,9 +154,9 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
   while (true) {
     if (node >= 0 && nq <= kLeafQ - 4) {
       if (COUNT) tc.nodes++;
-      const float4 *nd = P.nodes4 + 8 * (size_t)node;
-      const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-      const int4 ref = *reinterpret_cast<const int4 *>(nd + 6);
+      float4 lx, hx, ly, hy, lz, hz;
+      int4 ref;
+      load_node4q(P.nodes4 + 4 * (size_t)node, lx, hx, ly, hy, lz, hz, ref);
       const float thi = fminf(far_t, b.t[K - 1]);
       const float hi_w = thi + fabsf(thi) * kWideRel + kWideAbs;
       float tn0, tf0, tn1, tf1, tn2, tf2, tn3, tf3;
@@ -239,9 +261,12 @@ TT_D float probe_tile(const TraceParams &P, int tile, int *s_st, int stride) {
   int sp = 0, node = P.n_leaves > 0 ? 0 : -1;
   const float inf = __builtin_huge_valf();
   while (node >= 0) {
-    const float4 *nd = P.nodes4 + 8 * (size_t)node;
-    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-    const int4 ref = *reinterpret_cast<const int4 *>(nd + 6), cnt = *reinterpret_cast<const int4 *>(nd + 7);
+    float4 lx, hx, ly, hy, lz, hz;
+    int4 ref;
+    load_node4q(P.nodes4 + 4 * (size_t)node, lx, hx, ly, hy, lz, hz, ref);
+    const float4 q2 = P.nodes4[4 * (size_t)node + 2];  // leaves under each child, 16 bits each (saturated)
+    const uint32_t c01 = __float_as_uint(q2.z), c23 = __float_as_uint(q2.w);
+    const int4 cnt = make_int4((int)(c01 & 0xffffu), (int)(c01 >> 16), (int)(c23 & 0xffffu), (int)(c23 >> 16));
     float tn[4], tf[4];
     slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn[0], tf[0]);
     slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn[1], tf[1]);
