@@ -273,14 +273,6 @@ TT_D void slab(const RayPre &r, float lx, float hx, float ly, float hy, float lz
 TT_D bool box_ok(float tn, float tf, float tlo_c, float thi_c) {
   return tn <= tf && tf >= tlo_c && tn <= thi_c;
 }
-// The slab interval without the per-box widening, for box_ok_wide.
-TT_D void slab_raw(const RayPre &r, float lx, float hx, float ly, float hy, float lz, float hz, float &tn, float &tf) {
-  const float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
-  const float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
-  const float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
-  tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-  tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-}
 // The test slab() + box_ok() make, with the widening moved out of the per-box work: one
 // FMA per box instead of two FMAs and two adds. Widening constants kWideRel / kWideAbs
 // exceed twice slab()'s (1e-6, 1e-7), which keeps every box slab() + box_ok() accept

@@ -67,15 +67,16 @@ TT_D TraceParams camera_view(const TraceParams &P, const MultiCams &mc, int tile
   return Q;
 }
 
-// Cost estimate of every global tile (probe_tile on its camera), one lane per tile.
+// Cost estimate of every global tile (probe_tile on its camera), one wave per tile.
 template <int BLOCK>
 __global__ __launch_bounds__(64) void k_probe_multi(TraceParams P, MultiCams mc, float *cost) {
-  __shared__ int s_st[kProbeStack * 64];
-  const int t = blockIdx.x * 64 + threadIdx.x;
-  if (t >= mc.total_tiles) return;
+  __shared__ int s_front[2 * kProbeFront];
+  __shared__ int s_count;
+  const int t = blockIdx.x;
   int lt;
   const TraceParams Q = camera_view(P, mc, t, lt);
-  cost[t] = probe_tile<BLOCK>(Q, lt, s_st + threadIdx.x, 64);
+  const float est = probe_tile<BLOCK>(Q, lt, s_front, &s_count);
+  if (threadIdx.x == 0) cost[t] = est;
 }
 
 template <int K, int BLOCK>
@@ -209,7 +210,7 @@ std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, 
   MultiCams m = mc;
   const int n = mc.total_tiles;
   if (!backward && p.tile_cost && p.probe_order) {  // heaviest tiles first, by the probe's estimate
-    const dim3 g((n + 63) / 64), b(64);
+    const dim3 g(n), b(64);
     if (block == 64) hipLaunchKernelGGL(k_probe_multi<64>, g, b, 0, stream, p, mc, p.tile_cost);
     if (block == 128) hipLaunchKernelGGL(k_probe_multi<128>, g, b, 0, stream, p, mc, p.tile_cost);
     if (block == 256) hipLaunchKernelGGL(k_probe_multi<256>, g, b, 0, stream, p, mc, p.tile_cost);

@@ -76,13 +76,12 @@ __global__ __launch_bounds__(1024) void k_tile_offsets(const int *hits, int n, i
   if (t == 1023) off[n] = (int)min(s_part[1023], (long long)0x7fffffff);
 }
 
-// Cost estimate of every tile before the forward (probe_tile), one lane per tile.
-constexpr int kProbeBlock = 64;
-__global__ __launch_bounds__(kProbeBlock) void k_tile_probe(TraceParams P, float *cost) {
-  __shared__ int s_st[kProbeStack * kProbeBlock];
-  const int tile = blockIdx.x * kProbeBlock + threadIdx.x;
-  if (tile >= P.n_tiles) return;
-  cost[tile] = probe_tile<kBlock>(P, tile, s_st + threadIdx.x, kProbeBlock);
+// Cost estimate of every tile before the forward (probe_tile), one wave per tile.
+__global__ __launch_bounds__(64) void k_tile_probe(TraceParams P, float *cost) {
+  __shared__ int s_front[2 * kProbeFront];
+  __shared__ int s_count;
+  const float est = probe_tile<kBlock>(P, blockIdx.x, s_front, &s_count);
+  if (threadIdx.x == 0) cost[blockIdx.x] = est;
 }
 
 // Tiles ordered by cost, most expensive first (counting sort over 1024 buckets of
@@ -219,7 +218,7 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
   if (k != 8 && k != 16 && k != 32) return "k must be 8, 16 or 32";
   p.tile_sched = nullptr;
   if (p.tile_cost && p.probe_order) {  // the forward's schedule: estimated tile costs
-    hipLaunchKernelGGL(k_tile_probe, dim3((p.n_tiles + kProbeBlock - 1) / kProbeBlock), dim3(kProbeBlock), 0, stream,
+    hipLaunchKernelGGL(k_tile_probe, dim3(p.n_tiles), dim3(64), 0, stream,
                        p, p.tile_cost);
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, p.tile_cost, p.n_tiles, p.probe_order);
     e = hipGetLastError();

@@ -132,11 +132,66 @@ TT_D void load_node4q(const float4 *q, float4 &lx, float4 &hx, float4 &ly, float
   hz = dec4(c.y, sz, a.z);
 }
 
+// The walk's slab test reads a quantized node in ray space instead of decoding its planes:
+// the plane at step q of axis a lies at t = q * (2^e / d_a) + (origin_a - o_a) / d_a, where
+// 2^e / d_a is exact, so a plane costs one fma (the decode-then-slab form costs four).
+// Against the decoded planes the build verified, such a t errs by one rounding of the
+// decode (u |plane|, u = 2^-24), two of (origin - o) / d and one of the pad below, at most
+// 4.1 u (C + |o|) / |d_a| together (C bounds every coordinate of the scene), plus one
+// rounding of t itself, which the interval widening of box_ok_wide covers. Lower planes
+// are moved down and upper planes up by pad = 2^-19 (C + |o|), 32 u, in world space: the
+// walk stays conservative, and hit lists do not change.
+struct NodePad {
+  v3f pinv;  // pad / d per axis (signed)
+};
+TT_D NodePad node_pad(const TraceParams &P, const RayPre &r) {
+  // the root's origin and 256 steps of its scale bound every coordinate of the scene
+  const float4 root = P.nodes4[0];
+  const uint32_t eb = __float_as_uint(root.w);
+  auto reach = [](float org, uint32_t e) { return fmaf(256.0f, __uint_as_float(((e & 0xffu) - 1u) << 23), fabsf(org)); };
+  const float C = fmaxf(fmaxf(reach(root.x, eb), reach(root.y, eb >> 8)), reach(root.z, eb >> 16));
+  const float pad = 0x1p-19f * (C + fmaxf(fmaxf(fabsf(r.o.x), fabsf(r.o.y)), fabsf(r.o.z)));
+  NodePad np;
+  np.pinv = mk(pad * r.inv.x, pad * r.inv.y, pad * r.inv.z);
+  return np;
+}
+// Entry and exit distances of the 4 children of a quantized node (unwidened, as slab_raw).
+TT_D void slab4q(const RayPre &r, const NodePad &np, const float4 *q, float4 &tn, float4 &tf, int4 &ref) {
+  const float4 a = q[0], b = q[1], c = q[2];
+  ref = *reinterpret_cast<const int4 *>(q + 3);
+  const uint32_t eb = __float_as_uint(a.w);
+  constexpr float kBig = 3.0e38f;  // 2^e / d overflows only for e near 120 and d near 1e-30
+  auto step = [&](uint32_t e, float inv) {
+    return fminf(fmaxf(__uint_as_float(((e & 0xffu) - 1u) << 23) * inv, -kBig), kBig);
+  };
+  const float sx = step(eb, r.inv.x), sy = step(eb >> 8, r.inv.y), sz = step(eb >> 16, r.inv.z);
+  const float bx = (a.x - r.o.x) * r.inv.x, by = (a.y - r.o.y) * r.inv.y, bz = (a.z - r.o.z) * r.inv.z;
+  auto t4 = [](float w, float s, float o) {
+    const uint32_t u = __float_as_uint(w);
+    return make_float4(fmaf((float)(u & 0xffu), s, o), fmaf((float)((u >> 8) & 0xffu), s, o),
+                       fmaf((float)((u >> 16) & 0xffu), s, o), fmaf((float)(u >> 24), s, o));
+  };
+  const float4 lx = t4(b.x, sx, bx - np.pinv.x), hx = t4(b.y, sx, bx + np.pinv.x);
+  const float4 ly = t4(b.z, sy, by - np.pinv.y), hy = t4(b.w, sy, by + np.pinv.y);
+  const float4 lz = t4(c.x, sz, bz - np.pinv.z), hz = t4(c.y, sz, bz + np.pinv.z);
+  auto near = [](float l0, float h0, float l1, float h1, float l2, float h2) {
+    return fmaxf(fmaxf(fminf(l0, h0), fminf(l1, h1)), fminf(l2, h2));
+  };
+  auto far = [](float l0, float h0, float l1, float h1, float l2, float h2) {
+    return fminf(fminf(fmaxf(l0, h0), fmaxf(l1, h1)), fmaxf(l2, h2));
+  };
+  tn = make_float4(near(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x), near(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y),
+                   near(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z), near(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w));
+  tf = make_float4(far(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x), far(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y),
+                   far(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z), far(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w));
+}
+
 template <int K, int BLOCK, bool COUNT>
 TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo, float far_t, KBuf<K> &b,
                  int *s_node, TravCount &tc) {
   kinit(b);
   if (P.n_leaves == 0) return;
+  const NodePad np = node_pad(P, r);
   const int lane = threadIdx.x;
   int *ov = P.ov_stack + ((size_t)blockIdx.x * BLOCK + lane) * P.ov_depth;
   int *s_leaf = s_node + kStack * BLOCK;
@@ -154,16 +209,12 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
   while (true) {
     if (node >= 0 && nq <= kLeafQ - 4) {
       if (COUNT) tc.nodes++;
-      float4 lx, hx, ly, hy, lz, hz;
+      float4 tn, tf;
       int4 ref;
-      load_node4q(P.nodes4 + 4 * (size_t)node, lx, hx, ly, hy, lz, hz, ref);
+      slab4q(r, np, P.nodes4 + 4 * (size_t)node, tn, tf, ref);
       const float thi = fminf(far_t, b.t[K - 1]);
       const float hi_w = thi + fabsf(thi) * kWideRel + kWideAbs;
-      float tn0, tf0, tn1, tf1, tn2, tf2, tn3, tf3;
-      slab_raw(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn0, tf0);
-      slab_raw(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn1, tf1);
-      slab_raw(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn2, tf2);
-      slab_raw(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn3, tf3);
+      const float tn0 = tn.x, tf0 = tf.x, tn1 = tn.y, tf1 = tf.y, tn2 = tn.z, tf2 = tf.z, tn3 = tn.w, tf3 = tf.w;
       bool h0 = (ref.x != kEmptyChild) & box_ok_wide(tn0, tf0, lo_w, hi_w);
       bool h1 = (ref.y != kEmptyChild) & box_ok_wide(tn1, tf1, lo_w, hi_w);
       bool h2 = (ref.z != kEmptyChild) & box_ok_wide(tn2, tf2, lo_w, hi_w);
@@ -244,45 +295,65 @@ TT_D void pixel_ray(const TraceParams &P, size_t pix, v3f &o, v3f &d) {
 // Cost estimate of tile `tile` for scheduling: the tile's centre ray walks the 4-wide
 // BVH down to subtrees of at most kProbeCut leaves and sums the leaf counts of the
 // subtrees and leaves whose boxes its line crosses -- how much geometry the tile's rays
-// pass, at a few dozen node visits (a few percent of a full trace). s_st: this lane's
-// LDS stack of kProbeStack entries `stride` apart; entries past it are dropped (the
-// estimate then undercounts; it only orders tiles).
-constexpr int kProbeCut = 32, kProbeStack = 48;
+// pass, at a few dozen node visits (a few percent of a full trace). One wave probes one
+// tile, level by level: the nodes of a level are spread over the lanes and the children
+// to open form the next level, so the chain of dependent node fetches is the tree's
+// depth rather than the walk's length. Called by all 64 lanes of a one-wave block with
+// two LDS frontiers of kProbeFront entries; nodes past a full frontier are dropped (the
+// estimate then undercounts; it only orders tiles). Every lane returns the estimate.
+constexpr int kProbeCut = 32, kProbeFront = 512;
 template <int BLOCK>
-TT_D float probe_tile(const TraceParams &P, int tile, int *s_st, int stride) {
+TT_D float probe_tile(const TraceParams &P, int tile, int *s_front, int *s_count) {
   const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
   constexpr int TW = TileShape<BLOCK>::W, TH = TileShape<BLOCK>::H;
   const int x = min(tx * TW + TW / 2, P.W - 1), y = min(ty * TH + TH / 2, P.H - 1);
   const size_t pix = (size_t)y * P.W + x;
+  const int lane = threadIdx.x;
   v3f ro, rd;
   pixel_ray(P, pix, ro, rd);
   const RayPre r = prep_ray(ro, rd);
-  float est = 0.0f;
-  int sp = 0, node = P.n_leaves > 0 ? 0 : -1;
   const float inf = __builtin_huge_valf();
-  while (node >= 0) {
-    float4 lx, hx, ly, hy, lz, hz;
-    int4 ref;
-    load_node4q(P.nodes4 + 4 * (size_t)node, lx, hx, ly, hy, lz, hz, ref);
-    const float4 q2 = P.nodes4[4 * (size_t)node + 2];  // leaves under each child, 16 bits each (saturated)
-    const uint32_t c01 = __float_as_uint(q2.z), c23 = __float_as_uint(q2.w);
-    const int4 cnt = make_int4((int)(c01 & 0xffffu), (int)(c01 >> 16), (int)(c23 & 0xffffu), (int)(c23 >> 16));
-    float tn[4], tf[4];
-    slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn[0], tf[0]);
-    slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn[1], tf[1]);
-    slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn[2], tf[2]);
-    slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn[3], tf[3]);
-    const int rf[4] = {ref.x, ref.y, ref.z, ref.w}, ct[4] = {cnt.x, cnt.y, cnt.z, cnt.w};
-    for (int c = 0; c < 4; c++) {
-      if (rf[c] == kEmptyChild || !box_ok(tn[c], tf[c], 0.0f, inf)) continue;
-      if (rf[c] < 0 || ct[c] <= kProbeCut)
-        est += (float)ct[c];
-      else if (sp < kProbeStack)
-        s_st[sp++ * stride] = rf[c];
+  uint32_t est = 0;
+  int *cur = s_front, *nxt = s_front + kProbeFront;
+  if (lane == 0) cur[0] = 0, *s_count = 0;
+  int n = P.n_leaves > 0 ? 1 : 0;
+  __syncthreads();
+  while (n > 0) {
+    for (int i = lane; i < n; i += 64) {
+      const int node = cur[i];
+      float4 lx, hx, ly, hy, lz, hz;
+      int4 ref;
+      load_node4q(P.nodes4 + 4 * (size_t)node, lx, hx, ly, hy, lz, hz, ref);
+      const float4 q2 = P.nodes4[4 * (size_t)node + 2];  // leaves under each child, 16 bits each (saturated)
+      const uint32_t c01 = __float_as_uint(q2.z), c23 = __float_as_uint(q2.w);
+      const int ct[4] = {(int)(c01 & 0xffffu), (int)(c01 >> 16), (int)(c23 & 0xffffu), (int)(c23 >> 16)};
+      float tn[4], tf[4];
+      slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn[0], tf[0]);
+      slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tn[1], tf[1]);
+      slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tn[2], tf[2]);
+      slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tn[3], tf[3]);
+      const int rf[4] = {ref.x, ref.y, ref.z, ref.w};
+      for (int c = 0; c < 4; c++) {
+        if (rf[c] == kEmptyChild || !box_ok(tn[c], tf[c], 0.0f, inf)) continue;
+        if (rf[c] < 0 || ct[c] <= kProbeCut) {
+          est += (uint32_t)ct[c];
+        } else {
+          const int slot = atomicAdd(s_count, 1);
+          if (slot < kProbeFront) nxt[slot] = rf[c];
+        }
+      }
     }
-    node = sp > 0 ? s_st[--sp * stride] : -1;
+    __syncthreads();
+    n = min(*s_count, kProbeFront);
+    __syncthreads();
+    if (lane == 0) *s_count = 0;
+    int *t = cur;
+    cur = nxt;
+    nxt = t;
+    __syncthreads();
   }
-  return est;
+  for (int off = 32; off > 0; off >>= 1) est += __shfl_xor(est, off);
+  return (float)est;
 }
 
 // Tile of a block in a one-shot launch: workgroups go to the 8 XCDs round-robin,
