@@ -300,12 +300,14 @@ __global__ __launch_bounds__(256) void k_level_boxes(const int *__restrict__ lev
 // a wide one: it opens the boxes a ray is most likely to hit). A 4-wide step descends at
 // least one binary level, so a walk's path holds at most `height` 4-wide nodes.
 //
-// Stored quantized, 64 B per node (4 x 16 B; 128 B as floats):
+// Stored quantized, 80 B per node (5 x 16 B; 128 B as floats):
 //   [0] float origin.x, origin.y, origin.z; exponents e_x, e_y, e_z (bytes, biased by 128)
-//   [1] x planes: byte c of .x = child c's lo, of .y = its hi; y planes in .z / .w
-//   [2] z planes in .x / .y; leaves under each child (16 bits, saturated) in .z (children
-//       0, 1) and .w (2, 3)
-//   [3] int child[4] (>= 0 binary node, < 0 leaf ~s, kEmptyChild unused)
+//   [1] x planes: step counts q in 0..255, each an f16 integer (exact): .x = lo of children
+//       0, 1 (low half, high half), .y = lo of children 2, 3, .z / .w = their hi
+//   [2] y planes, [3] z planes, laid out as [1]
+//   [4] int child[4] (>= 0 binary node, < 0 leaf ~s, kEmptyChild unused)
+// and, in counts4, the leaves under each child (16 bits, saturated): .x children 0, 1, .y 2, 3.
+// f16 step counts let the walk turn a plane into a float inside its fma (v_fma_mix_f32).
 // Child c's box decodes per axis as fmaf(q, 2^e, origin) (q * 2^e is exact: one rounding).
 // origin is the least lo of the node's children, and every q is chosen here with that same
 // decode so that the decoded box contains the child's box (lo rounded down, hi up, the
@@ -314,7 +316,8 @@ __global__ __launch_bounds__(256) void k_level_boxes(const int *__restrict__ lev
 __device__ __forceinline__ float q_scale(int e) {
   return __int_as_float((e + 127) << 23);
 }
-__global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ nodes, int ni, float4 *nodes4) {
+__global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ nodes, int ni, float4 *nodes4,
+                                                    uint2 *counts4) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ni) return;
   float lo[3][4], hi[3][4];
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ no
     ref[e] = kEmptyChild;
     cnt[e] = 0;
   }
-  uint32_t qlo[3], qhi[3], eb = 0;
+  uint32_t qlo[3][4], qhi[3][4], eb = 0;
   float org[3];
   for (int a = 0; a < 3; a++) {
     float o = __builtin_huge_valf(), top = -__builtin_huge_valf();
@@ -368,11 +371,10 @@ __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ no
     const double ext = (double)top - (double)o;
     int sc_e = -120;
     if (ext > 0.0) sc_e = max(-120, min(120, (int)ceil(log2(ext / 254.0))));
-    uint32_t wl = 0, wh = 0;
+    uint32_t wl[4], wh[4];
     for (; sc_e <= 120; sc_e++) {  // the smallest scale whose steps reach every child's top after rounding
       const float sc = q_scale(sc_e);
       bool ok = true;
-      wl = wh = 0;
       for (int c = 0; c < 4 && ok; c++) {
         uint32_t ql = 255, qh = 0;
         if (ref[c] != kEmptyChild) {
@@ -386,24 +388,27 @@ __global__ __launch_bounds__(256) void k_collapse4(const float4 *__restrict__ no
           ql = (uint32_t)l;
           qh = (uint32_t)h;
         }
-        wl |= ql << (8 * c);
-        wh |= qh << (8 * c);
+        wl[c] = ql;
+        wh[c] = qh;
       }
       if (ok) break;
     }
     org[a] = o;
-    qlo[a] = wl;
-    qhi[a] = wh;
+    for (int c = 0; c < 4; c++) qlo[a][c] = wl[c], qhi[a][c] = wh[c];
     eb |= (uint32_t)(sc_e + 128) << (8 * a);
   }
   auto c16 = [&](int i) { return (uint32_t)min(cnt[i], 0xffff); };
-  float4 *out = nodes4 + 4 * (size_t)n;
+  auto h2 = [](uint32_t a, uint32_t b) {  // two step counts as f16 integers (low half, high half)
+    return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)a) |
+                           (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)b) << 16);
+  };
+  float4 *out = nodes4 + 5 * (size_t)n;
   out[0] = make_float4(org[0], org[1], org[2], __uint_as_float(eb));
-  out[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qhi[0]), __uint_as_float(qlo[1]),
-                       __uint_as_float(qhi[1]));
-  out[2] = make_float4(__uint_as_float(qlo[2]), __uint_as_float(qhi[2]), __uint_as_float(c16(0) | c16(1) << 16),
-                       __uint_as_float(c16(2) | c16(3) << 16));
-  *reinterpret_cast<int4 *>(out + 3) = make_int4(ref[0], ref[1], ref[2], ref[3]);
+  for (int a = 0; a < 3; a++)
+    out[1 + a] = make_float4(h2(qlo[a][0], qlo[a][1]), h2(qlo[a][2], qlo[a][3]), h2(qhi[a][0], qhi[a][1]),
+                             h2(qhi[a][2], qhi[a][3]));
+  *reinterpret_cast<int4 *>(out + 4) = make_int4(ref[0], ref[1], ref[2], ref[3]);
+  counts4[n] = make_uint2(c16(0) | c16(1) << 16, c16(2) | c16(3) << 16);
 }
 
 template <class T>
@@ -481,7 +486,7 @@ std::string bvh_cost(Bvh &b, float *out, hipStream_t stream) {
 }
 
 void bvh_free(Bvh &b) {
-  void *ps[] = {b.nodes, b.nodes4, b.leaves, b.leaf_parent, b.keys[0], b.keys[1], b.vals[0], b.vals[1], b.hist,
+  void *ps[] = {b.nodes, b.nodes4, b.counts4, b.leaves, b.leaf_parent, b.keys[0], b.keys[1], b.vals[0], b.vals[1], b.hist,
                 b.bounds, b.centroids, b.hword, b.height, b.level_nodes, b.level_meta, b.sah_part};
   for (void *p : ps)
     if (p) (void)hipFree(p);
@@ -504,7 +509,7 @@ static std::string refit_levels(Bvh &b, const float *verts, const int *cumsum, h
                        b.leaves, b.nodes);
   }
   hipLaunchKernelGGL(k_collapse4, dim3((b.n_internal + 255) / 256), dim3(256), 0, stream, b.nodes, b.n_internal,
-                     b.nodes4);
+                     b.nodes4, b.counts4);
   TT_TRY(hipGetLastError());
   return "";
 }
@@ -529,16 +534,17 @@ std::string bvh_build(Bvh &b, const float *verts, int num_vertices, const int *c
   const int ni = nl - 1;
   const size_t need = (size_t)nl;
   if (b.cap_leaves < need || !b.nodes) {
-    void *ps[] = {b.nodes, b.nodes4, b.leaves, b.leaf_parent, b.keys[0], b.keys[1], b.vals[0], b.vals[1],
+    void *ps[] = {b.nodes, b.nodes4, b.counts4, b.leaves, b.leaf_parent, b.keys[0], b.keys[1], b.vals[0], b.vals[1],
                   b.centroids, b.hword, b.height, b.level_nodes};
     for (void *p : ps)
       if (p) (void)hipFree(p);
-    b.nodes = nullptr, b.nodes4 = nullptr, b.leaves = nullptr, b.leaf_parent = nullptr;
+    b.nodes = nullptr, b.nodes4 = nullptr, b.counts4 = nullptr, b.leaves = nullptr, b.leaf_parent = nullptr;
     b.keys[0] = b.keys[1] = b.vals[0] = b.vals[1] = nullptr;
     b.centroids = nullptr, b.hword = nullptr, b.height = nullptr, b.level_nodes = nullptr;
     b.cap_leaves = 0;
     TT_TRY(hipMalloc(&b.nodes, sizeof(float4) * 4 * need));
-    TT_TRY(hipMalloc(&b.nodes4, sizeof(float4) * 4 * need));
+    TT_TRY(hipMalloc(&b.nodes4, sizeof(float4) * 5 * need));
+    TT_TRY(hipMalloc(&b.counts4, sizeof(uint2) * need));
     TT_TRY(hipMalloc(&b.leaves, sizeof(float4) * 3 * need));
     TT_TRY(hipMalloc(&b.leaf_parent, sizeof(int) * need));
     TT_TRY(hipMalloc(&b.keys[0], sizeof(uint32_t) * need));

@@ -303,6 +303,7 @@ void fill_common(tt::TraceParams &p, const tt_tracer *tr, int W, int H, const fl
   p.sigma = sigma;
   p.nodes = tr->bvh.nodes;
   p.nodes4 = tr->bvh.nodes4;
+  p.counts4 = tr->bvh.counts4;
   p.leaves = tr->bvh.leaves;
   p.n_leaves = P > 0 ? tr->bvh.n_leaves : 0;
   p.chunk_mode = chunk_mode;

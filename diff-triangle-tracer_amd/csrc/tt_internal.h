@@ -17,19 +17,22 @@ constexpr int kEmptyChild = 0x7fffffff;  // bound on LBVH height bookkeeping (a 
 //            [2] (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)   [3] int (child0, child1, parent, 0)
 //            child >= 0: internal node index; child < 0: leaf ~s (sorted position s)
 //   leaves : 3 x 16 B per leaf in Morton order: (A.xyz, prim bits) (B.xyz, 0) (C.xyz, 0)
-//   nodes4 : 4 x 16 B per binary node n, the traversal's 4-wide node: n's children, with
+//   nodes4 : 5 x 16 B per binary node n, the traversal's 4-wide node: n's children, with
 //            each internal child replaced by its own two children (2 to 4 entries), the
-//            child boxes as 8-bit planes over a per-axis origin and power-of-two scale
-//            [0] origin xyz + exponents [1] x, y planes [2] z planes + leaf counts
-//            [3] int child[4] (>= 0 binary node, < 0 leaf ~s, kEmptyChild unused)
+//            child boxes quantized to 256 steps over a per-axis origin and power-of-two
+//            scale, each step count stored as an f16 integer
+//            [0] origin xyz + exponents [1] x planes [2] y planes [3] z planes
+//            [4] int child[4] (>= 0 binary node, < 0 leaf ~s, kEmptyChild unused)
+//   counts4: per binary node n, the leaves under each of nodes4[n]'s children (16 bits each)
 struct Bvh {
   int P = -1;           // triangles in the structure (-1: none built)
   int n_leaves = 0;     // P, or 2 when P == 1 (second leaf is an unhittable dummy)
   int n_internal = 0;   // n_leaves - 1
   size_t cap_leaves = 0;
   float4 *nodes = nullptr;
-  float4 *nodes4 = nullptr;   // 4-wide view, 4 x 16 B per binary node n: the up-to-4 grandchildren of n,
-                              // their boxes quantized to 8-bit planes (layout at k_collapse4)
+  float4 *nodes4 = nullptr;   // 4-wide view, 5 x 16 B per binary node n: the up-to-4 grandchildren of n,
+                              // their boxes quantized to 256 steps (layout at k_collapse4)
+  uint2 *counts4 = nullptr;   // leaves under each child of nodes4[n] (the tile probe's estimate)
   float4 *leaves = nullptr;
   int *leaf_parent = nullptr;
   unsigned long long *hword = nullptr;  // per node: the two children's reported heights
@@ -68,6 +71,7 @@ struct TraceParams {
   int M, deg;
   const float *opac, *sigma;
   const float4 *nodes, *leaves, *nodes4;
+  const uint2 *counts4;
   int n_leaves;
   int *ov_stack;  // traversal stack spill: ov_depth entries per launched lane
   int ov_depth;

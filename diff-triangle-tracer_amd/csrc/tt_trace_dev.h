@@ -110,26 +110,31 @@ TT_D void cswap(float &da, int &ca, float &db, int &cb) {
 // so the intersection code runs with most lanes busy instead of once per leaf-holding
 // lane. The K-buffer keeps the K smallest keys whatever the insertion order, so
 // postponing a leaf only weakens the culling bound for a while.
+// Two f16 integers (low half, high half) of a quantized node's plane word.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+TT_D half2_t plane_pair(float w) { return __builtin_bit_cast(half2_t, w); }
+
 // A 4-wide node (quantized, layout at k_collapse4) decoded to its child boxes' planes: per
 // axis fmaf(q, 2^e, origin), the builder's own decode, so the boxes contain the exact ones.
 TT_D void load_node4q(const float4 *q, float4 &lx, float4 &hx, float4 &ly, float4 &hy, float4 &lz, float4 &hz,
                       int4 &ref) {
-  const float4 a = q[0], b = q[1], c = q[2];
-  ref = *reinterpret_cast<const int4 *>(q + 3);
+  const float4 a = q[0];
+  ref = *reinterpret_cast<const int4 *>(q + 4);
   const uint32_t eb = __float_as_uint(a.w);
+  auto dec4 = [](float w0, float w1, float s, float o) {
+    const half2_t u = plane_pair(w0), v = plane_pair(w1);
+    return make_float4(fmaf((float)u.x, s, o), fmaf((float)u.y, s, o), fmaf((float)v.x, s, o),
+                       fmaf((float)v.y, s, o));
+  };
+  const float4 px = q[1], py = q[2], pz = q[3];
   const float sx = __uint_as_float(((eb & 0xffu) - 1u) << 23), sy = __uint_as_float((((eb >> 8) & 0xffu) - 1u) << 23),
               sz = __uint_as_float((((eb >> 16) & 0xffu) - 1u) << 23);
-  auto dec4 = [](float w, float s, float o) {
-    const uint32_t u = __float_as_uint(w);
-    return make_float4(fmaf((float)(u & 0xffu), s, o), fmaf((float)((u >> 8) & 0xffu), s, o),
-                       fmaf((float)((u >> 16) & 0xffu), s, o), fmaf((float)(u >> 24), s, o));
-  };
-  lx = dec4(b.x, sx, a.x);
-  hx = dec4(b.y, sx, a.x);
-  ly = dec4(b.z, sy, a.y);
-  hy = dec4(b.w, sy, a.y);
-  lz = dec4(c.x, sz, a.z);
-  hz = dec4(c.y, sz, a.z);
+  lx = dec4(px.x, px.y, sx, a.x);
+  hx = dec4(px.z, px.w, sx, a.x);
+  ly = dec4(py.x, py.y, sy, a.y);
+  hy = dec4(py.z, py.w, sy, a.y);
+  lz = dec4(pz.x, pz.y, sz, a.z);
+  hz = dec4(pz.z, pz.w, sz, a.z);
 }
 
 // The walk's slab test reads a quantized node in ray space instead of decoding its planes:
@@ -157,8 +162,8 @@ TT_D NodePad node_pad(const TraceParams &P, const RayPre &r) {
 }
 // Entry and exit distances of the 4 children of a quantized node (unwidened, as slab_raw).
 TT_D void slab4q(const RayPre &r, const NodePad &np, const float4 *q, float4 &tn, float4 &tf, int4 &ref) {
-  const float4 a = q[0], b = q[1], c = q[2];
-  ref = *reinterpret_cast<const int4 *>(q + 3);
+  const float4 a = q[0], px = q[1], py = q[2], pz = q[3];
+  ref = *reinterpret_cast<const int4 *>(q + 4);
   const uint32_t eb = __float_as_uint(a.w);
   constexpr float kBig = 3.0e38f;  // 2^e / d overflows only for e near 120 and d near 1e-30
   auto step = [&](uint32_t e, float inv) {
@@ -166,14 +171,14 @@ TT_D void slab4q(const RayPre &r, const NodePad &np, const float4 *q, float4 &tn
   };
   const float sx = step(eb, r.inv.x), sy = step(eb >> 8, r.inv.y), sz = step(eb >> 16, r.inv.z);
   const float bx = (a.x - r.o.x) * r.inv.x, by = (a.y - r.o.y) * r.inv.y, bz = (a.z - r.o.z) * r.inv.z;
-  auto t4 = [](float w, float s, float o) {
-    const uint32_t u = __float_as_uint(w);
-    return make_float4(fmaf((float)(u & 0xffu), s, o), fmaf((float)((u >> 8) & 0xffu), s, o),
-                       fmaf((float)((u >> 16) & 0xffu), s, o), fmaf((float)(u >> 24), s, o));
+  auto t4 = [](float w0, float w1, float s, float o) {  // f16 step counts converted inside the fma
+    const half2_t u = plane_pair(w0), v = plane_pair(w1);
+    return make_float4(fmaf((float)u.x, s, o), fmaf((float)u.y, s, o), fmaf((float)v.x, s, o),
+                       fmaf((float)v.y, s, o));
   };
-  const float4 lx = t4(b.x, sx, bx - np.pinv.x), hx = t4(b.y, sx, bx + np.pinv.x);
-  const float4 ly = t4(b.z, sy, by - np.pinv.y), hy = t4(b.w, sy, by + np.pinv.y);
-  const float4 lz = t4(c.x, sz, bz - np.pinv.z), hz = t4(c.y, sz, bz + np.pinv.z);
+  const float4 lx = t4(px.x, px.y, sx, bx - np.pinv.x), hx = t4(px.z, px.w, sx, bx + np.pinv.x);
+  const float4 ly = t4(py.x, py.y, sy, by - np.pinv.y), hy = t4(py.z, py.w, sy, by + np.pinv.y);
+  const float4 lz = t4(pz.x, pz.y, sz, bz - np.pinv.z), hz = t4(pz.z, pz.w, sz, bz + np.pinv.z);
   auto near = [](float l0, float h0, float l1, float h1, float l2, float h2) {
     return fmaxf(fmaxf(fminf(l0, h0), fminf(l1, h1)), fminf(l2, h2));
   };
@@ -211,7 +216,7 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       if (COUNT) tc.nodes++;
       float4 tn, tf;
       int4 ref;
-      slab4q(r, np, P.nodes4 + 4 * (size_t)node, tn, tf, ref);
+      slab4q(r, np, P.nodes4 + 5 * (size_t)node, tn, tf, ref);
       const float thi = fminf(far_t, b.t[K - 1]);
       const float hi_w = thi + fabsf(thi) * kWideRel + kWideAbs;
       const float tn0 = tn.x, tf0 = tf.x, tn1 = tn.y, tf1 = tf.y, tn2 = tn.z, tf2 = tf.z, tn3 = tn.w, tf3 = tf.w;
@@ -323,9 +328,9 @@ TT_D float probe_tile(const TraceParams &P, int tile, int *s_front, int *s_count
       const int node = cur[i];
       float4 lx, hx, ly, hy, lz, hz;
       int4 ref;
-      load_node4q(P.nodes4 + 4 * (size_t)node, lx, hx, ly, hy, lz, hz, ref);
-      const float4 q2 = P.nodes4[4 * (size_t)node + 2];  // leaves under each child, 16 bits each (saturated)
-      const uint32_t c01 = __float_as_uint(q2.z), c23 = __float_as_uint(q2.w);
+      load_node4q(P.nodes4 + 5 * (size_t)node, lx, hx, ly, hy, lz, hz, ref);
+      const uint2 cw = P.counts4[node];  // leaves under each child, 16 bits each (saturated)
+      const uint32_t c01 = cw.x, c23 = cw.y;
       const int ct[4] = {(int)(c01 & 0xffffu), (int)(c01 >> 16), (int)(c23 & 0xffffu), (int)(c23 >> 16)};
       float tn[4], tf[4];
       slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tn[0], tf[0]);
