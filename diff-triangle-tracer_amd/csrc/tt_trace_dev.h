@@ -1482,35 +1482,56 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   __syncthreads();
 }
 
-// Sums of records sorted by triangle (keys) -- rows[j] is the record row of sorted entry j
-// -- with one thread per kRedChunk consecutive entries: each run of one triangle inside a
-// chunk is summed in registers and flushed once (FP32 atomics; a run crossing chunks is
-// flushed once per chunk). Entries with key 0xffffffff (rows never written) end the list.
-constexpr int kRedChunk = 64;
-__global__ __launch_bounds__(256) void k_rec_reduce_sorted(TraceParams P, const uint32_t *keys, const uint32_t *rows,
-                                                           int n) {
-  const long long b = ((long long)blockIdx.x * 256 + threadIdx.x) * kRedChunk;
-  if (b >= n) return;
-  const long long e = min((long long)n, b + kRedChunk);
-  uint32_t cur = 0xffffffffu;
-  float acc[11];
-  ShSink sh;
-  auto reset = [&]() {
+// Sums of records sorted by triangle (keys; rows[j] is the record row of sorted entry j;
+// entries with key 0xffffffff, rows never written, sort last). One wave per window of 64
+// sorted entries (a grid-stride loop over the windows that ends at the first window of
+// unwritten rows): each lane rebuilds one record's 59 gradient values into LDS, then for
+// each run of one triangle in the window the lanes sum one field each down the run and
+// flush it with one FP32 atomic instruction (lane = field), so a triangle costs one atomic
+// per field per window it appears in.
+constexpr int kRedRow = 61;  // floats per staged record (59 used), odd: rows start in distinct banks
+__global__ __launch_bounds__(64) void k_rec_reduce_sorted(TraceParams P, const uint32_t *keys, const uint32_t *rows,
+                                                          int n) {
+  __shared__ float s_v[64 * kRedRow];
+  const int lane = threadIdx.x;
+  const int nsh3 = 3 * (P.deg + 1) * (P.deg + 1);
+  const long long n_win = ((long long)n + 63) / 64;
+  for (long long w = blockIdx.x; w < n_win; w += gridDim.x) {
+    const long long j = w * 64 + lane;
+    const uint32_t key = j < n ? keys[j] : 0xffffffffu;
+    if (__shfl(key, 0) == 0xffffffffu) break;  // sorted: no written rows from here on
+    float acc[11];
+    ShSink sh;
     for (int f = 0; f < 11; f++) acc[f] = 0.0f;
     for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
-  };
-  reset();
-  for (long long j = b; j < e; j++) {
-    const uint32_t key = keys[j];
-    if (key == 0xffffffffu) break;
-    if (key != cur) {
-      if (cur != 0xffffffffu) flush_sums(P, cur, acc, sh);
-      reset();
-      cur = key;
+    if (key != 0xffffffffu) add_record(P, rows[j], acc, sh);
+    float *mine = s_v + lane * kRedRow;
+    for (int f = 0; f < 48; f++) mine[f] = sh.v[f];
+    for (int f = 0; f < 11; f++) mine[48 + f] = acc[f];
+    const uint32_t prev = __shfl_up(key, 1);
+    unsigned long long heads = __ballot(lane == 0 || prev != key);
+    __syncthreads();
+    while (heads) {  // runs [s, e) of one key, in order
+      const int s = __ffsll(heads) - 1;
+      heads &= heads - 1;
+      const int e = heads ? __ffsll(heads) - 1 : 64;
+      const uint32_t prim = __shfl(key, s);
+      if (prim == 0xffffffffu || lane >= 59) continue;
+      float sum = 0.0f;
+      for (int r = s; r < e; r++) sum += s_v[r * kRedRow + lane];
+      if (sum == 0.0f) continue;
+      if (lane < 48) {
+        if (lane < nsh3) unsafeAtomicAdd(P.dsh + (size_t)prim * P.M * 3 + lane, sum);
+      } else if (lane < 57) {
+        unsafeAtomicAdd(P.dverts + 3 * (size_t)P.cumsum[prim] + (lane - 48), sum);
+      } else if (lane == 57) {
+        unsafeAtomicAdd(P.dopac + prim, sum);
+      } else {
+        unsafeAtomicAdd(P.dsigma + prim, sum);
+      }
     }
-    add_record(P, rows[j], acc, sh);
+    __syncthreads();
   }
-  if (cur != 0xffffffffu) flush_sums(P, cur, acc, sh);
 }
 
 __global__ void k_iota(uint32_t *p, int n) {
