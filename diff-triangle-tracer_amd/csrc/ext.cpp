@@ -11,7 +11,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
       .def(pybind11::init<>())
       .def_readonly("data", &FwdTraceResult::data)
       .def_readonly("hits", &FwdTraceResult::hits)
-      .def_readonly("hit_counts", &FwdTraceResult::hit_counts);
+      .def_readonly("hit_counts", &FwdTraceResult::hit_counts)
+      .def_readonly("hit_colors", &FwdTraceResult::hit_colors);
 
   pybind11::class_<BwdTraceResult>(m, "BwdTraceResult")
       .def(pybind11::init<>())
@@ -28,7 +29,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
            pybind11::arg("colors"), pybind11::arg("others"), pybind11::arg("number_of_points"), pybind11::arg("shs"),
            pybind11::arg("degree"), pybind11::arg("opacities"), pybind11::arg("dL_dpixels"),
            pybind11::arg("dL_dothers"), pybind11::arg("scaling"), pybind11::arg("debug"),
-           pybind11::arg("hits") = pybind11::none(), pybind11::arg("hit_counts") = pybind11::none())
+           pybind11::arg("hits") = pybind11::none(), pybind11::arg("hit_counts") = pybind11::none(),
+           pybind11::arg("hit_colors") = pybind11::none())
       .def("set_options", &OptixTracer::set_options, pybind11::arg("k") = 16, pybind11::arg("cutoff") = 1e-4,
            pybind11::arg("chunk_mode") = 0, pybind11::arg("hit_cache") = 64)
       .def("get_options", &OptixTracer::get_options)

@@ -227,13 +227,15 @@ FwdTraceResult OptixTracer::trace_fwd(const Tensor &ray_o, const Tensor &ray_d, 
   a.max_blending_dev = max_blending.data_ptr<float>();
   a.density_dev = dens.data_ptr<float>();
   a.scaling_dev = scal.data_ptr<float>();
-  Tensor hits, hit_counts;
+  Tensor hits, hit_counts, hit_colors;
   if (hit_cache_ > 0 && store_hits_ && variant_ == TT_VARIANT_FORWARD && (long long)H * W > 0) {
     auto iopts = torch::TensorOptions().dtype(torch::kInt32).device(torch::kCUDA, dev);
     hits = torch::empty({hit_cache_, (long long)H * W, 2}, iopts);
     hit_counts = torch::empty({(long long)H * W}, iopts);
+    hit_colors = torch::empty({hit_cache_, (long long)H * W, 3}, fopts);
     a.hitlist_dev = reinterpret_cast<unsigned int *>(hits.data_ptr<int>());
     a.hitlist_count_dev = hit_counts.data_ptr<int>();
+    a.hitlist_color_dev = hit_colors.data_ptr<float>();
     a.hitlist_cap = hit_cache_;
   }
 
@@ -262,6 +264,7 @@ FwdTraceResult OptixTracer::trace_fwd(const Tensor &ray_o, const Tensor &ray_d, 
   res.data = std::make_tuple(out_color, out_others, gpu_time, scaling, density_factor, max_blending);
   res.hits = hits;
   res.hit_counts = hit_counts;
+  res.hit_colors = hit_colors;
   return res;
 }
 
@@ -272,7 +275,8 @@ BwdTraceResult OptixTracer::trace_bwd(const Tensor &ray_o, const Tensor &ray_d, 
                                       const torch::Tensor &others, const int number_of_points, const Tensor &shs,
                                       const int sh_degree, const Tensor &opacities, const Tensor &dL_dpixels,
                                       const Tensor &dL_dothers, const Tensor &scaling, const bool debug,
-                                      const c10::optional<Tensor> &hits, const c10::optional<Tensor> &hit_counts) {
+                                      const c10::optional<Tensor> &hits, const c10::optional<Tensor> &hit_counts,
+                                      const c10::optional<Tensor> &hit_colors) {
   (void)background;
   (void)scaling;
   (void)debug;
@@ -348,6 +352,13 @@ BwdTraceResult OptixTracer::trace_bwd(const Tensor &ray_o, const Tensor &ray_d, 
     a.hitlist_dev = reinterpret_cast<const unsigned int *>(hl.data_ptr<int>());
     a.hitlist_count_dev = hc.data_ptr<int>();
     a.hitlist_cap = (int)hl.size(0);
+    if (hit_colors.has_value() && hit_colors->defined()) {
+      const Tensor &hk = *hit_colors;
+      if (hk.dim() != 3 || hk.size(0) != hl.size(0) || hk.size(1) != hl.size(1) || hk.size(2) != 3 ||
+          hk.scalar_type() != torch::kFloat32 || !hk.is_contiguous() || hk.device() != ro.device())
+        throw Exception("hit_colors must be the forward's float32 [cap, H*W, 3] tensor");
+      a.hitlist_color_dev = hk.data_ptr<float>();
+    }
   }
   check_tt(tt_trace_bwd(tr_, &a, cur_stream(dev)), "trace_backward");
   BwdTraceResult res;

@@ -32,9 +32,10 @@ struct TracerState {
 
 struct FwdTraceResult {
   std::tuple<torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor, torch::Tensor> data;
-  // The forward's stored blended hits (int32 [cap, H*W, 2]) and counts (int32 [H*W]) for
-  // trace_bwd; undefined when the hit cache is off or the program is not "forward".
-  torch::Tensor hits, hit_counts;
+  // The forward's stored blended hits (int32 [cap, H*W, 2]), counts (int32 [H*W]) and
+  // colours (float [cap, H*W, 3]) for trace_bwd; undefined when the hit cache is off or the
+  // program is not "forward".
+  torch::Tensor hits, hit_counts, hit_colors;
 };
 
 struct BwdTraceResult {
@@ -69,7 +70,8 @@ class OptixTracer {
                            const int degree, const Tensor &opacities, const Tensor &dL_dpixels,
                            const Tensor &dL_dothers, const Tensor &scaling, const bool debug,
                            const c10::optional<Tensor> &hits = c10::nullopt,
-                           const c10::optional<Tensor> &hit_counts = c10::nullopt);
+                           const c10::optional<Tensor> &hit_counts = c10::nullopt,
+                           const c10::optional<Tensor> &hit_colors = c10::nullopt);
 
   // Options this tracer adds (the reference hard-codes CHUNK_SIZE 16 and a 1e-4
   // transmittance cutoff, config.h:10 and forward.cu:134). hit_cache: blended hits per

@@ -576,6 +576,7 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.ival_fb_end = a->interval_fallback_end;
   if (a->hitlist_dev) {
     p.hl = reinterpret_cast<uint2 *>(a->hitlist_dev);
+    p.hl_rgb = a->hitlist_color_dev;
     p.hl_count = a->hitlist_count_dev;
     p.hl_cap = a->hitlist_cap;
     p.hl_stride = (size_t)a->width * a->height;
@@ -692,6 +693,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
   p.ival_bwd = reinterpret_cast<const float4 *>(a->interval_dev);
   if (a->hitlist_dev) {
     p.hl_in = reinterpret_cast<const uint2 *>(a->hitlist_dev);
+    p.hl_rgb_in = a->hitlist_color_dev;
     p.hl_count_in = a->hitlist_count_dev;
     p.hl_cap = a->hitlist_cap;
     p.hl_stride = (size_t)a->width * a->height;
@@ -807,10 +809,12 @@ int multi_params(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::Multi
     p.hl_stride = (size_t)rays;
     if (!backward) {
       p.hl = reinterpret_cast<uint2 *>(a->hitlist_dev);
+      p.hl_rgb = a->hitlist_color_dev;
       p.hl_count = a->hitlist_count_dev;
       p.hl_cap = a->hitlist_cap;
     } else {
       p.hl_in = reinterpret_cast<const uint2 *>(a->hitlist_dev);
+      p.hl_rgb_in = a->hitlist_color_dev;
       p.hl_count_in = a->hitlist_count_dev;
       p.hl_cap = a->hitlist_cap;
     }

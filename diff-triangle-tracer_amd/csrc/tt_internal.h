@@ -136,6 +136,8 @@ struct TraceParams {
   int *hl_count;
   int hl_cap;
   const uint2 *hl_in;
+  float *hl_rgb;          // optional: colours of the stored hits, [entry][3] (clamped channel: -0.0f)
+  const float *hl_rgb_in;
   const int *hl_count_in;
   size_t hl_stride;  // entries between hit i and hit i + 1 of a pixel (the launch's total rays)
   int rec_lane_rows;  // rows per launched lane of a fixed record region (no per-tile regions)

@@ -62,6 +62,8 @@ TT_D TraceParams camera_view(const TraceParams &P, const MultiCams &mc, int tile
   if (P.hl) Q.hl = P.hl + off;  // hit lists keep the launch-wide stride P.hl_stride
   if (P.hl_count) Q.hl_count = P.hl_count + off;
   if (P.hl_in) Q.hl_in = P.hl_in + off;
+  if (P.hl_rgb) Q.hl_rgb = P.hl_rgb + 3 * off;
+  if (P.hl_rgb_in) Q.hl_rgb_in = P.hl_rgb_in + 3 * off;
   if (P.hl_count_in) Q.hl_count_in = P.hl_count_in + off;
   local_tile = tile - mc.tile_begin[c];
   return Q;

@@ -532,8 +532,16 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
             st.D += bw * t;
           }
           st.T = testT;
-          if (hseg >= 0 && P.hl && st.blended < P.hl_cap)  // hit list of segment hseg
-            P.hl[((size_t)hseg * P.hl_cap + st.blended) * P.hl_stride + ray] = make_uint2(prim, __float_as_uint(t));
+          if (hseg >= 0 && P.hl && st.blended < P.hl_cap) {  // hit list of segment hseg
+            const size_t e = ((size_t)hseg * P.hl_cap + st.blended) * P.hl_stride + ray;
+            P.hl[e] = make_uint2(prim, __float_as_uint(t));
+            if (P.hl_rgb) {  // the colour as blended; a clamped channel as -0.0f (+ 0.0f: no other -0)
+              float *c = P.hl_rgb + 3 * e;
+              c[0] = c0 ? -0.0f : col.x + 0.0f;
+              c[1] = c1 ? -0.0f : col.y + 0.0f;
+              c[2] = c2 ? -0.0f : col.z + 0.0f;
+            }
+          }
           st.blended++;
           code = 2;
         }
@@ -1131,10 +1139,12 @@ template <int K, int BLOCK, int MODE>
 TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], const float g[3], float dL_ddepth,
                         v3f dL_dn2, float final_depth, v3f final_normal, const float *Lr, const float *Lt, float tlo,
                         long long plo, float far_t, int *s_node, int *s_rec,
-                        const uint2 *hl = nullptr, int n_hl = 0, size_t stride = 0) {
+                        const uint2 *hl = nullptr, int n_hl = 0, size_t stride = 0,
+                        const float *hlc = nullptr) {
   float C[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f}, D = 0.0f, T = 1.0f;
-  // one hit at or past the near plane; false: the ray terminates before blending it
-  auto blend = [&](float t, uint32_t prim) -> bool {
+  // one hit at or past the near plane; false: the ray terminates before blending it. rgb:
+  // its colour from the forward's stored hits (clamped channel -0.0f), or null: evaluated here
+  auto blend = [&](float t, uint32_t prim, const float *rgb) -> bool {
     const int off = P.cumsum[prim];
     const float *vp = P.verts + 3 * (size_t)off;
     const v3f V[3] = {ld3(vp), ld3(vp + 3), ld3(vp + 6)};
@@ -1147,7 +1157,14 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
     if (testT < P.cutoff) return false;
     const float bw = alpha * T;
     bool c0, c1, c2;
-    const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
+    v3f col;
+    if (rgb) {
+      const float r0 = rgb[0], r1 = rgb[1], r2 = rgb[2];
+      c0 = signbit(r0), c1 = signbit(r1), c2 = signbit(r2);
+      col = mk(fabsf(r0), fabsf(r1), fabsf(r2));
+    } else {
+      col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
+    }
     float cc[3] = {col.x, col.y, col.z};
     if (MODE == 1) {
       const float rho = P.refl[prim];
@@ -1190,7 +1207,7 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
   if (hl) {
     for (int i = 0; i < n_hl; i++) {
       const uint2 h = hl[(size_t)i * stride];
-      if (!blend(__uint_as_float(h.y), h.x)) break;
+      if (!blend(__uint_as_float(h.y), h.x, hlc ? hlc + 3 * (size_t)i * stride : nullptr)) break;
     }
     return T;
   }
@@ -1211,7 +1228,7 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
       lp = prim;
       last = fmaxf((float)((double)last + kStepEps), t);
       if (t < P.near_t) continue;
-      if (!blend(t, prim)) return T;
+      if (!blend(t, prim, nullptr)) return T;
     }
     if (P.chunk_mode == 1) {
       tlo = lt;
@@ -1305,8 +1322,9 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
       plo = (long long)__float_as_uint(iv.w) - 1;
       far_t = iv.y;
     }
+    const float *hlc = hl && P.hl_rgb_in ? P.hl_rgb_in + 3 * (size_t)pix : nullptr;
     backward_seg<K, BLOCK, 0>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal, nullptr,
-                              nullptr, tlo, plo, far_t, s_node, s_rec, hl, n_hl, P.hl_stride);
+                              nullptr, tlo, plo, far_t, s_node, s_rec, hl, n_hl, P.hl_stride, hlc);
   } else {
     constexpr bool kPrim = VARIANT == 4 || VARIANT == 5;
     constexpr bool kRefl = VARIANT == 4 || VARIANT == 6 || VARIANT == 8;
@@ -1316,17 +1334,22 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
     for (int c = 0; c < 3; c++) Lr[c] = a[c], Lt[c] = a[3 + c];
     const float Rw = a[6], Qw = a[7];
     const uint2 *hls[3] = {nullptr, nullptr, nullptr};  // the forward's hit lists of the three segments
+    const float *hlcs[3] = {nullptr, nullptr, nullptr};  // and their colours
     int nhl[3] = {0, 0, 0};
     if (P.hl_in)
       for (int sg = 0; sg < 3; sg++) {
         if (!(sg == 0 ? kPrim : (sg == 1 ? kRefl : kRefr))) continue;
         nhl[sg] = P.hl_count_in[sg * P.hl_stride + pix];
-        if (nhl[sg] >= 0) hls[sg] = P.hl_in + (size_t)sg * P.hl_cap * P.hl_stride + pix;
+        if (nhl[sg] >= 0) {
+          const size_t e0 = (size_t)sg * P.hl_cap * P.hl_stride + pix;
+          hls[sg] = P.hl_in + e0;
+          if (P.hl_rgb_in) hlcs[sg] = P.hl_rgb_in + 3 * e0;
+        }
       }
     if (kPrim) {
       const float Tp = backward_seg<K, BLOCK, 1>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
                                                  final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_rec, hls[0],
-                                                 nhl[0], P.hl_stride);
+                                                 nhl[0], P.hl_stride, hlcs[0]);
       const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
       if (env_g) {  // the caller aggregates it across the wave (env_scatter_wave)
         for (int c = 0; c < 3; c++) env_g[c] = ge[c];
@@ -1345,7 +1368,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
         const float gs[3] = {wgt * dL_dpixel[0], wgt * dL_dpixel[1], wgt * dL_dpixel[2]};
         const float Ts = backward_seg<K, BLOCK, 2>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
                                                    nullptr, bounce_start(P), -1, P.far_t, s_node, s_rec, hls[1 + s],
-                                                   nhl[1 + s], P.hl_stride);
+                                                   nhl[1 + s], P.hl_stride, hlcs[1 + s]);
         const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
         env_scatter(P, sd, gse);
       };

@@ -82,7 +82,8 @@ class _Tracer(torch.autograd.Function):
                 tracer.set_store_hits(True)
         color, others, gpu_time, scaling, density_factor, max_blending = res.data
 
-        ctx.hits = (res.hits, res.hit_counts)  # stored blended hits (None: the backward re-traverses)
+        # stored blended hits and their colours (None: the backward re-traverses)
+        ctx.hits = (res.hits, res.hit_counts, res.hit_colors)
         ctx.tracer = tracer
         ctx.tracer_settings = tracer_settings
         ctx.number_of_points = number_of_points

@@ -53,7 +53,7 @@ class FwdArgs(C.Structure):
         ("interval_hard_cap", C.c_int), ("interval_fallback_start", C.c_float), ("interval_fallback_end", C.c_float),
         ("reflectivity_dev", _fp), ("ior_dev", _fp), ("envmap_dev", _fp), ("env_width", C.c_int),
         ("env_height", C.c_int), ("bounce_aux_dev", _fp),
-        ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int),
+        ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int), ("hitlist_color_dev", _fp),
     ]
 
 
@@ -73,7 +73,7 @@ class BwdArgs(C.Structure):
         ("interval_dev", _fp), ("variant", C.c_int), ("reflectivity_dev", _fp), ("envmap_dev", _fp),
         ("env_width", C.c_int), ("env_height", C.c_int), ("bounce_aux_dev", _fp), ("dL_dreflectivity_dev", _fp),
         ("dL_denvmap_dev", _fp),
-        ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int),
+        ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int), ("hitlist_color_dev", _fp),
     ]
 
 
@@ -93,6 +93,7 @@ class MultiArgs(C.Structure):
         ("dL_dvertices_dev", _fp), ("dL_dsigma_dev", _fp), ("dL_dopacity_dev", _fp), ("dL_dsh_dev", _fp),
         ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int),
         ("cameras_dev", _fp), ("target_dev", _fp), ("loss_grad_dev", _fp), ("loss_dev", _fp), ("loss_kind", C.c_int),
+        ("hitlist_color_dev", _fp),
     ]
 
 
@@ -456,12 +457,23 @@ def densify_apply(P, P_out, offset, cumsum, arrays, npts_out=None, cumsum_out=No
     check(lib().tt_densify_apply(C.byref(a), stream), "tt_densify_apply")
 
 
-def hitlist_buffers(R, cap, device, segments=1):
+def hitlist_buffers(R, cap, device, segments=1, colors=True):
     """Hit list of a forward for its backward: (hits int32 [segments, cap, R, 2], counts int32
-    [segments, R]); segments = 3 for the bounce variant (primary, reflected, refracted)."""
+    [segments, R][, colours float32 [segments, cap, R, 3]]); segments = 3 for the bounce variant
+    (primary, reflected, refracted). With the colours the backward reads each hit's colour
+    instead of evaluating its SH coefficients again."""
     import torch
-    return (torch.empty(int(segments), int(cap), int(R), 2, dtype=torch.int32, device=device),
-            torch.empty(int(segments), int(R), dtype=torch.int32, device=device))
+    out = (torch.empty(int(segments), int(cap), int(R), 2, dtype=torch.int32, device=device),
+           torch.empty(int(segments), int(R), dtype=torch.int32, device=device))
+    if colors:
+        out += (torch.empty(int(segments), int(cap), int(R), 3, dtype=torch.float32, device=device),)
+    return out
+
+
+def _set_hitlist(a, hitlist):
+    hits, counts = hitlist[0], hitlist[1]
+    a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[-3])
+    a.hitlist_color_dev = ptr(hitlist[2]) if len(hitlist) > 2 else None
 
 
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
@@ -469,7 +481,8 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
              log=None, near=0.2, far=1000.0, trav_stats=None, interval_in=None, interval_out=None,
              hard_cap=False, fallback_start=0.0, fallback_end=0.0, bounce=None, hitlist=None) -> FwdArgs:
     """bounce: dict(reflectivity [P], ior [P], envmap [H, W, 3], aux [R, 20]) for variant "bounce".
-    hitlist: (hits int32 [cap, R, 2], counts int32 [R]) written for the backward (see hitlist_buffers)."""
+    hitlist: (hits int32 [cap, R, 2], counts int32 [R][, colours float32 [cap, R, 3]]) written for the
+    backward (see hitlist_buffers)."""
     a = FwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -497,8 +510,7 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
         a.envmap_dev, a.bounce_aux_dev = ptr(bounce["envmap"]), ptr(bounce["aux"])
         a.env_height, a.env_width = int(bounce["envmap"].shape[0]), int(bounce["envmap"].shape[1])
     if hitlist is not None:
-        hits, counts = hitlist
-        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[-3])
+        _set_hitlist(a, hitlist)
     if log is not None:
         prim, t, code, count = log
         a.hit_log_cap = int(prim.shape[-1])
@@ -530,8 +542,7 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
     a.interval_dev = ptr(interval)
     if hitlist is not None:
-        hits, counts = hitlist
-        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[-3])
+        _set_hitlist(a, hitlist)
     if bounce is not None:
         a.variant = VARIANTS["bounce"]
         a.reflectivity_dev, a.envmap_dev = ptr(bounce["reflectivity"]), ptr(bounce["envmap"])
@@ -577,8 +588,7 @@ def multi_args(widths, heights, ray_o, ray_d, verts, npts, cumsum, shs, sh_degre
     a.dL_dvertices_rows = int(dverts.numel() // 3) if dverts is not None else 0
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
     if hitlist is not None:
-        hits, counts = hitlist
-        a.hitlist_dev, a.hitlist_count_dev, a.hitlist_cap = ptr(hits), ptr(counts), int(hits.shape[-3])
+        _set_hitlist(a, hitlist)
     a.cameras_dev = ptr(cameras)
     a.target_dev, a.loss_grad_dev, a.loss_dev = ptr(target), ptr(loss_grad), ptr(loss)
     a.loss_kind = LOSS[loss_kind] if target is not None else 0

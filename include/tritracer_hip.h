@@ -154,6 +154,11 @@ typedef struct {
   unsigned int *hitlist_dev;     /* [segments][hitlist_cap][R][2] written */
   int *hitlist_count_dev;        /* [segments][R] written */
   int hitlist_cap;
+  /* Optional with hitlist_dev: each stored hit's SH colour (the FP32 value the forward blended;
+     a channel clamped at zero is stored as -0.0f), at hitlist_color[3 * e + c] for the entry e
+     that hitlist[2 * e] holds. Handing it to tt_trace_bwd spares the backward the SH
+     coefficients' read and evaluation per hit (same gradients). */
+  float *hitlist_color_dev;      /* [segments][hitlist_cap][R][3] written */
 } tt_fwd_args;
 
 /* Replaces OptixTracer::trace_fwd's launch (tracer.cpp:105-132) and the raygen programs
@@ -199,6 +204,7 @@ typedef struct {
   const unsigned int *hitlist_dev;
   const int *hitlist_count_dev;
   int hitlist_cap;
+  const float *hitlist_color_dev; /* optional: the forward's hitlist_color_dev */
 } tt_bwd_args;
 
 /* Replaces OptixTracer::trace_bwd's launch (tracer.cpp:199-222) and backward.cu:191-391:
@@ -255,6 +261,9 @@ typedef struct {
   float *loss_grad_dev;
   float *loss_dev;
   int loss_kind;                /* TT_LOSS_* */
+  /* optional with hitlist_dev: the stored hits' colours, as tt_fwd_args.hitlist_color_dev
+     with R = sum R_c (written by tt_multi_forward, read by tt_multi_backward) */
+  float *hitlist_color_dev;     /* [hitlist_cap][sum R_c][3] */
 } tt_multi_args;
 #define TT_CAMERA_DOUBLES 14
 #define TT_LOSS_NONE 0

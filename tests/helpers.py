@@ -87,7 +87,7 @@ class GpuScene:
             out["interval"] = ivo.cpu().numpy()
         if hl is not None:
             out["_hitlist"] = hl
-            out["hitlist"] = (hl[0].cpu().numpy(), hl[1].cpu().numpy())
+            out["hitlist"] = tuple(x.cpu().numpy() for x in hl)
         return out
 
     def backward(self, ray_o, ray_d, color, others, dL_dpix, dL_dothers, k=16, cutoff=1e-3, chunk_mode="compat",

@@ -121,9 +121,9 @@ def test_unknown_program_names_raise():
 
 
 class _FakeResult:
-    def __init__(self, data, hits=None, hit_counts=None):
+    def __init__(self, data, hits=None, hit_counts=None, hit_colors=None):
         self.data = data
-        self.hits, self.hit_counts = hits, hit_counts
+        self.hits, self.hit_counts, self.hit_colors = hits, hit_counts, hit_colors
 
 
 class _FakeTracer:
@@ -149,7 +149,8 @@ class _FakeTracer:
         color = torch.full((3, H, W), 0.5)
         others = torch.zeros(7, H, W)
         return _FakeResult((color, others, torch.zeros(1), args[11], args[12], torch.zeros(P)),
-                           torch.zeros(4, H * W, 2, dtype=torch.int32), torch.zeros(H * W, dtype=torch.int32))
+                           torch.zeros(4, H * W, 2, dtype=torch.int32), torch.zeros(H * W, dtype=torch.int32),
+                           torch.zeros(4, H * W, 3))
 
     def trace_backward(self, *args):
         self.bwd_args = args
@@ -184,8 +185,8 @@ def test_autograd_wiring_and_gradient_layouts():
     # backward argument order (reference __init__.py:162-180) and gradient layouts (:205-227)
     ba = fake.bwd_args
     assert ba[9] == x["P"] and ba[11] == 3 and ba[13].shape == (3, 2, 3) and ba[14].shape == (7, 2, 3)
-    # the forward's stored hits travel to the backward (trailing optional arguments)
-    assert len(ba) == 19 and ba[17].shape == (4, 6, 2) and ba[18].shape == (6,)
+    # the forward's stored hits and their colours travel to the backward (trailing optional arguments)
+    assert len(ba) == 20 and ba[17].shape == (4, 6, 2) and ba[18].shape == (6,) and ba[19].shape == (4, 6, 3)
     assert torch.all(x["verts"].grad == 1.0) and x["verts"].grad.shape == x["verts"].shape
     assert torch.all(x["sigma"].grad == 2.0) and x["sigma"].grad.shape == (5, 1)
     assert torch.all(x["opac"].grad == 3.0) and torch.all(x["shs"].grad == 4.0)

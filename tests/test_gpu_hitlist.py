@@ -10,6 +10,10 @@ Bars:
   backward from the list ................. elementwise within 1e-3 max(|ref|, 1e-3 max|ref|) of the FP64
                                            oracle, also at a capacity of 3 (pixels with more blended hits
                                            are re-traversed: a mixed list / re-traversal launch)
+  stored colours ......................... every stored hit's colour >= 0 or -0.0 (a clamped channel);
+                                           the backward that reads them equals, within 1e-3 (FP32 atomics
+                                           add in any order; a wrong clamp bit errs by O(1)), the backward
+                                           that evaluates the SH again
 """
 import numpy as np
 import pytest
@@ -77,3 +81,18 @@ def test_hitlist_forward_and_backward(small, mode, cap):
         ok, worst = grad_close(gb[key], rb[key])
         _say(f"  {key}: worst ratio vs FP64 {worst:.3g}")
         assert ok, key
+    cols = out["hitlist"][2][0]  # [cap, R, 3] of segment 0
+    stored = np.zeros((cap, R), bool)
+    for r in range(R):
+        stored[: max(0, min(int(counts[r]) if counts[r] >= 0 else cap, cap)), r] = True
+    c = cols[stored]
+    clamped = (c == 0.0) & np.signbit(c)
+    _say(f"  stored colours: {c.shape[0]} hits, {int(clamped.sum())} clamped channels")
+    assert np.all((c > 0.0) | (c == 0.0)) and np.all(np.isfinite(c))
+    _say("  backward from the hit list without its colours (SH evaluated again)")
+    gn = g.backward(ro, rd, out["_color"], out["_others"], dpix, doth, k=16, cutoff=1e-3, chunk_mode=mode,
+                    hitlist=out["_hitlist"][:2])
+    for key in ("dopac", "dsigma", "dsh", "dverts"):
+        ok, worst = grad_close(gb[key], gn[key])
+        _say(f"  {key}: worst ratio vs the SH-evaluating backward {worst:.3g}")
+        assert ok, (key, worst)

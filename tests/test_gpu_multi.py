@@ -80,7 +80,7 @@ def _multi_args(mv, B, k=16, block=128, tpf=1, hitlist_cap=0):
     if hitlist_cap:
         import torch
         hl = capi.hitlist_buffers(Rt, hitlist_cap, S["ro"].device)
-        B["_hits"], B["_hit_counts"] = hl
+        B["_hits"], B["_hit_counts"], B["_hit_colors"] = hl
     return capi.multi_args(w, h, S["ro"], S["rd"], S["verts"], S["npts"], S["cumsum"], S["shs"], 3, S["opac"],
                            S["sigma"], S["bg"], B["color"], B["others"], B["maxb"], B["dens"], B["scal"],
                            dL_dpix=S["dpix"], dL_dothers=S["doth"], dverts=B["dverts"], dsigma=B["dsig"],

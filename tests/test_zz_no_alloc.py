@@ -29,7 +29,8 @@ def _batch(torch, cuda, sc, S, sizes, seed, cap):
     hl = capi.hitlist_buffers(Rt, cap, cuda)
     hl[0].fill_(-7)
     hl[1].fill_(-7)
-    B["hits"], B["counts"] = hl
+    hl[2].fill_(-7)
+    B["hits"], B["counts"], B["colors"] = hl
     a = capi.multi_args(w, h, B["ro"], B["rd"], S["verts"], S["npts"], S["cumsum"], S["shs"], 3, S["opac"], S["sigma"],
                         S["bg"], B["color"], B["others"], B["maxb"], B["dens"], B["scal"], k=16, cutoff=1e-3,
                         chunk_mode="exact", block=128, tiles_per_fetch=1, stats=B["stats"], hitlist=hl)
