@@ -795,7 +795,8 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     fa_compat = fr.fwd_args("compat")
     ba_compat = fr.bwd_args("compat")
     fa_emit = fr.fwd_args("exact", interval_out=fr.ival)
-    fa_soft = fr.fwd_args("exact", interval_in=fr.prop, interval_out=fr.ival)
+    # the proposal's forward reads its interval and emits none: the hit-list backward re-traverses nothing
+    fa_soft = fr.fwd_args("exact", interval_in=fr.prop)
     ba_bound = fr.bwd_args("exact", interval=fr.ival)
     # the proposal's ceiling: each ray's own emitted interval (plain forward of this frame) as
     # (start, cap) -- start at the first blended (or terminating) hit, cap at the terminating or
@@ -810,7 +811,7 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
                                torch.where(torch.isfinite(stop_t), stop_t, torch.zeros_like(stop_t))], 1).contiguous()
     # hard mode without fallbacks: the trace ends at the cap and rays that blend nothing (start
     # past the far plane) are not traced; with these intervals the image is plain's
-    fa_ceiling = fr.fwd_args("exact", interval_in=perfect, interval_out=fr.ival, hard=True)
+    fa_ceiling = fr.fwd_args("exact", interval_in=perfect, hard=True)
     runs = {
         "plain": [("fwd", lambda: fr.h.fwd(fa_plain, sp)), ("zero", fr.zero_grads), ("bwd", lambda: fr.h.bwd(ba_plain, sp))],
         "plain_retrace": [("fwd", lambda: fr.h.fwd(fa_retrace, sp)), ("zero", fr.zero_grads),
@@ -820,20 +821,19 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
         "handoff_only": [("fwd", lambda: fr.h.fwd(fa_emit, sp)), ("zero", fr.zero_grads),
                          ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
         "proposal_ceiling": [("fwd", lambda: fr.h.fwd(fa_ceiling, sp)), ("zero", fr.zero_grads),
-                             ("bwd", lambda: fr.h.bwd(ba_bound, sp))],
+                             ("bwd", lambda: fr.h.bwd(ba_plain, sp))],
     }
     if net is not None:
         ts = float(net.hp.t_scale)
-        fa_hard = fr.fwd_args("exact", interval_in=fr.prop2, interval_out=fr.ival, hard=True, fb_start=ms * ts,
-                              fb_end=me * ts)
+        fa_hard = fr.fwd_args("exact", interval_in=fr.prop2, hard=True, fb_start=ms * ts, fb_end=me * ts)
         # exact: proposed cap only (start 0, continuation past the cap): the image equals plain
         propose = lambda: net.forward(fr.ro, fr.rd, fr.prop, capi.PROPOSAL_CAP, ms, me, stream=sp)
         # lossy: proposed start and hard cap, with the fallbacks
         propose2 = lambda: net.forward(fr.ro, fr.rd, fr.prop2, capi.PROPOSAL_INTERVAL, ms, me, stream=sp)
         runs["proposal"] = [("mlp", propose), ("fwd", lambda: fr.h.fwd(fa_soft, sp)), ("zero", fr.zero_grads),
-                            ("bwd", lambda: fr.h.bwd(ba_bound, sp))]
+                            ("bwd", lambda: fr.h.bwd(ba_plain, sp))]
         runs["proposal_hard"] = [("mlp", propose2), ("fwd", lambda: fr.h.fwd(fa_hard, sp)), ("zero", fr.zero_grads),
-                                 ("bwd", lambda: fr.h.bwd(ba_bound, sp))]
+                                 ("bwd", lambda: fr.h.bwd(ba_plain, sp))]
     runs = {k: v for k, v in runs.items() if want(k)}
     res, images = {}, {}
     for name, launches in runs.items():
@@ -899,7 +899,9 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     bb = 104 * R + 48 * Hc + 428 * Hb
     step = "proposal" if "proposal" in res else next(iter(res))
     rs = res[step]
-    kfwd = f"k_forward<{args.k},0,false,{'true' if step != 'plain' and step != 'plain_retrace' and step != 'plain_compat' else 'false'}>"
+    # the forward's interval features per step (template bits: 1 interval read, 2 emitted, 4 hard cap)
+    ival_bits = {"proposal": 1, "handoff_only": 3, "proposal_ceiling": 5, "proposal_hard": 5}.get(step, 0)
+    kfwd = f"k_forward<{args.k},0,false,{ival_bits}>"
     if rs["k_bwd"] >= rs["k_fwd"]:
         dom, dom_bytes, dom_ms = f"k_backward_rec<{args.k},0>", bb, rs["k_bwd"]
     else:

@@ -18,7 +18,7 @@ constexpr int kTileW = TileShape<kBlock>::W, kTileH = TileShape<kBlock>::H;
 constexpr int kBlockB = 256;
 static_assert(TileShape<kBlockB>::W == kTileW && TileShape<kBlockB>::H == 2 * kTileH, "backward tile = 2 forward tiles");
 
-template <int K, int VARIANT, bool COUNT = false, bool IVAL = false>
+template <int K, int VARIANT, bool COUNT = false, int IVAL = 0>
 __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlock];
   const long long c0 = P.tile_cost ? wall_clock64() : 0;
@@ -153,9 +153,15 @@ hipError_t fwd_k(const TraceParams &p, int variant, hipStream_t s) {
   switch (variant) {
     case 0:
       if (p.trav_stats)
-        hipLaunchKernelGGL((k_forward<K, 0, true, true>), g, bl, 0, s, p);
-      else if (p.ival_in || p.ival_out)
-        hipLaunchKernelGGL((k_forward<K, 0, false, true>), g, bl, 0, s, p);
+        hipLaunchKernelGGL((k_forward<K, 0, true, kIvalIn | kIvalOut | kIvalHard>), g, bl, 0, s, p);
+      else if (p.ival_hard && p.ival_out)
+        hipLaunchKernelGGL((k_forward<K, 0, false, kIvalIn | kIvalOut | kIvalHard>), g, bl, 0, s, p);
+      else if (p.ival_hard)
+        hipLaunchKernelGGL((k_forward<K, 0, false, kIvalIn | kIvalHard>), g, bl, 0, s, p);
+      else if (p.ival_out)
+        hipLaunchKernelGGL((k_forward<K, 0, false, kIvalIn | kIvalOut>), g, bl, 0, s, p);
+      else if (p.ival_in)
+        hipLaunchKernelGGL((k_forward<K, 0, false, kIvalIn>), g, bl, 0, s, p);
       else
         hipLaunchKernelGGL((k_forward<K, 0>), g, bl, 0, s, p);
       break;

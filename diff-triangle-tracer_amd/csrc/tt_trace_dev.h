@@ -436,20 +436,25 @@ TT_D void seg_init(SegState &st) {
 // up to the cap) is traced again from 0 without cap, before anything is blended;
 // and a ray still open at the cap that has no blended hit yet, or whose last
 // blended hit lies within ival_fb_end of the cap, goes on past it.
-// IVAL: the interval features (start / cap, emitted interval) are compiled in.
-template <int K, int BLOCK, bool FULL, bool COUNT, bool BOUNCE = false, bool IVAL = false>
+// IVAL bits select the interval features compiled in: kIvalIn (a (start, cap) interval per
+// ray is read), kIvalOut (the blended hits' interval is recorded for ival_out), kIvalHard (the
+// cap may be hard, with its fallbacks, when P.ival_hard is set). Without kIvalHard a cap is
+// exact (the trace continues past it), with fewer live registers.
+constexpr int kIvalIn = 1, kIvalOut = 2, kIvalHard = 4;
+template <int K, int BLOCK, bool FULL, bool COUNT, bool BOUNCE = false, int IVAL = 0>
 TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node,
                         float tstart = 0.0f, float cap = kInfDist, int hseg = -1) {
   RayPre r = prep_ray(o, d);
   KBuf<K> b;
   float last = 0.0f, tlo = tstart;
   long long plo = -1;
-  bool capped = IVAL && cap < P.far_t && cap >= tstart;  // a cap before the start is no cap
-  bool hard = IVAL && P.ival_hard && capped;
-  bool probe = IVAL && P.ival_hard && tstart > 0.0f;  // first gather of a hard interval with a start
+  constexpr bool IN = (IVAL & kIvalIn) != 0, OUT = (IVAL & kIvalOut) != 0, HARD = IN && (IVAL & kIvalHard) != 0;
+  bool capped = IN && cap < P.far_t && cap >= tstart;  // a cap before the start is no cap
+  bool hard = HARD && P.ival_hard && capped;
+  bool probe = HARD && P.ival_hard && tstart > 0.0f;  // first gather of a hard interval with a start
   const bool logging = FULL && P.log_cap > 0;
   TravCount tc;
-  if (IVAL && tstart > P.far_t) {  // an interval past the far plane: nothing to gather, no fallback
+  if (IN && tstart > P.far_t) {  // an interval past the far plane: nothing to gather, no fallback
     if (COUNT) P.trav_stats[2 * (size_t)ray] = P.trav_stats[2 * (size_t)ray + 1] = 0;
     return;
   }
@@ -503,13 +508,11 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
               P.log_t[(size_t)ray * P.log_cap + slot] = t;
               P.log_code[(size_t)ray * P.log_cap + slot] = 3;
             }
-            if (IVAL) st.t_term = t;
+            if (OUT) st.t_term = t;
             return;
           }
-          if (IVAL) {
-            if (st.blended == 0) st.t_first = t, st.p_first = prim;
-            st.t_last = t;
-          }
+          if (OUT && st.blended == 0) st.t_first = t, st.p_first = prim;
+          if (OUT || HARD) st.t_last = t;
           const float bw = alpha * st.T;
           if (FULL) {
             P.density[prim] = 1.0f;
@@ -607,7 +610,7 @@ TT_D uint32_t ray_bin_key(v3f dir, int x, int y) {
 // variant's primary segment only -- the colour buffer then holds the primary's composite
 // C and aux its transmittance (slot 5), R, Q and the two secondary rays, and ray_keys the
 // pixel's sort key (of its reflected ray), for bounce_secondary.
-template <int K, int BLOCK, int VARIANT, bool COUNT, bool IVAL = false>
+template <int K, int BLOCK, int VARIANT, bool COUNT, int IVAL = 0>
 TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss = nullptr) {
   v3f o, d;
   pixel_ray(P, (size_t)pix, o, d);
@@ -615,13 +618,13 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
   seg_init(st);
   if (VARIANT == 0) {
     float tstart = 0.0f, cap = kInfDist;
-    if (IVAL && P.ival_in) {
+    if ((IVAL & kIvalIn) && P.ival_in) {
       const float2 iv = P.ival_in[pix];
       tstart = iv.x;
       cap = iv.y;
     }
     trace_segment<K, BLOCK, true, COUNT, false, IVAL>(P, o, d, st, pix, o, s_node, tstart, cap, 0);
-    if (IVAL && P.ival_out)
+    if ((IVAL & kIvalOut) && P.ival_out)
       P.ival_out[pix] = make_float4(st.t_first, st.t_last, st.t_term, __uint_as_float(st.p_first));
     if (P.hl_count) P.hl_count[pix] = st.blended <= P.hl_cap ? st.blended : -1;
   } else if (VARIANT == 5) {
