@@ -21,6 +21,8 @@ struct tt_tracer {
   tt::Bvh bvh;
   int *ov_stack = nullptr;  // traversal stack spill (entries below the LDS part)
   size_t ov_cap = 0;
+  float4 *tri_win = nullptr;  // the one-shot forward's window records (128 B per triangle)
+  size_t tri_win_cap = 0;
   int *heads = nullptr;                    // 8 work-queue heads of the persistent launches
   std::map<long long, int> grids;          // persistent grid size per (k, block, backward)
   hipStream_t capture_stream = nullptr;    // private stream used only to record graphs
@@ -185,6 +187,24 @@ int prepare_spill(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
   return prepare_spill_lanes(tr, tt::trace_lanes(W, H), p);
 }
 
+// Window records of the one-shot forward: kWinRec float4 per triangle, rewritten by every
+// forward from the vertices it is given.
+int prepare_tri_win(tt_tracer *tr, tt::TraceParams &p) {
+  const size_t n = (size_t)(p.P > 0 ? p.P : 0);
+  if (n > tr->tri_win_cap) {
+    if (tr->tri_win) (void)hipFree(tr->tri_win);
+    tr->tri_win = nullptr;
+    tr->tri_win_cap = 0;
+    hipError_t e = hipMalloc(&tr->tri_win, n * tt::kWinRec * sizeof(float4));
+    if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("window record allocation failed: ") +
+                                                     hipGetErrorString(e));
+    tr->tri_win_cap = n;
+    tr->mem_gen++;
+  }
+  p.tri_win = tr->tri_win;
+  return TT_OK;
+}
+
 void free_rec_sort(tt_tracer *tr) {
   for (uint32_t **q : {&tr->rec_keys, &tr->sort_keys2, &tr->sort_vals, &tr->sort_vals2, &tr->sort_hist}) {
     if (*q) (void)hipFree(*q);
@@ -347,6 +367,7 @@ int tt_destroy(tt_tracer *tr) {
   (void)hipSetDevice(tr->device);
   tt::bvh_free(tr->bvh);
   if (tr->ov_stack) (void)hipFree(tr->ov_stack);
+  if (tr->tri_win) (void)hipFree(tr->tri_win);
   if (tr->heads) (void)hipFree(tr->heads);
   if (tr->graph) (void)hipGraphExecDestroy(tr->graph);
   if (tr->capture_stream) (void)hipStreamDestroy(tr->capture_stream);
@@ -590,6 +611,7 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
     p.aux = a->bounce_aux_dev;
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
+  if ((rc = prepare_tri_win(tr, p)) != TT_OK) return rc;
   if (a->variant == TT_VARIANT_BOUNCE && (long long)a->width * a->height > 0) {  // secondary-ray binning
     if ((rc = prepare_rec_sort(tr, p, (size_t)a->width * a->height)) != TT_OK) return rc;
     p.ray_keys = p.rec_keys;
@@ -708,6 +730,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
     p.denv = a->dL_denvmap_dev;
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
+  if ((rc = prepare_tri_win(tr, p)) != TT_OK) return rc;
   {  // 16 rows per lane, or the largest record total a forward has needed so far
     poll_hits_total(tr);
     const size_t lanes = tt::trace_lanes(a->width, a->height);

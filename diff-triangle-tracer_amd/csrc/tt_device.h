@@ -299,33 +299,51 @@ struct Win {
 // formed in FP64 (add/sub/mul only) -- in FP32 that dot product cancels to
 // ~eps*|hit - p_i| absolute, which the window's diverging slope near an edge
 // amplifies into percent-level gradient error.
-TT_D void window32(v3f A, v3f B, v3f C, v3f o, v3f d, float t, float sigma, Win &w) {
+//
+// Split in two: the terms of the triangle alone (the FP64 edge-plane normals n_i, their
+// FP32 inverse lengths and the incenter's signed distance to edge 0) and the terms of one
+// hit. A forward computes the first part once per triangle into a 128-byte record
+// (window_record) and each hit reads it (window_hit); window32 runs both parts in turn. The
+// operations are the same either way, so the results are identical bit for bit.
+struct TriWin {
+  v3d n[3];     // ec x (p_i - p_{i+1}) in FP64, ec = (B - A) x (C - A)
+  float il[3];  // 1 / |toflt(n_i)|
+  float dist;   // dot(toflt(n_0) il_0, incenter - A)
+};
+TT_D void window_tri(v3f A, v3f B, v3f C, TriWin &tw) {
   v3f eAB = B - A, eAC = C - A, eBC = C - B;
   float a = vlen(eBC);
   float b = vlen(eAC);
   float c = vlen(eAB);
   v3f incA = divs(b * eAB + c * eAC, (a + b) + c);
-  v3f ec = cross(eAB, eAC);
-  w.n = normalize(ec);
+  v3d Ad = todbl(A), Bd = todbl(B), Cd = todbl(C);
+  v3d ecd = cross(Bd - Ad, Cd - Ad);
+  tw.n[0] = cross(ecd, Ad - Bd);
+  tw.n[1] = cross(ecd, Bd - Cd);
+  tw.n[2] = cross(ecd, Cd - Ad);
+  v3f m0 = toflt(tw.n[0]), m1 = toflt(tw.n[1]), m2 = toflt(tw.n[2]);
+  tw.il[0] = 1.0f / sqrtf(dot(m0, m0));
+  tw.il[1] = 1.0f / sqrtf(dot(m1, m1));
+  tw.il[2] = 1.0f / sqrtf(dot(m2, m2));
+  tw.dist = dot(m0 * tw.il[0], incA);
+}
+// The unit face normal the forward accumulates for the "others" output.
+TT_D v3f face_normal(v3f A, v3f B, v3f C) { return normalize(cross(B - A, C - A)); }
+// The terms of one hit at distance t along (o, d): everything of Win except n.
+TT_D void window_hit(const TriWin &tw, v3f A, v3f B, v3f C, v3f o, v3f d, float t, float sigma, Win &w) {
   v3d Ad = todbl(A), Bd = todbl(B), Cd = todbl(C);
   v3d od = todbl(o), td = todbl(d) * (double)t;
   v3d rA = (od - Ad) + td, rB = (od - Bd) + td, rC = (od - Cd) + td;
-  v3d ecd = cross(Bd - Ad, Cd - Ad);
-  v3d n0 = cross(ecd, Ad - Bd), n1 = cross(ecd, Bd - Cd), n2 = cross(ecd, Cd - Ad);
-  float q0 = (float)dot(n0, rA), q1 = (float)dot(n1, rB), q2 = (float)dot(n2, rC);
-  v3f m0 = toflt(n0), m1 = toflt(n1), m2 = toflt(n2);
-  float il0 = 1.0f / sqrtf(dot(m0, m0));
-  float il1 = 1.0f / sqrtf(dot(m1, m1));
-  float il2 = 1.0f / sqrtf(dot(m2, m2));
-  float dist = dot(m0 * il0, incA);
-  float ed0 = q0 * il0;
+  float q0 = (float)dot(tw.n[0], rA), q1 = (float)dot(tw.n[1], rB), q2 = (float)dot(tw.n[2], rC);
+  float dist = tw.dist;
+  float ed0 = q0 * tw.il[0];
   if (dist > 0.0f) {
     dist = -dist;
     ed0 = -ed0;
   }
   w.ed[0] = ed0;
-  w.ed[1] = q1 * il1;
-  w.ed[2] = q2 * il2;
+  w.ed[1] = q1 * tw.il[1];
+  w.ed[2] = q2 * tw.il[2];
   w.rel[0] = toflt(rA);
   w.rel[1] = toflt(rB);
   w.rel[2] = toflt(rC);
@@ -334,6 +352,50 @@ TT_D void window32(v3f A, v3f B, v3f C, v3f o, v3f d, float t, float sigma, Win 
   w.dist = dist;
   w.phi_final = phi / dist;
   w.Cx = fmaxf(0.0f, pow_tt(w.phi_final, sigma));
+}
+TT_D void window32(v3f A, v3f B, v3f C, v3f o, v3f d, float t, float sigma, Win &w) {
+  TriWin tw;
+  window_tri(A, B, C, tw);
+  w.n = face_normal(A, B, C);
+  window_hit(tw, A, B, C, o, d, t, sigma, w);
+}
+
+// Window records: 8 float4 (128 B, one cache line) per triangle, written by the forward's
+// record pass and read once per consumed hit:
+//   [0] n0.x n0.y  [1] n0.z n1.x  [2] n1.y n1.z  [3] n2.x n2.y   (FP64 pairs)
+//   [4] n2.z (FP64), A.x, A.y     [5] A.z, B.x, B.y, B.z
+//   [6] C.x, C.y, C.z, il0        [7] il1, il2, dist, sigma
+TT_D void window_record_store(float4 *q, v3f A, v3f B, v3f C, float sigma) {
+  TriWin tw;
+  window_tri(A, B, C, tw);
+  double2 *qd = reinterpret_cast<double2 *>(q);
+  qd[0] = make_double2(tw.n[0].x, tw.n[0].y);
+  qd[1] = make_double2(tw.n[0].z, tw.n[1].x);
+  qd[2] = make_double2(tw.n[1].y, tw.n[1].z);
+  qd[3] = make_double2(tw.n[2].x, tw.n[2].y);
+  const double z2 = tw.n[2].z;
+  const uint64_t zb = __double_as_longlong(z2);
+  q[4] = make_float4(__uint_as_float((uint32_t)zb), __uint_as_float((uint32_t)(zb >> 32)), A.x, A.y);
+  q[5] = make_float4(A.z, B.x, B.y, B.z);
+  q[6] = make_float4(C.x, C.y, C.z, tw.il[0]);
+  q[7] = make_float4(tw.il[1], tw.il[2], tw.dist, sigma);
+}
+TT_D void window_record_load(const float4 *q, TriWin &tw, v3f &A, v3f &B, v3f &C, float &sigma) {
+  const double2 *qd = reinterpret_cast<const double2 *>(q);
+  const double2 a = qd[0], b = qd[1], c = qd[2], e = qd[3];
+  const float4 f = q[4], g = q[5], h = q[6], k = q[7];
+  tw.n[0] = v3d{a.x, a.y, b.x};
+  tw.n[1] = v3d{b.y, c.x, c.y};
+  const double z2 = __longlong_as_double((long long)((uint64_t)__float_as_uint(f.x) | (uint64_t)__float_as_uint(f.y) << 32));
+  tw.n[2] = v3d{e.x, e.y, z2};
+  A = mk(f.z, f.w, g.x);
+  B = mk(g.y, g.z, g.w);
+  C = mk(h.x, h.y, h.z);
+  tw.il[0] = h.w;
+  tw.il[1] = k.x;
+  tw.il[2] = k.y;
+  tw.dist = k.z;
+  sigma = k.w;
 }
 
 // --------------------------------------------------------------- SH colour

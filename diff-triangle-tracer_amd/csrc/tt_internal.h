@@ -60,6 +60,8 @@ std::string bvh_cost(Bvh &b, float *out, hipStream_t stream);
 std::string bvh_build(Bvh &b, const float *verts, int num_vertices, const int *cumsum, int P, int mode,
                       hipStream_t stream);
 
+constexpr int kWinRec = 8;  // float4 per window record (128 B per triangle)
+
 struct TraceParams {
   int W, H, R;
   int tiles_x, n_tiles;
@@ -70,6 +72,8 @@ struct TraceParams {
   const float *shs;
   int M, deg;
   const float *opac, *sigma;
+  // per-triangle window records (kWinRec float4 each) of this forward, or null
+  float4 *tri_win;
   const float4 *nodes, *leaves, *nodes4;
   const uint2 *counts4;
   int n_leaves;

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_forward(TraceParams P) {
   const int tile = P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles);
   int x, y, blended = 0;
   if (tile_pixel<kBlock>(P, tile, x, y))
-    blended = forward_pixel<K, kBlock, VARIANT, COUNT, IVAL>(P, y * P.W + x, s_node);
+    blended = forward_pixel<K, kBlock, VARIANT, COUNT, IVAL, true>(P, y * P.W + x, s_node);
   if (P.tile_cost) {  // the tile's wall time (100 MHz ticks, the backward's schedule key) and blended hits
     for (int off = 32; off > 0; off >>= 1) blended += __shfl_xor(blended, off);
     __syncthreads();
@@ -50,7 +50,17 @@ __global__ __launch_bounds__(kBlock, 4) void k_bounce_secondary(TraceParams P, c
                                                                 const uint32_t *order) {
   __shared__ int s_node[kLdsInts * kBlock];
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < P.R && keys[i] != 0xffffffffu) bounce_secondary<K, kBlock>(P, (int)order[i], s_node);
+  if (i < P.R && keys[i] != 0xffffffffu) bounce_secondary<K, kBlock, true>(P, (int)order[i], s_node);
+}
+
+// The forward's window records: the terms of each triangle's window function that do not
+// depend on the ray (FP64 edge-plane normals, inverse lengths, incenter distance) with its
+// vertices and sigma, one 128-byte record per triangle, read by every consumed hit.
+__global__ __launch_bounds__(256) void k_tri_win(TraceParams P) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.P) return;
+  const float *vp = P.verts + 3 * (size_t)P.cumsum[i];
+  window_record_store(P.tri_win + kWinRec * (size_t)i, ld3(vp), ld3(vp + 3), ld3(vp + 6), P.sigma[i]);
 }
 
 // Exclusive prefix sums of the tiles' blended-hit counts (one block): the backward's
@@ -128,7 +138,7 @@ __global__ __launch_bounds__(kBlockB, 2) void k_backward_rec(TraceParams P) {
   int x, y;
   const int tile = P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles);
   const bool valid = tile_pixel<kBlockB>(P, tile, x, y);
-  backward_block_rec<K, kBlockB, VARIANT>(P, valid, y * P.W + x, tile, s_node, s_rec, s_tab);
+  backward_block_rec<K, kBlockB, VARIANT, true>(P, valid, y * P.W + x, tile, s_node, s_rec, s_tab);
 }
 
 // Statistics of the forward's 16x8 tiles folded into the backward's 16x16 tiles: blended
@@ -223,6 +233,8 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
   if (p.n_tiles == 0) return "";
   hipError_t e;
   if (k != 8 && k != 16 && k != 32) return "k must be 8, 16 or 32";
+  if (p.P > 0 && !p.tri_win) return "the forward needs its window-record buffer";
+  if (p.P > 0) hipLaunchKernelGGL(k_tri_win, dim3((p.P + 255) / 256), dim3(256), 0, stream, p);
   p.tile_sched = nullptr;
   if (p.tile_cost && p.probe_order) {  // the forward's schedule: estimated tile costs
     hipLaunchKernelGGL(k_tile_probe, dim3(p.n_tiles), dim3(64), 0, stream,
@@ -279,6 +291,9 @@ std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream
   if (p.n_tiles == 0) return "";
   dim3 g(p.n_tiles), bl(kBlockB);
   if (k != 8 && k != 16 && k != 32) return "k must be 8, 16 or 32";
+  // the window records of the vertices this backward is given (a forward's may be stale)
+  if (p.P > 0 && !p.tri_win) return "the backward needs its window-record buffer";
+  if (p.P > 0) hipLaunchKernelGGL(k_tri_win, dim3((p.P + 255) / 256), dim3(256), 0, stream, p);
   if (ev) (void)hipEventRecord(ev->before, stream);
   if (p.rec_rows) {
     if (variant == 4) {

@@ -441,7 +441,9 @@ TT_D void seg_init(SegState &st) {
 // cap may be hard, with its fallbacks, when P.ival_hard is set). Without kIvalHard a cap is
 // exact (the trace continues past it), with fewer live registers.
 constexpr int kIvalIn = 1, kIvalOut = 2, kIvalHard = 4;
-template <int K, int BLOCK, bool FULL, bool COUNT, bool BOUNCE = false, int IVAL = 0>
+// WREC: each consumed hit reads its triangle's window record (P.tri_win, written by the
+// forward's record pass) instead of forming the triangle's terms itself; same results.
+template <int K, int BLOCK, bool FULL, bool COUNT, bool BOUNCE = false, int IVAL = 0, bool WREC = false>
 TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ray, v3f scaling_org, int *s_node,
                         float tstart = 0.0f, float cap = kInfDist, int hseg = -1) {
   RayPre r = prep_ray(o, d);
@@ -493,10 +495,17 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
       if (t < P.near_t) {
         code = 0;
       } else {
-        const int off = P.cumsum[prim];
-        const float *vp = P.verts + 3 * (size_t)off;
         Win w;
-        window32(ld3(vp), ld3(vp + 3), ld3(vp + 6), o, d, t, P.sigma[prim], w);
+        v3f VA, VB, VC;
+        if (WREC) {
+          TriWin tw;
+          float sg;
+          window_record_load(P.tri_win + kWinRec * (size_t)prim, tw, VA, VB, VC, sg);
+          window_hit(tw, VA, VB, VC, o, d, t, sg, w);
+        } else {
+          const float *vp = P.verts + 3 * (size_t)P.cumsum[prim];
+          window32(ld3(vp), ld3(vp + 3), ld3(vp + 6), o, d, t, P.sigma[prim], w);
+        }
         const float alpha = fminf(0.99f, P.opac[prim] * w.Cx);
         if (alpha < 1.0f / 255.0f) {
           code = 1;
@@ -518,9 +527,10 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
             P.density[prim] = 1.0f;
             compute_scaling(P, prim, scaling_org);
             atomicMax(reinterpret_cast<int *>(P.maxb) + prim, __float_as_int(bw));
-            st.N[0] += w.n.x * bw;
-            st.N[1] += w.n.y * bw;
-            st.N[2] += w.n.z * bw;
+            const v3f nrm = WREC ? face_normal(VA, VB, VC) : w.n;
+            st.N[0] += nrm.x * bw;
+            st.N[1] += nrm.y * bw;
+            st.N[2] += nrm.z * bw;
           }
           bool c0, c1, c2;
           const v3f col = sh_color(P.shs + (size_t)prim * P.M * 3, P.deg, d, c0, c1, c2);
@@ -610,7 +620,7 @@ TT_D uint32_t ray_bin_key(v3f dir, int x, int y) {
 // variant's primary segment only -- the colour buffer then holds the primary's composite
 // C and aux its transmittance (slot 5), R, Q and the two secondary rays, and ray_keys the
 // pixel's sort key (of its reflected ray), for bounce_secondary.
-template <int K, int BLOCK, int VARIANT, bool COUNT, int IVAL = 0>
+template <int K, int BLOCK, int VARIANT, bool COUNT, int IVAL = 0, bool WREC = false>
 TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss = nullptr) {
   v3f o, d;
   pixel_ray(P, (size_t)pix, o, d);
@@ -623,12 +633,12 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
       tstart = iv.x;
       cap = iv.y;
     }
-    trace_segment<K, BLOCK, true, COUNT, false, IVAL>(P, o, d, st, pix, o, s_node, tstart, cap, 0);
+    trace_segment<K, BLOCK, true, COUNT, false, IVAL, WREC>(P, o, d, st, pix, o, s_node, tstart, cap, 0);
     if ((IVAL & kIvalOut) && P.ival_out)
       P.ival_out[pix] = make_float4(st.t_first, st.t_last, st.t_term, __uint_as_float(st.p_first));
     if (P.hl_count) P.hl_count[pix] = st.blended <= P.hl_cap ? st.blended : -1;
   } else if (VARIANT == 5) {
-    trace_segment<K, BLOCK, true, COUNT, true>(P, o, d, st, pix, o, s_node, 0.0f, kInfDist, 0);
+    trace_segment<K, BLOCK, true, COUNT, true, 0, WREC>(P, o, d, st, pix, o, s_node, 0.0f, kInfDist, 0);
     const Bounce g = bounce_rays(o, d, st.T, st.N, st.D, st.eta);
     if (P.hl_count) {
       P.hl_count[pix] = st.blended <= P.hl_cap ? st.blended : -1;
@@ -655,7 +665,7 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
       P.ray_keys[pix] = 0xffffffffu;
     }
   } else if (VARIANT == 4) {
-    trace_segment<K, BLOCK, true, COUNT, true>(P, o, d, st, pix, o, s_node, 0.0f, kInfDist, 0);
+    trace_segment<K, BLOCK, true, COUNT, true, 0, WREC>(P, o, d, st, pix, o, s_node, 0.0f, kInfDist, 0);
     int nb[3] = {st.blended, 0, 0};  // blended hits of the primary, reflected and refracted segment
     const Bounce g = bounce_rays(o, d, st.T, st.N, st.D, st.eta);
     float Lr[3] = {0.0f, 0.0f, 0.0f}, Lt[3] = {0.0f, 0.0f, 0.0f};
@@ -663,12 +673,12 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
       float E[3];
       SegState sr;
       seg_init(sr);
-      trace_segment<K, BLOCK, false, false>(P, g.orr, g.dr, sr, pix, g.orr, s_node, bounce_start(P), kInfDist, 1);
+      trace_segment<K, BLOCK, false, false, false, 0, WREC>(P, g.orr, g.dr, sr, pix, g.orr, s_node, bounce_start(P), kInfDist, 1);
       env_value(P.env, P.env_w, P.env_h, g.dr, E);
       for (int c = 0; c < 3; c++) Lr[c] = sr.C[c] + sr.T * E[c];
       SegState stt;
       seg_init(stt);
-      trace_segment<K, BLOCK, false, false>(P, g.ot, g.dt, stt, pix, g.ot, s_node, bounce_start(P), kInfDist, 2);
+      trace_segment<K, BLOCK, false, false, false, 0, WREC>(P, g.ot, g.dt, stt, pix, g.ot, s_node, bounce_start(P), kInfDist, 2);
       env_value(P.env, P.env_w, P.env_h, g.dt, E);
       for (int c = 0; c < 3; c++) Lt[c] = stt.C[c] + stt.T * E[c];
       st.consumed += sr.consumed + stt.consumed;
@@ -701,7 +711,7 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
         o = o + nrm * eps;
         d = reflect3(normalize(d), nrm);
       }
-      trace_segment<K, BLOCK, false, false>(P, o, d, st, pix, o, s_node);
+      trace_segment<K, BLOCK, false, false, false, 0, WREC>(P, o, d, st, pix, o, s_node);
     } else if (VARIANT == 2) {  // refract.cu:189-213
       float sth;
       if (hit_sphere(rad, cnt, o, d, sth)) {
@@ -717,7 +727,7 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
         o = hp + nrm * eps;
         d = nd;
       }
-      trace_segment<K, BLOCK, false, false>(P, o, d, st, pix, o, s_node);
+      trace_segment<K, BLOCK, false, false, false, 0, WREC>(P, o, d, st, pix, o, s_node);
     } else {  // envlight.cu:116-184
       const int samples = (int)(P.extra_len > 4 ? P.extra[4] : 0.0f);
       float sth;
@@ -733,7 +743,7 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
           const v3f on = o + dn * eps;
           SegState ss;
           seg_init(ss);
-          trace_segment<K, BLOCK, false, false>(P, on, dn, ss, pix, on, s_node);
+          trace_segment<K, BLOCK, false, false, false, 0, WREC>(P, on, dn, ss, pix, on, s_node);
           for (int c = 0; c < 3; c++) radc[c] += ss.C[c] + ss.T * P.bg[c];
           st.consumed += ss.consumed;
           st.blended += ss.blended;
@@ -742,7 +752,7 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
         for (int c = 0; c < 3; c++) st.C[c] += radc[c] * inv;
         st.T = 0.0f;
       } else {
-        trace_segment<K, BLOCK, false, false>(P, o, d, st, pix, o, s_node);
+        trace_segment<K, BLOCK, false, false, false, 0, WREC>(P, o, d, st, pix, o, s_node);
       }
     }
   }
@@ -784,7 +794,7 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
 // and refracted rays from aux, their radiance L = C' + T' E(d') into aux 0..5, the final
 // colour ((C + R L_r) + Q L_t) + T E(d) (the operation order of VARIANT 4), the hit counts
 // of both segments, and the statistics and the tile's blended hits added to.
-template <int K, int BLOCK>
+template <int K, int BLOCK, bool WREC = false>
 TT_D void bounce_secondary(const TraceParams &P, int pix, int *s_node) {
   float *a = P.aux + (size_t)kBounceAux * pix;
   const v3f orr = mk(a[8], a[9], a[10]), dr = mk(a[11], a[12], a[13]);
@@ -792,12 +802,12 @@ TT_D void bounce_secondary(const TraceParams &P, int pix, int *s_node) {
   float E[3], Lr[3], Lt[3];
   SegState sr;
   seg_init(sr);
-  trace_segment<K, BLOCK, false, false>(P, orr, dr, sr, pix, orr, s_node, bounce_start(P), kInfDist, 1);
+  trace_segment<K, BLOCK, false, false, false, 0, WREC>(P, orr, dr, sr, pix, orr, s_node, bounce_start(P), kInfDist, 1);
   env_value(P.env, P.env_w, P.env_h, dr, E);
   for (int c = 0; c < 3; c++) Lr[c] = sr.C[c] + sr.T * E[c];
   SegState stt;
   seg_init(stt);
-  trace_segment<K, BLOCK, false, false>(P, ot, dt, stt, pix, ot, s_node, bounce_start(P), kInfDist, 2);
+  trace_segment<K, BLOCK, false, false, false, 0, WREC>(P, ot, dt, stt, pix, ot, s_node, bounce_start(P), kInfDist, 2);
   env_value(P.env, P.env_w, P.env_h, dt, E);
   for (int c = 0; c < 3; c++) Lt[c] = stt.C[c] + stt.T * E[c];
   if (P.hl_count) {
@@ -1138,7 +1148,7 @@ TT_D void flush_sums(const TraceParams &P, uint32_t prim, const float acc[11], c
 // bounce variant's primary (per-hit colours extended by rho L_r + (1 - rho) L_t, with the
 // reflectivity gradient); 2: a bounce secondary ray (colour only). Returns the segment's
 // final transmittance.
-template <int K, int BLOCK, int MODE>
+template <int K, int BLOCK, int MODE, bool WREC = false>
 TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], const float g[3], float dL_ddepth,
                         v3f dL_dn2, float final_depth, v3f final_normal, const float *Lr, const float *Lt, float tlo,
                         long long plo, float far_t, int *s_node, int *s_rec,
@@ -1149,10 +1159,20 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
   // its colour from the forward's stored hits (clamped channel -0.0f), or null: evaluated here
   auto blend = [&](float t, uint32_t prim, const float *rgb) -> bool {
     const int off = P.cumsum[prim];
-    const float *vp = P.verts + 3 * (size_t)off;
-    const v3f V[3] = {ld3(vp), ld3(vp + 3), ld3(vp + 6)};
+    v3f V[3];
     Win w;
-    window32(V[0], V[1], V[2], o, d, t, P.sigma[prim], w);
+    float sigma;
+    if (WREC) {
+      TriWin tw;
+      window_record_load(P.tri_win + kWinRec * (size_t)prim, tw, V[0], V[1], V[2], sigma);
+      window_hit(tw, V[0], V[1], V[2], o, d, t, sigma, w);
+      if (MODE != 2) w.n = face_normal(V[0], V[1], V[2]);
+    } else {
+      const float *vp = P.verts + 3 * (size_t)off;
+      V[0] = ld3(vp), V[1] = ld3(vp + 3), V[2] = ld3(vp + 6);
+      sigma = P.sigma[prim];
+      window32(V[0], V[1], V[2], o, d, t, sigma, w);
+    }
     const float opacity = P.opac[prim];
     const float alpha = fminf(0.99f, opacity * w.Cx);
     if (alpha < 1.0f / 255.0f) return true;
@@ -1196,7 +1216,7 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
       dL_dalpha += sumv(hmul(dL_dn2, T * w.n - (final_normal - Nf) * inv1a));
     }
     const float dL_dC = opacity * dL_dalpha;
-    const float dL_dphi = ((P.sigma[prim] / w.phi_max) * dL_dC) * w.Cx;
+    const float dL_dphi = ((sigma / w.phi_max) * dL_dC) * w.Cx;
     const v3f dL_dn3 = MODE != 2 ? dL_dn2 * bw : mk(0.0f, 0.0f, 0.0f);
     HitSink hs;
     for (int f = 0; f < 11; f++) hs.v[f] = 0.0f;
@@ -1295,7 +1315,7 @@ TT_D void env_scatter_wave(const TraceParams &P, v3f d, const float g[3], bool o
 // VARIANT 5, 6, 7: only its primary (with the environment term), only the reflected, only
 // the refracted segment -- three launches whose sum is VARIANT 4, each with the register
 // footprint of one segment; VARIANT 8: both secondary segments (records grouped globally).
-template <int K, int BLOCK, int VARIANT>
+template <int K, int BLOCK, int VARIANT, bool WREC = false>
 TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, float *env_g = nullptr) {
   const size_t R = (size_t)P.R;
   v3f o, d;
@@ -1326,7 +1346,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
       far_t = iv.y;
     }
     const float *hlc = hl && P.hl_rgb_in ? P.hl_rgb_in + 3 * (size_t)pix : nullptr;
-    backward_seg<K, BLOCK, 0>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal, nullptr,
+    backward_seg<K, BLOCK, 0, WREC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth, final_normal, nullptr,
                               nullptr, tlo, plo, far_t, s_node, s_rec, hl, n_hl, P.hl_stride, hlc);
   } else {
     constexpr bool kPrim = VARIANT == 4 || VARIANT == 5;
@@ -1350,7 +1370,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
         }
       }
     if (kPrim) {
-      const float Tp = backward_seg<K, BLOCK, 1>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
+      const float Tp = backward_seg<K, BLOCK, 1, WREC>(P, o, d, final_color, dL_dpixel, dL_ddepth, dL_dn2, final_depth,
                                                  final_normal, Lr, Lt, 0.0f, -1, P.far_t, s_node, s_rec, hls[0],
                                                  nhl[0], P.hl_stride, hlcs[0]);
       const float ge[3] = {Tp * dL_dpixel[0], Tp * dL_dpixel[1], Tp * dL_dpixel[2]};
@@ -1369,7 +1389,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
         const v3f so = s == 0 ? mk(a[8], a[9], a[10]) : mk(a[14], a[15], a[16]);
         const v3f sd = s == 0 ? dr : mk(a[17], a[18], a[19]);
         const float gs[3] = {wgt * dL_dpixel[0], wgt * dL_dpixel[1], wgt * dL_dpixel[2]};
-        const float Ts = backward_seg<K, BLOCK, 2>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
+        const float Ts = backward_seg<K, BLOCK, 2, WREC>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
                                                    nullptr, bounce_start(P), -1, P.far_t, s_node, s_rec, hls[1 + s],
                                                    nhl[1 + s], P.hl_stride, hlcs[1 + s]);
         const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
@@ -1392,7 +1412,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
 // with one lane per (triangle, field), so that one atomic instruction covers a triangle's
 // consecutive gradient fields. LDS: s_node (traversal stack and leaf queue, reused for the
 // grouping), s_rec (kRecLds ints), s_tab (rec_tab_ints ints: table + stage).
-template <int K, int BLOCK, int VARIANT>
+template <int K, int BLOCK, int VARIANT, bool WREC = false>
 TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile, int *s_node, int *s_rec,
                              int *s_tab) {
   constexpr int kBatch = rec_batch<BLOCK>(), kAggSlots = agg_slots<BLOCK>();
@@ -1415,12 +1435,12 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   __syncthreads();
   if (VARIANT == 5) {  // the bounce primaries: environment gradient aggregated per wave
     float eg[3] = {0.0f, 0.0f, 0.0f};
-    if (valid) backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_rec, eg);
+    if (valid) backward_ray<K, BLOCK, VARIANT, WREC>(P, pix, s_node, s_rec, eg);
     v3f po = mk(0.0f, 0.0f, 0.0f), pd = mk(0.0f, 0.0f, 1.0f);
     if (valid) pixel_ray(P, (size_t)pix, po, pd);
     env_scatter_wave(P, pd, eg, valid);
   } else if (valid) {
-    backward_ray<K, BLOCK, VARIANT>(P, pix, s_node, s_rec);
+    backward_ray<K, BLOCK, VARIANT, WREC>(P, pix, s_node, s_rec);
   }
   __syncthreads();
   if (VARIANT == 8) return;  // records grouped globally (sorted by triangle) by the caller
