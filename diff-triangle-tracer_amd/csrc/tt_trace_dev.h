@@ -161,13 +161,12 @@ TT_D NodePad node_pad(const TraceParams &P, const RayPre &r) {
   return np;
 }
 // Entry and exit distances of the 4 children of a quantized node (unwidened, as slab_raw).
-TT_D void slab4q(const RayPre &r, const NodePad &np, const float4 *q, float4 &tn, float4 &tf, int4 &ref) {
-  const float4 a = q[0], px = q[1], py = q[2], pz = q[3];
-  ref = *reinterpret_cast<const int4 *>(q + 4);
+TT_D void slab4q(const RayPre &r, const NodePad &np, const float4 a, const float4 px, const float4 py,
+                 const float4 pz, float4 &tn, float4 &tf) {
   const uint32_t eb = __float_as_uint(a.w);
   constexpr float kBig = 3.0e38f;  // 2^e / d overflows only for e near 120 and d near 1e-30
-  auto step = [&](uint32_t e, float inv) {
-    return fminf(fmaxf(__uint_as_float(((e & 0xffu) - 1u) << 23) * inv, -kBig), kBig);
+  auto step = [&](uint32_t e, float inv) {  // 2^(e - 128) / d, exact below overflow
+    return fminf(fmaxf(__builtin_amdgcn_ldexpf(inv, (int)(e & 0xffu) - 128), -kBig), kBig);
   };
   const float sx = step(eb, r.inv.x), sy = step(eb >> 8, r.inv.y), sz = step(eb >> 16, r.inv.z);
   const float bx = (a.x - r.o.x) * r.inv.x, by = (a.y - r.o.y) * r.inv.y, bz = (a.z - r.o.z) * r.inv.z;
@@ -198,6 +197,17 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
   if (P.n_leaves == 0) return;
   const NodePad np = node_pad(P, r);
   const int lane = threadIdx.x;
+  // the data of `node`, fetched right after the node step that chose it and after the loads
+  // of the leaf tested in the same iteration, so that the leaf test's work (and the rest of
+  // the iteration) runs while the node's data is in flight
+  float4 qa, qx, qy, qz;
+  int4 qref;
+  auto fetch = [&](int n) {
+    const float4 *q = P.nodes4 + 5 * (size_t)n;
+    qa = q[0], qx = q[1], qy = q[2], qz = q[3];
+    qref = *reinterpret_cast<const int4 *>(q + 4);
+  };
+  fetch(0);
   int *ov = P.ov_stack + ((size_t)blockIdx.x * BLOCK + lane) * P.ov_depth;
   int *s_leaf = s_node + kStack * BLOCK;
   const float lo_w = tlo - fabsf(tlo) * kWideRel - kWideAbs;
@@ -215,8 +225,8 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
     if (node >= 0 && nq <= kLeafQ - 4) {
       if (COUNT) tc.nodes++;
       float4 tn, tf;
-      int4 ref;
-      slab4q(r, np, P.nodes4 + 5 * (size_t)node, tn, tf, ref);
+      const int4 ref = qref;
+      slab4q(r, np, qa, qx, qy, qz, tn, tf);
       const float thi = fminf(far_t, b.t[K - 1]);
       const float hi_w = thi + fabsf(thi) * kWideRel + kWideAbs;
       const float tn0 = tn.x, tf0 = tf.x, tn1 = tn.y, tf1 = tf.y, tn2 = tn.z, tf2 = tf.z, tn3 = tn.w, tf3 = tf.w;
@@ -269,10 +279,15 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
     }
     const bool walked = node < 0;
     const int busy = __popcll(__ballot(nq > 0)), active = __popcll(__ballot(1));
-    if (nq > 0 && (walked || nq > kLeafQ - 4 || 2 * busy >= active)) {
+    const bool test = nq > 0 && (walked || nq > kLeafQ - 4 || 2 * busy >= active);
+    float4 a, bb, c;
+    if (test) {
       const int s = s_leaf[--nq * BLOCK + lane];
+      a = P.leaves[3 * s], bb = P.leaves[3 * s + 1], c = P.leaves[3 * s + 2];
+    }
+    fetch(node >= 0 ? node : 0);  // every iteration: the root once the walk is over (never read)
+    if (test) {
       if (COUNT) tc.tris++;
-      const float4 a = P.leaves[3 * s], bb = P.leaves[3 * s + 1], c = P.leaves[3 * s + 2];
       const uint32_t prim = __float_as_uint(a.w);
       float t;
       if (intersect(r, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), mk(c.x, c.y, c.z), t) && t <= far_t &&
