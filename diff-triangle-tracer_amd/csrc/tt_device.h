@@ -380,13 +380,39 @@ TT_D void window_record_store(float4 *q, v3f A, v3f B, v3f C, float sigma) {
   q[6] = make_float4(C.x, C.y, C.z, tw.il[0]);
   q[7] = make_float4(tw.il[1], tw.il[2], tw.dist, sigma);
 }
+// A record as loaded (decoded by window_record_decode): lets a caller issue the loads of
+// the next hit's record before it works on the current one.
+struct WinRec {
+  float4 q[8];
+};
+TT_D void window_record_fetch(const float4 *p, WinRec &r) {
+#pragma unroll
+  for (int j = 0; j < 8; j++) r.q[j] = p[j];
+}
+TT_D double dbl_of(float lo, float hi) {
+  return __longlong_as_double((long long)((uint64_t)__float_as_uint(lo) | (uint64_t)__float_as_uint(hi) << 32));
+}
+TT_D void window_record_decode(const WinRec &r, TriWin &tw, v3f &A, v3f &B, v3f &C, float &sigma) {
+  const float4 a = r.q[0], b = r.q[1], c = r.q[2], e = r.q[3], f = r.q[4], g = r.q[5], h = r.q[6], k = r.q[7];
+  tw.n[0] = v3d{dbl_of(a.x, a.y), dbl_of(a.z, a.w), dbl_of(b.x, b.y)};
+  tw.n[1] = v3d{dbl_of(b.z, b.w), dbl_of(c.x, c.y), dbl_of(c.z, c.w)};
+  tw.n[2] = v3d{dbl_of(e.x, e.y), dbl_of(e.z, e.w), dbl_of(f.x, f.y)};
+  A = mk(f.z, f.w, g.x);
+  B = mk(g.y, g.z, g.w);
+  C = mk(h.x, h.y, h.z);
+  tw.il[0] = h.w;
+  tw.il[1] = k.x;
+  tw.il[2] = k.y;
+  tw.dist = k.z;
+  sigma = k.w;
+}
 TT_D void window_record_load(const float4 *q, TriWin &tw, v3f &A, v3f &B, v3f &C, float &sigma) {
   const double2 *qd = reinterpret_cast<const double2 *>(q);
   const double2 a = qd[0], b = qd[1], c = qd[2], e = qd[3];
   const float4 f = q[4], g = q[5], h = q[6], k = q[7];
   tw.n[0] = v3d{a.x, a.y, b.x};
   tw.n[1] = v3d{b.y, c.x, c.y};
-  const double z2 = __longlong_as_double((long long)((uint64_t)__float_as_uint(f.x) | (uint64_t)__float_as_uint(f.y) << 32));
+  const double z2 = dbl_of(f.x, f.y);
   tw.n[2] = v3d{e.x, e.y, z2};
   A = mk(f.z, f.w, g.x);
   B = mk(g.y, g.z, g.w);

@@ -1172,23 +1172,35 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
   float C[3] = {0.0f, 0.0f, 0.0f}, N[3] = {0.0f, 0.0f, 0.0f}, D = 0.0f, T = 1.0f;
   // one hit at or past the near plane; false: the ray terminates before blending it. rgb:
   // its colour from the forward's stored hits (clamped channel -0.0f), or null: evaluated here
-  auto blend = [&](float t, uint32_t prim, const float *rgb) -> bool {
+  // With WREC the hit's inputs (window record, opacity, stored colour) come in `hi`, loaded
+  // by the caller (fetch_hit) ahead of time; without, they are read here.
+  struct HitIn {
+    WinRec w;
+    float opacity, rgb[3];
+  };
+  auto fetch_hit = [&](uint32_t prim, const float *rgb, HitIn &hi) {
+    window_record_fetch(P.tri_win + kWinRec * (size_t)prim, hi.w);
+    hi.opacity = P.opac[prim];
+    if (rgb) hi.rgb[0] = rgb[0], hi.rgb[1] = rgb[1], hi.rgb[2] = rgb[2];
+  };
+  auto blend = [&](float t, uint32_t prim, const float *rgb, const HitIn &hi) -> bool {
     const int off = P.cumsum[prim];
     v3f V[3];
     Win w;
-    float sigma;
+    float sigma, opacity;
     if (WREC) {
       TriWin tw;
-      window_record_load(P.tri_win + kWinRec * (size_t)prim, tw, V[0], V[1], V[2], sigma);
+      window_record_decode(hi.w, tw, V[0], V[1], V[2], sigma);
       window_hit(tw, V[0], V[1], V[2], o, d, t, sigma, w);
       if (MODE != 2) w.n = face_normal(V[0], V[1], V[2]);
+      opacity = hi.opacity;
     } else {
       const float *vp = P.verts + 3 * (size_t)off;
       V[0] = ld3(vp), V[1] = ld3(vp + 3), V[2] = ld3(vp + 6);
       sigma = P.sigma[prim];
       window32(V[0], V[1], V[2], o, d, t, sigma, w);
+      opacity = P.opac[prim];
     }
-    const float opacity = P.opac[prim];
     const float alpha = fminf(0.99f, opacity * w.Cx);
     if (alpha < 1.0f / 255.0f) return true;
     const float testT = T * (1.0f - alpha);
@@ -1197,7 +1209,7 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
     bool c0, c1, c2;
     v3f col;
     if (rgb) {
-      const float r0 = rgb[0], r1 = rgb[1], r2 = rgb[2];
+      const float r0 = WREC ? hi.rgb[0] : rgb[0], r1 = WREC ? hi.rgb[1] : rgb[1], r2 = WREC ? hi.rgb[2] : rgb[2];
       c0 = signbit(r0), c1 = signbit(r1), c2 = signbit(r2);
       col = mk(fabsf(r0), fabsf(r1), fabsf(r2));
     } else {
@@ -1243,9 +1255,35 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
     return true;
   };
   if (hl) {
-    for (int i = 0; i < n_hl; i++) {
-      const uint2 h = hl[(size_t)i * stride];
-      if (!blend(__uint_as_float(h.y), h.x, hlc ? hlc + 3 * (size_t)i * stride : nullptr)) break;
+    if (WREC) {
+      // software pipeline over the stored hits: while hit i is blended, the inputs of hit i + 1
+      // and the entry of hit i + 2 are in flight (indices clamped to the last stored hit, so
+      // every load reads a written entry; the surplus loads are never used)
+      const int last = n_hl - 1;
+      auto entry = [&](int j) { return hl[(size_t)j * stride]; };
+      auto colours = [&](int j) { return hlc ? hlc + 3 * (size_t)j * stride : nullptr; };
+      HitIn cur, nxt;
+      uint2 hc = make_uint2(0u, 0u), hn = make_uint2(0u, 0u);
+      if (n_hl > 0) {
+        hc = entry(0);
+        fetch_hit(hc.x, colours(0), cur);
+        hn = entry(min(1, last));
+      }
+      for (int i = 0; i < n_hl; i++) {
+        const int i1 = min(i + 1, last);
+        fetch_hit(hn.x, colours(i1), nxt);
+        const uint2 h2 = entry(min(i + 2, last));
+        if (!blend(__uint_as_float(hc.y), hc.x, colours(i), cur)) break;
+        cur = nxt;
+        hc = hn;
+        hn = h2;
+      }
+    } else {
+      HitIn none;
+      for (int i = 0; i < n_hl; i++) {
+        const uint2 h = hl[(size_t)i * stride];
+        if (!blend(__uint_as_float(h.y), h.x, hlc ? hlc + 3 * (size_t)i * stride : nullptr, none)) break;
+      }
     }
     return T;
   }
@@ -1266,7 +1304,9 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
       lp = prim;
       last = fmaxf((float)((double)last + kStepEps), t);
       if (t < P.near_t) continue;
-      if (!blend(t, prim, nullptr)) return T;
+      HitIn hi;  // fetched here: a fetch ahead of the blend spills at this kernel's register bound
+      if (WREC) fetch_hit(prim, nullptr, hi);
+      if (!blend(t, prim, nullptr, hi)) return T;
     }
     if (P.chunk_mode == 1) {
       tlo = lt;
