@@ -21,7 +21,7 @@ struct tt_tracer {
   tt::Bvh bvh;
   int *ov_stack = nullptr;  // traversal stack spill (entries below the LDS part)
   size_t ov_cap = 0;
-  float4 *tri_win = nullptr;  // the one-shot forward's window records (128 B per triangle)
+  float4 *tri_win = nullptr;  // window records of the traversal launches (128 B per triangle)
   size_t tri_win_cap = 0;
   int *heads = nullptr;                    // 8 work-queue heads of the persistent launches
   std::map<long long, int> grids;          // persistent grid size per (k, block, backward)
@@ -896,6 +896,7 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
     const int rc2 = prepare_records(tr, (size_t)grid * a->block, p, 64);
     if (rc2 != TT_OK) return rc2;
   }
+  if ((rc = prepare_tri_win(tr, p)) != TT_OK) return rc;
   return prepare_spill_lanes(tr, (size_t)grid * a->block, p);
 }
 
@@ -962,7 +963,7 @@ int tt_multi_forward_noalloc(tt_tracer *tr, const tt_multi_args *a, void *stream
   if (!base) return fail(TT_ERR_STATE, "the static scratch is not set up on this device (device index >= 64)");
   const int ki = a->k == 8 ? 0 : (a->k == 16 ? 1 : 2), bi = a->block == 64 ? 0 : (a->block == 128 ? 1 : 2);
   int &g = tr->noalloc_grid[ki][bi];
-  if (g == 0) g = tt::multi_grid(a->k, a->block, false);
+  if (g == 0) g = tt::multi_grid(a->k, a->block, false, false);
   if (g <= 0) return fail(TT_ERR_HIP, "could not size the persistent grid");
   int grid = g;
   const int depth = 3 * tr->bvh.height_max + 1 - tt::kLdsStack;

@@ -182,7 +182,7 @@ struct MultiCams {
 // Tile shape of a persistent launch with `block` lanes per block (64 -> 8x8, 128 -> 16x8, 256 -> 16x16).
 void multi_tile_shape(int block, int &tw, int &th);
 // Blocks that fill the device for the persistent kernel (k, block, backward?), 0 if unsupported.
-int multi_grid(int k, int block, bool backward);
+int multi_grid(int k, int block, bool backward, bool wrec = true);
 // Persistent launch over all cameras; `heads` = 8 device ints (reset inside, graph-safe).
 // A forward with a fused loss (p.loss_part) writes the batch's loss to *loss_out.
 std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, int k, int block, int tiles_per_fetch,
@@ -198,6 +198,8 @@ int trace_tiles(int W, int H);
 int backward_tiles(int W, int H);
 // Tiles ordered by cost, most expensive first (one block), into order[n].
 std::string launch_tile_order(const float *cost, int n, int *order, hipStream_t stream);
+// The window records of p.tri_win from p's vertices and sigma (one per triangle).
+std::string launch_tri_win(const TraceParams &p, hipStream_t stream);
 // Lanes a trace launch uses for an image of W x H (the stack spill needs one slot each).
 size_t trace_lanes(int W, int H);
 constexpr int kLdsStack = 16;  // traversal stack entries per lane kept in LDS; deeper ones spill

@@ -81,7 +81,9 @@ __global__ __launch_bounds__(64) void k_probe_multi(TraceParams P, MultiCams mc,
   if (threadIdx.x == 0) cost[t] = est;
 }
 
-template <int K, int BLOCK>
+// WREC: the consumed hits read the launch's window records (P.tri_win); the allocation-free
+// forward has none and forms each triangle's terms per hit.
+template <int K, int BLOCK, bool WREC>
 __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, MultiCams mc, int *heads, int G) {
   __shared__ int s_node[kLdsInts * BLOCK];
   __shared__ int s_range[2];
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, Multi
       int lt, x, y;
       const TraceParams Q = camera_view(P, mc, t, lt);
       float loss = 0.0f;
-      if (tile_pixel<BLOCK>(Q, lt, x, y)) forward_pixel<K, BLOCK, 0, false>(Q, y * Q.W + x, s_node, &loss);
+      if (tile_pixel<BLOCK>(Q, lt, x, y)) forward_pixel<K, BLOCK, 0, false, 0, WREC>(Q, y * Q.W + x, s_node, &loss);
       if (P.loss_part) {  // the tile's loss: lanes summed in a fixed order (wave tree, then the waves)
         for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
         __syncthreads();
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_backward_multi(TraceParams P, Mult
       int lt, x, y;
       const TraceParams Q = camera_view(P, mc, t, lt);
       const bool valid = tile_pixel<BLOCK>(Q, lt, x, y);
-      backward_block_rec<K, BLOCK, 0>(Q, valid, y * Q.W + x, lt, s_node, s_rec, s_tab);
+      backward_block_rec<K, BLOCK, 0, true>(Q, valid, y * Q.W + x, lt, s_node, s_rec, s_tab);
     }
   }
 }
@@ -166,23 +168,24 @@ __global__ __launch_bounds__(1024) void k_loss_total(const float *part, int n, f
 }
 
 template <int K, int BLOCK>
-const void *kernel_of(bool backward) {
-  return backward ? reinterpret_cast<const void *>(&k_backward_multi<K, BLOCK>)
-                  : reinterpret_cast<const void *>(&k_forward_multi<K, BLOCK>);
+const void *kernel_of(bool backward, bool wrec) {
+  if (backward) return reinterpret_cast<const void *>(&k_backward_multi<K, BLOCK>);
+  return wrec ? reinterpret_cast<const void *>(&k_forward_multi<K, BLOCK, true>)
+              : reinterpret_cast<const void *>(&k_forward_multi<K, BLOCK, false>);
 }
 
 template <int K>
-const void *kernel_k(int block, bool backward) {
-  if (block == 64) return kernel_of<K, 64>(backward);
-  if (block == 128) return kernel_of<K, 128>(backward);
-  if (block == 256) return kernel_of<K, 256>(backward);
+const void *kernel_k(int block, bool backward, bool wrec) {
+  if (block == 64) return kernel_of<K, 64>(backward, wrec);
+  if (block == 128) return kernel_of<K, 128>(backward, wrec);
+  if (block == 256) return kernel_of<K, 256>(backward, wrec);
   return nullptr;
 }
 
-const void *multi_kernel(int k, int block, bool backward) {
-  if (k == 8) return kernel_k<8>(block, backward);
-  if (k == 16) return kernel_k<16>(block, backward);
-  if (k == 32) return kernel_k<32>(block, backward);
+const void *multi_kernel(int k, int block, bool backward, bool wrec) {
+  if (k == 8) return kernel_k<8>(block, backward, wrec);
+  if (k == 16) return kernel_k<16>(block, backward, wrec);
+  if (k == 32) return kernel_k<32>(block, backward, wrec);
   return nullptr;
 }
 
@@ -193,8 +196,8 @@ void multi_tile_shape(int block, int &tw, int &th) {
   th = block == 256 ? 16 : 8;
 }
 
-int multi_grid(int k, int block, bool backward) {
-  const void *fn = multi_kernel(k, block, backward);
+int multi_grid(int k, int block, bool backward, bool wrec) {
+  const void *fn = multi_kernel(k, block, backward, wrec);
   if (!fn) return 0;
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -205,9 +208,15 @@ int multi_grid(int k, int block, bool backward) {
 
 std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, int k, int block, int G, int grid,
                          bool backward, hipStream_t stream, float *loss_out) {
-  const void *fn = multi_kernel(k, block, backward);
+  // the backward always reads window records; a forward does when it is given them
+  if (backward && p.P > 0 && !p.tri_win) return "the backward needs its window-record buffer";
+  const void *fn = multi_kernel(k, block, backward, p.tri_win != nullptr);
   if (!fn) return "unsupported (k, block): k in {8, 16, 32}, block in {64, 128, 256}";
   if (mc.total_tiles == 0) return "";
+  if (p.tri_win && p.P > 0) {
+    std::string e = launch_tri_win(p, stream);
+    if (!e.empty()) return e;
+  }
   TraceParams pp = p;
   MultiCams m = mc;
   const int n = mc.total_tiles;

@@ -269,6 +269,15 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
   return "";
 }
 
+std::string launch_tri_win(const TraceParams &p, hipStream_t stream) {
+  if (p.P <= 0) return "";
+  if (!p.tri_win) return "no window-record buffer";
+  hipLaunchKernelGGL(k_tri_win, dim3((p.P + 255) / 256), dim3(256), 0, stream, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return std::string("window record launch failed: ") + hipGetErrorString(e);
+  return "";
+}
+
 std::string launch_tile_order(const float *cost, int n, int *order, hipStream_t stream) {
   if (n == 0) return "";
   hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, stream, cost, n, order);
