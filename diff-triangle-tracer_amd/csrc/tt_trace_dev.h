@@ -147,7 +147,8 @@ TT_D void load_node4q(const float4 *q, float4 &lx, float4 &hx, float4 &ly, float
 // are moved down and upper planes up by pad = 2^-19 (C + |o|), 32 u, in world space: the
 // walk stays conservative, and hit lists do not change.
 struct NodePad {
-  v3f pinv;  // pad / d per axis (signed)
+  v3f pinv;                   // pad / |d| per axis
+  bool neg_x, neg_y, neg_z;   // d < 0 per axis: the upper plane is the entry plane
 };
 TT_D NodePad node_pad(const TraceParams &P, const RayPre &r) {
   // the root's origin and 256 steps of its scale bound every coordinate of the scene
@@ -157,7 +158,8 @@ TT_D NodePad node_pad(const TraceParams &P, const RayPre &r) {
   const float C = fmaxf(fmaxf(reach(root.x, eb), reach(root.y, eb >> 8)), reach(root.z, eb >> 16));
   const float pad = 0x1p-19f * (C + fmaxf(fmaxf(fabsf(r.o.x), fabsf(r.o.y)), fabsf(r.o.z)));
   NodePad np;
-  np.pinv = mk(pad * r.inv.x, pad * r.inv.y, pad * r.inv.z);
+  np.pinv = mk(pad * fabsf(r.inv.x), pad * fabsf(r.inv.y), pad * fabsf(r.inv.z));
+  np.neg_x = r.inv.x < 0.0f, np.neg_y = r.inv.y < 0.0f, np.neg_z = r.inv.z < 0.0f;
   return np;
 }
 // Entry and exit distances of the 4 children of a quantized node (unwidened, as slab_raw).
@@ -175,19 +177,18 @@ TT_D void slab4q(const RayPre &r, const NodePad &np, const float4 a, const float
     return make_float4(fmaf((float)u.x, s, o), fmaf((float)u.y, s, o), fmaf((float)v.x, s, o),
                        fmaf((float)v.y, s, o));
   };
-  const float4 lx = t4(px.x, px.y, sx, bx - np.pinv.x), hx = t4(px.z, px.w, sx, bx + np.pinv.x);
-  const float4 ly = t4(py.x, py.y, sy, by - np.pinv.y), hy = t4(py.z, py.w, sy, by + np.pinv.y);
-  const float4 lz = t4(pz.x, pz.y, sz, bz - np.pinv.z), hz = t4(pz.z, pz.w, sz, bz + np.pinv.z);
-  auto near = [](float l0, float h0, float l1, float h1, float l2, float h2) {
-    return fmaxf(fmaxf(fminf(l0, h0), fminf(l1, h1)), fminf(l2, h2));
-  };
-  auto far = [](float l0, float h0, float l1, float h1, float l2, float h2) {
-    return fminf(fminf(fmaxf(l0, h0), fmaxf(l1, h1)), fmaxf(l2, h2));
-  };
-  tn = make_float4(near(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x), near(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y),
-                   near(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z), near(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w));
-  tf = make_float4(far(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x), far(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y),
-                   far(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z), far(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w));
+  // entry / exit plane of each axis by the sign of d: fma is monotone in q and in the offset,
+  // so these are exactly the min / max of the lower and upper plane distances
+  const float4 nx = t4(np.neg_x ? px.z : px.x, np.neg_x ? px.w : px.y, sx, bx - np.pinv.x);
+  const float4 fx = t4(np.neg_x ? px.x : px.z, np.neg_x ? px.y : px.w, sx, bx + np.pinv.x);
+  const float4 ny = t4(np.neg_y ? py.z : py.x, np.neg_y ? py.w : py.y, sy, by - np.pinv.y);
+  const float4 fy = t4(np.neg_y ? py.x : py.z, np.neg_y ? py.y : py.w, sy, by + np.pinv.y);
+  const float4 nz = t4(np.neg_z ? pz.z : pz.x, np.neg_z ? pz.w : pz.y, sz, bz - np.pinv.z);
+  const float4 fz = t4(np.neg_z ? pz.x : pz.z, np.neg_z ? pz.y : pz.w, sz, bz + np.pinv.z);
+  tn = make_float4(fmaxf(fmaxf(nx.x, ny.x), nz.x), fmaxf(fmaxf(nx.y, ny.y), nz.y), fmaxf(fmaxf(nx.z, ny.z), nz.z),
+                   fmaxf(fmaxf(nx.w, ny.w), nz.w));
+  tf = make_float4(fminf(fminf(fx.x, fy.x), fz.x), fminf(fminf(fx.y, fy.y), fz.y), fminf(fminf(fx.z, fy.z), fz.z),
+                   fminf(fminf(fx.w, fy.w), fz.w));
 }
 
 template <int K, int BLOCK, bool COUNT>
