@@ -650,7 +650,8 @@ class Frame:
         return self.capi.bwd_args(self.W, self.H, self.ro, self.rd, self.verts, self.npts, self.cumsum, self.shs, 3,
                                   self.opac, self.sigma, self.color, self.others, self.dpix, self.doth, self.dverts,
                                   self.dsig, self.dop, self.dsh, k=self.k, cutoff=self.cutoff, chunk_mode=mode,
-                                  zero_grads=False, interval=interval, hitlist=self.hl if hitlist else None)
+                                  zero_grads=False, interval=interval, hitlist=self.hl if hitlist else None,
+                                  same_scene=True)  # the frame's scene does not change between fwd and bwd
 
     def zero_grads(self):
         for t in (self.dverts, self.dsig, self.dop, self.dsh):

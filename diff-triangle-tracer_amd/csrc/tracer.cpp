@@ -359,6 +359,10 @@ BwdTraceResult OptixTracer::trace_bwd(const Tensor &ray_o, const Tensor &ray_d, 
         throw Exception("hit_colors must be the forward's float32 [cap, H*W, 3] tensor");
       a.hitlist_color_dev = hk.data_ptr<float>();
     }
+    // the forward's hit list is only valid for the scene that forward traced, so handing it
+    // over already vouches for unchanged vertices and sigma: the backward may reuse that
+    // forward's window records (the library still checks the pointers)
+    a.same_scene_as_forward = 1;
   }
   check_tt(tt_trace_bwd(tr_, &a, cur_stream(dev)), "trace_backward");
   BwdTraceResult res;

@@ -23,6 +23,10 @@ struct tt_tracer {
   size_t ov_cap = 0;
   float4 *tri_win = nullptr;  // window records of the traversal launches (128 B per triangle)
   size_t tri_win_cap = 0;
+  // the inputs the last record pass was issued for (a backward told the scene is unchanged
+  // reuses the records when its inputs are these)
+  const void *win_verts = nullptr, *win_cumsum = nullptr, *win_sigma = nullptr;
+  int win_P = -1;
   int *heads = nullptr;                    // 8 work-queue heads of the persistent launches
   std::map<long long, int> grids;          // persistent grid size per (k, block, backward)
   hipStream_t capture_stream = nullptr;    // private stream used only to record graphs
@@ -187,14 +191,17 @@ int prepare_spill(tt_tracer *tr, int W, int H, tt::TraceParams &p) {
   return prepare_spill_lanes(tr, tt::trace_lanes(W, H), p);
 }
 
-// Window records of the one-shot forward: kWinRec float4 per triangle, rewritten by every
-// forward from the vertices it is given.
-int prepare_tri_win(tt_tracer *tr, tt::TraceParams &p) {
+// Window records of a traversal launch: kWinRec float4 per triangle, formed by a record pass
+// before the launch from the vertices and sigma it is given -- or, with `reuse`, taken from the
+// last pass when that pass was issued for the same three input pointers and P (the caller
+// vouches that their contents did not change).
+int prepare_tri_win(tt_tracer *tr, tt::TraceParams &p, bool reuse = false) {
   const size_t n = (size_t)(p.P > 0 ? p.P : 0);
   if (n > tr->tri_win_cap) {
     if (tr->tri_win) (void)hipFree(tr->tri_win);
     tr->tri_win = nullptr;
     tr->tri_win_cap = 0;
+    tr->win_P = -1;
     hipError_t e = hipMalloc(&tr->tri_win, n * tt::kWinRec * sizeof(float4));
     if (e != hipSuccess) return fail(TT_ERR_HIP, std::string("window record allocation failed: ") +
                                                      hipGetErrorString(e));
@@ -202,6 +209,12 @@ int prepare_tri_win(tt_tracer *tr, tt::TraceParams &p) {
     tr->mem_gen++;
   }
   p.tri_win = tr->tri_win;
+  p.tri_win_ready = reuse && tr->win_P == p.P && tr->win_verts == p.verts && tr->win_cumsum == p.cumsum &&
+                    tr->win_sigma == p.sigma;
+  tr->win_verts = p.verts;
+  tr->win_cumsum = p.cumsum;
+  tr->win_sigma = p.sigma;
+  tr->win_P = p.P;
   return TT_OK;
 }
 
@@ -730,7 +743,7 @@ int tt_trace_bwd(tt_tracer *tr, const tt_bwd_args *a, void *stream) {
     p.denv = a->dL_denvmap_dev;
   }
   if ((rc = prepare_spill(tr, a->width, a->height, p)) != TT_OK) return rc;
-  if ((rc = prepare_tri_win(tr, p)) != TT_OK) return rc;
+  if ((rc = prepare_tri_win(tr, p, a->same_scene_as_forward != 0)) != TT_OK) return rc;
   {  // 16 rows per lane, or the largest record total a forward has needed so far
     poll_hits_total(tr);
     const size_t lanes = tt::trace_lanes(a->width, a->height);
@@ -849,7 +862,7 @@ int multi_params(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::Multi
 // multi_params plus the tracer's persistent grid, work-queue heads, tile schedule, record
 // rows and stack spill (allocated or grown here).
 int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiCams &mc, tt::TraceParams &p,
-                int &grid) {
+                int &grid, bool reuse_records = false) {
   long long rays = 0;
   int rc = multi_params(tr, a, backward, mc, p, rays);
   if (rc != TT_OK) return rc;
@@ -896,7 +909,7 @@ int multi_setup(tt_tracer *tr, const tt_multi_args *a, bool backward, tt::MultiC
     const int rc2 = prepare_records(tr, (size_t)grid * a->block, p, 64);
     if (rc2 != TT_OK) return rc2;
   }
-  if ((rc = prepare_tri_win(tr, p)) != TT_OK) return rc;
+  if ((rc = prepare_tri_win(tr, p, reuse_records)) != TT_OK) return rc;
   return prepare_spill_lanes(tr, (size_t)grid * a->block, p);
 }
 
@@ -920,8 +933,8 @@ int multi_issue(tt_tracer *tr, const tt_multi_args *a, int what, hipStream_t s) 
                                      a->loss_dev);
     if (!e.empty()) return fail(TT_ERR_HIP, e);
   }
-  if (what & 2) {
-    if ((rc = multi_setup(tr, a, true, mc, p, grid)) != TT_OK) return rc;
+  if (what & 2) {  // after this call's forward, the backward reads the records it formed
+    if ((rc = multi_setup(tr, a, true, mc, p, grid, (what & 1) != 0)) != TT_OK) return rc;
     if ((rc = zero_grads(a, s)) != TT_OK) return rc;
     if (a->P > 0) {
       std::string e = tt::launch_multi(p, mc, tr->heads, a->k, a->block, a->tiles_per_fetch, grid, true, s);

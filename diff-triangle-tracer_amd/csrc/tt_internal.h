@@ -72,8 +72,10 @@ struct TraceParams {
   const float *shs;
   int M, deg;
   const float *opac, *sigma;
-  // per-triangle window records (kWinRec float4 each) of this forward, or null
+  // per-triangle window records (kWinRec float4 each) of this launch, or null; tri_win_ready:
+  // they already hold this launch's inputs (the host skips the record pass)
   float4 *tri_win;
+  int tri_win_ready;
   const float4 *nodes, *leaves, *nodes4;
   const uint2 *counts4;
   int n_leaves;

@@ -213,7 +213,7 @@ std::string launch_multi(const TraceParams &p, const MultiCams &mc, int *heads, 
   const void *fn = multi_kernel(k, block, backward, p.tri_win != nullptr);
   if (!fn) return "unsupported (k, block): k in {8, 16, 32}, block in {64, 128, 256}";
   if (mc.total_tiles == 0) return "";
-  if (p.tri_win && p.P > 0) {
+  if (p.tri_win && p.P > 0 && !p.tri_win_ready) {
     std::string e = launch_tri_win(p, stream);
     if (!e.empty()) return e;
   }

@@ -300,9 +300,10 @@ std::string launch_backward(const TraceParams &p0, int k, int variant, hipStream
   if (p.n_tiles == 0) return "";
   dim3 g(p.n_tiles), bl(kBlockB);
   if (k != 8 && k != 16 && k != 32) return "k must be 8, 16 or 32";
-  // the window records of the vertices this backward is given (a forward's may be stale)
+  // the window records of the vertices this backward is given, unless the host knows the
+  // last forward's records hold them
   if (p.P > 0 && !p.tri_win) return "the backward needs its window-record buffer";
-  if (p.P > 0) hipLaunchKernelGGL(k_tri_win, dim3((p.P + 255) / 256), dim3(256), 0, stream, p);
+  if (p.P > 0 && !p.tri_win_ready) hipLaunchKernelGGL(k_tri_win, dim3((p.P + 255) / 256), dim3(256), 0, stream, p);
   if (ev) (void)hipEventRecord(ev->before, stream);
   if (p.rec_rows) {
     if (variant == 4) {

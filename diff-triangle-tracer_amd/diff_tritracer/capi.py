@@ -74,6 +74,7 @@ class BwdArgs(C.Structure):
         ("env_width", C.c_int), ("env_height", C.c_int), ("bounce_aux_dev", _fp), ("dL_dreflectivity_dev", _fp),
         ("dL_denvmap_dev", _fp),
         ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int), ("hitlist_color_dev", _fp),
+        ("same_scene_as_forward", C.c_int),
     ]
 
 
@@ -521,9 +522,11 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
 
 def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, color, others, dL_dpix,
              dL_dothers, dverts, dsigma, dopac, dsh, k=16, cutoff=1e-4, chunk_mode="compat", zero_grads=True,
-             near=0.2, far=1000.0, interval=None, bounce=None, hitlist=None) -> BwdArgs:
+             near=0.2, far=1000.0, interval=None, bounce=None, hitlist=None, same_scene=False) -> BwdArgs:
     """bounce: dict(reflectivity, envmap, aux, dreflectivity [P], denvmap [H, W, 3]) selects the
-    bounce variant's backward. hitlist: the forward's (hits, counts): read instead of re-traversing."""
+    bounce variant's backward. hitlist: the forward's (hits, counts): read instead of re-traversing.
+    same_scene: verts, cumsum and sigma hold what they held at the tracer's last forward (the
+    backward then reuses that forward's window records when the pointers match)."""
     a = BwdArgs()
     a.width, a.height = int(W), int(H)
     a.ray_o_dev, a.ray_d_dev = ptr(ray_o), ptr(ray_d)
@@ -541,6 +544,7 @@ def bwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.dL_dvertices_rows = int(dverts.numel() // 3)
     a.dL_dvertices_dev, a.dL_dsigma_dev, a.dL_dopacity_dev, a.dL_dsh_dev = ptr(dverts), ptr(dsigma), ptr(dopac), ptr(dsh)
     a.interval_dev = ptr(interval)
+    a.same_scene_as_forward = 1 if same_scene else 0
     if hitlist is not None:
         _set_hitlist(a, hitlist)
     if bounce is not None:

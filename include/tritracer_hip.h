@@ -205,6 +205,13 @@ typedef struct {
   const int *hitlist_count_dev;
   int hitlist_cap;
   const float *hitlist_color_dev; /* optional: the forward's hitlist_color_dev */
+  /* nonzero: the caller guarantees that vertices_dev, cumsum_dev and sigma_dev hold the values
+     they held at this tracer's last tt_trace_fwd. When that forward was given the same three
+     pointers and P, the backward then reads the per-triangle window records that forward formed
+     (128 B per triangle: the ray-independent terms of the window function) instead of forming
+     them again; otherwise, and when zero, it forms them from its own inputs. No counterpart in
+     the reference (which keeps no per-triangle state between its launches). */
+  int same_scene_as_forward;
 } tt_bwd_args;
 
 /* Replaces OptixTracer::trace_bwd's launch (tracer.cpp:199-222) and backward.cu:191-391:

@@ -91,7 +91,7 @@ class GpuScene:
         return out
 
     def backward(self, ray_o, ray_d, color, others, dL_dpix, dL_dothers, k=16, cutoff=1e-3, chunk_mode="compat",
-                 interval=None, hitlist=None):
+                 interval=None, hitlist=None, same_scene=False):
         from diff_tritracer import capi
 
         torch = self.torch
@@ -110,7 +110,7 @@ class GpuScene:
                           oth, gp, go, dverts, dsig, dop, dsh, k=k, cutoff=cutoff, chunk_mode=chunk_mode,
                           zero_grads=True,
                           interval=to_dev(np.asarray(interval, np.float32), dev) if interval is not None else None,
-                          hitlist=hitlist)
+                          hitlist=hitlist, same_scene=same_scene)
         self.h.bwd(a, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         return dict(dverts=dverts.cpu().numpy(), dsigma=dsig.cpu().numpy(), dopac=dop.cpu().numpy(),

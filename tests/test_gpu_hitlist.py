@@ -14,6 +14,11 @@ Bars:
                                            the backward that reads them equals, within 1e-3 (FP32 atomics
                                            add in any order; a wrong clamp bit errs by O(1)), the backward
                                            that evaluates the SH again
+  window records reused ................. a backward told the scene is unchanged reuses the forward's
+                                           per-triangle window records only for the same input pointers:
+                                           with other vertices it forms their records (same gradients as
+                                           without the flag); with the forward's own, the same gradients
+                                           as forming them again (within 1e-3: FP32 atomic order)
 """
 import numpy as np
 import pytest
@@ -96,3 +101,38 @@ def test_hitlist_forward_and_backward(small, mode, cap):
         ok, worst = grad_close(gb[key], gn[key])
         _say(f"  {key}: worst ratio vs the SH-evaluating backward {worst:.3g}")
         assert ok, (key, worst)
+
+
+def test_backward_reuses_window_records_only_for_the_forward_inputs(small):
+    import torch
+
+    sc, ro, rd, g, S = small
+    R = ro.shape[0] * ro.shape[1]
+    out = g.forward(ro, rd, k=16, cutoff=1e-3, chunk_mode="exact", hitlist_cap=64)
+    rng = np.random.Generator(np.random.PCG64(12))
+    dpix = rng.standard_normal((3, R)).astype(np.float32)
+    doth = np.zeros((7, R), np.float32)
+    args = (ro, rd, out["_color"], out["_others"], dpix, doth)
+    kw = dict(k=16, cutoff=1e-3, chunk_mode="exact", hitlist=out["_hitlist"])
+    _say("backward reusing the forward's window records, and forming them again")
+    reuse = g.backward(*args, same_scene=True, **kw)
+    formed = g.backward(*args, same_scene=False, **kw)
+    for key in ("dopac", "dsigma", "dsh", "dverts"):
+        ok, worst = grad_close(reuse[key], formed[key])
+        assert ok, (key, worst)
+    # other vertices (a different tensor, moved by 1e-3) on the same tracer: records reused from
+    # the forward would be those of the old vertices; the flag must not apply to them
+    saved = g.verts
+    try:
+        g.verts = saved + 1e-3 * torch.randn(saved.shape, generator=torch.Generator().manual_seed(5)).to(saved.device)
+        _say("backward with other vertices and the flag set, and without it")
+        flagged = g.backward(*args, same_scene=True, **kw)
+        plain = g.backward(*args, same_scene=False, **kw)
+    finally:
+        g.verts = saved
+    moved = 0
+    for key in ("dopac", "dsigma", "dsh", "dverts"):
+        ok, worst = grad_close(flagged[key], plain[key])
+        assert ok, (key, worst)
+        moved += not grad_close(flagged[key], reuse[key])[0]
+    assert moved > 0  # the moved vertices do change the gradients: the check above can tell
