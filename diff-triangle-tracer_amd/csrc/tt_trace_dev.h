@@ -1084,6 +1084,7 @@ TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const
   base = __shfl(base, leader);
   const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
   if (idx < s_rec[2]) {
+    if (idx < rec_batch<BLOCK>()) s_rec[kRecLds + idx] = (int)prim;  // the grouping's first batch reads it here
     if (P.rec_keys) P.rec_keys[(size_t)s_rec[1] + idx] = prim;
     float4 *row = P.rec_rows + 5 * ((size_t)s_rec[1] + idx);
     const int cl = (c0 ? 1 : 0) | (c1 ? 2 : 0) | (c2 ? 4 : 0);
@@ -1114,9 +1115,10 @@ TT_D uint32_t add_record(const TraceParams &P, size_t r, float acc[11], ShSink &
 
 // Stage row of one triangle's sums (64 floats): SH 0..47, vertices 48..56, opacity 57,
 // sigma 58, triangle id 63 (-1: no triangle).
-TT_D void stage_sums(float *row, int prim, const float acc[11], const ShSink &sh) {
+TT_D void stage_sums(float *row, int prim, int vrow, const float acc[11], const ShSink &sh) {
   for (int f = 0; f < 48; f++) row[f] = sh.v[f];
   for (int f = 0; f < 11; f++) row[48 + f] = acc[f];
+  row[59] = __int_as_float(vrow);  // the triangle's first vertex row (cumsum)
   row[63] = __int_as_float(prim);
 }
 
@@ -1128,18 +1130,14 @@ TT_D void flush_stage(const TraceParams &P, const float *stage) {
   for (int p = threadIdx.x; p < 64 * kStageSlots; p += BLOCK) {
     const int f = p & 63;
     const int prim = __float_as_int(stage[p | 63]);
-    if (prim < 0 || f >= 59) continue;
+    if (prim < 0 || f >= 59 || (f < 48 && f >= nsh3)) continue;
     const float v = stage[p];
     if (v == 0.0f) continue;
-    if (f < 48) {
-      if (f < nsh3) unsafeAtomicAdd(P.dsh + (size_t)prim * P.M * 3 + f, v);
-    } else if (f < 57) {
-      unsafeAtomicAdd(P.dverts + 3 * (size_t)P.cumsum[prim] + (f - 48), v);
-    } else if (f == 57) {
-      unsafeAtomicAdd(P.dopac + prim, v);
-    } else {
-      unsafeAtomicAdd(P.dsigma + prim, v);
-    }
+    // the field's address, picked per lane: one atomic instruction covers all of a row's fields
+    float *dst = f < 48 ? P.dsh + (size_t)prim * P.M * 3 + f
+                        : (f < 57 ? P.dverts + 3 * (size_t)__float_as_int(stage[(p & ~63) | 59]) + (f - 48)
+                                  : (f == 57 ? P.dopac + prim : P.dsigma + prim));
+    unsafeAtomicAdd(dst, v);
   }
 }
 
@@ -1511,7 +1509,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
       __syncthreads();
     }
     for (int i = threadIdx.x; i < n; i += BLOCK) {
-      const uint32_t prim = __float_as_uint(P.rec_rows[5 * (first + i)].x);
+      const uint32_t prim = b0 == 0 ? (uint32_t)s_rec[kRecLds + i] : __float_as_uint(P.rec_rows[5 * (first + i)].x);
       const int sl = claim_slot<kAggSlots>(t_keys, prim);
       slot_of[i] = (short)sl;
       if (sl >= 0) atomicAdd(&t_cnt[sl], 1);
@@ -1557,7 +1555,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
     const int n_items = s_rec[3];
     for (int i0 = 0; i0 < n_items; i0 += BLOCK) {  // work item = (slot, segment of <= kSegRecords records)
       const int it = i0 + threadIdx.x;
-      int prim = -1;
+      int prim = -1, vrow = 0;
       float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       ShSink sh;
       for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
@@ -1569,12 +1567,13 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
           else hi = mid;
         }
         prim = t_keys[lo];
+        vrow = P.cumsum[prim];
         const int b = t_off[lo] + (it - t_item[lo]) * kSegRecords, e = min(t_off[lo] + t_cnt[lo], b + kSegRecords);
 #pragma unroll 2
         for (int j = b; j < e; j++) add_record(P, first + list[j], acc, sh);
       }
       for (int q = 0; q < BLOCK / kStageSlots; q++) {
-        if (threadIdx.x / kStageSlots == q) stage_sums(stage + 64 * (threadIdx.x % kStageSlots), prim, acc, sh);
+        if (threadIdx.x / kStageSlots == q) stage_sums(stage + 64 * (threadIdx.x % kStageSlots), prim, vrow, acc, sh);
         __syncthreads();
         flush_stage<BLOCK>(P, stage);
         __syncthreads();
