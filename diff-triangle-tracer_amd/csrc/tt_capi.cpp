@@ -604,6 +604,7 @@ int tt_trace_fwd(tt_tracer *tr, const tt_fwd_args *a, void *stream) {
   p.log_code = a->hit_log_code_dev;
   p.log_count = a->hit_log_count_dev;
   p.ival_in = reinterpret_cast<const float2 *>(a->interval_in_dev);
+  p.ival_ready = a->interval_ready_event;
   p.ival_out = reinterpret_cast<float4 *>(a->interval_out_dev);
   p.ival_hard = a->interval_in_dev && a->interval_hard_cap ? 1 : 0;
   p.ival_fb_start = a->interval_fallback_start;

@@ -101,6 +101,7 @@ struct TraceParams {
   // depth intervals: forward input (start, cap) and output (first, last, term, prim_first);
   // backward input = the forward's output
   const float2 *ival_in;
+  void *ival_ready;  // host side: a hipEvent_t the traversal launch waits for (ival_in produced elsewhere)
   int ival_hard;  // 1: hits past the cap are dropped (the proposal is trusted); 0: exact continuation
   float ival_fb_start, ival_fb_end;  // hard mode fallbacks (distances), see trace_segment
   float4 *ival_out;

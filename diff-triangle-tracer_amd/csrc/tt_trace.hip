@@ -244,6 +244,10 @@ std::string launch_forward(const TraceParams &p0, int k, int variant, hipStream_
     if (e != hipSuccess) return std::string("tile probe launch failed: ") + hipGetErrorString(e);
     p.tile_sched = p.probe_order;
   }
+  if (p.ival_ready) {  // the proposed intervals come from another stream: wait only now
+    e = hipStreamWaitEvent(stream, static_cast<hipEvent_t>(p.ival_ready), 0);
+    if (e != hipSuccess) return std::string("hipStreamWaitEvent on interval_ready_event failed: ") + hipGetErrorString(e);
+  }
   if (ev) (void)hipEventRecord(ev->before, stream);
   if (k == 8)
     e = fwd_k<8>(p, variant, stream);

@@ -54,6 +54,7 @@ class FwdArgs(C.Structure):
         ("reflectivity_dev", _fp), ("ior_dev", _fp), ("envmap_dev", _fp), ("env_width", C.c_int),
         ("env_height", C.c_int), ("bounce_aux_dev", _fp),
         ("hitlist_dev", _fp), ("hitlist_count_dev", _fp), ("hitlist_cap", C.c_int), ("hitlist_color_dev", _fp),
+        ("interval_ready_event", _fp),
     ]
 
 
@@ -480,7 +481,8 @@ def _set_hitlist(a, hitlist):
 def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigma, bg, color, others, maxb, density,
              scaling, k=16, cutoff=1e-4, chunk_mode="compat", variant="forward", extra=None, stats=None,
              log=None, near=0.2, far=1000.0, trav_stats=None, interval_in=None, interval_out=None,
-             hard_cap=False, fallback_start=0.0, fallback_end=0.0, bounce=None, hitlist=None) -> FwdArgs:
+             hard_cap=False, fallback_start=0.0, fallback_end=0.0, bounce=None, hitlist=None,
+             interval_event=None) -> FwdArgs:
     """bounce: dict(reflectivity [P], ior [P], envmap [H, W, 3], aux [R, 20]) for variant "bounce".
     hitlist: (hits int32 [cap, R, 2], counts int32 [R][, colours float32 [cap, R, 3]]) written for the
     backward (see hitlist_buffers)."""
@@ -505,6 +507,8 @@ def fwd_args(W, H, ray_o, ray_d, verts, npts, cumsum, shs, sh_degree, opac, sigm
     a.trav_stats_dev = ptr(trav_stats)
     a.interval_in_dev, a.interval_out_dev = ptr(interval_in), ptr(interval_out)
     a.interval_hard_cap = 1 if hard_cap else 0
+    # a torch.cuda.Event recorded where interval_in is produced: the traversal waits for it
+    a.interval_ready_event = interval_event.cuda_event if interval_event is not None else None
     a.interval_fallback_start, a.interval_fallback_end = float(fallback_start), float(fallback_end)
     if bounce is not None:
         a.reflectivity_dev, a.ior_dev = ptr(bounce["reflectivity"]), ptr(bounce["ior"])

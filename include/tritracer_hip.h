@@ -159,6 +159,12 @@ typedef struct {
      that hitlist[2 * e] holds. Handing it to tt_trace_bwd spares the backward the SH
      coefficients' read and evaluation per hit (same gradients). */
   float *hitlist_color_dev;      /* [segments][hitlist_cap][R][3] written */
+  /* Optional hipEvent_t (no counterpart in the reference): the forward waits for it on its
+     stream (hipStreamWaitEvent) right before the traversal kernel, after its window-record pass
+     and tile probe, which do not read interval_in. A caller that computes interval_in on
+     another stream (the proposal network, tt_proposal_forward) records this event there and
+     so overlaps that work with those passes. NULL: no wait. */
+  void *interval_ready_event;
 } tt_fwd_args;
 
 /* Replaces OptixTracer::trace_fwd's launch (tracer.cpp:105-132) and the raygen programs

@@ -227,3 +227,47 @@ def test_trained_proposal_ablation(cuda):
     assert n3 < 0.95 * n0  # measured 0.913x
     assert n2 <= n0
     assert p > 25.0
+
+
+def test_forward_waits_for_intervals_made_on_another_stream(cuda):
+    """interval_ready_event: the proposal network runs on a second stream while the forward
+    forms its window records and probes its tiles; the forward's traversal waits for the
+    event. Three rounds, each checked bit for bit against the same network and forward run
+    one after the other on one stream (exact cap mode: the image is the plain trace's)."""
+    import torch
+
+    sc, ro, rd = scenes.small_config(0, 128, 128)
+    g = GpuScene(sc, cuda)
+    R = 128 * 128
+    net = capi.Proposal(seed=7)
+    o, d = to_dev(ro, cuda), to_dev(rd, cuda)
+    main = torch.cuda.current_stream()
+    s2 = torch.cuda.Stream()
+    fork, ready = torch.cuda.Event(), torch.cuda.Event()
+    ready.record(s2)
+
+    def render(interval_in, event=None):
+        color, others = torch.zeros(3, R, device=cuda), torch.zeros(7, R, device=cuda)
+        maxb, dens, scal = (torch.zeros(g.P, device=cuda) for _ in range(3))
+        a = capi.fwd_args(128, 128, o, d, g.verts, g.npts, g.cumsum, g.shs, 3, g.opac, g.sigma,
+                          torch.zeros(3, device=cuda), color, others, maxb, dens, scal, k=16, cutoff=1e-3,
+                          chunk_mode="exact", interval_in=interval_in, interval_event=event)
+        g.h.fwd(a, main.cuda_stream)
+        return color
+
+    for rnd in range(3):
+        cap_serial = torch.empty(R, 2, device=cuda)
+        net.forward(o, d, cap_serial, capi.PROPOSAL_CAP, 0.05, 0.05, stream=main.cuda_stream)
+        c_serial = render(cap_serial)
+        cap_over = torch.full((R, 2), float("nan"), device=cuda)
+        fork.record(main)
+        s2.wait_event(fork)
+        net.forward(o, d, cap_over, capi.PROPOSAL_CAP, 0.05, 0.05, stream=s2.cuda_stream)
+        ready.record(s2)
+        c_over = render(cap_over, ready)
+        torch.cuda.synchronize()
+        same_cap = np.array_equal(cap_serial.cpu().numpy().view(np.uint32), cap_over.cpu().numpy().view(np.uint32))
+        same_img = np.array_equal(c_serial.cpu().numpy().view(np.uint32), c_over.cpu().numpy().view(np.uint32))
+        print(f"[gpu-proposal] round {rnd}: intervals equal {same_cap}, image equal {same_img}", flush=True)
+        assert same_cap and same_img
+        torch.randn(1000, device=cuda).sum().item()  # eager work between rounds
