@@ -61,9 +61,9 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 # valu_busy = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x GRBM_GUI_ACTIVE / XCDs).
 PROFILE_SOURCE = "profiles/trace_fwd_bwd_1M_1080p.md"
 MEASURED = {
-    "k_forward<16,0,false,1>": dict(traffic=6.156e9, avg_ms=4.429, l2_hit=0.692, valu_busy=0.74),
-    "k_forward<16,0,false,0>": dict(traffic=6.171e9, avg_ms=4.422, l2_hit=0.692, valu_busy=0.75),
-    "k_backward_rec<16,0>": dict(traffic=6.314e9, avg_ms=2.873, l2_hit=0.601, valu_busy=0.37),
+    "k_forward<16,0,false,1>": dict(traffic=6.128e9, avg_ms=4.315, l2_hit=0.692, valu_busy=0.72),
+    "k_forward<16,0,false,0>": dict(traffic=6.136e9, avg_ms=4.364, l2_hit=0.693, valu_busy=0.72),
+    "k_backward_rec<16,0>": dict(traffic=5.738e9, avg_ms=2.265, l2_hit=0.621, valu_busy=0.49),
 }
 # densify and scene-file kernels at 1M splats (profiles/densify_sceneio_1M.md)
 MEASURED_TRAFFIC = {"k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
