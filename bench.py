@@ -13,8 +13,9 @@ cutoff 1e-3, random N(0,1) image-space loss gradient; exact chunk mode.
                  cores, trained first on the intervals the tracer emits for 16
                  other camera poses of the same scene; the measured frame is held
                  out) -> forward capped at the proposed end with exact continuation
-                 past it (same image as plain, bit for bit) -> backward bounded by
-                 the interval the forward emitted (same gradients)
+                 past it (same image as plain, bit for bit) -> backward from the
+                 stored hits (same gradients); the network pass that proposes the
+                 next step's intervals runs on a second stream beside the backward
   proposal_hard: lossy variant in config: proposed start and a hard cap, with
                  fallbacks to the full trace; its PSNR against plain is reported
   proposal_ceiling: each ray's own emitted interval of this frame as (start, cap), trusted
@@ -664,11 +665,12 @@ def _median(xs):
     return 0.5 * (xs[(n - 1) // 2] + xs[n // 2]) if n else float("nan")
 
 
-def timed(fr, launches, steps, warmup, stats=None):
+def timed(fr, launches, steps, warmup, stats=None, tail=None):
     """Run `launches` (list of (name, fn)) as one step: warmup, then `steps` timed steps
     with HIP events around every launch on the launch stream (torch's current stream, which
     every launch here uses). Returns (median ms/step, {name: median ms}); `stats`, if given,
-    receives the mean ms/step over the whole timed region."""
+    receives the mean ms/step over the whole timed region, which ends after `tail()` (work
+    that the steps left on another stream joined to the launch stream)."""
     torch = fr.torch
     for _ in range(warmup):
         for _, fn in launches:
@@ -682,6 +684,8 @@ def timed(fr, launches, steps, warmup, stats=None):
         for j, (_, fn) in enumerate(launches):
             fn()
             evs[i][j + 1].record(fr.stream)
+    if tail is not None:
+        tail()
     stop.record(fr.stream)
     torch.cuda.synchronize()
     per = {name: _median([e[j].elapsed_time(e[j + 1]) for e in evs]) for j, (name, _) in enumerate(launches)}
@@ -832,9 +836,11 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
         propose = lambda: net.forward(fr.ro, fr.rd, fr.prop, capi.PROPOSAL_CAP, ms, me, stream=sp)
         # lossy: proposed start and hard cap, with the fallbacks
         propose2 = lambda: net.forward(fr.ro, fr.rd, fr.prop2, capi.PROPOSAL_INTERVAL, ms, me, stream=sp)
-        # in the timed steps the network runs on a second stream (default priority), after the
-        # step's start, while the forward forms its window records and probes its tiles; the
-        # forward's traversal waits for it (interval_ready_event)
+        # in the timed steps the network runs on a second stream (default priority) beside the
+        # backward and proposes the intervals that the next step's forward reads (the views of a
+        # training schedule are known ahead, so the next view's proposal overlaps this view's
+        # backward); the forward's traversal waits for them (interval_ready_event). Every timed
+        # step runs one network pass, one forward and one backward.
         s2 = torch.cuda.Stream()
         fork, ready, ready2 = torch.cuda.Event(), torch.cuda.Event(), torch.cuda.Event()
         for e in (ready, ready2):
@@ -843,26 +849,31 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
         fa_hard_ev = fr.fwd_args("exact", interval_in=fr.prop2, hard=True, fb_start=ms * ts, fb_end=me * ts,
                                  interval_event=ready2)
 
-        def overlapped(out, mode, ev):
+        def beside_backward(out, mode, ev, bwd):
             def run():
-                fork.record(fr.stream)
+                fork.record(fr.stream)  # after this step's forward, which has read `out`
                 s2.wait_event(fork)
                 net.forward(fr.ro, fr.rd, out, mode, ms, me, stream=s2.cuda_stream)
                 ev.record(s2)
+                bwd()
             return run
 
-        runs["proposal"] = [("mlp", overlapped(fr.prop, capi.PROPOSAL_CAP, ready)),
-                            ("fwd", lambda: fr.h.fwd(fa_soft_ev, sp)), ("zero", fr.zero_grads),
-                            ("bwd", lambda: fr.h.bwd(ba_plain, sp))]
-        runs["proposal_hard"] = [("mlp", overlapped(fr.prop2, capi.PROPOSAL_INTERVAL, ready2)),
-                                 ("fwd", lambda: fr.h.fwd(fa_hard_ev, sp)), ("zero", fr.zero_grads),
-                                 ("bwd", lambda: fr.h.bwd(ba_plain, sp))]
+        runs["proposal"] = [("fwd", lambda: fr.h.fwd(fa_soft_ev, sp)), ("zero", fr.zero_grads),
+                            ("bwd", beside_backward(fr.prop, capi.PROPOSAL_CAP, ready,
+                                                    lambda: fr.h.bwd(ba_plain, sp)))]
+        runs["proposal_hard"] = [("fwd", lambda: fr.h.fwd(fa_hard_ev, sp)), ("zero", fr.zero_grads),
+                                 ("bwd", beside_backward(fr.prop2, capi.PROPOSAL_INTERVAL, ready2,
+                                                         lambda: fr.h.bwd(ba_plain, sp)))]
+        tails = {"proposal": lambda: fr.stream.wait_event(ready),
+                 "proposal_hard": lambda: fr.stream.wait_event(ready2)}
     runs = {k: v for k, v in runs.items() if want(k)}
+    if net is None:
+        tails = {}
     res, images = {}, {}
     for name, launches in runs.items():
         fr.h.set_kernel_timing(True)
         st_ = {}
-        ms_step, per = timed(fr, launches, args.steps, args.warmup, st_)
+        ms_step, per = timed(fr, launches, args.steps, args.warmup, st_, tails.get(name))
         # the traversal kernels alone (library events around them, over the timed steps)
         kf, kb = fr.h.kernel_times("fwd", args.steps), fr.h.kernel_times("bwd", args.steps)
         fr.h.set_kernel_timing(False)
@@ -961,13 +972,14 @@ def frame_benchmark(args, dev, scene, cam_eye, label):
     cfg["traversal_kernel_ms"] = {k: {"fwd": round(v["k_fwd"], 4), "bwd": round(v["k_bwd"], 4)} for k, v in res.items()}
     if net is not None and "proposal" in res:
         mlp_flops = 2 * 14848 * R
-        # the network alone, on the launch stream (in the timed step it overlaps the forward's passes)
+        # the network alone, on the launch stream (in the timed step it runs beside the backward)
         mlp_ms, _ = timed(fr, [("mlp", propose)], args.steps, args.warmup)
         tfl = mlp_flops / (mlp_ms * 1e-3) / 1e12
         # training: forward + backward ~ 3x the forward's MACs per ray (dW and dH per layer)
         train_flops = 3 * 2 * 14848 * args.proposal_batch * args.proposal_steps
         ttf = train_flops / (pinfo["train_ms"] * 1e-3) / 1e12
         cfg.update(mlp_ms=round(mlp_ms, 4), mlp_overlapped_in_step=True,
+                   mlp_schedule="the next view's proposal on a second stream beside this view's backward",
                    mlp_tflops=round(tfl, 2), mlp_mfma_frac=round(tfl / BF16_PEAK_TFLOPS, 5),
                    mlp_train_tflops=round(ttf, 3), mlp_train_mfma_frac=round(ttf / BF16_PEAK_TFLOPS, 6),
                    proposal_training=pinfo)

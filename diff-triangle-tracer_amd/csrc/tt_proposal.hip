@@ -118,20 +118,56 @@ __device__ __forceinline__ void store_relu(const f32x4 acc[NT], bf16 *H, int sh,
 constexpr int SX = kInPad + 8, SH = kHid + 8;  // LDS row strides (bf16), 16-byte multiples
 
 // ---------------------------------------------------------------- inference
-// One wave per block, kInferTiles tiles of 16 rays per block. The kernel is latency-bound
-// (each layer waits on its weight fragments and on LDS): one tile per block, the most
-// waves in flight, measured 0.26 ms for 2M rays against 0.40 ms at 4 tiles and 0.43 at 16.
-constexpr int kInferTiles = 1;
-__global__ __launch_bounds__(64) void k_prop_infer(const float *ro, const float *rd, int R, const bf16x8 *F,
-                                                   const float *W, Hyper hp, float ms, float me, int mode,
-                                                   float *out) {
+// Forward of one dense layer for 16 rays (rows 0..15 of Ain) with the layer's weight
+// fragments and this lane's bias values held in registers: acc = bias + A * W^T.
+template <int KT, int NT>
+__device__ __forceinline__ void dense_reg(const bf16 *Ain, int sin, const bf16x8 (&w)[KT * NT], const float (&b)[NT],
+                                          f32x4 acc[NT]) {
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) acc[nt] = f32x4{b[nt], b[nt], b[nt], b[nt]};
+#pragma unroll
+  for (int kt = 0; kt < KT; kt++) {
+    const bf16x8 a = lds_a(Ain, sin, 0, 32 * kt);
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[nt] = mfma(a, w[kt * NT + nt], acc[nt]);
+  }
+}
+template <int NT>
+__device__ __forceinline__ void load_bias(const float *bias, int nvalid, float (&b)[NT]) {
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) {
+    const int col = 16 * nt + (threadIdx.x & 15);
+    b[nt] = col < nvalid ? bias[col] : 0.0f;
+  }
+}
+
+// Persistent waves (one per block, two per SIMD): each wave loads the network's 34 weight
+// fragments (136 registers a lane) and its bias values once and then runs 16-ray tiles
+// until the rays are done, so the layers read only LDS (the activations) and registers.
+// The products, their order and the roundings are those of the training kernel's forward.
+constexpr int kInferWaves = 2048;  // 256 CUs x 4 SIMDs x 2 waves
+__global__ __launch_bounds__(64, 2) void k_prop_infer(const float *ro, const float *rd, int R, const bf16x8 *F,
+                                                      const float *W, Hyper hp, float ms, float me, int mode,
+                                                      float *out) {
   __shared__ __attribute__((aligned(16))) bf16 X[16 * SX];
   __shared__ __attribute__((aligned(16))) bf16 Ha[16 * SH];
   __shared__ __attribute__((aligned(16))) bf16 Hb[16 * SH];
   const int l = threadIdx.x;
-  for (int sub = 0; sub < kInferTiles; sub++) {
-    const long long base = ((long long)blockIdx.x * kInferTiles + sub) * 16;
-    if (base >= R) break;  // uniform across the block
+  bf16x8 w1[16], w2[8], w3[8], w4[2];
+#pragma unroll
+  for (int i = 0; i < 16; i++) w1[i] = frag(F, fL1 + i);
+#pragma unroll
+  for (int i = 0; i < 8; i++) w2[i] = frag(F, fL2 + i), w3[i] = frag(F, fL3 + i);
+#pragma unroll
+  for (int i = 0; i < 2; i++) w4[i] = frag(F, fL4 + i);
+  float b1[4], b2[4], b3[4], b4[1];
+  load_bias(W + oB1, kHid, b1);
+  load_bias(W + oB2, kHid, b2);
+  load_bias(W + oB3, kHid, b3);
+  load_bias(W + oB4, kOut, b4);
+  const long long tiles = ((long long)R + 15) / 16;
+  for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const long long base = tile * 16;
     {  // encoding: lane l fills features 32 g .. 32 g + 31 of ray l % 16, g = l / 16 -- feature()
        // unrolled: feature 32 g + i is raw input i (g = 0, i < 6) or a sinusoid of input
        // 2g - 1 (i < 6), 2g (6 <= i < 22) or 2g + 1 (i >= 22), octave ((i + 10) & 15) / 2,
@@ -166,17 +202,17 @@ __global__ __launch_bounds__(64) void k_prop_infer(const float *ro, const float 
     }
     __syncthreads();
     f32x4 acc[4];
-    dense<4, 4>(X, SX, 0, F, fL1, W + oB1, kHid, acc);
+    dense_reg<4, 4>(X, SX, w1, b1, acc);
     store_relu<4>(acc, Ha, SH, 0);
     __syncthreads();
-    dense<2, 4>(Ha, SH, 0, F, fL2, W + oB2, kHid, acc);
+    dense_reg<2, 4>(Ha, SH, w2, b2, acc);
     store_relu<4>(acc, Hb, SH, 0);
     __syncthreads();
-    dense<2, 4>(Hb, SH, 0, F, fL3, W + oB3, kHid, acc);
+    dense_reg<2, 4>(Hb, SH, w3, b3, acc);
     store_relu<4>(acc, Ha, SH, 0);
     __syncthreads();
     f32x4 y[1];
-    dense<2, 1>(Ha, SH, 0, F, fL4, W + oB4, kOut, y);
+    dense_reg<2, 1>(Ha, SH, w4, b4, y);
     // lane l holds y[row 4(l/16)+i][col l%16]; columns 0 and 1 are the outputs
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -540,11 +576,11 @@ int tt_proposal_create(tt_proposal **out, const tt_proposal_hparams *hp, unsigne
 
 int tt_proposal_destroy(tt_proposal *p) {
   if (!p) return TT_OK;
-  hipFree(p->W);
-  hipFree(p->m);
-  hipFree(p->v);
-  hipFree(p->F);
-  hipFree(p->partials);
+  (void)hipFree(p->W);
+  (void)hipFree(p->m);
+  (void)hipFree(p->v);
+  (void)hipFree(p->F);
+  (void)hipFree(p->partials);
   delete p;
   return TT_OK;
 }
@@ -576,8 +612,9 @@ int tt_proposal_forward(tt_proposal *p, const float *ray_o_dev, const float *ray
   PARG(mode == TT_PROPOSAL_RAW || mode == TT_PROPOSAL_INTERVAL || mode == TT_PROPOSAL_CAP, "unknown mode");
   if (R == 0) return TT_OK;
   PARG(ray_o_dev && ray_d_dev && out_dev, "NULL buffer");
-  PARG(R <= (1ll << 31) - 16 * kInferTiles, "too many rays");
-  const long long blocks = (R + 16 * kInferTiles - 1) / (16 * kInferTiles);
+  PARG(R <= (1ll << 31) - 16, "too many rays");
+  const long long tiles = (R + 15) / 16;
+  const long long blocks = tiles < kInferWaves ? tiles : kInferWaves;
   hipLaunchKernelGGL(k_prop_infer, dim3((unsigned)blocks), dim3(64), 0, static_cast<hipStream_t>(stream), ray_o_dev,
                      ray_d_dev, (int)R, reinterpret_cast<const bf16x8 *>(p->F), p->W, hyper(p), margin_start,
                      margin_end, mode, out_dev);

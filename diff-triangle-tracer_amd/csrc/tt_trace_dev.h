@@ -1572,7 +1572,8 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
 #pragma unroll 2
         for (int j = b; j < e; j++) add_record(P, first + list[j], acc, sh);
       }
-      for (int q = 0; q < BLOCK / kStageSlots; q++) {
+      const int live = min(BLOCK, n_items - i0);  // work items of this pass (the rest of the rows are empty)
+      for (int q = 0; q * kStageSlots < live; q++) {
         if (threadIdx.x / kStageSlots == q) stage_sums(stage + 64 * (threadIdx.x % kStageSlots), prim, vrow, acc, sh);
         __syncthreads();
         flush_stage<BLOCK>(P, stage);
