@@ -127,7 +127,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_forward_multi(TraceParams P, Multi
 template <int K, int BLOCK>
 __global__ __launch_bounds__(BLOCK, 2) void k_backward_multi(TraceParams P, MultiCams mc, int *heads, int G) {
   __shared__ int s_node[kLdsInts * BLOCK];
-  __shared__ int s_rec[kRecLds + rec_batch<BLOCK>()];
+  __shared__ int s_rec[rec_lds_ints<BLOCK>()];
   __shared__ int s_tab[rec_tab_ints<BLOCK>()];
   __shared__ int s_range[2];
   while (true) {

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(1024) void k_tile_order(const float *cost, int n, i
 template <int K, int VARIANT>
 __global__ __launch_bounds__(kBlockB, 2) void k_backward_rec(TraceParams P) {
   __shared__ int s_node[kLdsInts * kBlockB];
-  __shared__ int s_rec[kRecLds + rec_batch<kBlockB>()];
+  __shared__ int s_rec[rec_lds_ints<kBlockB>()];
   __shared__ int s_tab[rec_tab_ints<kBlockB>()];
   int x, y;
   const int tile = P.tile_sched ? P.tile_sched[blockIdx.x] : xcd_tile(P.n_tiles);

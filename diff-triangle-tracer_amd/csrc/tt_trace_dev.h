@@ -961,16 +961,26 @@ struct GlobalSink {
 template <class Sink>
 TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, Sink &sink) {
   v3f g[3] = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
-  for (int k = 0; k < 3; k++) {
-    if (fabsf(w.ed[k] - w.phi_max) > 1e-6f) continue;
+  // The edges at the window's maximum (one, or two or three at ties), in ascending order.
+  // Each pass takes a lane's next such edge, with the vertices rotated so that it is edge
+  // (p1, p2): one copy of the Jacobian code runs per pass, however the edges differ across
+  // the wave's lanes (an unrolled loop over the three edges ran every edge's copy that any
+  // lane needed). The sums into g keep the ascending-edge order.
+  unsigned pend = 0;
+  for (int k = 0; k < 3; k++) pend |= (fabsf(w.ed[k] - w.phi_max) > 1e-6f ? 0u : 1u) << k;
+  const v3f V0 = V[0], V1 = V[1], V2 = V[2], R0 = w.rel[0], R1 = w.rel[1], R2 = w.rel[2];
+  auto pick = [](v3f a0, v3f a1, v3f a2, int i) { return i == 0 ? a0 : (i == 1 ? a1 : a2); };
+  while (pend) {
+    const int k = __builtin_ctz(pend);
+    pend &= pend - 1;
     const int k1 = k == 2 ? 0 : k + 1, k2 = k == 0 ? 2 : k - 1;
-    const v3f p1 = V[k], p2 = V[k1], p3 = V[k2];
+    const v3f p1 = pick(V0, V1, V2, k), p2 = pick(V0, V1, V2, k1), p3 = pick(V0, V1, V2, k2);
     const v3f n = cross(p1 - p3, p2 - p3);
     const v3f ne = cross(n, p1 - p2);
     const v3f Ne = normalize(ne);
     float J[9];
     dnormvdv(Ne, ne, J);
-    const v3f dL_dNe = dL_dphi * w.rel[k];
+    const v3f dL_dNe = dL_dphi * pick(R0, R1, R2, k);
     const v3f dL_dne = vecmat(dL_dNe, J);
     float j1[9], j2[9], j3[9];
     dne_dp1(p1, p2, p3, j1);
@@ -979,13 +989,7 @@ TT_D void window_bwd(const v3f V[3], const Win &w, float dL_dphi, v3f dL_dn3, Si
     const v3f g1 = vecmat(dL_dne, j1) - dL_dphi * Ne;
     const v3f g2 = vecmat(dL_dne, j2);
     const v3f g3 = vecmat(dL_dne, j3);
-    if (k == 0) {
-      g[0] = g[0] + g1, g[1] = g[1] + g2, g[2] = g[2] + g3;
-    } else if (k == 1) {
-      g[1] = g[1] + g1, g[2] = g[2] + g2, g[0] = g[0] + g3;
-    } else {
-      g[2] = g[2] + g1, g[0] = g[0] + g2, g[1] = g[1] + g3;
-    }
+    for (int j = 0; j < 3; j++) g[j] = g[j] + (j == k ? g1 : (j == k1 ? g2 : g3));
   }
   if (dL_dn3.x != 0.0f || dL_dn3.y != 0.0f || dL_dn3.z != 0.0f) {
     const v3f p1 = V[0], p2 = V[1], p3 = V[2];
@@ -1049,10 +1053,15 @@ TT_D void sh_bwd(int deg, v3f dir0, bool c0, bool c1, bool c2, const float dc[3]
   }
 }
 
-// Gradient record of one blended hit, 5 x float4 = 80 B:
+// Gradient record of one blended hit. Records that the emitting block sums itself are
+// 4 x float4 = 64 B, with the ray's direction kept once per lane in the block's LDS (rec_dirs):
+//   (prim, clamp bits | lane << 3, dL/dc[0], dL/dc[1]) (dL/dc[2], vertex 0..2) (vertex 3..6)
+//   (vertex 7, vertex 8, opacity, sigma)
+// Records summed over the whole launch after a sort by triangle (the bounce secondaries)
+// carry the direction, 5 x float4 = 80 B:
 //   (prim, clamp bits, d.x, d.y) (d.z, dL/dc[3]) (vertex 0..3) (vertex 4..7) (vertex 8, opacity, sigma, 0)
 // The SH gradient (48 values) is rebuilt from the direction, dL/dcolour and the clamp
-// bits when the block aggregates its records. A block writes its records into its own
+// bits when the records are summed. A block writes its records into its own
 // region of P.rec_rows, rows [s_rec[1], s_rec[1] + s_rec[2]); the row index comes from one
 // LDS atomic per wave (counter s_rec[0]). A one-shot backward that follows a forward of the
 // same image sizes each tile's region to that tile's blended-hit count in the forward
@@ -1067,13 +1076,22 @@ template <int BLOCK>
 constexpr int agg_slots() {  // triangles a block groups one batch of records into
   return 2 * BLOCK;
 }
+constexpr int kRowLocal = 4, kRowGlobal = 5;  // float4 per record: block-summed / launch-summed
+template <int BLOCK>
+constexpr int rec_lds_ints() {  // s_rec: header, the first batch's triangle ids, the lanes' ray directions
+  return kRecLds + rec_batch<BLOCK>() + 3 * BLOCK;
+}
+template <int BLOCK>
+TT_D float *rec_dirs(int *s_rec) {
+  return reinterpret_cast<float *>(s_rec + kRecLds + rec_batch<BLOCK>());
+}
 constexpr int kStageSlots = 32;  // triangles whose sums are staged in LDS per coalesced flush
 constexpr int kSegRecords = 16;  // records one thread sums at most (a triangle's longer runs are split)
 template <int BLOCK>
 constexpr int rec_tab_ints() {  // s_tab of backward_block_rec: grouping table + flush stage
   return 5 * agg_slots<BLOCK>() + 64 * kStageSlots;
 }
-template <int BLOCK>
+template <int BLOCK, bool LOCAL>
 TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const float dc[3], bool c0, bool c1,
                       bool c2, const HitSink &h, int *s_rec) {
   const unsigned long long m = __ballot(1);
@@ -1086,13 +1104,23 @@ TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const
   if (idx < s_rec[2]) {
     if (idx < rec_batch<BLOCK>()) s_rec[kRecLds + idx] = (int)prim;  // the grouping's first batch reads it here
     if (P.rec_keys) P.rec_keys[(size_t)s_rec[1] + idx] = prim;
-    float4 *row = P.rec_rows + 5 * ((size_t)s_rec[1] + idx);
     const int cl = (c0 ? 1 : 0) | (c1 ? 2 : 0) | (c2 ? 4 : 0);
-    row[0] = make_float4(__uint_as_float(prim), __int_as_float(cl), d.x, d.y);
-    row[1] = make_float4(d.z, dc[0], dc[1], dc[2]);
-    row[2] = make_float4(h.v[0], h.v[1], h.v[2], h.v[3]);
-    row[3] = make_float4(h.v[4], h.v[5], h.v[6], h.v[7]);
-    row[4] = make_float4(h.v[8], h.v[9], h.v[10], 0.0f);
+    if (LOCAL) {  // every record of this lane in this launch has this direction
+      float *sd = rec_dirs<BLOCK>(s_rec) + 3 * threadIdx.x;
+      sd[0] = d.x, sd[1] = d.y, sd[2] = d.z;
+      float4 *row = P.rec_rows + kRowLocal * ((size_t)s_rec[1] + idx);
+      row[0] = make_float4(__uint_as_float(prim), __int_as_float(cl | (int)threadIdx.x << 3), dc[0], dc[1]);
+      row[1] = make_float4(dc[2], h.v[0], h.v[1], h.v[2]);
+      row[2] = make_float4(h.v[3], h.v[4], h.v[5], h.v[6]);
+      row[3] = make_float4(h.v[7], h.v[8], h.v[9], h.v[10]);
+    } else {
+      float4 *row = P.rec_rows + kRowGlobal * ((size_t)s_rec[1] + idx);
+      row[0] = make_float4(__uint_as_float(prim), __int_as_float(cl), d.x, d.y);
+      row[1] = make_float4(d.z, dc[0], dc[1], dc[2]);
+      row[2] = make_float4(h.v[0], h.v[1], h.v[2], h.v[3]);
+      row[3] = make_float4(h.v[4], h.v[5], h.v[6], h.v[7]);
+      row[4] = make_float4(h.v[8], h.v[9], h.v[10], 0.0f);
+    }
   } else {
     GlobalSink g{P.dverts + 3 * (size_t)off, P.dopac + prim, P.dsigma + prim, P.dsh + (size_t)prim * P.M * 3};
     for (int f = 0; f < 11; f++) g.add(f, h.v[f]);
@@ -1100,9 +1128,23 @@ TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const
   }
 }
 
-// Adds record row r (an absolute row of P.rec_rows) to register sums; returns its triangle.
-TT_D uint32_t add_record(const TraceParams &P, size_t r, float acc[11], ShSink &sh) {
-  const float4 *row = P.rec_rows + 5 * r;
+// Adds record r (an absolute record of P.rec_rows) to register sums; returns its triangle.
+// LOCAL: a block-summed record, whose direction is in `dirs` (the block's rec_dirs).
+template <bool LOCAL>
+TT_D uint32_t add_record(const TraceParams &P, size_t r, float acc[11], ShSink &sh, const float *dirs = nullptr) {
+  if (LOCAL) {
+    const float4 *row = P.rec_rows + kRowLocal * r;
+    const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+    const int cl = __float_as_int(r0.y);
+    const float *sd = dirs + 3 * (cl >> 3);
+    const float dc[3] = {r0.z, r0.w, r1.x};
+    acc[0] += r1.y, acc[1] += r1.z, acc[2] += r1.w;
+    acc[3] += r2.x, acc[4] += r2.y, acc[5] += r2.z, acc[6] += r2.w;
+    acc[7] += r3.x, acc[8] += r3.y, acc[9] += r3.z, acc[10] += r3.w;
+    sh_bwd(P.deg, mk(sd[0], sd[1], sd[2]), (cl & 1) != 0, (cl & 2) != 0, (cl & 4) != 0, dc, sh);
+    return __float_as_uint(r0.x);
+  }
+  const float4 *row = P.rec_rows + kRowGlobal * r;
   const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4];
   const int cl = __float_as_int(r0.y);
   const float dc[3] = {r1.y, r1.z, r1.w};
@@ -1162,7 +1204,7 @@ TT_D void flush_sums(const TraceParams &P, uint32_t prim, const float acc[11], c
 // bounce variant's primary (per-hit colours extended by rho L_r + (1 - rho) L_t, with the
 // reflectivity gradient); 2: a bounce secondary ray (colour only). Returns the segment's
 // final transmittance.
-template <int K, int BLOCK, int MODE, bool WREC = false>
+template <int K, int BLOCK, int MODE, bool WREC = false, bool LOCAL = true>
 TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], const float g[3], float dL_ddepth,
                         v3f dL_dn2, float final_depth, v3f final_normal, const float *Lr, const float *Lt, float tlo,
                         long long plo, float far_t, int *s_node, int *s_rec,
@@ -1249,7 +1291,7 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
     hs.v[F_OPAC] = dL_dalpha * w.Cx;
     if (w.phi_final > 0.0f) hs.v[F_SIGMA] = (dL_dC * w.Cx) * log_tt(w.phi_final);
     window_bwd(V, w, dL_dphi, dL_dn3, hs);
-    emit_record<BLOCK>(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_rec);
+    emit_record<BLOCK, LOCAL>(P, prim, off, d, dL_dcolor, c0, c1, c2, hs, s_rec);
     T = testT;
     return true;
   };
@@ -1443,7 +1485,7 @@ TT_D void backward_ray(const TraceParams &P, int pix, int *s_node, int *s_rec, f
         const v3f so = s == 0 ? mk(a[8], a[9], a[10]) : mk(a[14], a[15], a[16]);
         const v3f sd = s == 0 ? dr : mk(a[17], a[18], a[19]);
         const float gs[3] = {wgt * dL_dpixel[0], wgt * dL_dpixel[1], wgt * dL_dpixel[2]};
-        const float Ts = backward_seg<K, BLOCK, 2, WREC>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
+        const float Ts = backward_seg<K, BLOCK, 2, WREC, VARIANT != 8>(P, so, sd, s == 0 ? Lr : Lt, gs, 0.0f, zero, 0.0f, zero, nullptr,
                                                    nullptr, bounce_start(P), -1, P.far_t, s_node, s_rec, hls[1 + s],
                                                    nhl[1 + s], P.hl_stride, hlcs[1 + s]);
         const float gse[3] = {Ts * gs[0], Ts * gs[1], Ts * gs[2]};
@@ -1471,6 +1513,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
                              int *s_tab) {
   constexpr int kBatch = rec_batch<BLOCK>(), kAggSlots = agg_slots<BLOCK>();
   static_assert(kLdsInts * BLOCK * 4 >= 2 * kBatch * 2, "the grouping reuses the traversal LDS");
+  static_assert(VARIANT != 4, "block-summed records hold one direction per lane: one segment kind per launch");
   int *t_keys = s_tab, *t_cnt = s_tab + kAggSlots, *t_off = s_tab + 2 * kAggSlots, *t_fill = s_tab + 3 * kAggSlots;
   int *t_item = s_tab + 4 * kAggSlots;  // per slot: its first work item (segment of <= kSegRecords records)
   float *stage = reinterpret_cast<float *>(s_tab + 5 * kAggSlots);
@@ -1509,7 +1552,8 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
       __syncthreads();
     }
     for (int i = threadIdx.x; i < n; i += BLOCK) {
-      const uint32_t prim = b0 == 0 ? (uint32_t)s_rec[kRecLds + i] : __float_as_uint(P.rec_rows[5 * (first + i)].x);
+      const uint32_t prim =
+          b0 == 0 ? (uint32_t)s_rec[kRecLds + i] : __float_as_uint(P.rec_rows[kRowLocal * (first + i)].x);
       const int sl = claim_slot<kAggSlots>(t_keys, prim);
       slot_of[i] = (short)sl;
       if (sl >= 0) atomicAdd(&t_cnt[sl], 1);
@@ -1547,7 +1591,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
         float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         ShSink sh;
         for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
-        const uint32_t prim = add_record(P, first + i, acc, sh);
+        const uint32_t prim = add_record<true>(P, first + i, acc, sh, rec_dirs<BLOCK>(s_rec));
         flush_sums(P, prim, acc, sh);
       }
     }
@@ -1570,7 +1614,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
         vrow = P.cumsum[prim];
         const int b = t_off[lo] + (it - t_item[lo]) * kSegRecords, e = min(t_off[lo] + t_cnt[lo], b + kSegRecords);
 #pragma unroll 2
-        for (int j = b; j < e; j++) add_record(P, first + list[j], acc, sh);
+        for (int j = b; j < e; j++) add_record<true>(P, first + list[j], acc, sh, rec_dirs<BLOCK>(s_rec));
       }
       const int live = min(BLOCK, n_items - i0);  // work items of this pass (the rest of the rows are empty)
       for (int q = 0; q * kStageSlots < live; q++) {
@@ -1606,7 +1650,7 @@ __global__ __launch_bounds__(64) void k_rec_reduce_sorted(TraceParams P, const u
     ShSink sh;
     for (int f = 0; f < 11; f++) acc[f] = 0.0f;
     for (int f = 0; f < 48; f++) sh.v[f] = 0.0f;
-    if (key != 0xffffffffu) add_record(P, rows[j], acc, sh);
+    if (key != 0xffffffffu) add_record<false>(P, rows[j], acc, sh);
     float *mine = s_v + lane * kRedRow;
     for (int f = 0; f < 48; f++) mine[f] = sh.v[f];
     for (int f = 0; f < 11; f++) mine[48 + f] = acc[f];
