@@ -243,7 +243,7 @@ TT_D bool intersect(const RayPre &r, v3f v0, v3f v1, v3f v2, float &t) {
   if (det == 0.0f) return false;
   float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
   float T = U * Az + V * Bz + W * Cz;
-  t = T / det;
+  t = T / det + 0.0f;  // -0.0 as +0.0: hit keys compare t as unsigned bits
   return true;
 }
 
@@ -251,9 +251,6 @@ TT_D bool intersect(const RayPre &r, v3f v0, v3f v1, v3f v2, float &t) {
 // (bitwise | and &: no short-circuit branches in the compiled code)
 TT_D bool after_key(float t, uint32_t prim, float tlo, long long plo) {
   return (t > tlo) | ((t == tlo) & ((long long)prim > plo));
-}
-TT_D bool hit_less(float t, uint32_t p, float bt, uint32_t bp) {
-  return (t < bt) | ((t == bt) & (p < bp));
 }
 
 // Conservative slab test: boxes are padded at build time and the interval is

@@ -48,41 +48,43 @@ struct TileShape {
 
 template <int K>
 struct KBuf {
-  float t[K];
-  uint32_t p[K];
+  uint64_t k[K];
+  TT_D float t(int i) const { return __uint_as_float((uint32_t)(k[i] >> 32)); }
+  TT_D uint32_t p(int i) const { return (uint32_t)k[i]; }
 };
+TT_D uint64_t hit_key(float t, uint32_t p) { return ((uint64_t)__float_as_uint(t) << 32) | p; }
 
 template <int K>
 TT_D void kinit(KBuf<K> &b) {
 #pragma unroll
-  for (int i = 0; i < K; i++) b.t[i] = kInfDist, b.p[i] = kInvalid;
+  for (int i = 0; i < K; i++) b.k[i] = hit_key(kInfDist, kInvalid);
 }
 
 // Sorted insertion (the anyhit compare-and-swap chain of closestk.h:111-147) with a
-// deterministic (t, prim) order instead of traversal order for equal t.
+// deterministic (t, prim) order instead of traversal order for equal t. A key is the 64-bit
+// (t bits, prim): for t >= 0 (every gathered hit: intersect returns -0.0 as +0.0 and the
+// gathers start at t >= 0) unsigned key order is the (t, prim) order.
 // Slot i takes slot i-1 if the key is less than slot i-1, the key if it is less than slot
 // i only, and keeps its value otherwise; slots are rewritten from the last down, each from
 // the old contents, so the steps do not depend on one another.
 template <int K>
 TT_D void kinsert(KBuf<K> &b, float t, uint32_t p) {
-  bool lt = hit_less(t, p, b.t[K - 1], b.p[K - 1]);  // key < slot i
+  const uint64_t key = hit_key(t, p);
+  bool lt = key < b.k[K - 1];  // key < slot i
   if (!lt) return;
 #pragma unroll
   for (int i = K - 1; i > 0; i--) {
-    const bool lt_prev = hit_less(t, p, b.t[i - 1], b.p[i - 1]);
-    b.t[i] = lt_prev ? b.t[i - 1] : (lt ? t : b.t[i]);
-    b.p[i] = lt_prev ? b.p[i - 1] : (lt ? p : b.p[i]);
+    const bool lt_prev = key < b.k[i - 1];
+    b.k[i] = lt_prev ? b.k[i - 1] : (lt ? key : b.k[i]);
     lt = lt_prev;
   }
-  b.t[0] = lt ? t : b.t[0];
-  b.p[0] = lt ? p : b.p[0];
+  b.k[0] = lt ? key : b.k[0];
 }
 template <int K>
 TT_D void kpop(KBuf<K> &b) {
 #pragma unroll
-  for (int i = 0; i < K - 1; i++) b.t[i] = b.t[i + 1], b.p[i] = b.p[i + 1];
-  b.t[K - 1] = kInfDist;
-  b.p[K - 1] = kInvalid;
+  for (int i = 0; i < K - 1; i++) b.k[i] = b.k[i + 1];
+  b.k[K - 1] = hit_key(kInfDist, kInvalid);
 }
 
 // Optional traversal counters (debug/profiling): nodes visited and triangles tested.
@@ -228,7 +230,7 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       float4 tn, tf;
       const int4 ref = qref;
       slab4q(r, np, qa, qx, qy, qz, tn, tf);
-      const float thi = fminf(far_t, b.t[K - 1]);
+      const float thi = fminf(far_t, b.t(K - 1));
       const float hi_w = thi + fabsf(thi) * kWideRel + kWideAbs;
       const float tn0 = tn.x, tf0 = tf.x, tn1 = tn.y, tf1 = tf.y, tn2 = tn.z, tf2 = tf.z, tn3 = tn.w, tf3 = tf.w;
       bool h0 = (ref.x != kEmptyChild) & box_ok_wide(tn0, tf0, lo_w, hi_w);
@@ -480,7 +482,7 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
     gather<K, BLOCK, COUNT>(P, r, tlo, plo, capped ? cap : P.far_t, b, s_node, tc);
     if (probe) {
       probe = false;
-      if (b.p[0] == kInvalid || b.t[0] < tstart + P.ival_fb_start) {  // the proposal may have missed: full trace
+      if (b.p(0) == kInvalid || b.t(0) < tstart + P.ival_fb_start) {  // the proposal may have missed: full trace
         tlo = 0.0f;
         plo = -1;
         capped = hard = false;
@@ -495,8 +497,8 @@ TT_D void trace_segment(const TraceParams &P, v3f o, v3f d, SegState &st, int ra
     uint32_t lp = 0;
     bool spent = false;
     for (int i = 0; i < K; i++) {
-      const float t = b.t[0];
-      const uint32_t prim = b.p[0];
+      const float t = b.t(0);
+      const uint32_t prim = b.p(0);
       kpop(b);
       if (prim == kInvalid) {
         spent = true;
@@ -646,7 +648,7 @@ TT_D int forward_pixel(const TraceParams &P, int pix, int *s_node, float *loss =
     float tstart = 0.0f, cap = kInfDist;
     if ((IVAL & kIvalIn) && P.ival_in) {
       const float2 iv = P.ival_in[pix];
-      tstart = iv.x;
+      tstart = fmaxf(iv.x, 0.0f);  // a start before the origin is the full trace (hit keys need t >= 0)
       cap = iv.y;
     }
     trace_segment<K, BLOCK, true, COUNT, false, IVAL, WREC>(P, o, d, st, pix, o, s_node, tstart, cap, 0);
@@ -1337,8 +1339,8 @@ TT_D float backward_seg(const TraceParams &P, v3f o, v3f d, const float fin[3], 
     float lt = 0.0f;
     uint32_t lp = 0;
     for (int i = 0; i < K; i++) {
-      const float t = b.t[0];
-      const uint32_t prim = b.p[0];
+      const float t = b.t(0);
+      const uint32_t prim = b.p(0);
       kpop(b);
       if (prim == kInvalid) return T;
       lt = t;

@@ -119,8 +119,9 @@ typedef struct {
   /* Depth intervals (no counterpart in the reference; TT_VARIANT_FORWARD, TT_CHUNK_EXACT).
      interval_in: optional [R][2] (start, cap) per ray. The gather starts at t >= start; the
      first gathers stop at cap and go on past it only while the ray is still open, so the
-     result equals a trace that starts at `start` (start <= near_t: the full trace; a cap
-     below start is ignored; a start beyond far_t: an empty trace, in both modes below).
+     result equals a trace that starts at `start` (start <= near_t: the full trace; a start
+     below 0 is taken as 0; a cap below start is ignored; a start beyond far_t: an empty
+     trace, in both modes below).
      interval_out: optional [R][4] per ray: t of the first and of the last blended hit, t of
      the hit that ended the ray (inf if none), first blended triangle id (int bits); the
      first field is inf for a ray with no blended hit. */

@@ -537,7 +537,7 @@ inline bool intersect(const RayPre &r, V3f v0, V3f v1, V3f v2, float &t) {
   if (det == 0.0f) return false;
   float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
   float T = U * Az + V * Bz + W * Cz;
-  t = T / det;
+  t = T / det + 0.0f;  // -0.0 as +0.0, as the kernels' hit keys need
   return true;
 }
 
@@ -1226,7 +1226,7 @@ int or_forward_f32(void *scene, const float *o, const float *d, int R, const or_
         trace_segment_f32(s, op, so, sd, Cs, *Ts, N, consumed, blended, nullptr, nullptr);
       else
         trace_segment_f32(s, op, so, sd, C, T, N, consumed, blended, op.variant == 0 ? &sk : nullptr, nullptr,
-                          op.tstart ? op.tstart[pix] : 0.0f);
+                          op.tstart ? std::fmax(op.tstart[pix], 0.0f) : 0.0f);
     };
     if (op.variant == 4) {
       BounceAcc acc;
@@ -1394,7 +1394,7 @@ void *or_record(void *scene, const float *o, const float *d, int R, const or_opt
       sg.weight = wgt;
       float Cl[3] = {0, 0, 0}, Tl = 1.0f, Nl[3] = {0, 0, 0};
       trace_segment_f32(s, op, so, sd, Cl, Tl, Nl, consumed, blended, nullptr, &sg.hits,
-                        (!env && op.tstart) ? op.tstart[pix] : 0.0f);
+                        (!env && op.tstart) ? std::fmax(op.tstart[pix], 0.0f) : 0.0f);
       if (env) {
         for (int c = 0; c < 3; c++) Cs[c] = Cl[c];
         *Ts = Tl;

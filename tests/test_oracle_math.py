@@ -234,6 +234,16 @@ def test_intersection_watertight_on_shared_edges():
             assert any(h is not None for h in hits)
 
 
+def test_intersection_at_the_origin_is_plus_zero():
+    # origin on the triangle's plane, inside it: t = 0 / det, whose sign follows the winding;
+    # both windings report +0.0 (hit keys order t as unsigned bits, -0.0 would sort last)
+    V = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
+    for tri in (V, V[::-1].copy()):
+        for d in ([0, 0, 1], [0.1, 0.2, -1]):
+            t = O.intersect([0.0, 0.0, 0.0], d, *tri)
+            assert t is not None and float(t) == 0.0 and not np.signbit(np.float32(t))
+
+
 def test_intersection_double_sided_and_parallel():
     V = np.array([[0, 0, 1], [1, 0, 1], [0, 1, 1]], np.float32)
     t1 = O.intersect([0.2, 0.2, 0], [0, 0, 1], *V)
