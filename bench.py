@@ -58,13 +58,13 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 # Counters of the default workload's traversal kernels (1M splats, 1920x1080, k=16), copied from
 # the rocprofv3 runs summarised in PROFILE_SOURCE (they cannot be read inside this process):
 # traffic = HBM bytes per launch, 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE (factors from a 256 MiB
-# copy); avg_ms = the profiler's average duration; l2_hit = TCC_HIT / (TCC_HIT + TCC_MISS);
+# copy); avg_ms = the profiler's median duration over the timed steps; l2_hit = TCC_HIT / (TCC_HIT + TCC_MISS);
 # valu_busy = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x GRBM_GUI_ACTIVE / XCDs).
 PROFILE_SOURCE = "profiles/trace_fwd_bwd_1M_1080p.md"
 MEASURED = {
-    "k_forward<16,0,false,1>": dict(traffic=6.128e9, avg_ms=4.315, l2_hit=0.692, valu_busy=0.72),
-    "k_forward<16,0,false,0>": dict(traffic=6.136e9, avg_ms=4.364, l2_hit=0.693, valu_busy=0.72),
-    "k_backward_rec<16,0>": dict(traffic=5.738e9, avg_ms=2.265, l2_hit=0.621, valu_busy=0.49),
+    "k_forward<16,0,false,1>": dict(traffic=6.169e9, avg_ms=4.312, l2_hit=0.692, valu_busy=0.71),
+    "k_forward<16,0,false,0>": dict(traffic=6.132e9, avg_ms=4.325, l2_hit=0.693, valu_busy=0.71),
+    "k_backward_rec<16,0>": dict(traffic=4.877e9, avg_ms=2.067, l2_hit=0.602, valu_busy=0.43),
 }
 # densify and scene-file kernels at 1M splats (profiles/densify_sceneio_1M.md)
 MEASURED_TRAFFIC = {"k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
