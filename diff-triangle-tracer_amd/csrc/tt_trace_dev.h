@@ -920,8 +920,8 @@ constexpr int F_OPAC = 9, F_SIGMA = 10, F_SH = 11;
 // probing, 8 probes), or -1 when full.
 template <int SLOTS>
 TT_D int claim_slot(int *keys, uint32_t prim) {
-  static_assert(SLOTS == 64 || SLOTS == 128 || SLOTS == 256 || SLOTS == 512, "table size");
-  constexpr int bits = SLOTS == 64 ? 6 : (SLOTS == 128 ? 7 : (SLOTS == 256 ? 8 : 9));
+  static_assert(SLOTS >= 64 && SLOTS <= 2048 && (SLOTS & (SLOTS - 1)) == 0, "table size");
+  constexpr int bits = __builtin_ctz(SLOTS);
   uint32_t h = (prim * 2654435761u) >> (32 - bits);
   for (int probe = 0; probe < 8; probe++) {
     const int old = atomicCAS(&keys[h], -1, (int)prim);
@@ -1076,7 +1076,7 @@ constexpr int rec_batch() {  // records grouped per pass (two shorts each in the
 }
 template <int BLOCK>
 constexpr int agg_slots() {  // triangles a block groups one batch of records into
-  return 2 * BLOCK;
+  return 8 * BLOCK;
 }
 constexpr int kRowLocal = 4, kRowGlobal = 5;  // float4 per record: block-summed / launch-summed
 template <int BLOCK>
@@ -1091,7 +1091,7 @@ constexpr int kStageSlots = 32;  // triangles whose sums are staged in LDS per c
 constexpr int kSegRecords = 16;  // records one thread sums at most (a triangle's longer runs are split)
 template <int BLOCK>
 constexpr int rec_tab_ints() {  // s_tab of backward_block_rec: grouping table + flush stage
-  return 5 * agg_slots<BLOCK>() + 64 * kStageSlots;
+  return 3 * agg_slots<BLOCK>() + 64 * kStageSlots;
 }
 template <int BLOCK, bool LOCAL>
 TT_D void emit_record(const TraceParams &P, uint32_t prim, int off, v3f d, const float dc[3], bool c0, bool c1,
@@ -1516,9 +1516,12 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
   constexpr int kBatch = rec_batch<BLOCK>(), kAggSlots = agg_slots<BLOCK>();
   static_assert(kLdsInts * BLOCK * 4 >= 2 * kBatch * 2, "the grouping reuses the traversal LDS");
   static_assert(VARIANT != 4, "block-summed records hold one direction per lane: one segment kind per launch");
-  int *t_keys = s_tab, *t_cnt = s_tab + kAggSlots, *t_off = s_tab + 2 * kAggSlots, *t_fill = s_tab + 3 * kAggSlots;
-  int *t_item = s_tab + 4 * kAggSlots;  // per slot: its first work item (segment of <= kSegRecords records)
-  float *stage = reinterpret_cast<float *>(s_tab + 5 * kAggSlots);
+  // per slot: triangle id, record count (low 16 bits) and records listed so far (high 16 bits),
+  // first list position and first work item (segment of <= kSegRecords records), 16 bits each
+  int *t_keys = s_tab, *t_cf = s_tab + kAggSlots;
+  short *t_off = reinterpret_cast<short *>(s_tab + 2 * kAggSlots), *t_item = t_off + kAggSlots;
+  float *stage = reinterpret_cast<float *>(s_tab + 3 * kAggSlots);
+  static_assert(kBatch < 32768 && kBatch / kSegRecords + kAggSlots < 32768, "16-bit list positions and items");
   if (threadIdx.x == 0) {
     const int lane_rows = P.rec_lane_rows > 0 ? P.rec_lane_rows : 16;
     int start = blockIdx.x * BLOCK * lane_rows, rows = BLOCK * lane_rows;
@@ -1530,7 +1533,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
     s_rec[1] = start;
     s_rec[2] = rows;
   }
-  for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cnt[i] = 0, t_fill[i] = 0;
+  for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cf[i] = 0;
   __syncthreads();
   if (VARIANT == 5) {  // the bounce primaries: environment gradient aggregated per wave
     float eg[3] = {0.0f, 0.0f, 0.0f};
@@ -1550,7 +1553,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
     const int n = min(kBatch, n_all - b0);
     const size_t first = start + b0;
     if (b0 > 0) {
-      for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cnt[i] = 0, t_fill[i] = 0;
+      for (int i = threadIdx.x; i < kAggSlots; i += BLOCK) t_keys[i] = -1, t_cf[i] = 0;
       __syncthreads();
     }
     for (int i = threadIdx.x; i < n; i += BLOCK) {
@@ -1558,7 +1561,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
           b0 == 0 ? (uint32_t)s_rec[kRecLds + i] : __float_as_uint(P.rec_rows[kRowLocal * (first + i)].x);
       const int sl = claim_slot<kAggSlots>(t_keys, prim);
       slot_of[i] = (short)sl;
-      if (sl >= 0) atomicAdd(&t_cnt[sl], 1);
+      if (sl >= 0) atomicAdd(&t_cf[sl], 1);
     }
     __syncthreads();
     if (threadIdx.x < 64) {  // exclusive scan of the slot counts: kAggSlots / 64 per lane + a wave scan
@@ -1566,7 +1569,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
       const int l = threadIdx.x;
       int c[per], sum = 0, items = 0;
       for (int j = 0; j < per; j++) {
-        c[j] = t_cnt[per * l + j];
+        c[j] = t_cf[per * l + j] & 0xffff;
         sum += c[j];
         items += (c[j] + kSegRecords - 1) / kSegRecords;
       }
@@ -1577,8 +1580,8 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
       }
       int run = inc - sum, run_items = inc_items - items;
       for (int j = 0; j < per; j++) {
-        t_off[per * l + j] = run;
-        t_item[per * l + j] = run_items;
+        t_off[per * l + j] = (short)run;
+        t_item[per * l + j] = (short)run_items;
         run += c[j];
         run_items += (c[j] + kSegRecords - 1) / kSegRecords;
       }
@@ -1588,7 +1591,7 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
     for (int i = threadIdx.x; i < n; i += BLOCK) {
       const int sl = slot_of[i];
       if (sl >= 0) {
-        list[t_off[sl] + atomicAdd(&t_fill[sl], 1)] = (short)i;
+        list[t_off[sl] + (atomicAdd(&t_cf[sl], 0x10000) >> 16)] = (short)i;
       } else {  // no slot left: this record goes out on its own
         float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         ShSink sh;
@@ -1614,7 +1617,8 @@ TT_D void backward_block_rec(const TraceParams &P, bool valid, int pix, int tile
         }
         prim = t_keys[lo];
         vrow = P.cumsum[prim];
-        const int b = t_off[lo] + (it - t_item[lo]) * kSegRecords, e = min(t_off[lo] + t_cnt[lo], b + kSegRecords);
+        const int b = t_off[lo] + (it - t_item[lo]) * kSegRecords,
+                  e = min(t_off[lo] + (t_cf[lo] & 0xffff), b + kSegRecords);
 #pragma unroll 2
         for (int j = b; j < e; j++) add_record<true>(P, first + list[j], acc, sh, rec_dirs<BLOCK>(s_rec));
       }
