@@ -214,6 +214,13 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
   int *ov = P.ov_stack + ((size_t)blockIdx.x * BLOCK + lane) * P.ov_depth;
   int *s_leaf = s_node + kStack * BLOCK;
   const float lo_w = tlo - fabsf(tlo) * kWideRel - kWideAbs;
+  // the upper culling bound (the K-th key's t, or far_t), widened; it changes only when a hit
+  // is inserted, so it is formed there rather than in every node step
+  auto upper = [&]() {
+    const float thi = fminf(far_t, b.t(K - 1));
+    return thi + fabsf(thi) * kWideRel + kWideAbs;
+  };
+  float hi_w = upper();
   int node = 0;
   int sp = 0;
   int nq = 0;  // queued leaves of this lane
@@ -230,23 +237,22 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       float4 tn, tf;
       const int4 ref = qref;
       slab4q(r, np, qa, qx, qy, qz, tn, tf);
-      const float thi = fminf(far_t, b.t(K - 1));
-      const float hi_w = thi + fabsf(thi) * kWideRel + kWideAbs;
       const float tn0 = tn.x, tf0 = tf.x, tn1 = tn.y, tf1 = tf.y, tn2 = tn.z, tf2 = tf.z, tn3 = tn.w, tf3 = tf.w;
       bool h0 = (ref.x != kEmptyChild) & box_ok_wide(tn0, tf0, lo_w, hi_w);
       bool h1 = (ref.y != kEmptyChild) & box_ok_wide(tn1, tf1, lo_w, hi_w);
       bool h2 = (ref.z != kEmptyChild) & box_ok_wide(tn2, tf2, lo_w, hi_w);
       bool h3 = (ref.w != kEmptyChild) & box_ok_wide(tn3, tf3, lo_w, hi_w);
       // hit leaves to the queue without branches: slots nq .. nq + 3 are free (nq <= kLeafQ - 4);
-      // a write that is not counted is overwritten by the next one or lies past the queue's end
+      // a write that is not counted is overwritten by the next one or lies past the queue's end.
+      // The queue holds the child references (~leaf), complemented when a leaf is tested.
       const bool l0 = h0 & (ref.x < 0), l1 = h1 & (ref.y < 0), l2 = h2 & (ref.z < 0), l3 = h3 & (ref.w < 0);
-      s_leaf[nq * BLOCK + lane] = ~ref.x;
+      s_leaf[nq * BLOCK + lane] = ref.x;
       nq += l0;
-      s_leaf[nq * BLOCK + lane] = ~ref.y;
+      s_leaf[nq * BLOCK + lane] = ref.y;
       nq += l1;
-      s_leaf[nq * BLOCK + lane] = ~ref.z;
+      s_leaf[nq * BLOCK + lane] = ref.z;
       nq += l2;
-      s_leaf[nq * BLOCK + lane] = ~ref.w;
+      s_leaf[nq * BLOCK + lane] = ref.w;
       nq += l3;
       h0 &= !l0;
       h1 &= !l1;
@@ -285,7 +291,7 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
     const bool test = nq > 0 && (walked || nq > kLeafQ - 4 || 2 * busy >= active);
     float4 a, bb, c;
     if (test) {
-      const int s = s_leaf[--nq * BLOCK + lane];
+      const int s = ~s_leaf[--nq * BLOCK + lane];
       a = P.leaves[3 * s], bb = P.leaves[3 * s + 1], c = P.leaves[3 * s + 2];
     }
     fetch(node >= 0 ? node : 0);  // every iteration: the root once the walk is over (never read)
@@ -294,8 +300,10 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       const uint32_t prim = __float_as_uint(a.w);
       float t;
       if (intersect(r, mk(a.x, a.y, a.z), mk(bb.x, bb.y, bb.z), mk(c.x, c.y, c.z), t) && t <= far_t &&
-          after_key(t, prim, tlo, plo))
+          after_key(t, prim, tlo, plo)) {
         kinsert(b, t, prim);
+        hi_w = upper();
+      }
     }
     if (walked && nq == 0) return;
   }
