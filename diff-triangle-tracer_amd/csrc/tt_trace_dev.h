@@ -238,30 +238,28 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       const int4 ref = qref;
       slab4q(r, np, qa, qx, qy, qz, tn, tf);
       const float tn0 = tn.x, tf0 = tf.x, tn1 = tn.y, tf1 = tf.y, tn2 = tn.z, tf2 = tf.z, tn3 = tn.w, tf3 = tf.w;
-      bool h0 = (ref.x != kEmptyChild) & box_ok_wide(tn0, tf0, lo_w, hi_w);
-      bool h1 = (ref.y != kEmptyChild) & box_ok_wide(tn1, tf1, lo_w, hi_w);
-      bool h2 = (ref.z != kEmptyChild) & box_ok_wide(tn2, tf2, lo_w, hi_w);
-      bool h3 = (ref.w != kEmptyChild) & box_ok_wide(tn3, tf3, lo_w, hi_w);
+      // boxes hit (kEmptyChild: no child); a hit child is a leaf (ref < 0) or internal
+      const bool h0 = (ref.x != kEmptyChild) & box_ok_wide(tn0, tf0, lo_w, hi_w);
+      const bool h1 = (ref.y != kEmptyChild) & box_ok_wide(tn1, tf1, lo_w, hi_w);
+      const bool h2 = (ref.z != kEmptyChild) & box_ok_wide(tn2, tf2, lo_w, hi_w);
+      const bool h3 = (ref.w != kEmptyChild) & box_ok_wide(tn3, tf3, lo_w, hi_w);
       // hit leaves to the queue without branches: slots nq .. nq + 3 are free (nq <= kLeafQ - 4);
       // a write that is not counted is overwritten by the next one or lies past the queue's end.
       // The queue holds the child references (~leaf), complemented when a leaf is tested.
-      const bool l0 = h0 & (ref.x < 0), l1 = h1 & (ref.y < 0), l2 = h2 & (ref.z < 0), l3 = h3 & (ref.w < 0);
       s_leaf[nq * BLOCK + lane] = ref.x;
-      nq += l0;
+      nq += h0 & (ref.x < 0);
       s_leaf[nq * BLOCK + lane] = ref.y;
-      nq += l1;
+      nq += h1 & (ref.y < 0);
       s_leaf[nq * BLOCK + lane] = ref.z;
-      nq += l2;
+      nq += h2 & (ref.z < 0);
       s_leaf[nq * BLOCK + lane] = ref.w;
-      nq += l3;
-      h0 &= !l0;
-      h1 &= !l1;
-      h2 &= !l2;
-      h3 &= !l3;
+      nq += h3 & (ref.w < 0);
+      // internal hit children by entry distance (the others at +inf, which no hit box reaches:
+      // tn <= hi_w, finite), so after the sort child i is to be visited iff d_i < inf
       const float inf = __builtin_huge_valf();
-      float d0 = h0 ? tn0 : inf, d1 = h1 ? tn1 : inf, d2 = h2 ? tn2 : inf, d3 = h3 ? tn3 : inf;
+      float d0 = (h0 & (ref.x >= 0)) ? tn0 : inf, d1 = (h1 & (ref.y >= 0)) ? tn1 : inf,
+            d2 = (h2 & (ref.z >= 0)) ? tn2 : inf, d3 = (h3 & (ref.w >= 0)) ? tn3 : inf;
       int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
-      const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
       cswap(d0, c0, d1, c1);
       cswap(d2, c2, d3, c3);
       cswap(d0, c0, d2, c2);
@@ -269,17 +267,17 @@ TT_D void gather(const TraceParams &P, const RayPre &r, float tlo, long long plo
       cswap(d1, c1, d2, c2);
       if (sp + 3 <= kStack) {  // the usual case: LDS slots sp .. sp + 2, written without branches
         s_node[sp * BLOCK + lane] = c3;
-        sp += cnt >= 4;
+        sp += d3 < inf;
         s_node[sp * BLOCK + lane] = c2;
-        sp += cnt >= 3;
+        sp += d2 < inf;
         s_node[sp * BLOCK + lane] = c1;
-        sp += cnt >= 2;
+        sp += d1 < inf;
       } else {
-        if (cnt >= 4) push(c3);
-        if (cnt >= 3) push(c2);
-        if (cnt >= 2) push(c1);
+        if (d3 < inf) push(c3);
+        if (d2 < inf) push(c2);
+        if (d1 < inf) push(c1);
       }
-      node = cnt >= 1 ? c0 : -1;
+      node = d0 < inf ? c0 : -1;
       if (node < 0 && sp > 0) {
         sp--;
         node = s_node[min(sp, kStack - 1) * BLOCK + lane];
