@@ -36,7 +36,7 @@ namespace tt {
 namespace {
 
 constexpr int kStack = kLdsStack;  // LDS entries per lane (deferred node index)
-constexpr int kLeafQ = 8;          // LDS queue of postponed leaves per lane
+constexpr int kLeafQ = 12;         // LDS queue of postponed leaves per lane
 constexpr int kLdsInts = kStack + kLeafQ;  // ints of LDS per lane the gather uses
 
 // Pixel tile of a block: one 8x8 quadrant per wave; 64 lanes -> 8x8, 128 -> 16x8, 256 -> 16x16.
