@@ -62,9 +62,9 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 # valu_busy = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x GRBM_GUI_ACTIVE / XCDs).
 PROFILE_SOURCE = "profiles/trace_fwd_bwd_1M_1080p.md"
 MEASURED = {
-    "k_forward<16,0,false,1>": dict(traffic=6.162e9, avg_ms=4.186, l2_hit=0.692, valu_busy=0.68),
-    "k_forward<16,0,false,0>": dict(traffic=6.190e9, avg_ms=4.207, l2_hit=0.692, valu_busy=0.69),
-    "k_backward_rec<16,0>": dict(traffic=4.692e9, avg_ms=1.792, l2_hit=0.632, valu_busy=0.49),
+    "k_forward<16,0,false,1>": dict(traffic=6.166e9, avg_ms=4.144, l2_hit=0.691, valu_busy=0.68),
+    "k_forward<16,0,false,0>": dict(traffic=6.197e9, avg_ms=4.161, l2_hit=0.691, valu_busy=0.68),
+    "k_backward_rec<16,0>": dict(traffic=4.692e9, avg_ms=1.788, l2_hit=0.632, valu_busy=0.49),
 }
 # densify and scene-file kernels at 1M splats (profiles/densify_sceneio_1M.md)
 MEASURED_TRAFFIC = {"k_apply": 1.058e9, "k_scene_unpack": 0.493e9}
