@@ -2,7 +2,7 @@
 formula does not restate: textbook constants and orthonormality, barycentric
 closed forms, exact rational arithmetic, numpy's libm.
 
-The reference holds no golden vectors or tests (SURVEY: no tests/, no fixtures),
+The reference holds no golden vectors or tests (it has no tests/ directory and no fixtures),
 so these closed forms and invariants are what fixes the oracle.
 """
 import math
