@@ -421,8 +421,18 @@ __global__ void k_prop_reduce(const float *partials, int tiles, int mode, float 
                               float lr, float b1, float b2, float eps, int t, float *loss_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > kParams) return;
+  // the tiles' rows summed in tile order; 16 rows loaded ahead of their adds (same order, so
+  // the same result bit for bit, with 16 loads in flight per thread instead of one)
   float g = 0.0f;
-  for (int i = 0; i < tiles; i++) g += partials[(size_t)i * kPartStride + p];
+  int i = 0;
+  for (; i + 16 <= tiles; i += 16) {
+    float x[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = partials[(size_t)(i + j) * kPartStride + p];
+#pragma unroll
+    for (int j = 0; j < 16; j++) g += x[j];
+  }
+  for (; i < tiles; i++) g += partials[(size_t)i * kPartStride + p];
   if (p == kParams) {
     if (loss_out) *loss_out = g;
     return;
@@ -634,7 +644,7 @@ int tt_proposal_loss_grad(tt_proposal *p, const float *ray_o_dev, const float *r
                      reinterpret_cast<const float4 *>(interval_dev), (long long)B, 0, 0ull,
                      reinterpret_cast<const bf16x8 *>(p->F), p->W, hyper(p), 1.0f / (float)B, p->partials);
   PCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_prop_reduce, dim3((kParams + 256) / 256), dim3(256), 0, s, p->partials, B / 32, 0, grad_dev,
+  hipLaunchKernelGGL(k_prop_reduce, dim3((kParams + 64) / 64), dim3(64), 0, s, p->partials, B / 32, 0, grad_dev,
                      p->W, p->m, p->v, 0.0f, 0.0f, 0.0f, 0.0f, 0, loss_dev);
   PCHK(hipGetLastError());
   return TT_OK;
@@ -655,7 +665,7 @@ int tt_proposal_train(tt_proposal *p, const float *ray_o_dev, const float *ray_d
     hipLaunchKernelGGL(k_prop_train, dim3(batch / 32), dim3(64), 0, s, ray_o_dev, ray_d_dev,
                        reinterpret_cast<const float4 *>(interval_dev), n_rays, 1, sd,
                        reinterpret_cast<const bf16x8 *>(p->F), p->W, hyper(p), 1.0f / (float)batch, p->partials);
-    hipLaunchKernelGGL(k_prop_reduce, dim3((kParams + 256) / 256), dim3(256), 0, s, p->partials, batch / 32, 1,
+    hipLaunchKernelGGL(k_prop_reduce, dim3((kParams + 64) / 64), dim3(64), 0, s, p->partials, batch / 32, 1,
                        nullptr, p->W, p->m, p->v, p->hp.lr, p->hp.beta1, p->hp.beta2, p->hp.eps, p->t,
                        loss_hist_dev ? loss_hist_dev + i : nullptr);
     hipLaunchKernelGGL(k_prop_pack, dim3((kFrags * 512 + 255) / 256), dim3(256), 0, s, p->W, p->F);
